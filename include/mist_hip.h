@@ -1,0 +1,182 @@
+/*
+ * mist_hip.h -- C ABI of libmist_hip.so, the MI355X (gfx950) implementation of the
+ * per-step hot path of Mist.jl / NS_Multiphase:
+ *   - semi-Lagrangian geometric VOF transport (ELVIRA normals, PLIC plane constants,
+ *     back-traced cell pre-images split into 5 tets, tet/mesh-plane/PLIC cutting), and
+ *   - the variable-density pressure Poisson multigrid solve (red-black Gauss-Seidel,
+ *     residual, 8-cell restriction, trilinear prolongation) in FP64 or as FP32 V-cycles
+ *     inside an FP64 defect-correction loop.
+ *
+ * Conventions
+ *   - Plain C: no C++ types, no torch types. Every entry point returns an int status
+ *     (MIST_OK = 0); mist_last_error() gives the message of the last failure.
+ *   - Fields live in device memory inside a context. Host arrays cross the boundary by
+ *     mist_upload / mist_download as flat copies of the reference's ghosted, column-major
+ *     (i fastest) OffsetArray parents, so a Julia caller passes `parent(A)` unchanged.
+ *   - Indices follow the reference: interior cells are 1..N per axis, ghosts extend to
+ *     1-nghost and N+nghost, nghost = 3 + max(0, ceil(CFL)-2) (src/Mesh.jl:66). The volume
+ *     fraction carries 3 extra ghost layers (src/Tools.jl:144); face density/viscosity
+ *     arrays carry one extra face on their own axis (src/Tools.jl:169-177).
+ *   - All work of one context is ordered on one HIP stream. One host thread per context.
+ *     download, reduce and the calls that return scalars synchronise that stream.
+ */
+#ifndef MIST_HIP_H
+#define MIST_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------------- */
+enum {
+    MIST_OK = 0,
+    MIST_ERR_PARAM = 1,     /* invalid parameter / unsupported option */
+    MIST_ERR_OOM = 2,       /* device allocation failed */
+    MIST_ERR_HIP = 3,       /* HIP runtime error */
+    MIST_ERR_NOCONV = 4,    /* solver stopped at its cycle limit above tolerance */
+    MIST_ERR_STATE = 5,     /* call order / missing field */
+    MIST_ERR_OVERFLOW = 6,  /* a geometric work bound was exceeded (cut depth, walk) */
+    MIST_ERR_NODEVICE = 7   /* no HIP device available */
+};
+
+/* ---- string-valued options of the reference `parameters` struct, as enums -------- */
+enum { MIST_PS_SEMI_LAGRANGIAN = 0, MIST_PS_FINITE_DIFFERENCE = 1 };      /* pressure_scheme */
+enum { MIST_NORMAL_ELVIRA = 0 };                                           /* normalMethod */
+enum { MIST_VEL_NONE = 0, MIST_VEL_DEFORMATION = 1, MIST_VEL_DEFORMATION3D = 2,
+       MIST_VEL_ROTATION = 3, MIST_VEL_DIVFLOW = 4, MIST_VEL_DIVFLOW3D = 5 }; /* VFVelocity */
+enum { MIST_PROJ_RK4 = 0, MIST_PROJ_EULER = 1, MIST_PROJ_MIDPOINT = 2 };    /* projection_method */
+enum { MIST_TESS_5_TETS = 0 };                                              /* tesselation */
+enum { MIST_CASE_GENERIC = 0, MIST_CASE_ZALESAK = 1 };                      /* test_case */
+enum { MIST_MG_FP64 = 0, MIST_MG_MIXED = 1 };                               /* new: V-cycle precision */
+enum { MIST_COARSE_FACE_AVERAGE = 0, MIST_COARSE_RESTRICTED_VF = 1 };       /* new: coarse operator */
+
+/*
+ * POD mirror of the fields of `parameters` (src/Parameters.jl:3-91) that the hot path
+ * reads. Strings become the enums above. Fields marked "new" have no reference
+ * counterpart; mist_params_default() fills them.
+ */
+typedef struct mist_params {
+    double mu_liq, mu_gas, rho_liq, rho_gas, sigma;
+    double grav_x, grav_y, grav_z;
+    double Lx, Ly, Lz;
+    double tFinal;
+    int64_t Nx, Ny, Nz;
+    int64_t stepMax;
+    double max_dt;          /* reference default Inf */
+    double CFL;
+    int64_t xper, yper, zper;
+    int64_t solveNS;
+    double tol;
+    int64_t pressure_scheme;
+    int64_t mg_lvl;
+    double VFlo, VFhi;      /* defaults 1e-10 and 1-VFlo */
+    int64_t nband;          /* default ceil(CFL) */
+    int64_t normal_method;
+    int64_t vf_velocity;
+    int64_t projection_method;
+    int64_t tesselation;
+    int64_t test_case;
+    /* new */
+    double vel_period;      /* T in cos(pi t / T) of the deformation fields; 2.0 = reference */
+    int64_t mg_precision;   /* MIST_MG_FP64 or MIST_MG_MIXED */
+    int64_t mg_nu1, mg_nu2; /* smoothing sweeps, reference 10 and 10 */
+    int64_t mg_max_cycles;  /* outer cycle limit, reference 10000 */
+    int64_t mg_coarse_op;   /* MIST_COARSE_FACE_AVERAGE or MIST_COARSE_RESTRICTED_VF */
+    int64_t mg_coarse_sweeps; /* RB-GS sweeps on the coarsest level */
+} mist_params;
+
+/* ---- field identifiers (device-resident arrays of a context) --------------------- */
+enum {
+    MIST_FIELD_VOF = 0,   /* VF: volume fraction, cell extents + 3 extra ghost layers, f64 */
+    MIST_FIELD_NX = 1, MIST_FIELD_NY = 2, MIST_FIELD_NZ = 3,   /* interface normal, f64 */
+    MIST_FIELD_PLIC_D = 4,  /* D: PLIC plane constant, n.x = D, f64 */
+    MIST_FIELD_BAND = 5,    /* band: int16 */
+    MIST_FIELD_U = 6, MIST_FIELD_V = 7, MIST_FIELD_W = 8,      /* cell-centre velocity */
+    MIST_FIELD_UF = 9, MIST_FIELD_VF = 10, MIST_FIELD_WF = 11, /* face velocity, cell extents */
+    MIST_FIELD_P = 12,      /* pressure */
+    MIST_FIELD_DIVG = 13,   /* divergence diagnostic */
+    MIST_FIELD_DENX = 14, MIST_FIELD_DENY = 15, MIST_FIELD_DENZ = 16, /* face density */
+    MIST_FIELD_RHS = 17,    /* pressure right-hand side b (interior cells are used) */
+    MIST_FIELD_COUNT = 18
+};
+
+enum { MIST_REDUCE_SUM = 0, MIST_REDUCE_MAX = 1, MIST_REDUCE_MIN = 2, MIST_REDUCE_MAXABS = 3 };
+
+enum { MIST_IC_CIRCLE = 0, MIST_IC_SPHERE = 1, MIST_IC_BUBBLE3D = 2 };
+
+typedef struct mist_ctx mist_ctx;
+
+/* Fill *p with the reference defaults (src/Parameters.jl:3-91) plus the new fields. */
+int mist_params_default(mist_params* p);
+
+/* Create a context on HIP device `device`: builds the mesh (src/Mesh.jl:55-196, one rank)
+ * and allocates every field in device memory (src/Tools.jl:136-191). */
+int mist_ctx_create(const mist_params* p, int device, mist_ctx** out);
+int mist_ctx_destroy(mist_ctx* ctx);
+const char* mist_last_error(const mist_ctx* ctx);   /* ctx may be NULL: last global error */
+
+/* Extents of a field: lo = lowest (1-based reference) index per axis, n = length per axis,
+ * elem = element size in bytes. */
+int mist_field_extents(const mist_ctx* ctx, int field, int64_t lo[3], int64_t n[3], int64_t* elem);
+/* Device pointer of a field (for zero-copy interop; owned by the context). */
+int mist_field_device_ptr(mist_ctx* ctx, int field, void** dptr);
+
+int mist_upload(mist_ctx* ctx, int field, const void* host, size_t nbytes);
+int mist_download(mist_ctx* ctx, int field, void* host, size_t nbytes);
+
+/* Initial volume fraction over the cell extents (IC! loops over imino_:imaxo_):
+ * MIST_IC_CIRCLE args = {rad, xo, yo}          (VFcircle, src/Tools.jl:790-902)
+ * MIST_IC_SPHERE args = {rad, xo, yo, zo}      (VFsphere, src/Tools.jl:907-922)
+ * MIST_IC_BUBBLE3D args = {rad, xo, yo, zo}    (VFbubble3d, src/Tools.jl:1061-1076) */
+int mist_init_vof(mist_ctx* ctx, int shape, const double* args, int nargs);
+
+/* Stage entry points (one each per reference function). */
+int mist_neumann(mist_ctx* ctx, int field);             /* Neumann!  src/Tools.jl:4-18 */
+int mist_update_borders(mist_ctx* ctx, int field);      /* update_borders!  src/Parallel.jl:63-143 */
+int mist_define_velocity(mist_ctx* ctx, double t);      /* defineVelocity!  src/Tools.jl:648-693 */
+int mist_compute_dt(mist_ctx* ctx, double* dt);         /* compute_dt  src/Tools.jl:317-337 */
+int mist_compute_band(mist_ctx* ctx);                   /* computeBand!  src/VF.jl:16-74 */
+int mist_compute_normal(mist_ctx* ctx);                 /* computeNormal!/ELVIRA!  src/VF.jl:79, src/ELVIRA.jl:6-137 */
+int mist_compute_plic(mist_ctx* ctx);                   /* computePLIC!  src/VF.jl:94-103 */
+int mist_transport(mist_ctx* ctx, double dt, double t); /* VOF part of transport!  src/Transport.jl:2-268 */
+int mist_compute_props(mist_ctx* ctx);                  /* compute_dens!  src/Tools.jl:1212-1232 */
+int mist_pressure_rhs(mist_ctx* ctx);                   /* RHS = div(u*)  src/Pressure.jl:16-22 */
+/* Variable-density Poisson solve  A p = b,  A = dt div(rho_face^-1 grad) with wall faces
+ * dropped (src/Pressure.jl:909-1033 scaling), b = RHS field, to ||b - A p||_inf <= tol.
+ * Replaces pressure_solver!/mg_cycler (src/Pressure.jl:5-30, src/Multigrid.jl:333-371).
+ * precision: MIST_MG_FP64 or MIST_MG_MIXED. P (MIST_FIELD_P) is the initial guess and
+ * result (mean-zero gauge). */
+int mist_pressure_solve(mist_ctx* ctx, double dt, double tol, int64_t precision,
+                        int64_t* cycles, double* final_resid);
+int mist_corrector(mist_ctx* ctx, double dt);           /* corrector!  src/Velocity.jl:3-20 */
+int mist_reduce(mist_ctx* ctx, int kind, int field, double* out); /* interior cells, FP64 */
+
+/* One step of run_solver with solveNS=false (src/NavierStokes_Parallel.jl:125-178):
+ * dt = compute_dt, t += dt, defineVelocity!(t), transport!, computeBand!.
+ * t is read and advanced; dt_used returns the step. If clamp_to > 0 the step is
+ * clamped so t does not pass clamp_to (test/test_deformation.jl:200-205). */
+int mist_vof_step(mist_ctx* ctx, double* t, double clamp_to, double* dt_used);
+/* Initialisation of run_solver for the VOF fields (src/NavierStokes_Parallel.jl:73-98):
+ * Neumann!(VF), update_VF_borders!, computeBand!, computeNormal!, computePLIC!. */
+int mist_vof_init(mist_ctx* ctx);
+
+/* Batched PLIC primitives (first slice; host arrays of length m).
+ * computeDist (src/VF.jl:109-156) for cells (i,j,k) with normals n and fractions vf. */
+int mist_plic_dist_batch(mist_ctx* ctx, const int64_t* ijk, const double* nrm,
+                         const double* vf, double* dist_out, int64_t m);
+/* computePLIC2VF (src/VF.jl:219-278): liquid fraction of cell (i,j,k) under plane (n, D). */
+int mist_plic_vf_batch(mist_ctx* ctx, const int64_t* ijk, const double* nrm,
+                       const double* dist, double* vf_out, int64_t m);
+
+/* Per-stage device time of the last calls (ms), for profiling: kinds are the stage
+ * entry points in order normal, transport, plic, band, mg. */
+int mist_stage_times(const mist_ctx* ctx, double* ms, int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MIST_HIP_H */

@@ -1,0 +1,302 @@
+// oracle_capi.cpp -- TEST INFRASTRUCTURE ONLY (the oracle): extern "C" entry points used by
+// tests/ (through ctypes) and by bench.py's cpu_baseline leg. They mirror the product's
+// C ABI (include/mist_hip.h) on host arrays, plus single-cell primitives for the pins.
+#include <cstring>
+#include <memory>
+#include <string>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "oracle_core.h"
+
+using namespace ora;
+
+namespace ora {
+CutScratch* new_cut_scratch();
+void delete_cut_scratch(CutScratch* s);
+}
+
+static std::string g_err;
+
+static int check_params(const mist_params* p, std::string& err) {
+    if (p->Nx < 1 || p->Ny < 1 || p->Nz < 1) { err = "Nx, Ny, Nz must be >= 1"; return MIST_ERR_PARAM; }
+    if (!(p->Lx > 0 && p->Ly > 0 && p->Lz > 0)) { err = "Lx, Ly, Lz must be > 0"; return MIST_ERR_PARAM; }
+    if (p->tesselation != MIST_TESS_5_TETS) { err = "incompatible tesselation called"; return MIST_ERR_PARAM; }
+    if (p->normal_method != MIST_NORMAL_ELVIRA) { err = "Unknown method to copute interface normal"; return MIST_ERR_PARAM; }
+    if (p->projection_method < 0 || p->projection_method > 2) { err = "Unknown projection_method in project!"; return MIST_ERR_PARAM; }
+    return MIST_OK;
+}
+
+extern "C" {
+
+void ora_params_default(mist_params* p) {
+    std::memset(p, 0, sizeof(*p));
+    p->max_dt = INFINITY;
+    p->solveNS = 1;
+    p->pressure_scheme = MIST_PS_SEMI_LAGRANGIAN;
+    p->mg_lvl = 1;
+    p->VFlo = 1e-10;
+    p->VFhi = 1.0 - 1e-10;
+    p->nband = -1;
+    p->vel_period = 2.0;
+    p->mg_precision = MIST_MG_FP64;
+    p->mg_nu1 = 10;
+    p->mg_nu2 = 10;
+    p->mg_max_cycles = 10000;
+    p->mg_coarse_op = MIST_COARSE_FACE_AVERAGE;
+    p->mg_coarse_sweeps = 64;
+}
+
+int ora_ctx_create(const mist_params* p, void** out) {
+    *out = nullptr;
+    std::string err;
+    int st = check_params(p, err);
+    if (st) { g_err = err; return st; }
+    auto c = std::make_unique<Ctx>();
+    c->p = *p;
+    if (c->p.nband < 0) c->p.nband = (int64_t)std::ceil(c->p.CFL);
+    c->m.build(c->p, 1);
+    const Ext ce = c->m.cell_ext();
+    c->VF.init(c->m.vof_ext());
+    for (Arr3<double>* a : {&c->nx, &c->ny, &c->nz, &c->D, &c->u, &c->v, &c->w, &c->uf, &c->vf, &c->wf,
+                            &c->P, &c->divg, &c->RHS})
+        a->init(ce);
+    c->denx.init(c->m.face_ext(0));
+    c->deny.init(c->m.face_ext(1));
+    c->denz.init(c->m.face_ext(2));
+    c->band.init(ce);
+    *out = c.release();
+    return MIST_OK;
+}
+
+void ora_ctx_destroy(void* h) { delete (Ctx*)h; }
+
+const char* ora_last_error(void* h) { return h ? ((Ctx*)h)->err.c_str() : g_err.c_str(); }
+
+int ora_field_extents(void* h, int field, int64_t lo[3], int64_t n[3], int64_t* elem) {
+    Ctx& c = *(Ctx*)h;
+    const Ext* e = nullptr;
+    if (field == MIST_FIELD_BAND) { e = &c.band.e; *elem = 2; }
+    else { Arr3<double>* a = field_f64(c, field); if (!a) return MIST_ERR_PARAM; e = &a->e; *elem = 8; }
+    for (int d = 0; d < 3; ++d) { lo[d] = e->lo[d]; n[d] = e->n[d]; }
+    return MIST_OK;
+}
+
+int ora_set_field(void* h, int field, const void* src, size_t nbytes) {
+    Ctx& c = *(Ctx*)h;
+    if (field == MIST_FIELD_BAND) {
+        if (nbytes != c.band.a.size() * 2) return MIST_ERR_PARAM;
+        std::memcpy(c.band.a.data(), src, nbytes);
+        return MIST_OK;
+    }
+    Arr3<double>* a = field_f64(c, field);
+    if (!a || nbytes != a->a.size() * 8) return MIST_ERR_PARAM;
+    std::memcpy(a->a.data(), src, nbytes);
+    return MIST_OK;
+}
+
+int ora_get_field(void* h, int field, void* dst, size_t nbytes) {
+    Ctx& c = *(Ctx*)h;
+    if (field == MIST_FIELD_BAND) {
+        if (nbytes != c.band.a.size() * 2) return MIST_ERR_PARAM;
+        std::memcpy(dst, c.band.a.data(), nbytes);
+        return MIST_OK;
+    }
+    Arr3<double>* a = field_f64(c, field);
+    if (!a || nbytes != a->a.size() * 8) return MIST_ERR_PARAM;
+    std::memcpy(dst, a->a.data(), nbytes);
+    return MIST_OK;
+}
+
+int ora_init_vof(void* h, int shape, const double* args, int nargs) {
+    Ctx& c = *(Ctx*)h;
+    const Mesh& m = c.m;
+    if ((shape == MIST_IC_CIRCLE && nargs < 3) || (shape != MIST_IC_CIRCLE && nargs < 4)) return MIST_ERR_PARAM;
+#pragma omp parallel for collapse(2) schedule(dynamic, 4)
+    for (int64_t k = m.kmino_; k <= m.kmaxo_; ++k)
+        for (int64_t j = m.jmino_; j <= m.jmaxo_; ++j)
+            for (int64_t i = m.imino_; i <= m.imaxo_; ++i) {
+                double v;
+                if (shape == MIST_IC_CIRCLE)
+                    v = vf_circle(m.x(i), m.x(i + 1), m.y(j), m.y(j + 1), args[0], args[1], args[2]);
+                else
+                    v = vf_sphere(m.x(i), m.x(i + 1), m.y(j), m.y(j + 1), m.z(k), m.z(k + 1), args[0], args[1],
+                                  args[2], args[3], shape == MIST_IC_BUBBLE3D);
+                c.VF(i, j, k) = v;
+            }
+    return MIST_OK;
+}
+
+int ora_neumann(void* h, int field) {
+    Ctx& c = *(Ctx*)h;
+    if (field == MIST_FIELD_BAND) { neumann_i16(c, c.band); return MIST_OK; }
+    Arr3<double>* a = field_f64(c, field);
+    if (!a) return MIST_ERR_PARAM;
+    neumann(c, *a);
+    return MIST_OK;
+}
+
+int ora_update_borders(void* h, int field) {
+    Ctx& c = *(Ctx*)h;
+    if (field == MIST_FIELD_BAND) { update_borders_i16(c, c.band); return MIST_OK; }
+    Arr3<double>* a = field_f64(c, field);
+    if (!a) return MIST_ERR_PARAM;
+    update_borders(c, *a);
+    return MIST_OK;
+}
+
+int ora_define_velocity(void* h, double t) { define_velocity(*(Ctx*)h, t); return MIST_OK; }
+int ora_compute_dt(void* h, double* dt) { *dt = compute_dt(*(Ctx*)h); return MIST_OK; }
+int ora_compute_band(void* h) { compute_band(*(Ctx*)h); return MIST_OK; }
+int ora_compute_normal(void* h) { compute_normal(*(Ctx*)h); return MIST_OK; }
+int ora_compute_plic(void* h) { compute_plic(*(Ctx*)h); return MIST_OK; }
+int ora_transport(void* h, double dt, double /*t*/) { return transport_vof(*(Ctx*)h, dt); }
+int ora_compute_props(void* h) { compute_dens(*(Ctx*)h); return MIST_OK; }
+int ora_pressure_rhs(void* h) { pressure_rhs(*(Ctx*)h); return MIST_OK; }
+int ora_corrector(void* h, double dt) { corrector(*(Ctx*)h, dt); return MIST_OK; }
+int ora_pressure_solve(void* h, double dt, double tol, int64_t precision, int64_t* cycles, double* resid) {
+    return pressure_solve(*(Ctx*)h, dt, tol, (int)precision, cycles, resid);
+}
+int64_t ora_max_cut_level(void* h) { return ((Ctx*)h)->max_cut_level; }
+
+// run_solver initialisation of the VOF fields (src/NavierStokes_Parallel.jl:73-98)
+int ora_vof_init(void* h) {
+    Ctx& c = *(Ctx*)h;
+    neumann(c, c.VF);
+    update_borders(c, c.VF);
+    compute_band(c);
+    compute_normal(c);
+    compute_plic(c);
+    return MIST_OK;
+}
+
+// One run_solver step with solveNS = false (src/NavierStokes_Parallel.jl:125-146)
+int ora_vof_step(void* h, double* t, double clamp_to, double* dt_used) {
+    Ctx& c = *(Ctx*)h;
+    double dt = compute_dt(c);
+    if (clamp_to > 0.0 && *t + dt > clamp_to) dt = clamp_to - *t;
+    *t += dt;
+    define_velocity(c, *t);
+    int st = transport_vof(c, dt);
+    compute_band(c);
+    *dt_used = dt;
+    return st;
+}
+
+// Reductions over interior cells in FP64 (sum in long double for a stable checker).
+int ora_reduce(void* h, int kind, int field, double* out) {
+    Ctx& c = *(Ctx*)h;
+    const Mesh& m = c.m;
+    Arr3<double>* a = field_f64(c, field);
+    if (!a) return MIST_ERR_PARAM;
+    long double s = 0.0L;
+    double r = kind == MIST_REDUCE_MIN ? INFINITY : (kind == MIST_REDUCE_MAX ? -INFINITY : 0.0);
+    for (int64_t k = m.kmin_; k <= m.kmax_; ++k)
+        for (int64_t j = m.jmin_; j <= m.jmax_; ++j)
+            for (int64_t i = m.imin_; i <= m.imax_; ++i) {
+                const double v = (*a)(i, j, k);
+                if (kind == MIST_REDUCE_SUM) s += v;
+                else if (kind == MIST_REDUCE_MAX) r = jmax(r, v);
+                else if (kind == MIST_REDUCE_MIN) r = jmin(r, v);
+                else r = jmax(r, std::fabs(v));
+            }
+    *out = kind == MIST_REDUCE_SUM ? (double)s : r;
+    return MIST_OK;
+}
+
+// ---- single-cell primitives for the pins ---------------------------------------------
+double ora_compute_alpha(double m0, double m1, double m2, double VF) { return compute_alpha(m0, m1, m2, VF); }
+double ora_compute_dist(void* h, int64_t i, int64_t j, int64_t k, double nx, double ny, double nz, double VF) {
+    return compute_dist(*(Ctx*)h, i, j, k, nx, ny, nz, VF);
+}
+double ora_plic2vf(void* h, int64_t i, int64_t j, int64_t k, double nx, double ny, double nz, double dist) {
+    return plic2vf(*(Ctx*)h, i, j, k, nx, ny, nz, dist);
+}
+int ora_d2case(const double d[4], int inplace, double dout[4]) {
+    double t[4] = {d[0], d[1], d[2], d[3]};
+    int cs = d2case_inplace(t);
+    if (dout) for (int n = 0; n < 4; ++n) dout[n] = inplace ? t[n] : d[n];
+    return cs + 1;  // reference numbering
+}
+double ora_tet_vol(const double v[12]) {
+    double t[4][3];
+    std::memcpy(t, v, sizeof(t));
+    return tet_vol(t);
+}
+double ora_cell2verts(void* h, int64_t i, int64_t j, int64_t k, double out[24]) {
+    double v[8][3];
+    const double s = cell2verts(*(Ctx*)h, i, j, k, v);
+    std::memcpy(out, v, sizeof(v));
+    return s;
+}
+void ora_project(void* h, double pt[3], int64_t i, int64_t j, int64_t k, double dt) { project(*(Ctx*)h, pt, i, j, k, dt); }
+int ora_pt2index(void* h, const double pt[3], int64_t i, int64_t j, int64_t k, int64_t out[3]) {
+    return pt2index(*(Ctx*)h, out, pt, i, j, k) ? 1 : 0;
+}
+void ora_vel_face(void* h, const double pt[3], int64_t i, int64_t j, int64_t k, double out[3]) {
+    Ctx& c = *(Ctx*)h;
+    out[0] = vel_uface(c, pt, i, j, k);
+    out[1] = vel_vface(c, pt, i, j, k);
+    out[2] = vel_wface(c, pt, i, j, k);
+}
+// cutTet of one tet with vertex indices: out = (vol, vLiq, vU, vV, vW, vrho); returns level.
+int ora_cut_tet(void* h, const double tet_in[12], const int64_t ind_in[12], double out[6]) {
+    Ctx& c = *(Ctx*)h;
+    double tet[4][3];
+    int64_t ind[4][3];
+    std::memcpy(tet, tet_in, sizeof(tet));
+    std::memcpy(ind, ind_in, sizeof(ind));
+    std::unique_ptr<CutScratch, void (*)(CutScratch*)> s(new_cut_scratch(), delete_cut_scratch);
+    CutSums r = cut_tet(c, tet, ind, false, false, false, 1, *s);
+    out[0] = r.vol; out[1] = r.vLiq; out[2] = r.vU; out[3] = r.vV; out[4] = r.vW; out[5] = r.vrho;
+    return r.lvl;
+}
+// Semi-Lagrangian volume of one band cell (src/Transport.jl:117-145): out = (vol, vLiq).
+int ora_sl_cell(void* h, int64_t i, int64_t j, int64_t k, double dt, double out[2]) {
+    Ctx& c = *(Ctx*)h;
+    double verts[8][3], tets[5][4][3];
+    int64_t vInds[8][3], inds[5][4][3];
+    const double tetsign = cell2verts(c, i, j, k, verts);
+    for (int n = 0; n < 8; ++n) project(c, verts[n], i, j, k, dt);
+    verts2tets5(verts, tets);
+    for (int n = 0; n < 8; ++n) pt2index(c, vInds[n], verts[n], i, j, k);
+    inds2tets5(vInds, inds);
+    std::unique_ptr<CutScratch, void (*)(CutScratch*)> s(new_cut_scratch(), delete_cut_scratch);
+    double vol = 0.0, vLiq = 0.0;
+    int lvl = 0;
+    for (int t = 0; t < 5; ++t) {
+        CutSums r = cut_tet(c, tets[t], inds[t], false, false, false, 1, *s);
+        vol += tetsign * r.vol;
+        vLiq += tetsign * r.vLiq;
+        lvl = std::max(lvl, r.lvl);
+    }
+    out[0] = vol;
+    out[1] = vLiq;
+    return lvl;
+}
+int ora_elvira_cell(void* h, int64_t i, int64_t j, int64_t k, double out[3]) {
+    int best = -1;
+    out[0] = out[1] = out[2] = 0.0;
+    elvira_calc(*(Ctx*)h, i, j, k, &out[0], &out[1], &out[2], &best);
+    return best;
+}
+
+void ora_set_count_flops(int on) { g_count_flops = on != 0; }
+void ora_get_flops(uint64_t out[2]) { out[0] = t_flops.flops; out[1] = t_flops.special; }
+void ora_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+int ora_get_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+}  // extern "C"

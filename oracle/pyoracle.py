@@ -1,0 +1,296 @@
+"""ctypes wrapper of the oracle (oracle/liboracle.so) -- TEST INFRASTRUCTURE ONLY.
+
+The oracle is the FP64 CPU restatement of the reference algorithm (see oracle_core.h). Only
+tests/, the package's smoke() and bench.py's cpu_baseline leg use it, and only as the
+checker. ``OracleContext`` offers the same stage methods as the product's ``Context`` so a
+parity test runs one sequence of calls on both.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from ns_multiphase_amd.params import FIELD, IC_SHAPES, REDUCE, MistParams, create_mesh
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("MIST_ORACLE_LIB", os.path.join(HERE, "liboracle.so"))
+
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the Makefile beside this file."""
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-C", HERE, "-B" if force else "liboracle.so"], check=True,
+                       stdout=subprocess.DEVNULL)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        vp, d, i64, i32 = ctypes.c_void_p, ctypes.c_double, ctypes.c_int64, ctypes.c_int
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int64)
+        sig = {
+            "ora_params_default": (None, [ctypes.POINTER(MistParams)]),
+            "ora_ctx_create": (i32, [ctypes.POINTER(MistParams), ctypes.POINTER(vp)]),
+            "ora_ctx_destroy": (None, [vp]),
+            "ora_last_error": (ctypes.c_char_p, [vp]),
+            "ora_field_extents": (i32, [vp, i32, ip, ip, ip]),
+            "ora_set_field": (i32, [vp, i32, vp, ctypes.c_size_t]),
+            "ora_get_field": (i32, [vp, i32, vp, ctypes.c_size_t]),
+            "ora_init_vof": (i32, [vp, i32, dp, i32]),
+            "ora_neumann": (i32, [vp, i32]),
+            "ora_update_borders": (i32, [vp, i32]),
+            "ora_define_velocity": (i32, [vp, d]),
+            "ora_compute_dt": (i32, [vp, dp]),
+            "ora_compute_band": (i32, [vp]),
+            "ora_compute_normal": (i32, [vp]),
+            "ora_compute_plic": (i32, [vp]),
+            "ora_transport": (i32, [vp, d, d]),
+            "ora_compute_props": (i32, [vp]),
+            "ora_pressure_rhs": (i32, [vp]),
+            "ora_corrector": (i32, [vp, d]),
+            "ora_pressure_solve": (i32, [vp, d, d, i64, ip, dp]),
+            "ora_max_cut_level": (i64, [vp]),
+            "ora_vof_init": (i32, [vp]),
+            "ora_vof_step": (i32, [vp, dp, d, dp]),
+            "ora_reduce": (i32, [vp, i32, i32, dp]),
+            "ora_compute_alpha": (d, [d, d, d, d]),
+            "ora_compute_dist": (d, [vp, i64, i64, i64, d, d, d, d]),
+            "ora_plic2vf": (d, [vp, i64, i64, i64, d, d, d, d]),
+            "ora_d2case": (i32, [dp, i32, dp]),
+            "ora_tet_vol": (d, [dp]),
+            "ora_cell2verts": (d, [vp, i64, i64, i64, dp]),
+            "ora_project": (None, [vp, dp, i64, i64, i64, d]),
+            "ora_pt2index": (i32, [vp, dp, i64, i64, i64, ip]),
+            "ora_vel_face": (None, [vp, dp, i64, i64, i64, dp]),
+            "ora_cut_tet": (i32, [vp, dp, ip, dp]),
+            "ora_sl_cell": (i32, [vp, i64, i64, i64, d, dp]),
+            "ora_elvira_cell": (i32, [vp, i64, i64, i64, dp]),
+            "ora_set_count_flops": (None, [i32]),
+            "ora_get_flops": (None, [ctypes.POINTER(ctypes.c_uint64)]),
+            "ora_set_threads": (None, [i32]),
+            "ora_get_threads": (i32, []),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+class OracleContext:
+    """The oracle's context: reference-layout arrays on the host."""
+
+    def __init__(self, param):
+        self.param = param
+        self.mesh = create_mesh(param)
+        cp = param.to_c()
+        h = ctypes.c_void_p()
+        st = lib().ora_ctx_create(ctypes.byref(cp), ctypes.byref(h))
+        if st:
+            raise OracleError(lib().ora_last_error(None).decode())
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().ora_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st:
+            raise OracleError(f"oracle status {st}: {lib().ora_last_error(self.h).decode()}")
+
+    # ---- fields -------------------------------------------------------------------
+    def extents(self, name):
+        lo = np.zeros(3, np.int64)
+        n = np.zeros(3, np.int64)
+        el = ctypes.c_int64()
+        self._check(lib().ora_field_extents(self.h, FIELD[name], _ip(lo), _ip(n), ctypes.byref(el)))
+        return lo, n, el.value
+
+    def get(self, name):
+        lo, n, el = self.extents(name)
+        a = np.zeros(tuple(n[::-1]), dtype=np.int16 if el == 2 else np.float64)
+        self._check(lib().ora_get_field(self.h, FIELD[name], a.ctypes.data, a.nbytes))
+        return a.transpose(2, 1, 0)  # index as [i, j, k]
+
+    def set(self, name, arr):
+        lo, n, el = self.extents(name)
+        a = np.ascontiguousarray(np.asarray(arr, dtype=np.int16 if el == 2 else np.float64).transpose(2, 1, 0))
+        if a.shape != tuple(n[::-1]):
+            raise ValueError(f"{name}: shape {arr.shape} != {tuple(n)}")
+        self._check(lib().ora_set_field(self.h, FIELD[name], a.ctypes.data, a.nbytes))
+
+    # ---- stages (same names as the product Context) -----------------------------
+    def init_vof(self, shape, *args):
+        a = np.asarray(args, np.float64)
+        self._check(lib().ora_init_vof(self.h, IC_SHAPES[shape], _dp(a), len(a)))
+
+    def neumann(self, name):
+        self._check(lib().ora_neumann(self.h, FIELD[name]))
+
+    def update_borders(self, name):
+        self._check(lib().ora_update_borders(self.h, FIELD[name]))
+
+    def define_velocity(self, t):
+        self._check(lib().ora_define_velocity(self.h, t))
+
+    def compute_dt(self):
+        dt = ctypes.c_double()
+        self._check(lib().ora_compute_dt(self.h, ctypes.byref(dt)))
+        return dt.value
+
+    def compute_band(self):
+        self._check(lib().ora_compute_band(self.h))
+
+    def compute_normal(self):
+        self._check(lib().ora_compute_normal(self.h))
+
+    def compute_plic(self):
+        self._check(lib().ora_compute_plic(self.h))
+
+    def transport(self, dt, t=0.0):
+        self._check(lib().ora_transport(self.h, dt, t))
+
+    def compute_props(self):
+        self._check(lib().ora_compute_props(self.h))
+
+    def pressure_rhs(self):
+        self._check(lib().ora_pressure_rhs(self.h))
+
+    def corrector(self, dt):
+        self._check(lib().ora_corrector(self.h, dt))
+
+    def pressure_solve(self, dt, tol, precision="fp64", check=True):
+        cyc = ctypes.c_int64()
+        res = ctypes.c_double()
+        prec = {"fp64": 0, "mixed": 1}[precision]
+        st = lib().ora_pressure_solve(self.h, dt, tol, prec, ctypes.byref(cyc), ctypes.byref(res))
+        if check:
+            self._check(st)
+        return cyc.value, res.value
+
+    def vof_init(self):
+        self._check(lib().ora_vof_init(self.h))
+
+    def vof_step(self, t, clamp_to=0.0):
+        tt = ctypes.c_double(t)
+        dt = ctypes.c_double()
+        self._check(lib().ora_vof_step(self.h, ctypes.byref(tt), clamp_to, ctypes.byref(dt)))
+        return tt.value, dt.value
+
+    def reduce(self, kind, name):
+        out = ctypes.c_double()
+        self._check(lib().ora_reduce(self.h, REDUCE[kind], FIELD[name], ctypes.byref(out)))
+        return out.value
+
+    def max_cut_level(self):
+        return lib().ora_max_cut_level(self.h)
+
+    # ---- single-cell primitives ---------------------------------------------------
+    def compute_dist(self, i, j, k, n, VF):
+        return lib().ora_compute_dist(self.h, i, j, k, n[0], n[1], n[2], VF)
+
+    def plic2vf(self, i, j, k, n, dist):
+        return lib().ora_plic2vf(self.h, i, j, k, n[0], n[1], n[2], dist)
+
+    def cell2verts(self, i, j, k):
+        out = np.zeros(24)
+        s = lib().ora_cell2verts(self.h, i, j, k, _dp(out))
+        return s, out.reshape(8, 3)
+
+    def project(self, pt, i, j, k, dt):
+        p = np.array(pt, np.float64)
+        lib().ora_project(self.h, _dp(p), i, j, k, dt)
+        return p
+
+    def pt2index(self, pt, i, j, k):
+        p = np.array(pt, np.float64)
+        out = np.zeros(3, np.int64)
+        ok = lib().ora_pt2index(self.h, _dp(p), i, j, k, _ip(out))
+        return bool(ok), out
+
+    def vel_face(self, pt, i, j, k):
+        p = np.array(pt, np.float64)
+        out = np.zeros(3)
+        lib().ora_vel_face(self.h, _dp(p), i, j, k, _dp(out))
+        return out
+
+    def cut_tet(self, tet, ind):
+        t = np.ascontiguousarray(np.asarray(tet, np.float64).reshape(4, 3))
+        n = np.ascontiguousarray(np.asarray(ind, np.int64).reshape(4, 3))
+        out = np.zeros(6)
+        lvl = lib().ora_cut_tet(self.h, _dp(t), _ip(n), _dp(out))
+        return out, lvl
+
+    def sl_cell(self, i, j, k, dt):
+        out = np.zeros(2)
+        lvl = lib().ora_sl_cell(self.h, i, j, k, dt, _dp(out))
+        return out, lvl
+
+    def elvira_cell(self, i, j, k):
+        out = np.zeros(3)
+        best = lib().ora_elvira_cell(self.h, i, j, k, _dp(out))
+        return out, best
+
+
+def compute_alpha(m, VF):
+    return lib().ora_compute_alpha(float(m[0]), float(m[1]), float(m[2]), float(VF))
+
+
+def d2case(d, inplace=True):
+    a = np.asarray(d, np.float64).copy()
+    out = np.zeros(4)
+    c = lib().ora_d2case(_dp(a), 1 if inplace else 0, _dp(out))
+    return c, out
+
+
+def tet_vol(v):
+    a = np.ascontiguousarray(np.asarray(v, np.float64).reshape(4, 3))
+    return lib().ora_tet_vol(_dp(a))
+
+
+def set_threads(n):
+    lib().ora_set_threads(int(n))
+
+
+def get_threads():
+    return lib().ora_get_threads()
+
+
+def flops(reset=False):
+    out = (ctypes.c_uint64 * 2)()
+    lib().ora_get_flops(out)
+    return int(out[0]), int(out[1])
+
+
+def count_flops(on=True):
+    lib().ora_set_count_flops(1 if on else 0)

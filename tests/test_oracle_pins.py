@@ -1,0 +1,332 @@
+"""Pins of the oracle (CPU): the reference's own PLIC tests, exact-rational golden vectors,
+closed forms and invariants. A mistake in the oracle's restatement (a dropped term, a wrong
+sign or index, a transposed operand) fails at least one of these.
+"""
+import json
+import os
+from fractions import Fraction as F
+
+import numpy as np
+import pytest
+
+from ns_multiphase_amd.params import parameters
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = json.load(open(os.path.join(HERE, "golden", "geometry_vectors.json")))
+
+
+def ci_param(**kw):
+    """Parameters of the reference's CI test (test/test_PLIC_dist_cutting.jl:3-34)."""
+    base = dict(mu_liq=0.1, mu_gas=1e-4, rho_liq=1.0, rho_gas=1.0, sigma=1.0, Lx=3.0, Ly=1.0,
+                Lz=1.0, tFinal=100.0, Nx=10, Ny=10, Nz=1, stepMax=20, CFL=0.1, out_period=10,
+                tol=1e-3, nprocx=1, nprocy=1, nprocz=1, xper=False, yper=False, zper=False)
+    base.update(kw)
+    return parameters(**base)
+
+
+def cube_param(N=16, **kw):
+    base = dict(mu_liq=1.0, mu_gas=1.0, rho_liq=1.0, rho_gas=1.0, sigma=0.0, Lx=1.0, Ly=1.0,
+                Lz=1.0, tFinal=1.0, Nx=N, Ny=N, Nz=N, stepMax=10, CFL=0.5, out_period=1,
+                tol=1e-8, nprocx=1, nprocy=1, nprocz=1, xper=False, yper=False, zper=False)
+    base.update(kw)
+    return parameters(**base)
+
+
+# ---- reference CI: PLIC round trip (test/runtests.jl:32-45) -------------------------------
+def test_plic_roundtrip_reference_ci(oracle):
+    c = oracle.OracleContext(ci_param())
+    rng = np.random.default_rng(1234)  # the reference draws unseeded rand(); seeded here
+    for _ in range(1000):
+        n = rng.random(3) * 2.0 - 1.0
+        VF = rng.random()
+        n = n / np.sqrt(n[0] ** 2 + n[1] ** 2 + n[2] ** 2)
+        dist = c.compute_dist(1, 1, 1, n, VF)
+        assert abs(c.plic2vf(1, 1, 1, n, dist) - VF) <= 1e-12
+
+
+# ---- reference known answer: axis-aligned PLIC distance (test/test_PLIC_dist.jl:43-51) ---
+@pytest.mark.parametrize("VF", [1e-9, 0.1, 0.3, 0.5, 0.7, 0.999])
+def test_plic_dist_axis_aligned(oracle, VF):
+    c = oracle.OracleContext(ci_param())
+    m = c.mesh
+    i = j = k = 1
+    x_i = m.x[m.nghost + i - 1]
+    y_j = m.y[m.nghost + j - 1]
+    assert abs(c.compute_dist(i, j, k, (1.0, 0.0, 0.0), VF) - (x_i + VF * m.dx)) <= 1e-12
+    assert abs(c.compute_dist(i, j, k, (0.0, 1.0, 0.0), VF) - (y_j + VF * m.dy)) <= 1e-12
+
+
+def test_compute_dist_empty_full(oracle):
+    c = oracle.OracleContext(ci_param())
+    L = 3.0 + 1.0 + 1.0
+    assert c.compute_dist(1, 1, 1, (0.6, 0.8, 0.0), 0.0) == -L
+    assert c.compute_dist(1, 1, 1, (0.6, 0.8, 0.0), 1.0) == L
+
+
+# ---- computeAlpha against the exact forward volume (every branch) -------------------------
+def test_compute_alpha_exact(oracle):
+    worst = 0.0
+    for case in GOLD["alpha"]:
+        a = oracle.compute_alpha(case["m"], case["V"])
+        worst = max(worst, abs(a - case["alpha"]))
+    assert worst <= 2e-12, worst
+    # the set covers every branch of src/VF.jl:183-211
+    br = set()
+    for case in GOLD["alpha"]:
+        m1, m2, m3 = sorted(case["m"])
+        al = case["alpha"]
+        br.add(0 if al < m1 else 1 if al < m2 else 2 if al < min(m1 + m2, m3) else 3)
+    assert br == {0, 1, 2, 3}
+
+
+# ---- computePLIC2VF against exact cube-plane volumes ------------------------------------
+def test_plic2vf_exact(oracle):
+    c = oracle.OracleContext(cube_param(16))
+    worst = 0.0
+    for case in GOLD["cube"]:
+        i, j, k = case["ijk"]
+        v = c.plic2vf(i, j, k, case["n"], case["D"])
+        worst = max(worst, abs(v - case["frac"]))
+    assert worst <= 1e-12, worst
+
+
+def test_five_tet_decomposition_volume(oracle):
+    """sum tetsign * tet_vol over the 5 tets = cell volume, even and odd cells."""
+    from golden.exact_geometry import tet_volume
+    c = oracle.OracleContext(ci_param())
+    tets5 = [(1, 2, 4, 6), (1, 4, 3, 7), (1, 5, 6, 7), (4, 7, 6, 8), (1, 4, 7, 6)]
+    m = c.mesh
+    for (i, j, k) in [(1, 1, 1), (2, 1, 1), (3, 4, 1), (4, 4, 1)]:
+        s, v = c.cell2verts(i, j, k)
+        assert s == (1.0 if (i + j + k) % 2 == 0 else -1.0)
+        tot = sum(F(s) * tet_volume([v[a - 1] for a in t]) for t in tets5)
+        vol = F(m.dx) * F(m.dy) * F(m.dz)
+        assert abs(float(tot - vol)) <= 1e-15
+        # every tet has the same orientation sign
+        for t in tets5:
+            assert tet_volume([v[a - 1] for a in t]) * F(s) > 0
+
+
+def test_d2case_snapping(oracle):
+    # |d| < eps takes the majority sign; ties go negative (src/VFgeom.jl:693-708)
+    c, d = oracle.d2case([1.0, 1.0, -1.0, 0.0])
+    assert d[3] > 0 and c == 1 + 1 + 2 + 8
+    c, d = oracle.d2case([-1.0, 1.0, 0.0, 0.0])
+    assert d[2] < 0 and d[3] < 0 and c == 1 + 2
+    c, _ = oracle.d2case([-1.0, -2.0, -3.0, -4.0])
+    assert c == 1
+    c, _ = oracle.d2case([1.0, 2.0, 3.0, 4.0])
+    assert c == 16
+
+
+# ---- tet cutting: PLIC cut of one tet against the exact simplex formula -------------------
+def _set_uniform_plane(c, n, D):
+    nx = c.get("nx"); ny = c.get("ny"); nz = c.get("nz"); Dd = c.get("D")
+    nx[...] = n[0]; ny[...] = n[1]; nz[...] = n[2]; Dd[...] = D
+    c.set("nx", nx); c.set("ny", ny); c.set("nz", nz); c.set("D", Dd)
+
+
+def test_cut_tet_single_cell_exact(oracle):
+    c = oracle.OracleContext(cube_param(4, Lx=4.0, Ly=4.0, Lz=4.0))
+    worst = 0.0
+    for case in GOLD["tet"]:
+        _set_uniform_plane(c, case["n"], case["D"])
+        ind = [[1, 1, 1]] * 4  # all vertices in one cell: no mesh cut, one PLIC cut
+        out, lvl = c.cut_tet(case["v"], ind)
+        assert lvl == 1
+        assert abs(out[0] - case["vol"]) <= 1e-14
+        worst = max(worst, abs(out[1] - case["liq"]))
+    assert worst <= 1e-14, worst
+
+
+def test_cut_tet_mesh_cuts_tile_the_tet(oracle):
+    """A tet spanning several cells: the sub-tets of the x/y/z mesh cuts tile it, and with the
+    same plane in every cell the liquid volume equals the exact simplex cut."""
+    from golden.exact_geometry import tet_fraction, tet_volume
+    c = oracle.OracleContext(cube_param(8))
+    rng = np.random.default_rng(7)
+    m = c.mesh
+    nchk = 0
+    for _ in range(300):
+        v = (rng.integers(8, 56, size=(4, 3)) / 64.0)  # dyadic vertices inside the cube
+        vol = tet_volume(v.tolist())
+        if abs(vol) < F(1, 4096):
+            continue
+        n = rng.integers(-64, 64, size=3) / 64.0
+        D = float(rng.integers(-32, 32)) / 64.0
+        try:
+            fr = tet_fraction(v.tolist(), n.tolist(), D)
+        except ValueError:
+            continue
+        ind = [[int(np.searchsorted(m.x, p[0], side="right") - 1 - m.nghost + 1),
+                int(np.searchsorted(m.y, p[1], side="right") - 1 - m.nghost + 1),
+                int(np.searchsorted(m.z, p[2], side="right") - 1 - m.nghost + 1)] for p in v]
+        _set_uniform_plane(c, n, D)
+        out, lvl = c.cut_tet(v, ind)
+        assert abs(out[0] - float(vol)) <= 1e-14
+        assert abs(out[1] - float(fr * vol)) <= 1e-14
+        nchk += 1
+    assert nchk > 100
+
+
+# ---- back-trace: interpolation and RK4 closed forms ----------------------------------------
+def test_face_interpolation_exact_for_trilinear_fields(oracle):
+    c = oracle.OracleContext(cube_param(8))
+    m = c.mesh
+    X, Y, Z = np.meshgrid(m.x[:-1], m.ym, m.zm, indexing="ij")
+    f = lambda x, y, z: 0.3 + 1.5 * x - 0.25 * y + 2.0 * z + 0.5 * x * y - 0.75 * y * z + x * y * z
+    c.set("uf", f(X, Y, Z))
+    X, Y, Z = np.meshgrid(m.xm, m.y[:-1], m.zm, indexing="ij")
+    c.set("vf", f(X, Y, Z))
+    X, Y, Z = np.meshgrid(m.xm, m.ym, m.z[:-1], indexing="ij")
+    c.set("wf", f(X, Y, Z))
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        pt = rng.random(3) * 0.9 + 0.05
+        i, j, k = (int(pt[q] * 8) + 1 for q in range(3))
+        out = c.vel_face(pt, i, j, k)
+        assert np.allclose(out, f(*pt), rtol=0, atol=1e-14)
+
+
+def test_rk4_backtrace_matches_rk4_polynomial(oracle):
+    """For the linear rotation field the interpolation is exact, so RK4 (src/Tools.jl:400-405)
+    maps x to c + P(-dt A)(x - c) with P(z) = I + z + z^2/2 + z^3/6 + z^4/24 exactly."""
+    p = cube_param(16, Nz=1, Lz=1.0 / 16, zper=True, VFVelocity="rotation")
+    c = oracle.OracleContext(p)
+    c.define_velocity(0.0)
+    w = 2.0 * np.pi
+    A = np.array([[0.0, -w, 0.0], [w, 0.0, 0.0], [0.0, 0.0, 0.0]])
+    dt = 1.0 / 64
+    Z = -dt * A
+    P = np.eye(3) + Z + Z @ Z / 2 + Z @ Z @ Z / 6 + Z @ Z @ Z @ Z / 24
+    cen = np.array([0.5, 0.5, 0.0])
+    rng = np.random.default_rng(5)
+    for _ in range(100):
+        x = np.array([0.3 + 0.4 * rng.random(), 0.3 + 0.4 * rng.random(), 0.5 / 16])
+        i, j, k = int(x[0] * 16) + 1, int(x[1] * 16) + 1, 1
+        got = c.project(x, i, j, k, dt)
+        exp = cen + P @ (x - cen)
+        assert np.allclose(got, exp, rtol=0, atol=5e-15)
+
+
+def test_uniform_flow_backtrace(oracle):
+    p = cube_param(8, projection_method="RK4")
+    c = oracle.OracleContext(p)
+    U = (0.7, -0.4, 0.25)
+    for name, val in zip(("uf", "vf", "wf"), U):
+        a = c.get(name); a[...] = val; c.set(name, a)
+    for meth in ("RK4", "Euler", "Midpoint"):
+        cc = oracle.OracleContext(cube_param(8, projection_method=meth))
+        for name, val in zip(("uf", "vf", "wf"), U):
+            a = cc.get(name); a[...] = val; cc.set(name, a)
+        x = np.array([0.41, 0.52, 0.63])
+        got = cc.project(x, 4, 5, 6, 0.01)
+        assert np.allclose(got, x - 0.01 * np.array(U), rtol=0, atol=1e-15)
+
+
+def test_pt2index_eps_slack(oracle):
+    c = oracle.OracleContext(cube_param(8))
+    m = c.mesh
+    xf = m.x[m.nghost + 3]  # face between cells 3 and 4
+    # a point exactly on the face stays in the start cell from either side
+    ok, I = c.pt2index([xf, 0.5, 0.5], 3, 4, 4)
+    assert ok and I[0] == 3
+    ok, I = c.pt2index([xf, 0.5, 0.5], 4, 4, 4)
+    assert ok and I[0] == 4
+    ok, I = c.pt2index([xf + 1e-9, 0.5, 0.5], 3, 4, 4)
+    assert ok and I[0] == 4
+
+
+# ---- whole SL-VOF step: planar interface in uniform flow ----------------------------------
+def planar_vf(mesh, n, D):
+    """Exact cell fractions of the half-space n.x <= D on every cell of the VF array."""
+    from golden.exact_geometry import cube_fraction
+    g = mesh.nghost + 3
+    shape = mesh.vof_shape()
+    VF = np.zeros(shape)
+    fx = lambda q: mesh.x[0] + (q - 3) * mesh.dx
+    h = (mesh.dx, mesh.dy, mesh.dz)
+    for a in range(shape[0]):
+        x0 = mesh.x[0] + (a - 3) * mesh.dx
+        for b in range(shape[1]):
+            y0 = mesh.y[0] + (b - 3) * mesh.dy
+            for cc in range(shape[2]):
+                z0 = mesh.z[0] + (cc - 3) * mesh.dz
+                # fast path: fully on one side
+                corners = [n[0] * (x0 + u * h[0]) + n[1] * (y0 + v * h[1]) + n[2] * (z0 + w * h[2])
+                           for u in (0, 1) for v in (0, 1) for w in (0, 1)]
+                if max(corners) <= D:
+                    VF[a, b, cc] = 1.0
+                elif min(corners) >= D:
+                    VF[a, b, cc] = 0.0
+                else:
+                    VF[a, b, cc] = float(cube_fraction(n, D, (x0, y0, z0), h))
+    return VF
+
+
+@pytest.mark.parametrize("nrm", [(1.0, 0.5, 0.25), (-0.5, 1.0, 0.75), (0.25, -0.75, 1.0)])
+def test_sl_step_planar_interface_uniform_flow(oracle, nrm):
+    N = 12
+    p = cube_param(N, CFL=0.5)
+    c = oracle.OracleContext(p)
+    m = c.mesh
+    n = np.array(nrm)
+    D0 = float(n @ np.array([0.5, 0.5, 0.5]))
+    VF = planar_vf(m, n, D0)
+    c.set("VF", VF)
+    c.vof_init()
+    U = np.array([0.3, -0.2, 0.1]) * (0.5 / N) / 0.01 / 0.5  # |U| dt ~ 0.3 cell
+    dt = 0.01
+    for name, val in zip(("uf", "vf", "wf"), U):
+        a = c.get(name); a[...] = val; c.set(name, a)
+    for name, val in zip(("u", "v", "w"), U):
+        a = c.get(name); a[...] = val; c.set(name, a)
+    # ELVIRA reproduces the plane normal in interface cells away from the walls
+    nx, ny, nz = c.get("nx"), c.get("ny"), c.get("nz")
+    VFg = c.get("VF")
+    g = m.nghost
+    nn = n / np.linalg.norm(n)
+    cnt = 0
+    for i in range(3, N - 1):
+        for j in range(3, N - 1):
+            for k in range(3, N - 1):
+                v = VFg[i + g + 2, j + g + 2, k + g + 2]
+                if 1e-6 < v < 1 - 1e-6:
+                    got = np.array([nx[i + g - 1, j + g - 1, k + g - 1], ny[i + g - 1, j + g - 1, k + g - 1],
+                                    nz[i + g - 1, j + g - 1, k + g - 1]])
+                    assert np.allclose(got, nn, atol=1e-12), (i, j, k, got, nn)
+                    cnt += 1
+    assert cnt > 10
+    c.transport(dt)
+    VF1 = c.get("VF")
+    exact = planar_vf(m, n, D0 + float(n @ U) * dt)
+    # cells 4..N-3: their pre-images and the ELVIRA/PLIC stencils they read stay two cells
+    # from the walls, where Neumann ghosts and zero ghost normals break planarity
+    sl = slice(4 + g + 2, N - 3 + g + 3)
+    err = np.abs(VF1[sl, sl, sl] - exact[sl, sl, sl]).max()
+    assert err <= 1e-12, err
+    moved = np.abs(VF1[sl, sl, sl] - VFg[sl, sl, sl]).max()
+    assert moved > 0.05  # the interface did move through the checked cells
+
+
+# ---- band flags --------------------------------------------------------------------------
+def test_band_flags(oracle):
+    N = 16
+    p = cube_param(N, CFL=1.5)  # nband = 2
+    c = oracle.OracleContext(p)
+    c.init_vof("sphere", 0.25, 0.5, 0.5, 0.5)
+    c.vof_init()
+    b = c.get("band")
+    VF = c.get("VF")
+    g = c.mesh.nghost
+    bi = b[g:g + N, g:g + N, g:g + N].astype(int)
+    vi = VF[g + 3:g + 3 + N, g + 3:g + 3 + N, g + 3:g + 3 + N]
+    inter = (vi >= 1e-10) & (vi <= 1 - 1e-10)
+    assert np.all(bi[inter] == 0)
+    assert set(np.unique(bi)) <= {-3, -2, -1, 0, 1, 2, 3}
+    assert np.all(np.sign(bi[vi < 1e-10]) <= 0) and np.all(np.sign(bi[vi > 1 - 1e-10]) >= 0)
+    # |band| grows by at most one per cell step (Chebyshev neighbourhood)
+    pad = np.pad(np.abs(bi), 1, mode="edge")
+    for s in [(a, b_, d) for a in (-1, 0, 1) for b_ in (-1, 0, 1) for d in (-1, 0, 1)]:
+        nb = pad[1 + s[0]:1 + s[0] + N, 1 + s[1]:1 + s[1] + N, 1 + s[2]:1 + s[2] + N]
+        assert np.all(np.abs(bi) <= np.minimum(nb + 1, 3))
