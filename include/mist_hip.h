@@ -172,8 +172,23 @@ int mist_plic_vf_batch(mist_ctx* ctx, const int64_t* ijk, const double* nrm,
                        const double* dist, double* vf_out, int64_t m);
 
 /* Per-stage device time of the last calls (ms), for profiling: kinds are the stage
- * entry points in order normal, transport, plic, band, mg. */
+ * entry points in order normal, transport, plic, band, mg. Filled only when the process
+ * runs with MIST_STAGE_TIMING set (each stage then synchronises). */
 int mist_stage_times(const mist_ctx* ctx, double* ms, int n);
+
+/* Kernel profile: HIP events on the context's stream around every launch of the kernels
+ * below while enabled, and device-side totals of the work units each launch processed.
+ * No reference counterpart (the reference has only @time around whole runs); used by the
+ * benchmark to compute achieved FLOP/s and bytes/s per kernel. */
+enum { MIST_PROF_ELVIRA = 0,     /* units: interface cells scored */
+       MIST_PROF_SL_CUT = 1,     /* units: band cells transported */
+       MIST_PROF_PLIC = 2,       /* units: cells (incl. ghosts) */
+       MIST_PROF_PROJECT = 3,    /* units: cell vertices visited */
+       MIST_PROF_MG_RBGS = 4,    /* units: cells relaxed (one colour sweep = half a level) */
+       MIST_PROF_MG_RESID = 5,   /* units: cells */
+       MIST_PROF_COUNT = 6 };
+int mist_profile_enable(mist_ctx* ctx, int on);   /* on != 0 also resets the totals */
+int mist_profile_read(mist_ctx* ctx, int kernel, double* total_ms, int64_t* launches, int64_t* units);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,314 @@
+// mist_device.h -- device-side geometry shared by the SL-VOF kernels (gfx950).
+//
+// The arithmetic of every function follows the reference operation by operation (same
+// association order, literal powers as products, no FMA: the library is compiled with
+// -ffp-contract=off) so that results agree with an FP64 evaluation of the reference to the
+// last few ulps; cbrt/acos/sin/cos come from the device math library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mist {
+
+constexpr double EPS = 2.220446049250313e-16;  // Julia eps() for Float64
+
+// Geometry of one mesh level as seen by kernels: reference indices (interior 1..N, ghosts
+// 1-ng..N+ng), coordinate pointers pre-offset so x[i] addresses reference index i.
+struct DevMesh {
+    int Nx, Ny, Nz, ng;
+    int cnx, cny, cnz;   // cell-field extents (N + 2 ng)
+    int vnx, vny, vnz;   // VF extents (N + 2 ng + 6)
+    double dx, dy, dz;
+    double Lx, Ly, Lz;
+    const double* x;  const double* y;  const double* z;    // faces, index imino..imaxo+1
+    const double* xm; const double* ym; const double* zm;   // centres, index imino..imaxo
+    __host__ __device__ inline int64_t cell(int i, int j, int k) const {
+        return (int64_t)(i - 1 + ng) + (int64_t)cnx * ((int64_t)(j - 1 + ng) + (int64_t)cny * (k - 1 + ng));
+    }
+    __host__ __device__ inline int64_t vof(int i, int j, int k) const {
+        return (int64_t)(i + 2 + ng) + (int64_t)vnx * ((int64_t)(j + 2 + ng) + (int64_t)vny * (k + 2 + ng));
+    }
+    __host__ __device__ inline int imino() const { return 1 - ng; }
+    __host__ __device__ inline int imaxo() const { return Nx + ng; }
+    __host__ __device__ inline int jmaxo() const { return Ny + ng; }
+    __host__ __device__ inline int kmaxo() const { return Nz + ng; }
+};
+
+struct DevParams {
+    double VFlo, VFhi;
+    int nband;
+    int projection;      // MIST_PROJ_*
+    int zalesak;         // test_case == "Zalesak"
+    double rho_liq, rho_gas;
+};
+
+// Julia min/max for non-NaN arguments with signed zeros ordered -0.0 < 0.0.
+__device__ __forceinline__ double jmax(double a, double b) {
+    if (a != a) return a;
+    if (b != b) return b;
+    if (a == b) return signbit(a) ? b : a;
+    return a > b ? a : b;
+}
+__device__ __forceinline__ double jmin(double a, double b) {
+    if (a != a) return a;
+    if (b != b) return b;
+    if (a == b) return signbit(a) ? a : b;
+    return a < b ? a : b;
+}
+// a /^ b  (src/Tools.jl:375-377)
+__device__ __forceinline__ double sdiv(double a, double b) {
+    return fabs(b) <= EPS ? __builtin_huge_val() : a / b;
+}
+
+// ---- cut tables (src/VFgeom.jl:54-165), case c = reference case - 1 ----------------------
+// Vertex numbers 1..8 of the reference are stored 0-based. The 4 vertices of tet n of case c
+// are packed as 4 nibbles of CUT_VTET[c][n]; CUT_EDGE packs the (v1, v2) pairs of the cut
+// vertices as nibbles v1 | v2 << 4 per byte.
+__constant__ const unsigned char CUT_NVERT[16] = {0, 3, 3, 4, 3, 4, 4, 3, 3, 4, 4, 3, 4, 3, 3, 0};
+__constant__ const unsigned char CUT_NTETS[16] = {1, 4, 4, 6, 4, 6, 6, 4, 4, 6, 6, 4, 6, 4, 4, 1};
+__constant__ const unsigned char CUT_NNTET[16] = {1, 2, 2, 4, 2, 4, 4, 4, 2, 4, 4, 4, 4, 4, 4, 2};
+
+#define E2(a, b) (unsigned char)(((a) - 1) | (((b) - 1) << 4))
+__constant__ const unsigned char CUT_EDGE[16][4] = {
+    {0, 0, 0, 0},
+    {E2(1, 2), E2(1, 3), E2(1, 4), 0},
+    {E2(2, 3), E2(2, 4), E2(2, 1), 0},
+    {E2(1, 4), E2(2, 4), E2(1, 3), E2(2, 3)},
+    {E2(3, 4), E2(3, 1), E2(3, 2), 0},
+    {E2(1, 4), E2(3, 4), E2(1, 2), E2(3, 2)},
+    {E2(2, 4), E2(3, 4), E2(2, 1), E2(3, 1)},
+    {E2(4, 1), E2(4, 2), E2(4, 3), 0},
+    {E2(4, 1), E2(4, 2), E2(4, 3), 0},
+    {E2(1, 3), E2(4, 3), E2(1, 2), E2(4, 2)},
+    {E2(2, 3), E2(4, 3), E2(2, 1), E2(4, 1)},
+    {E2(3, 4), E2(3, 1), E2(3, 2), 0},
+    {E2(3, 2), E2(4, 2), E2(3, 1), E2(4, 1)},
+    {E2(2, 3), E2(2, 4), E2(2, 1), 0},
+    {E2(1, 2), E2(1, 3), E2(1, 4), 0},
+    {0, 0, 0, 0}};
+#undef E2
+
+#define T4(a, b, c, d) (unsigned short)(((a) - 1) | (((b) - 1) << 4) | (((c) - 1) << 8) | (((d) - 1) << 12))
+__constant__ const unsigned short CUT_VTET[16][6] = {
+    {T4(1, 2, 3, 4), 0, 0, 0, 0, 0},
+    {T4(5, 7, 6, 1), T4(6, 2, 3, 4), T4(4, 2, 5, 6), T4(5, 6, 7, 4), 0, 0},
+    {T4(7, 5, 6, 2), T4(1, 3, 4, 6), T4(1, 5, 3, 6), T4(5, 7, 6, 1), 0, 0},
+    {T4(5, 8, 6, 2), T4(5, 7, 8, 1), T4(5, 1, 8, 2), T4(5, 6, 8, 4), T4(5, 8, 7, 3), T4(5, 8, 3, 4)},
+    {T4(6, 5, 7, 3), T4(2, 1, 4, 6), T4(6, 5, 4, 2), T4(6, 7, 5, 2), 0, 0},
+    {T4(5, 6, 8, 3), T4(5, 8, 7, 1), T4(5, 8, 1, 3), T4(5, 8, 6, 4), T4(5, 7, 8, 2), T4(5, 8, 4, 2)},
+    {T4(8, 6, 5, 3), T4(5, 7, 8, 2), T4(8, 5, 2, 3), T4(8, 5, 6, 4), T4(5, 8, 7, 1), T4(5, 8, 1, 4)},
+    {T4(1, 2, 3, 7), T4(1, 2, 7, 6), T4(5, 7, 6, 1), T4(5, 6, 7, 4), 0, 0},
+    {T4(5, 6, 7, 4), T4(1, 2, 3, 6), T4(5, 1, 3, 6), T4(5, 7, 6, 3), 0, 0},
+    {T4(5, 8, 6, 4), T4(5, 7, 8, 1), T4(5, 8, 4, 1), T4(5, 6, 8, 3), T4(5, 8, 7, 2), T4(5, 8, 2, 3)},
+    {T4(8, 5, 6, 4), T4(5, 8, 7, 2), T4(8, 2, 5, 4), T4(8, 6, 5, 3), T4(5, 7, 8, 1), T4(5, 8, 3, 1)},
+    {T4(1, 4, 2, 7), T4(4, 1, 6, 7), T4(6, 7, 5, 4), T4(6, 5, 7, 3), 0, 0},
+    {T4(8, 6, 5, 4), T4(5, 7, 8, 3), T4(8, 4, 5, 3), T4(8, 5, 6, 2), T4(5, 8, 7, 1), T4(5, 8, 1, 2)},
+    {T4(3, 4, 1, 7), T4(7, 6, 3, 4), T4(7, 6, 5, 3), T4(7, 5, 6, 2), 0, 0},
+    {T4(7, 4, 2, 3), T4(2, 3, 6, 7), T4(5, 6, 7, 2), T4(5, 7, 6, 1), 0, 0},
+    {T4(1, 2, 3, 4), 0, 0, 0, 0, 0}};
+#undef T4
+// side of the mesh plane of each sub-tet (1 = below, 2 = above), bit n set when side 2
+__constant__ const unsigned char CUT_SIDE2[16] = {
+    0x00, 0x01, 0x01, 0x07, 0x01, 0x07, 0x07, 0x07, 0x01, 0x07, 0x07, 0x07, 0x07, 0x07, 0x07, 0x01};
+
+// 5-tet decomposition (src/VFgeom.jl:235-256), 0-based corner numbers packed per tet.
+__constant__ const unsigned short TET5[5] = {
+    (0 | (1 << 4) | (3 << 8) | (5 << 12)), (0 | (3 << 4) | (2 << 8) | (6 << 12)),
+    (0 | (4 << 4) | (5 << 8) | (6 << 12)), (3 | (6 << 4) | (5 << 8) | (7 << 12)),
+    (0 | (3 << 4) | (6 << 8) | (5 << 12))};
+
+__device__ __forceinline__ int nib(unsigned v, int q) { return (v >> (4 * q)) & 15; }
+
+// tet_vol (src/VFgeom.jl:669-681)
+__device__ __forceinline__ double tet_vol(const double* p0, const double* p1, const double* p2, const double* p3) {
+    const double a1 = p0[0] - p3[0], a2 = p0[1] - p3[1], a3 = p0[2] - p3[2];
+    const double b1 = p1[0] - p3[0], b2 = p1[1] - p3[1], b3 = p1[2] - p3[2];
+    const double c1 = p2[0] - p3[0], c2 = p2[1] - p3[1], c3 = p2[2] - p3[2];
+    return (-1.0 / 6.0) * ((a1 * (b2 * c3 - c2 * b3) - a2 * (b1 * c3 - c1 * b3)) + a3 * (b1 * c2 - c1 * b2));
+}
+
+// d2case (src/VFgeom.jl:693-708) on a copy held in registers; d is snapped in place.
+__device__ __forceinline__ int d2case_snap(double d[4]) {
+    int npos = 0, nneg = 0;
+#pragma unroll
+    for (int n = 0; n < 4; ++n) { npos += d[n] > 0.0; nneg += d[n] < 0.0; }
+    const double s = npos > nneg ? EPS : -EPS;
+#pragma unroll
+    for (int n = 0; n < 4; ++n)
+        if (fabs(d[n]) < EPS) d[n] = s;
+    return (d[0] > 0.0) | ((d[1] > 0.0) << 1) | ((d[2] > 0.0) << 2) | ((d[3] > 0.0) << 3);
+}
+
+// computeAlpha (src/VF.jl:164-214), Scardovelli & Zaleski (2000).
+__device__ inline double compute_alpha(double a, double b, double c, double VF) {
+    // sort ascending
+    double m1 = a, m2 = b, m3 = c, t;
+    if (m1 > m2) { t = m1; m1 = m2; m2 = t; }
+    if (m2 > m3) { t = m2; m2 = m3; m3 = t; }
+    if (m1 > m2) { t = m1; m1 = m2; m2 = t; }
+    const double m12 = m1 + m2;
+    const double V1 = (m1 * m1) / jmax((6.0 * m2) * m3, EPS);
+    const double V2 = V1 + sdiv(0.5 * (m2 - m1), m3);
+    double V3;
+    if (m12 <= m3) V3 = sdiv(0.5 * m12, m3);
+    else V3 = sdiv(((m3 * m3) * (3.0 * m12 - m3) + (m1 * m1) * (m1 - 3.0 * m3)) + (m2 * m2) * (m2 - 3.0 * m3),
+                   ((6.0 * m1) * m2) * m3);
+    double alpha;
+    if (0.0 <= VF && VF < V1) {
+        alpha = cbrt((((6.0 * m1) * m2) * m3) * VF);
+    } else if (V1 <= VF && VF < V2) {
+        alpha = 0.5 * (m1 + sqrt((m1 * m1) + ((8.0 * m2) * m3) * (VF - V1)));
+    } else if (V2 <= VF && VF < V3) {
+        double a0 = ((m1 * m1) * m1 + (m2 * m2) * m2) - (((6.0 * m1) * m2) * m3) * VF;
+        double a1 = -3.0 * ((m1 * m1) + (m2 * m2));
+        double a2 = 3.0 * m12;
+        const double a3 = -1.0;
+        a0 = a0 / a3; a1 = a1 / a3; a2 = a2 / a3;
+        const double p0 = a1 / 3.0 - (a2 * a2) / 9.0;
+        const double q0 = (a1 * a2 - 3.0 * a0) / 6.0 - ((a2 * a2) * a2) / 27.0;
+        const double theta = acos(jmin(1.0, jmax(-1.0, q0 / sqrt(-1.0 * ((p0 * p0) * p0))))) / 3.0;
+        alpha = sqrt(-p0) * (sqrt(3.0) * sin(theta) - cos(theta)) - a2 / 3.0;
+    } else if (m12 <= m3) {
+        alpha = m3 * VF + 0.5 * m12;
+    } else {
+        double a0 = (((m1 * m1) * m1 + (m2 * m2) * m2) + (m3 * m3) * m3) - (((6.0 * m1) * m2) * m3) * VF;
+        double a1 = -3.0 * (((m1 * m1) + (m2 * m2)) + (m3 * m3));
+        double a2 = 3.0;
+        const double a3 = -2.0;
+        a0 = a0 / a3; a1 = a1 / a3; a2 = a2 / a3;
+        const double p0 = a1 / 3.0 - (a2 * a2) / 9.0;
+        const double q0 = (a1 * a2 - 3.0 * a0) / 6.0 - ((a2 * a2) * a2) / 27.0;
+        const double theta = acos(jmin(1.0, jmax(-1.0, q0 / sqrt(-((p0 * p0) * p0))))) / 3.0;
+        alpha = sqrt(-p0) * (sqrt(3.0) * sin(theta) - cos(theta)) - a2 / 3.0;
+    }
+    return alpha;
+}
+
+// computeDist (src/VF.jl:109-156)
+__device__ inline double compute_dist(const DevMesh& m, const DevParams& p, int i, int j, int k, double nx,
+                                      double ny, double nz, double VF) {
+    if (VF < p.VFlo) return -((m.Lx + m.Ly) + m.Lz);
+    if (VF > p.VFhi) return +((m.Lx + m.Ly) + m.Lz);
+    double mm0 = nx * m.dx, mm1 = ny * m.dy, mm2 = nz * m.dz;
+    const double norm = ((fabs(mm0) + fabs(mm1)) + fabs(mm2)) + EPS;
+    mm0 = mm0 / norm; mm1 = mm1 / norm; mm2 = mm2 / norm;
+    double factor = 0.0;
+    if (mm0 < 0.0) factor += mm0;
+    if (mm1 < 0.0) factor += mm1;
+    if (mm2 < 0.0) factor += mm2;
+    const double VFo = VF > 0.5 ? 1.0 - VF : VF;
+    double alpha = compute_alpha(fabs(mm0), fabs(mm1), fabs(mm2), VFo);
+    alpha = VF > 0.5 ? 1.0 - alpha : alpha;
+    alpha += factor;
+    alpha -= 0.5 * ((mm0 + mm1) + mm2);
+    double dist = alpha * norm;
+    dist += (nx * m.xm[i] + ny * m.ym[j]) + nz * m.zm[k];
+    return dist;
+}
+
+// Corner c (bits: x = c&1, y = c>>1&1, z = c>>2) of cell (i,j,k) in the reference vertex
+// numbering of cell2verts! (src/VFgeom.jl:190-216): even cells number corners in lexical
+// order, odd cells mirror x. Returns the lexical corner index of reference vertex v.
+__device__ __forceinline__ int vert_corner(int v, bool even) { return even ? v : (v ^ 1); }
+
+// computePLIC2VF (src/VF.jl:219-278) given the 8 corner coordinates and corner distances
+// d = n.x - dist of a cell (lexical corner order). `even` selects the vertex numbering and
+// tetsign. d2case snaps the per-tet copy of d, and the snapped values are the ones
+// interpolated, as in the reference.
+__device__ inline double plic2vf_corners(const double cx[2], const double cy[2], const double cz[2],
+                                         const double dc[8], bool even, double inv_vol_unused,
+                                         double vol) {
+    const double tetsign = even ? 1.0 : -1.0;
+    double VF = 0.0;
+#pragma unroll 1
+    for (int t = 0; t < 5; ++t) {
+        const unsigned tv = TET5[t];
+        double vert[8][3];
+        double d[4];
+#pragma unroll
+        for (int n = 0; n < 4; ++n) {
+            const int c = vert_corner(nib(tv, n), even);
+            vert[n][0] = cx[c & 1]; vert[n][1] = cy[(c >> 1) & 1]; vert[n][2] = cz[c >> 2];
+            d[n] = dc[c];
+        }
+        const int cs = d2case_snap(d);
+        const int nv = CUT_NVERT[cs];
+        for (int n = 0; n < nv; ++n) {
+            const unsigned e = CUT_EDGE[cs][n];
+            const int v1 = e & 15, v2 = e >> 4;
+            const double mu = jmin(1.0, jmax(0.0, sdiv(-d[v1], d[v2] - d[v1])));
+            vert[4 + n][0] = (1.0 - mu) * vert[v1][0] + mu * vert[v2][0];
+            vert[4 + n][1] = (1.0 - mu) * vert[v1][1] + mu * vert[v2][1];
+            vert[4 + n][2] = (1.0 - mu) * vert[v1][2] + mu * vert[v2][2];
+        }
+        const int hi = CUT_NTETS[cs], lo = CUT_NNTET[cs];
+        for (int n = hi; n >= lo; --n) {
+            const unsigned q = CUT_VTET[cs][n - 1];
+            VF += (tetsign * tet_vol(vert[nib(q, 0)], vert[nib(q, 1)], vert[nib(q, 2)], vert[nib(q, 3)])) / vol;
+        }
+    }
+    return VF;
+}
+
+// pt2index! (src/Tools.jl:424-434). The walk is bounded by the coordinate arrays; a walk
+// that would leave them sets *ovf and stops at the last valid cell.
+__device__ __forceinline__ void pt2index(const DevMesh& m, int I[3], const double pt[3], int i, int j, int k,
+                                         int* ovf) {
+    const int lo = 1 - m.ng;
+    I[0] = i; I[1] = j; I[2] = k;
+    while (pt[0] > m.x[I[0] + 1] + EPS) { if (I[0] + 1 > m.Nx + m.ng) { *ovf = 1; break; } I[0] += 1; }
+    while (pt[0] < m.x[I[0]] - EPS) { if (I[0] - 1 < lo) { *ovf = 1; break; } I[0] -= 1; }
+    while (pt[1] > m.y[I[1] + 1] + EPS) { if (I[1] + 1 > m.Ny + m.ng) { *ovf = 1; break; } I[1] += 1; }
+    while (pt[1] < m.y[I[1]] - EPS) { if (I[1] - 1 < lo) { *ovf = 1; break; } I[1] -= 1; }
+    while (pt[2] > m.z[I[2] + 1] + EPS) { if (I[2] + 1 > m.Nz + m.ng) { *ovf = 1; break; } I[2] += 1; }
+    while (pt[2] < m.z[I[2]] - EPS) { if (I[2] - 1 < lo) { *ovf = 1; break; } I[2] -= 1; }
+}
+
+// Index search of get_velocity_{u,v,w}face (src/Tools.jl:520-540): i with c[i] <= p < c[i+1]
+// clamped to [lo, hi-1]. The result does not depend on the starting index.
+__device__ __forceinline__ int find_cell(double p, int i, int lo, int hi, const double* c) {
+    while (p - c[i] < 0.0 && i > lo) i = i - 1;
+    while (p - c[i + 1] >= 0.0 && i + 1 < hi) i = i + 1;
+    return i;
+}
+
+__device__ __forceinline__ double trilin(const double* A, const DevMesh& m, int i, int j, int k, double wx1,
+                                         double wx2, double wy1, double wy2, double wz1, double wz2) {
+    const int64_t sx = 1, sy = m.cnx, sz = (int64_t)m.cnx * m.cny;
+    const int64_t b = m.cell(i, j, k);
+    return wz1 * (wy1 * (wx1 * A[b + sx + sy + sz] + wx2 * A[b + sy + sz]) +
+                  wy2 * (wx1 * A[b + sx + sz] + wx2 * A[b + sz])) +
+           wz2 * (wy1 * (wx1 * A[b + sx + sy] + wx2 * A[b + sy]) +
+                  wy2 * (wx1 * A[b + sx] + wx2 * A[b]));
+}
+
+// get_velocity_{u,v,w}face (src/Tools.jl:516-641), starting index (i,j,k).
+__device__ inline void vel_face(const DevMesh& m, const double* uf, const double* vf, const double* wf,
+                                const double pt[3], int i0, int j0, int k0, double v[3]) {
+    const int lo = 1 - m.ng;
+    int i, j, k;
+    double wx1, wy1, wz1;
+    // u on x faces
+    i = find_cell(pt[0], i0, lo, m.Nx + m.ng, m.x);
+    j = find_cell(pt[1], j0, lo, m.Ny + m.ng, m.ym);
+    k = find_cell(pt[2], k0, lo, m.Nz + m.ng, m.zm);
+    wx1 = sdiv(pt[0] - m.x[i], m.x[i + 1] - m.x[i]);
+    wy1 = sdiv(pt[1] - m.ym[j], m.ym[j + 1] - m.ym[j]);
+    wz1 = sdiv(pt[2] - m.zm[k], m.zm[k + 1] - m.zm[k]);
+    v[0] = trilin(uf, m, i, j, k, wx1, 1.0 - wx1, wy1, 1.0 - wy1, wz1, 1.0 - wz1);
+    // v on y faces
+    i = find_cell(pt[0], i0, lo, m.Nx + m.ng, m.xm);
+    j = find_cell(pt[1], j0, lo, m.Ny + m.ng, m.y);
+    wx1 = sdiv(pt[0] - m.xm[i], m.xm[i + 1] - m.xm[i]);
+    wy1 = sdiv(pt[1] - m.y[j], m.y[j + 1] - m.y[j]);
+    v[1] = trilin(vf, m, i, j, k, wx1, 1.0 - wx1, wy1, 1.0 - wy1, wz1, 1.0 - wz1);
+    // w on z faces
+    j = find_cell(pt[1], j0, lo, m.Ny + m.ng, m.ym);
+    k = find_cell(pt[2], k0, lo, m.Nz + m.ng, m.z);
+    wy1 = sdiv(pt[1] - m.ym[j], m.ym[j + 1] - m.ym[j]);
+    wz1 = sdiv(pt[2] - m.z[k], m.z[k + 1] - m.z[k]);
+    v[2] = trilin(wf, m, i, j, k, wx1, 1.0 - wx1, wy1, 1.0 - wy1, wz1, 1.0 - wz1);
+}
+
+}  // namespace mist

@@ -1,0 +1,96 @@
+// mist_internal.h -- host-side context of libmist_hip.so and the launchers each .hip file
+// provides. Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/mist_hip.h"
+#include "mist_device.h"
+
+#define MIST_HIP_TRY(expr)                                                              \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            ctx->err = std::string(#expr) + ": " + hipGetErrorString(e_);              \
+            return MIST_ERR_HIP;                                                        \
+        }                                                                               \
+    } while (0)
+
+namespace mist {
+struct MGState;
+void mg_free(MGState* s);
+}
+
+enum { MIST_STAGE_NORMAL = 0, MIST_STAGE_TRANSPORT, MIST_STAGE_PLIC, MIST_STAGE_BAND, MIST_STAGE_MG, MIST_STAGE_COUNT };
+
+// Per-kernel timing with HIP events recorded on the context's stream around each launch of
+// the profiled kernels (enabled by mist_profile_enable); elapsed times are collected when
+// the profile is read. Work units per launch (interface cells for ELVIRA, band cells for
+// the cut, cells for the rest) are accumulated on the device.
+struct KernelProfile {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[MIST_PROF_COUNT];
+    double ms[MIST_PROF_COUNT] = {};
+    int64_t launches[MIST_PROF_COUNT] = {};
+    int64_t host_units[MIST_PROF_COUNT] = {};
+    unsigned long long* d_units = nullptr;  // device accumulators, one per kernel
+};
+
+struct mist_ctx {
+    mist_params p{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    mist::DevMesh m{};
+    mist::DevParams dp{};
+    std::vector<double> hx, hy, hz, hxm, hym, hzm;  // host copies, index q = reference i - imino
+    double* geom = nullptr;
+    void* f[MIST_FIELD_COUNT] = {};
+    int64_t flo[MIST_FIELD_COUNT][3] = {};
+    int64_t fn[MIST_FIELD_COUNT][3] = {};
+    int64_t felem[MIST_FIELD_COUNT] = {};
+    // work buffers
+    double* pv = nullptr;        // projected cell vertices, (Nx+1)(Ny+1)(Nz+1) x 3
+    int* band_list = nullptr;    // interior cells with |band| <= nband (linear cell index)
+    int* iface_list = nullptr;   // interior interface cells (ELVIRA)
+    int* counters = nullptr;     // [0] band cells, [1] interface cells, [2] overflow flag, [3] max cut level
+    double* red = nullptr;       // reduction partials
+    int red_blocks = 0;
+    int* h_counters = nullptr;   // pinned
+    double* h_red = nullptr;     // pinned
+    uint64_t vf_version = 1, normals_version = 0, band_version = 0;
+    bool band_list_valid = false;
+    mist::MGState* mg = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double stage_ms[MIST_STAGE_COUNT] = {};
+    bool timing = false;
+    KernelProfile prof;
+    std::string err;
+};
+
+namespace mist {
+// profiling helpers (mist_capi.hip)
+hipEvent_t prof_begin(mist_ctx* ctx, int kernel);
+void prof_end(mist_ctx* ctx, int kernel, hipEvent_t a, const int* device_units, int64_t host_units);
+// mist_vof.hip
+int launch_define_velocity(mist_ctx* ctx, double t);
+int launch_maxabs_uvw(mist_ctx* ctx, double out[3]);
+int launch_neumann(mist_ctx* ctx, int field);
+int launch_borders(mist_ctx* ctx, int field);
+int launch_band(mist_ctx* ctx);
+int launch_normals(mist_ctx* ctx);
+int launch_plic(mist_ctx* ctx);
+int launch_transport(mist_ctx* ctx, double dt);
+int launch_init_vof(mist_ctx* ctx, int shape, const double* args);
+int launch_dens(mist_ctx* ctx);
+int launch_pressure_rhs(mist_ctx* ctx);
+int launch_corrector(mist_ctx* ctx, double dt);
+int launch_reduce(mist_ctx* ctx, int kind, int field, double* out);
+int launch_plic_dist_batch(mist_ctx* ctx, const int64_t* ijk, const double* nrm, const double* vf, double* out, int64_t m);
+int launch_plic_vf_batch(mist_ctx* ctx, const int64_t* ijk, const double* nrm, const double* dist, double* out, int64_t m);
+// mist_mg.hip
+int pressure_solve(mist_ctx* ctx, double dt, double tol, int precision, int64_t* cycles, double* resid);
+}  // namespace mist

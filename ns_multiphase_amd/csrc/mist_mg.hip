@@ -1,0 +1,521 @@
+// mist_mg.hip -- variable-density pressure Poisson multigrid on gfx950.
+//
+// Operator (src/Pressure.jl:909-1033 scaling): (A p)_c = sum_f c_f (p_nb - p_c) with
+// c_f = dt / (den_f h^2), wall faces dropped, periodic faces wrapped, b = div(u*)
+// (src/Pressure.jl:16-22). V-cycle of mg_vc_lin! (src/Multigrid.jl:678-764): red-black
+// Gauss-Seidel (src/Pressure.jl:1096-1133, colour (i+j+k)%2, red first, omega 1), 8-cell
+// average restriction (src/Multigrid.jl:183-197), trilinear prolongation from coarse centres
+// (src/Multigrid.jl:5-54). Coarse face coefficients are the mean of the fine faces covering
+// them. The outer loop evaluates the FP64 residual in flux form and applies one V-cycle in
+// precision T (double, or float for the mixed mode) to it per iteration, until
+// max|b - A p| <= tol; the solution is shifted to zero mean.
+//
+// Storage: interior cells only (n_x n_y n_z per level, x fastest); neighbours across a wall
+// have a zero coefficient and read the cell itself, so no ghost layers are needed.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "mist_internal.h"
+
+namespace mist {
+
+struct LevelGeom {
+    int n[3];
+    int per[3];
+    int coarsened[3];  // coarser than the level above along axis d
+    __host__ __device__ int64_t size() const { return (int64_t)n[0] * n[1] * n[2]; }
+};
+
+template <class T>
+struct LevelBuf {
+    T* c[3] = {nullptr, nullptr, nullptr};
+    T* e = nullptr;
+    T* b = nullptr;
+    T* r = nullptr;
+};
+
+struct MGState {
+    std::vector<LevelGeom> g;
+    std::vector<LevelBuf<double>> d;  // FP64 hierarchy
+    std::vector<LevelBuf<float>> s;   // FP32 hierarchy
+    double* c64[3] = {nullptr, nullptr, nullptr};  // fine FP64 operator
+    double* p64 = nullptr;
+    double* b64 = nullptr;
+    double* r64 = nullptr;
+    double* part = nullptr;
+    double* h_part = nullptr;  // pinned
+    int nblocks = 0;
+    int64_t nfine = 0;
+    std::vector<void*> allocs;
+};
+
+void mg_free(MGState* s) {
+    if (!s) return;
+    for (void* p : s->allocs) (void)hipFree(p);
+    if (s->h_part) (void)hipHostFree(s->h_part);
+    delete s;
+}
+
+// ---- kernels ---------------------------------------------------------------------------
+struct Nb {
+    int64_t id[6];
+};
+
+template <class T>
+__device__ __forceinline__ void stencil(const LevelGeom& g, T* const* C, int i, int j, int k, int64_t cell, int64_t nb[6],
+                                       T cf[6]) {
+    const int q[3] = {i, j, k};
+    const int64_t st[3] = {1, (int64_t)g.n[0], (int64_t)g.n[0] * g.n[1]};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const int nd = g.n[d];
+        cf[2 * d] = C[d][cell];
+        if (q[d] + 1 < nd) {
+            nb[2 * d + 1] = cell + st[d];
+            cf[2 * d + 1] = C[d][cell + st[d]];
+        } else if (g.per[d]) {
+            nb[2 * d + 1] = cell - (int64_t)(nd - 1) * st[d];
+            cf[2 * d + 1] = C[d][nb[2 * d + 1]];
+        } else {
+            nb[2 * d + 1] = cell;
+            cf[2 * d + 1] = T(0);
+        }
+        if (q[d] > 0) nb[2 * d] = cell - st[d];
+        else if (g.per[d]) nb[2 * d] = cell + (int64_t)(nd - 1) * st[d];
+        else nb[2 * d] = cell;
+    }
+}
+
+template <class T>
+__global__ void k_rbgs(LevelGeom g, T* cx, T* cy, T* cz, T* e, const T* b, int color) {
+    T* C[3] = {cx, cy, cz};
+    const int hx = (g.n[0] + 1) / 2;
+    const int64_t tot = (int64_t)hx * g.n[1] * g.n[2];
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
+        const int h = (int)(q % hx);
+        const int64_t row = q / hx;
+        const int j = (int)(row % g.n[1]);
+        const int k = (int)(row / g.n[1]);
+        const int i = 2 * h + ((color + 1 + j + k) & 1);  // ((i+1)+(j+1)+(k+1)) % 2 == color
+        if (i >= g.n[0]) continue;
+        const int64_t cell = i + (int64_t)g.n[0] * (j + (int64_t)g.n[1] * k);
+        int64_t nb[6];
+        T cf[6];
+        stencil(g, C, i, j, k, cell, nb, cf);
+        const T S = ((((cf[0] + cf[1]) + cf[2]) + cf[3]) + cf[4]) + cf[5];
+        if (!(S > T(0))) continue;
+        const T sum = ((((cf[0] * e[nb[0]] + cf[1] * e[nb[1]]) + cf[2] * e[nb[2]]) + cf[3] * e[nb[3]]) +
+                       cf[4] * e[nb[4]]) + cf[5] * e[nb[5]];
+        e[cell] = (sum - b[cell]) / S;
+    }
+}
+
+template <class T>
+__global__ void k_resid(LevelGeom g, T* cx, T* cy, T* cz, const T* e, const T* b, T* r) {
+    T* C[3] = {cx, cy, cz};
+    const int64_t tot = g.size();
+    for (int64_t cell = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; cell < tot;
+         cell += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(cell % g.n[0]);
+        const int j = (int)((cell / g.n[0]) % g.n[1]);
+        const int k = (int)(cell / ((int64_t)g.n[0] * g.n[1]));
+        int64_t nb[6];
+        T cf[6];
+        stencil(g, C, i, j, k, cell, nb, cf);
+        const T pc = e[cell];
+        const T Ap = ((((cf[0] * (e[nb[0]] - pc) + cf[1] * (e[nb[1]] - pc)) + cf[2] * (e[nb[2]] - pc)) +
+                       cf[3] * (e[nb[3]] - pc)) + cf[4] * (e[nb[4]] - pc)) + cf[5] * (e[nb[5]] - pc);
+        r[cell] = b[cell] - Ap;
+    }
+}
+
+template <class T>
+__global__ void k_restrict(LevelGeom F, LevelGeom C, const T* rf, T* bc) {
+    const int f0 = C.coarsened[0] ? 2 : 1, f1 = C.coarsened[1] ? 2 : 1, f2 = C.coarsened[2] ? 2 : 1;
+    const T inv = T(1) / T(f0 * f1 * f2);
+    const int64_t tot = C.size();
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
+        const int I = (int)(q % C.n[0]);
+        const int J = (int)((q / C.n[0]) % C.n[1]);
+        const int K = (int)(q / ((int64_t)C.n[0] * C.n[1]));
+        T s = T(0);
+        for (int c = 0; c < f2; ++c)
+            for (int b = 0; b < f1; ++b)
+                for (int a = 0; a < f0; ++a)
+                    s += rf[(I * f0 + a) + (int64_t)F.n[0] * ((J * f1 + b) + (int64_t)F.n[1] * (K * f2 + c))];
+        bc[q] = s * inv;
+    }
+}
+
+template <class T>
+__global__ void k_prolong_add(LevelGeom C, LevelGeom F, const T* ec, T* ef) {
+    const int64_t tot = F.size();
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
+        const int qq[3] = {(int)(q % F.n[0]), (int)((q / F.n[0]) % F.n[1]), (int)(q / ((int64_t)F.n[0] * F.n[1]))};
+        int lo[3], hi[3];
+        T w1[3], w2[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            if (!C.coarsened[d]) { lo[d] = hi[d] = qq[d]; w1[d] = T(1); w2[d] = T(0); continue; }
+            const int Q = qq[d] / 2;
+            if ((qq[d] & 1) == 0) { lo[d] = Q - 1; hi[d] = Q; w1[d] = T(0.75); w2[d] = T(0.25); }
+            else { lo[d] = Q; hi[d] = Q + 1; w1[d] = T(0.25); w2[d] = T(0.75); }
+            if (lo[d] < 0) lo[d] = C.per[d] ? C.n[d] - 1 : 0;
+            if (hi[d] >= C.n[d]) hi[d] = C.per[d] ? 0 : C.n[d] - 1;
+        }
+        auto E = [&](int a, int b, int c) { return ec[a + (int64_t)C.n[0] * (b + (int64_t)C.n[1] * c)]; };
+        const T v = w1[2] * (w1[1] * (w1[0] * E(hi[0], hi[1], hi[2]) + w2[0] * E(lo[0], hi[1], hi[2])) +
+                             w2[1] * (w1[0] * E(hi[0], lo[1], hi[2]) + w2[0] * E(lo[0], lo[1], hi[2]))) +
+                    w2[2] * (w1[1] * (w1[0] * E(hi[0], hi[1], lo[2]) + w2[0] * E(lo[0], hi[1], lo[2])) +
+                             w2[1] * (w1[0] * E(hi[0], lo[1], lo[2]) + w2[0] * E(lo[0], lo[1], lo[2])));
+        ef[q] += v;
+    }
+}
+
+template <class T>
+__global__ void k_coarsen_coef(LevelGeom F, LevelGeom C, int d, const T* cf, T* cc) {
+    const int f[3] = {C.coarsened[0] ? 2 : 1, C.coarsened[1] ? 2 : 1, C.coarsened[2] ? 2 : 1};
+    const int d1 = (d + 1) % 3, d2 = (d + 2) % 3;
+    const T scale = T(1) / T(f[d] * f[d]) / T(f[d1] * f[d2]);
+    const int64_t tot = C.size();
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
+        const int Q[3] = {(int)(q % C.n[0]), (int)((q / C.n[0]) % C.n[1]), (int)(q / ((int64_t)C.n[0] * C.n[1]))};
+        T s = T(0);
+        for (int b = 0; b < f[d2]; ++b)
+            for (int a = 0; a < f[d1]; ++a) {
+                int x[3];
+                x[d] = Q[d] * f[d];
+                x[d1] = Q[d1] * f[d1] + a;
+                x[d2] = Q[d2] * f[d2] + b;
+                s += cf[x[0] + (int64_t)F.n[0] * (x[1] + (int64_t)F.n[1] * x[2])];
+            }
+        cc[q] = s * scale;
+    }
+}
+
+// fine coefficients from the face densities (FP64), plus the rhs and initial guess
+__global__ void k_fine_setup(DevMesh m, LevelGeom g, double dt, const double* denx, const double* deny,
+                             const double* denz, const double* rhs, const double* P, double* cx, double* cy,
+                             double* cz, double* b, double* p) {
+    const int64_t tot = g.size();
+    const double h2[3] = {m.dx * m.dx, m.dy * m.dy, m.dz * m.dz};
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(q % g.n[0]) + 1;
+        const int j = (int)((q / g.n[0]) % g.n[1]) + 1;
+        const int k = (int)(q / ((int64_t)g.n[0] * g.n[1])) + 1;
+        const int a = i - 1 + m.ng, bb = j - 1 + m.ng, c = k - 1 + m.ng;
+        const double dx = denx[a + (int64_t)(m.cnx + 1) * (bb + (int64_t)m.cny * c)];
+        const double dy = deny[a + (int64_t)m.cnx * (bb + (int64_t)(m.cny + 1) * c)];
+        const double dz = denz[a + (int64_t)m.cnx * (bb + (int64_t)m.cny * c)];
+        cx[q] = (i > 1 || g.per[0]) ? dt / (dx * h2[0]) : 0.0;
+        cy[q] = (j > 1 || g.per[1]) ? dt / (dy * h2[1]) : 0.0;
+        cz[q] = (k > 1 || g.per[2]) ? dt / (dz * h2[2]) : 0.0;
+        b[q] = rhs[m.cell(i, j, k)];
+        p[q] = P[m.cell(i, j, k)];
+    }
+}
+
+template <class Ts, class Td>
+__global__ void k_convert(const Ts* a, Td* b, int64_t n) {
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        b[q] = (Td)a[q];
+}
+
+// b_T = (T) r64, e_T = 0
+template <class T>
+__global__ void k_start_cycle(const double* r64, T* b, T* e, int64_t n) {
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        b[q] = (T)r64[q];
+        e[q] = T(0);
+    }
+}
+
+template <class T>
+__global__ void k_add_correction(const T* e, double* p, int64_t n) {
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        p[q] += (double)e[q];
+}
+
+template <class T>
+__global__ void k_zero(T* a, int64_t n) {
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        a[q] = T(0);
+}
+
+__global__ void k_shift(double* a, int64_t n, const double* sum) {
+    const double mean = *sum / (double)n;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        a[q] -= mean;
+}
+
+constexpr int MG_RED = 256;
+__device__ double mg_block_sum(double v, bool maxabs) {
+    __shared__ double s[MG_RED];
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = MG_RED / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            s[threadIdx.x] = maxabs ? fmax(s[threadIdx.x], s[threadIdx.x + w]) : s[threadIdx.x] + s[threadIdx.x + w];
+        __syncthreads();
+    }
+    const double r = s[0];
+    __syncthreads();
+    return r;
+}
+
+template <class T>
+__global__ __launch_bounds__(MG_RED) void k_sum_part(const T* a, int64_t n, int maxabs, double* part) {
+    double v = 0.0;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        v = maxabs ? fmax(v, fabs((double)a[q])) : v + (double)a[q];
+    const double r = mg_block_sum(v, maxabs != 0);
+    if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+__global__ __launch_bounds__(MG_RED) void k_sum_final(const double* part, int n, int maxabs, double* out) {
+    double v = 0.0;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) v = maxabs ? fmax(v, part[q]) : v + part[q];
+    const double r = mg_block_sum(v, maxabs != 0);
+    if (threadIdx.x == 0) *out = r;
+}
+
+// Coarsest level: remove the mean of b, zero e, then `sweeps` red-black sweeps, all in one
+// work-group (the level has at most a few hundred cells).
+template <class T>
+__global__ __launch_bounds__(256) void k_coarse_solve(LevelGeom g, T* cx, T* cy, T* cz, T* e, T* b, int sweeps) {
+    __shared__ double s[256];
+    T* C[3] = {cx, cy, cz};
+    const int64_t tot = g.size();
+    double v = 0.0;
+    for (int64_t q = threadIdx.x; q < tot; q += blockDim.x) v += (double)b[q];
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    const T mean = (T)(s[0] / (double)tot);
+    for (int64_t q = threadIdx.x; q < tot; q += blockDim.x) { b[q] -= mean; e[q] = T(0); }
+    __syncthreads();
+    for (int sw = 0; sw < sweeps; ++sw)
+        for (int color = 0; color < 2; ++color) {
+            for (int64_t cell = threadIdx.x; cell < tot; cell += blockDim.x) {
+                const int i = (int)(cell % g.n[0]);
+                const int j = (int)((cell / g.n[0]) % g.n[1]);
+                const int k = (int)(cell / ((int64_t)g.n[0] * g.n[1]));
+                if (((i + 1) + (j + 1) + (k + 1)) % 2 != color) continue;
+                int64_t nb[6];
+                T cf[6];
+                stencil(g, C, i, j, k, cell, nb, cf);
+                const T S = ((((cf[0] + cf[1]) + cf[2]) + cf[3]) + cf[4]) + cf[5];
+                if (!(S > T(0))) continue;
+                const T sum = ((((cf[0] * e[nb[0]] + cf[1] * e[nb[1]]) + cf[2] * e[nb[2]]) + cf[3] * e[nb[3]]) +
+                               cf[4] * e[nb[4]]) + cf[5] * e[nb[5]];
+                e[cell] = (sum - b[cell]) / S;
+            }
+            __syncthreads();
+        }
+}
+
+__global__ void k_store_p(DevMesh m, LevelGeom g, const double* p, double* P) {
+    const int64_t tot = g.size();
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(q % g.n[0]) + 1;
+        const int j = (int)((q / g.n[0]) % g.n[1]) + 1;
+        const int k = (int)(q / ((int64_t)g.n[0] * g.n[1])) + 1;
+        P[m.cell(i, j, k)] = p[q];
+    }
+}
+
+// ---- host driver -----------------------------------------------------------------------
+static inline int gridn(int64_t n, int b = 256) {
+    int64_t g = (n + b - 1) / b;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(g, 65536));
+}
+
+#define MGL(kernel, grid, block, ...)                                                    \
+    do {                                                                                 \
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, ctx->stream, __VA_ARGS__); \
+        MIST_HIP_TRY(hipGetLastError());                                                 \
+    } while (0)
+
+template <class T>
+static int alloc_level(mist_ctx* ctx, MGState* s, const LevelGeom& g, LevelBuf<T>& L) {
+    const size_t bytes = (size_t)g.size() * sizeof(T);
+    void** ptrs[6] = {(void**)&L.c[0], (void**)&L.c[1], (void**)&L.c[2], (void**)&L.e, (void**)&L.b, (void**)&L.r};
+    for (void** p : ptrs) {
+        if (hipMalloc(p, bytes) != hipSuccess) { ctx->err = "multigrid: device allocation failed"; return MIST_ERR_OOM; }
+        s->allocs.push_back(*p);
+    }
+    return MIST_OK;
+}
+
+static int build_state(mist_ctx* ctx) {
+    if (ctx->mg) return MIST_OK;
+    const mist_params& p = ctx->p;
+    MGState* s = new MGState();
+    LevelGeom g0{};
+    g0.n[0] = (int)p.Nx; g0.n[1] = (int)p.Ny; g0.n[2] = (int)p.Nz;
+    g0.per[0] = p.xper && p.Nx > 1; g0.per[1] = p.yper && p.Ny > 1; g0.per[2] = p.zper && p.Nz > 1;
+    int64_t nlev = p.mg_lvl;
+    if (nlev <= 1) {
+        nlev = 1;
+        int n[3] = {g0.n[0], g0.n[1], g0.n[2]};
+        for (;;) {
+            const int mx = std::max(n[0], std::max(n[1], n[2]));
+            bool ok = mx > 4;
+            for (int d = 0; d < 3; ++d) if (n[d] > 1 && (n[d] % 2)) ok = false;
+            if (!ok) break;
+            for (int d = 0; d < 3; ++d) if (n[d] > 1) n[d] /= 2;
+            ++nlev;
+        }
+    }
+    s->g.push_back(g0);
+    for (int64_t l = 1; l < nlev; ++l) {
+        LevelGeom c{};
+        const LevelGeom& f = s->g.back();
+        for (int d = 0; d < 3; ++d) {
+            if (f.n[d] > 1 && (f.n[d] % 2)) { mg_free(s); ctx->err = "multigrid: odd extent cannot be coarsened"; return MIST_ERR_PARAM; }
+            c.coarsened[d] = f.n[d] > 1;
+            c.n[d] = f.n[d] > 1 ? f.n[d] / 2 : 1;
+            c.per[d] = f.per[d] && c.n[d] > 1;
+        }
+        s->g.push_back(c);
+    }
+    if (s->g.back().size() > 4096) { mg_free(s); ctx->err = "multigrid: coarsest level too large (raise mg_lvl)"; return MIST_ERR_PARAM; }
+    s->d.resize(s->g.size());
+    s->s.resize(s->g.size());
+    int st;
+    for (size_t l = 0; l < s->g.size(); ++l) {
+        if ((st = alloc_level(ctx, s, s->g[l], s->d[l]))) { mg_free(s); return st; }
+        if ((st = alloc_level(ctx, s, s->g[l], s->s[l]))) { mg_free(s); return st; }
+    }
+    s->nfine = g0.size();
+    const size_t fb = (size_t)s->nfine * sizeof(double);
+    double** f64[6] = {&s->c64[0], &s->c64[1], &s->c64[2], &s->p64, &s->b64, &s->r64};
+    for (double** q : f64) {
+        if (hipMalloc((void**)q, fb) != hipSuccess) { mg_free(s); ctx->err = "multigrid: device allocation failed"; return MIST_ERR_OOM; }
+        s->allocs.push_back(*q);
+    }
+    s->nblocks = 1024;
+    if (hipMalloc((void**)&s->part, (s->nblocks + 8) * sizeof(double)) != hipSuccess) { mg_free(s); ctx->err = "multigrid: device allocation failed"; return MIST_ERR_OOM; }
+    s->allocs.push_back(s->part);
+    if (hipHostMalloc((void**)&s->h_part, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess) { mg_free(s); ctx->err = "multigrid: pinned allocation failed"; return MIST_ERR_OOM; }
+    ctx->mg = s;
+    return MIST_OK;
+}
+
+// sum (maxabs = 0) or max |.| (maxabs = 1) of a device vector, result in s->part[nblocks]
+template <class T>
+static int dev_reduce(mist_ctx* ctx, const T* a, int64_t n, int maxabs) {
+    MGState* s = ctx->mg;
+    const int g = std::min<int>(s->nblocks, gridn(n, MG_RED));
+    MGL(k_sum_part<T>, g, MG_RED, a, n, maxabs, s->part);
+    MGL(k_sum_final, 1, MG_RED, (const double*)s->part, g, maxabs, s->part + s->nblocks);
+    return MIST_OK;
+}
+
+template <class T>
+static int vcycle(mist_ctx* ctx, std::vector<LevelBuf<T>>& L, size_t l) {
+    MGState* s = ctx->mg;
+    const LevelGeom& g = s->g[l];
+    LevelBuf<T>& B = L[l];
+    const int nu1 = (int)ctx->p.mg_nu1, nu2 = (int)ctx->p.mg_nu2;
+    const int64_t n = g.size();
+    const int64_t nh = (int64_t)((g.n[0] + 1) / 2) * g.n[1] * g.n[2];
+    if (l + 1 == L.size()) {
+        MGL(k_coarse_solve<T>, 1, 256, g, B.c[0], B.c[1], B.c[2], B.e, B.b, (int)ctx->p.mg_coarse_sweeps);
+        return MIST_OK;
+    }
+    if (l > 0) MGL(k_zero<T>, gridn(n), 256, B.e, n);
+    for (int sw = 0; sw < nu1; ++sw)
+        for (int c = 0; c < 2; ++c) {
+            hipEvent_t ev = l == 0 ? prof_begin(ctx, MIST_PROF_MG_RBGS) : nullptr;
+            MGL(k_rbgs<T>, gridn(nh), 256, g, B.c[0], B.c[1], B.c[2], B.e, (const T*)B.b, c);
+            prof_end(ctx, MIST_PROF_MG_RBGS, ev, nullptr, n);
+        }
+    MGL(k_resid<T>, gridn(n), 256, g, B.c[0], B.c[1], B.c[2], (const T*)B.e, (const T*)B.b, B.r);
+    const LevelGeom& gc = s->g[l + 1];
+    MGL(k_restrict<T>, gridn(gc.size()), 256, g, gc, (const T*)B.r, L[l + 1].b);
+    int st = vcycle(ctx, L, l + 1);
+    if (st) return st;
+    MGL(k_prolong_add<T>, gridn(n), 256, gc, g, (const T*)L[l + 1].e, B.e);
+    for (int sw = 0; sw < nu2; ++sw)
+        for (int c = 0; c < 2; ++c) {
+            hipEvent_t ev = l == 0 ? prof_begin(ctx, MIST_PROF_MG_RBGS) : nullptr;
+            MGL(k_rbgs<T>, gridn(nh), 256, g, B.c[0], B.c[1], B.c[2], B.e, (const T*)B.b, c);
+            prof_end(ctx, MIST_PROF_MG_RBGS, ev, nullptr, n);
+        }
+    return MIST_OK;
+}
+
+template <class T>
+static int setup_hierarchy(mist_ctx* ctx, std::vector<LevelBuf<T>>& L) {
+    MGState* s = ctx->mg;
+    for (int d = 0; d < 3; ++d) MGL((k_convert<double, T>), gridn(s->nfine), 256, (const double*)s->c64[d], L[0].c[d], s->nfine);
+    for (size_t l = 1; l < L.size(); ++l)
+        for (int d = 0; d < 3; ++d)
+            MGL(k_coarsen_coef<T>, gridn(s->g[l].size()), 256, s->g[l - 1], s->g[l], d, (const T*)L[l - 1].c[d], L[l].c[d]);
+    return MIST_OK;
+}
+
+template <class T>
+static int solve_t(mist_ctx* ctx, std::vector<LevelBuf<T>>& L, double tol, int64_t* cycles, double* resid) {
+    MGState* s = ctx->mg;
+    int st = setup_hierarchy(ctx, L);
+    if (st) return st;
+    const LevelGeom& g0 = s->g[0];
+    const int64_t n = s->nfine;
+    const int64_t maxc = ctx->p.mg_max_cycles > 0 ? ctx->p.mg_max_cycles : 10000;
+    int64_t cyc = 0;
+    double res = 0.0;
+    int status = MIST_OK;
+    for (;;) {
+        hipEvent_t ev = prof_begin(ctx, MIST_PROF_MG_RESID);
+        MGL(k_resid<double>, gridn(n), 256, g0, s->c64[0], s->c64[1], s->c64[2], (const double*)s->p64,
+            (const double*)s->b64, s->r64);
+        prof_end(ctx, MIST_PROF_MG_RESID, ev, nullptr, n);
+        if ((st = dev_reduce<double>(ctx, s->r64, n, 1))) return st;
+        MIST_HIP_TRY(hipMemcpyAsync(s->h_part, s->part + s->nblocks, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        MIST_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        res = s->h_part[0];
+        if (!(res > tol)) break;  // also stops on NaN, reported through the residual
+        if (cyc >= maxc) { status = MIST_ERR_NOCONV; break; }
+        MGL(k_start_cycle<T>, gridn(n), 256, (const double*)s->r64, L[0].b, L[0].e, n);
+        if ((st = vcycle(ctx, L, 0))) return st;
+        MGL(k_add_correction<T>, gridn(n), 256, (const T*)L[0].e, s->p64, n);
+        ++cyc;
+    }
+    if (res != res) status = MIST_ERR_NOCONV;
+    *cycles = cyc;
+    *resid = res;
+    return status;
+}
+
+int pressure_solve(mist_ctx* ctx, double dt, double tol, int precision, int64_t* cycles, double* resid) {
+    int st = build_state(ctx);
+    if (st) return st;
+    MGState* s = ctx->mg;
+    const int64_t n = s->nfine;
+    MGL(k_fine_setup, gridn(n), 256, ctx->m, s->g[0], dt, (const double*)ctx->f[MIST_FIELD_DENX],
+        (const double*)ctx->f[MIST_FIELD_DENY], (const double*)ctx->f[MIST_FIELD_DENZ],
+        (const double*)ctx->f[MIST_FIELD_RHS], (const double*)ctx->f[MIST_FIELD_P], s->c64[0], s->c64[1], s->c64[2],
+        s->b64, s->p64);
+    // compatibility: remove the mean of b (constants are the null space of A)
+    if ((st = dev_reduce<double>(ctx, s->b64, n, 0))) return st;
+    MGL(k_shift, gridn(n), 256, s->b64, n, (const double*)(s->part + s->nblocks));
+    int status = precision == MIST_MG_MIXED ? solve_t<float>(ctx, s->s, tol, cycles, resid)
+                                            : solve_t<double>(ctx, s->d, tol, cycles, resid);
+    if (status != MIST_OK && status != MIST_ERR_NOCONV) return status;
+    // zero-mean gauge, back into P, then ghosts
+    if ((st = dev_reduce<double>(ctx, s->p64, n, 0))) return st;
+    MGL(k_shift, gridn(n), 256, s->p64, n, (const double*)(s->part + s->nblocks));
+    MGL(k_store_p, gridn(n), 256, ctx->m, s->g[0], (const double*)s->p64, (double*)ctx->f[MIST_FIELD_P]);
+    if ((st = launch_neumann(ctx, MIST_FIELD_P))) return st;
+    if ((st = launch_borders(ctx, MIST_FIELD_P))) return st;
+    if (status == MIST_ERR_NOCONV) ctx->err = "pressure solve did not converge";
+    return status;
+}
+
+}  // namespace mist
