@@ -158,7 +158,8 @@ def test_reduce_and_dt(mist, oracle):
     g.define_velocity(0.3)
     o.define_velocity(0.3)
     for name in ("u", "v", "w", "uf", "vf", "wf"):
-        assert np.abs(g.get(name) - o.get(name)).max() <= 4e-16, name
+        # device sin/cos may differ from the host libm by an ulp (values up to 2)
+        assert np.abs(g.get(name) - o.get(name)).max() <= 1e-15, name
     assert g.compute_dt() == o.compute_dt()
     for kind in ("sum", "max", "min", "maxabs"):
         a, b = g.reduce(kind, "VF"), o.reduce(kind, "VF")
