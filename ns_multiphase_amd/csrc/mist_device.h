@@ -251,6 +251,168 @@ __device__ inline double plic2vf_corners(const double cx[2], const double cy[2],
     return VF;
 }
 
+// ---- register-resident tet cutting -------------------------------------------------
+// The same tables as compile-time constants, so that a case-specialised cut indexes its
+// vertex arrays with constants and keeps them in registers.
+struct CutTab {
+    static constexpr unsigned char nvert[16] = {0, 3, 3, 4, 3, 4, 4, 3, 3, 4, 4, 3, 4, 3, 3, 0};
+    static constexpr unsigned char ntets[16] = {1, 4, 4, 6, 4, 6, 6, 4, 4, 6, 6, 4, 6, 4, 4, 1};
+    static constexpr unsigned char nntet[16] = {1, 2, 2, 4, 2, 4, 4, 4, 2, 4, 4, 4, 4, 4, 4, 2};
+    // cut vertex n of case c interpolates reference vertices (v1, v2): nibbles v1 | v2 << 4
+    static constexpr unsigned char edge[16][4] = {
+        {0, 0, 0, 0},
+        {0x10, 0x20, 0x30, 0},
+        {0x21, 0x31, 0x01, 0},
+        {0x30, 0x31, 0x20, 0x21},
+        {0x32, 0x02, 0x12, 0},
+        {0x30, 0x32, 0x10, 0x12},
+        {0x31, 0x32, 0x01, 0x02},
+        {0x03, 0x13, 0x23, 0},
+        {0x03, 0x13, 0x23, 0},
+        {0x20, 0x23, 0x10, 0x13},
+        {0x21, 0x23, 0x01, 0x03},
+        {0x32, 0x02, 0x12, 0},
+        {0x12, 0x13, 0x02, 0x03},
+        {0x21, 0x31, 0x01, 0},
+        {0x10, 0x20, 0x30, 0},
+        {0, 0, 0, 0}};
+};
+
+struct V3 {
+    double x, y, z;
+};
+
+__device__ __forceinline__ V3 lerp3(const V3& a, const V3& b, double mu) {
+    return V3{(1.0 - mu) * a.x + mu * b.x, (1.0 - mu) * a.y + mu * b.y, (1.0 - mu) * a.z + mu * b.z};
+}
+__device__ __forceinline__ double tet_vol3(const V3& p0, const V3& p1, const V3& p2, const V3& p3) {
+    const double a1 = p0.x - p3.x, a2 = p0.y - p3.y, a3 = p0.z - p3.z;
+    const double b1 = p1.x - p3.x, b2 = p1.y - p3.y, b3 = p1.z - p3.z;
+    const double c1 = p2.x - p3.x, c2 = p2.y - p3.y, c3 = p2.z - p3.z;
+    return (-1.0 / 6.0) * ((a1 * (b2 * c3 - c2 * b3) - a2 * (b1 * c3 - c1 * b3)) + a3 * (b1 * c2 - c1 * b2));
+}
+
+template <int V>
+struct IC {
+    static constexpr int value = V;
+};
+template <int B, int E, class F>
+__device__ __forceinline__ void sfor(F&& f) {
+    if constexpr (B < E) {
+        f(IC<B>{});
+        sfor<B + 1, E>(f);
+    }
+}
+
+// Vertex numbers 5..8 of a case: (0-based) tet numbers of the sub-tets, as constants.
+__host__ __device__ constexpr int vtet_of(int c, int n, int k) {
+    constexpr unsigned short T[16][6] = {
+        {0x3210, 0, 0, 0, 0, 0},
+        {0x0564, 0x3215, 0x5413, 0x3654, 0, 0},
+        {0x1546, 0x5320, 0x5240, 0x0564, 0, 0},
+        {0x1574, 0x0764, 0x1704, 0x3754, 0x2674, 0x3274},
+        {0x2645, 0x5301, 0x1345, 0x1465, 0, 0},
+        {0x2754, 0x0674, 0x2074, 0x3574, 0x1764, 0x1374},
+        {0x2457, 0x1764, 0x2147, 0x3547, 0x0674, 0x3074},
+        {0x6210, 0x5610, 0x0564, 0x3654, 0, 0},
+        {0x3654, 0x5210, 0x5204, 0x2564, 0, 0},
+        {0x3574, 0x0764, 0x0374, 0x2754, 0x1674, 0x2174},
+        {0x3547, 0x1674, 0x3417, 0x2457, 0x0764, 0x0274},
+        {0x6130, 0x6503, 0x3465, 0x2645, 0, 0},
+        {0x3457, 0x2764, 0x2437, 0x1547, 0x0674, 0x1074},
+        {0x6032, 0x3256, 0x2456, 0x1546, 0, 0},
+        {0x2136, 0x6521, 0x1654, 0x0564, 0, 0},
+        {0x3210, 0, 0, 0, 0, 0}};
+    return (T[c][n] >> (4 * k)) & 15;
+}
+
+// Liquid (negative side) and total volume of tet P cut by a plane with vertex distances dm,
+// sub-tets in the reference's order (src/VFgeom.jl:784-827): liquid tets from ntets down
+// to nntet, then gas tets 1..nntet-1. on_liq(tv) / on_gas(tv) receive each sub-tet volume.
+template <int CS, bool GAS, class FL, class FG>
+__device__ __forceinline__ void cut_case(const V3 (&P)[4], const double (&dm)[4], FL&& on_liq, FG&& on_gas) {
+    constexpr int NV = CutTab::nvert[CS], NT = CutTab::ntets[CS], NN = CutTab::nntet[CS];
+    V3 Q[8];
+    Q[0] = P[0]; Q[1] = P[1]; Q[2] = P[2]; Q[3] = P[3];
+    sfor<0, NV>([&](auto I) {
+        constexpr int n = decltype(I)::value;
+        constexpr int v1 = CutTab::edge[CS][n] & 15, v2 = CutTab::edge[CS][n] >> 4;
+        const double mu = jmin(1.0, jmax(0.0, sdiv(-dm[v1], dm[v2] - dm[v1])));
+        Q[4 + n] = lerp3(Q[v1], Q[v2], mu);
+    });
+    sfor<0, NT - NN + 1>([&](auto I) {
+        constexpr int n = NT - decltype(I)::value;  // 1-based sub-tet number
+        on_liq(tet_vol3(Q[vtet_of(CS, n - 1, 0)], Q[vtet_of(CS, n - 1, 1)], Q[vtet_of(CS, n - 1, 2)],
+                        Q[vtet_of(CS, n - 1, 3)]));
+    });
+    if constexpr (GAS) {
+        sfor<1, NN>([&](auto I) {
+            constexpr int n = decltype(I)::value;
+            on_gas(tet_vol3(Q[vtet_of(CS, n - 1, 0)], Q[vtet_of(CS, n - 1, 1)], Q[vtet_of(CS, n - 1, 2)],
+                            Q[vtet_of(CS, n - 1, 3)]));
+        });
+    }
+}
+
+template <bool GAS, class FL, class FG>
+__device__ __forceinline__ void cut_dispatch(int cs, const V3 (&P)[4], const double (&dm)[4], FL&& on_liq,
+                                             FG&& on_gas) {
+    switch (cs) {
+#define MIST_CASE(k) \
+    case k: cut_case<k, GAS>(P, dm, on_liq, on_gas); break;
+        MIST_CASE(0) MIST_CASE(1) MIST_CASE(2) MIST_CASE(3) MIST_CASE(4) MIST_CASE(5) MIST_CASE(6) MIST_CASE(7)
+        MIST_CASE(8) MIST_CASE(9) MIST_CASE(10) MIST_CASE(11) MIST_CASE(12) MIST_CASE(13) MIST_CASE(14) MIST_CASE(15)
+#undef MIST_CASE
+        default: break;
+    }
+}
+
+// computePLIC2VF (src/VF.jl:219-278) of the cell [x0,x1]x[y0,y1]x[z0,z1] for the plane
+// (nx,ny,nz,dist): 5 tets (parity `even` selects the vertex numbering and tetsign), d per tet
+// vertex as the reference computes it, snapped in place by d2case, then the liquid sub-tets.
+__device__ __forceinline__ void tet5_vertices(int t, bool even, double x0, double x1, double y0, double y1,
+                                              double z0, double z1, V3 (&P)[4]) {
+    const unsigned tv = TET5[t];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+        const int c = vert_corner(nib(tv, n), even);
+        P[n].x = (c & 1) ? x1 : x0;
+        P[n].y = (c & 2) ? y1 : y0;
+        P[n].z = (c & 4) ? z1 : z0;
+    }
+}
+
+__device__ __forceinline__ double plic2vf_reg(double x0, double x1, double y0, double y1, double z0, double z1,
+                                              double nx, double ny, double nz, double dist, bool even,
+                                              double cellvol) {
+    const double ts = even ? 1.0 : -1.0;
+    double VF = 0.0;
+    for (int t = 0; t < 5; ++t) {
+        V3 P[4];
+        tet5_vertices(t, even, x0, x1, y0, y1, z0, z1, P);
+        double d[4];
+#pragma unroll
+        for (int n = 0; n < 4; ++n) d[n] = ((nx * P[n].x + ny * P[n].y) + nz * P[n].z) - dist;
+        const int cs = d2case_snap(d);
+        cut_dispatch<false>(cs, P, d, [&](double v) { VF += (ts * v) / cellvol; }, [](double) {});
+    }
+    return VF;
+}
+
+// computePLIC2VF of a cell lying entirely on the liquid side: case 1 in every tet, so each
+// whole tet is one liquid sub-tet.
+__device__ __forceinline__ double plic2vf_full(double x0, double x1, double y0, double y1, double z0, double z1,
+                                               bool even, double cellvol) {
+    const double ts = even ? 1.0 : -1.0;
+    double VF = 0.0;
+    for (int t = 0; t < 5; ++t) {
+        V3 P[4];
+        tet5_vertices(t, even, x0, x1, y0, y1, z0, z1, P);
+        VF += (ts * tet_vol3(P[0], P[1], P[2], P[3])) / cellvol;
+    }
+    return VF;
+}
+
 // pt2index! (src/Tools.jl:424-434). The walk is bounded by the coordinate arrays; a walk
 // that would leave them sets *ovf and stops at the last valid cell.
 __device__ __forceinline__ void pt2index(const DevMesh& m, int I[3], const double pt[3], int i, int j, int k,

@@ -429,6 +429,131 @@ __global__ __launch_bounds__(64) void k_elvira(DevMesh m, DevParams p, const dou
     }
 }
 
+// ELVIRA with one interface cell per work-group of 256 threads. Threads 0..26 load the
+// 3x3x3 liquid fractions and build the 27 candidate normals and their PLIC constants once
+// (computeDist); then half-wave h scores every candidate on neighbours h, h+8, h+16, h+24,
+// writing (VF - VF_)^2 to LDS; the first half-wave sums each candidate's 27 terms in the
+// reference's neighbour order and takes the lowest-numbered minimum. Neighbours that every
+// candidate plane misses by a wide margin take the exact all-liquid / all-gas value.
+constexpr int ELV_THREADS = 256;
+
+__global__ __launch_bounds__(ELV_THREADS) void k_elvira2(DevMesh m, DevParams p, const double* VF, const int* list,
+                                                         const int* count, int fam_mask, double* nxo, double* nyo,
+                                                         double* nzo) {
+    __shared__ double sVF[27];
+    __shared__ double sFull[27];
+    __shared__ double sC[27][4];
+    __shared__ int sOk[27];
+    __shared__ double sE[27][28];
+    const int tid = threadIdx.x;
+    const int half = tid >> 5, lane = tid & 31;
+    const int ncell = *count;
+    const double cellvol = (m.dx * m.dy) * m.dz;
+    for (int cidx = blockIdx.x; cidx < ncell; cidx += gridDim.x) {
+        int i, j, k;
+        interior_of(m, list[cidx], i, j, k);
+        if (tid < 27) {
+            const int ii = i - 1 + tid % 3, jj = j - 1 + (tid / 3) % 3, kk = k - 1 + tid / 9;
+            sVF[tid] = VF[m.vof(ii, jj, kk)];
+            sFull[tid] = plic2vf_full(m.x[ii], m.x[ii + 1], m.y[jj], m.y[jj + 1], m.z[kk], m.z[kk + 1],
+                                      ((ii + jj + kk) % 2) == 0, cellvol);
+        }
+        __syncthreads();
+        if (tid < 27) {
+            const int fam = tid / 9, a = (tid % 9) / 3, b = tid % 3;
+            int ok = (fam_mask >> fam) & 1;
+            if (ok) {
+                auto at = [&](int di, int dj, int dk) { return sVF[(di + 1) + 3 * (dj + 1) + 9 * (dk + 1)]; };
+                // stencil pairs [[-1,0],[-1,1],[0,1]] of src/ELVIRA.jl:72
+                auto P0 = [](int q) { return q == 2 ? 0 : -1; };
+                auto P1 = [](int q) { return q == 0 ? 0 : 1; };
+                double cnx, cny, cnz;
+                if (fam == 0) {  // X-Y stencils (src/ELVIRA.jl:71-78)
+                    const int i1 = P0(b), i2 = P1(b), j1 = P0(a), j2 = P1(a);
+                    const double dz = m.dz;
+                    const double sx2 = (at(i2, 0, -1) * dz + at(i2, 0, 0) * dz) + at(i2, 0, 1) * dz;
+                    const double sx1 = (at(i1, 0, -1) * dz + at(i1, 0, 0) * dz) + at(i1, 0, 1) * dz;
+                    cnx = -(sx2 - sx1) / (m.xm[i + i2] - m.xm[i + i1]);
+                    const double sy2 = (at(0, j2, -1) * dz + at(0, j2, 0) * dz) + at(0, j2, 1) * dz;
+                    const double sy1 = (at(0, j1, -1) * dz + at(0, j1, 0) * dz) + at(0, j1, 1) * dz;
+                    cny = -(sy2 - sy1) / (m.ym[j + j2] - m.ym[j + j1]);
+                    cnz = at(0, 0, 1) > at(0, 0, -1) ? -1.0 : 1.0;
+                } else if (fam == 1) {  // X-Z stencils (src/ELVIRA.jl:81-88)
+                    const int i1 = P0(b), i2 = P1(b), k1 = P0(a), k2 = P1(a);
+                    const double dy = m.dy;
+                    const double sx2 = (at(i2, -1, 0) * dy + at(i2, 0, 0) * dy) + at(i2, 1, 0) * dy;
+                    const double sx1 = (at(i1, -1, 0) * dy + at(i1, 0, 0) * dy) + at(i1, 1, 0) * dy;
+                    cnx = -(sx2 - sx1) / (m.xm[i + i2] - m.xm[i + i1]);
+                    cny = at(0, 1, 0) > at(0, -1, 0) ? -1.0 : 1.0;
+                    const double sz2 = (at(0, -1, k2) * dy + at(0, 0, k2) * dy) + at(0, 1, k2) * dy;
+                    const double sz1 = (at(0, -1, k1) * dy + at(0, 0, k1) * dy) + at(0, 1, k1) * dy;
+                    cnz = -(sz2 - sz1) / (m.zm[k + k2] - m.zm[k + k1]);
+                } else {  // Y-Z stencils (src/ELVIRA.jl:91-98)
+                    const int j1 = P0(b), j2 = P1(b), k1 = P0(a), k2 = P1(a);
+                    const double dx = m.dx;
+                    cnx = at(1, 0, 0) > at(-1, 0, 0) ? -1.0 : 1.0;
+                    const double sy2 = (at(-1, j2, 0) * dx + at(0, j2, 0) * dx) + at(1, j2, 0) * dx;
+                    const double sy1 = (at(-1, j1, 0) * dx + at(0, j1, 0) * dx) + at(1, j1, 0) * dx;
+                    cny = -(sy2 - sy1) / (m.ym[j + j2] - m.ym[j + j1]);
+                    const double sz2 = (at(-1, 0, k2) * dx + at(0, 0, k2) * dx) + at(1, 0, k2) * dx;
+                    const double sz1 = (at(-1, 0, k1) * dx + at(0, 0, k1) * dx) + at(1, 0, k1) * dx;
+                    cnz = -(sz2 - sz1) / (m.zm[k + k2] - m.zm[k + k1]);
+                }
+                const double norm = sqrt((cnx * cnx + cny * cny) + cnz * cnz);
+                cnx = cnx / norm; cny = cny / norm; cnz = cnz / norm;
+                sC[tid][0] = cnx; sC[tid][1] = cny; sC[tid][2] = cnz;
+                sC[tid][3] = compute_dist(m, p, i, j, k, cnx, cny, cnz, at(0, 0, 0));
+            }
+            sOk[tid] = ok;
+        }
+        __syncthreads();
+        if (lane < 27 && sOk[lane]) {
+            const double cnx = sC[lane][0], cny = sC[lane][1], cnz = sC[lane][2], dist = sC[lane][3];
+            const double hx = fabs(cnx) * m.dx, hy = fabs(cny) * m.dy, hz = fabs(cnz) * m.dz;
+            for (int nb = half; nb < 27; nb += 8) {
+                const int ii = i - 1 + nb % 3, jj = j - 1 + (nb / 3) % 3, kk = k - 1 + nb / 9;
+                const double dcen = ((cnx * m.xm[ii] + cny * m.ym[jj]) + cnz * m.zm[kk]) - dist;
+                const double hext = 0.5 * ((hx + hy) + hz);
+                const double margin =
+                    1e-9 * (((1.0 + fabs(dist)) + fabs(m.xm[ii])) + (fabs(m.ym[jj]) + fabs(m.zm[kk])));
+                double vf_;
+                if (dcen - hext > margin) {
+                    vf_ = 0.0;
+                } else if (dcen + hext < -margin) {
+                    vf_ = sFull[nb];
+                } else {
+                    vf_ = plic2vf_reg(m.x[ii], m.x[ii + 1], m.y[jj], m.y[jj + 1], m.z[kk], m.z[kk + 1], cnx, cny,
+                                      cnz, dist, ((ii + jj + kk) % 2) == 0, cellvol);
+                }
+                const double df = sVF[nb] - vf_;
+                sE[lane][nb] = df * df;
+            }
+        }
+        __syncthreads();
+        if (tid < 32) {
+            double err = __builtin_huge_val();
+            if (lane < 27 && sOk[lane]) {
+                double e = 0.0;
+                for (int nb = 0; nb < 27; ++nb) e += sE[lane][nb];
+                err = e;
+            }
+            const bool ok = err < __builtin_huge_val();
+            double be = ok ? err : __builtin_huge_val();
+            int bl = ok ? lane : 64;
+            for (int off = 16; off > 0; off >>= 1) {
+                const double oe = __shfl_xor(be, off, 32);
+                const int ol = __shfl_xor(bl, off, 32);
+                if (ol < 64 && (bl == 64 || oe < be || (oe == be && ol < bl))) { be = oe; bl = ol; }
+            }
+            if (lane == 0 && bl < 64) {
+                const int64_t c = m.cell(i, j, k);
+                nxo[c] = sC[bl][0]; nyo[c] = sC[bl][1]; nzo[c] = sC[bl][2];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // computePLIC! (src/VF.jl:94-103) over every cell including ghosts
 __global__ void k_plic(DevMesh m, DevParams p, const double* VF, const double* nx, const double* ny,
                        const double* nz, double* D) {
@@ -696,6 +821,234 @@ __global__ __launch_bounds__(64) void k_sl_cut(DevMesh m, const double* pv, cons
 }
 
 // ------------------------------------------------------------------------------------
+// cutTet for tets whose vertices lie in at most two cells along each axis (all but a few
+// pre-image tets at CFL <= 1): then the reference recursion is at most one x cut, one y cut
+// and one z cut before the PLIC leaf, and every vertex index is lo or lo+1 per axis. The
+// three levels are nested loops over sub-tets with their vertices in registers; a vertex's
+// cell is three bits relative to lo. A cut vertex's index is the pt2index! walk clamped
+// between the indices of its edge ends (src/VFgeom.jl:853-862), which for two candidate
+// cells reduces to one comparison with the plane between them.
+// ------------------------------------------------------------------------------------
+struct FastTet {
+    V3 p[4];
+    unsigned bits;  // 3 bits per vertex (x, y, z offset from lo)
+};
+
+struct FastLevel {
+    V3 q[8];
+    unsigned bits1;  // side-1 bits of the 8 level vertices
+    int cs, nchild, axis;
+};
+
+__device__ __forceinline__ V3 sel8(const V3 (&q)[8], int v) {
+    V3 r = q[0];
+#pragma unroll
+    for (int k = 1; k < 8; ++k)
+        if (v == k) r = q[k];
+    return r;
+}
+__device__ __forceinline__ double comp(const V3& a, int axis) { return axis == 0 ? a.x : (axis == 1 ? a.y : a.z); }
+
+// True when the four vertices do not share their cell index along `axis`.
+__device__ __forceinline__ bool spans(unsigned bits, int axis) {
+    const unsigned b0 = (bits >> axis) & 1u, b1 = (bits >> (3 + axis)) & 1u, b2 = (bits >> (6 + axis)) & 1u,
+                   b3 = (bits >> (9 + axis)) & 1u;
+    return (b0 | b1 | b2 | b3) != (b0 & b1 & b2 & b3);
+}
+
+// Mesh-plane cut of node T along `axis` at the plane between lo and lo+1 (cut_ind = lo+1).
+__device__ __forceinline__ void fast_cut(const FastTet& T, int axis, const double planes[3], FastLevel& F) {
+    const double P = planes[axis];
+    double d[4], dd[4];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) { d[n] = comp(T.p[n], axis) - P; dd[n] = d[n]; }
+    const int cs = d2case_snap(dd);  // the reference snaps a copy
+    F.cs = cs;
+    F.axis = axis;
+    F.nchild = CUT_NTETS[cs];
+    // original vertices: side-1 index along the axis is min(ind, cut_ind - 1) = lo
+    unsigned b1 = T.bits & ~((1u << axis) | (1u << (3 + axis)) | (1u << (6 + axis)) | (1u << (9 + axis)));
+#pragma unroll
+    for (int n = 0; n < 4; ++n) F.q[n] = T.p[n];
+    const int nv = CUT_NVERT[cs];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+        if (n < nv) {
+            const unsigned e = CUT_EDGE[cs][n];
+            const int v1 = e & 15, v2 = e >> 4;
+            double d1 = d[0], d2 = d[0];
+#pragma unroll
+            for (int k = 1; k < 4; ++k) { if (v1 == k) d1 = d[k]; if (v2 == k) d2 = d[k]; }
+            V3 a = T.p[0], b = T.p[0];
+#pragma unroll
+            for (int k = 1; k < 4; ++k) { if (v1 == k) a = T.p[k]; if (v2 == k) b = T.p[k]; }
+            const double mu = jmin(1.0, jmax(0.0, sdiv(-d1, d2 - d1)));
+            const V3 c = lerp3(a, b, mu);
+            F.q[4 + n] = c;
+            // index of the cut vertex: walk from vertex n's side-1 index, clamp to [v1, v2]
+            const unsigned s = (b1 >> (3 * n)) & 7u;
+            const unsigned e1 = (b1 >> (3 * v1)) & 7u, e2 = (b1 >> (3 * v2)) & 7u;
+            unsigned nb = 0;
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+                const unsigned u1 = (e1 >> ax) & 1u, u2 = (e2 >> ax) & 1u;
+                unsigned bit;
+                if (ax == axis) bit = 0u;  // side 1: cut_ind - 1
+                else if (u1 == u2) bit = u1;
+                else {
+                    const double pc = comp(c, ax), pl = planes[ax];
+                    bit = ((s >> ax) & 1u) ? (pc < pl - EPS ? 0u : 1u) : (pc > pl + EPS ? 1u : 0u);
+                }
+                nb |= bit << ax;
+            }
+            b1 |= nb << (3 * (4 + n));
+        }
+    }
+    F.bits1 = b1;
+}
+
+__device__ __forceinline__ FastTet fast_child(const FastLevel& F, int n) {
+    FastTet T;
+    const unsigned q = CUT_VTET[F.cs][n];
+    const bool side2 = (CUT_SIDE2[F.cs] >> n) & 1;
+    unsigned bits = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int v = nib(q, k);
+        T.p[k] = sel8(F.q, v);
+        unsigned b = (F.bits1 >> (3 * v)) & 7u;
+        if (side2) b |= 1u << F.axis;
+        bits |= b << (3 * k);
+    }
+    T.bits = bits;
+    return T;
+}
+
+__device__ __forceinline__ void fast_leaf(const DevMesh& m, const double* nx, const double* ny, const double* nz,
+                                          const double* D, const FastTet& T, int lo0, int lo1, int lo2, double& vol,
+                                          double& liq) {
+    const unsigned b = T.bits & 7u;
+    const int64_t c = m.cell(lo0 + (b & 1u), lo1 + ((b >> 1) & 1u), lo2 + ((b >> 2) & 1u));
+    const double n0 = nx[c], n1 = ny[c], n2 = nz[c], Dc = D[c];
+    double d[4], dd[4];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+        d[n] = ((n0 * T.p[n].x + n1 * T.p[n].y) + n2 * T.p[n].z) - Dc;
+        dd[n] = d[n];
+    }
+    const int cs = d2case_snap(dd);
+    double V = 0.0, L = 0.0;
+    cut_dispatch<true>(cs, T.p, d, [&](double tv) { V += tv; L += tv; }, [&](double tv) { V += tv; });
+    vol = V;
+    liq = L;
+}
+
+// Volumes of the tet after the x, y, z and PLIC cuts, summed as the recursion nests them.
+__device__ void cut_tet_fast(const DevMesh& m, const double* nx, const double* ny, const double* nz, const double* D,
+                             const FastTet& root, int lo0, int lo1, int lo2, double& vol, double& liq) {
+    const double planes[3] = {m.x[lo0 + 1], m.y[lo1 + 1], m.z[lo2 + 1]};
+    FastLevel FX, FY, FZ;
+    const bool cx = spans(root.bits, 0);
+    if (cx) fast_cut(root, 0, planes, FX);
+    const int nxc = cx ? FX.nchild : 1;
+    double vX = 0.0, lX = 0.0;
+    for (int a = 0; a < nxc; ++a) {
+        const FastTet TY = cx ? fast_child(FX, a) : root;
+        const bool cy = spans(TY.bits, 1);
+        if (cy) fast_cut(TY, 1, planes, FY);
+        const int nyc = cy ? FY.nchild : 1;
+        double vY = 0.0, lY = 0.0;
+        for (int b = 0; b < nyc; ++b) {
+            const FastTet TZ = cy ? fast_child(FY, b) : TY;
+            const bool cz = spans(TZ.bits, 2);
+            if (cz) fast_cut(TZ, 2, planes, FZ);
+            const int nzc = cz ? FZ.nchild : 1;
+            double vZ = 0.0, lZ = 0.0;
+            for (int c = 0; c < nzc; ++c) {
+                const FastTet TL = cz ? fast_child(FZ, c) : TZ;
+                double v, l;
+                fast_leaf(m, nx, ny, nz, D, TL, lo0, lo1, lo2, v, l);
+                if (cz) { vZ += v; lZ += l; } else { vZ = v; lZ = l; }
+            }
+            if (cy) { vY += vZ; lY += lZ; } else { vY = vZ; lY = lZ; }
+        }
+        if (cx) { vX += vY; lX += lY; } else { vX = vY; lX = lY; }
+    }
+    vol = vX;
+    liq = lX;
+}
+
+// SL volume fraction of band cells: 12 cells per wave, one lane per (cell, tet). Cells with a
+// tet spanning more than two cells along an axis go to the general kernel (k_sl_cut) through
+// `defer`.
+__global__ __launch_bounds__(64) void k_sl_fast(DevMesh m, const double* pv, const int* list, const int* count,
+                                                const double* nx, const double* ny, const double* nz,
+                                                const double* D, double* VF, int* ovf, int* defer,
+                                                int* ndefer) {
+    __shared__ double s_vol[64], s_liq[64];
+    __shared__ int s_gen[64];
+    const int lane = threadIdx.x;
+    const int cl = lane / 5, t = lane - 5 * cl;
+    const int nb = *count;
+    for (int base = blockIdx.x * 12; base < nb; base += gridDim.x * 12) {
+        const int cidx = base + cl;
+        const bool active = lane < 60 && cidx < nb;
+        double vol = 0.0, liq = 0.0;
+        int general = 0;
+        int i = 1, j = 1, k = 1, q = 0;
+        if (active) {
+            q = list[cidx];
+            interior_of(m, q, i, j, k);
+            const bool even = ((i + j + k) % 2) == 0;
+            const unsigned tv = TET5[t];
+            FastTet T;
+            int ind[4][3];
+            int o = 0;
+            for (int n = 0; n < 4; ++n) {
+                const int c = vert_corner(nib(tv, n), even);
+                const int64_t vq = vtx_id(m, i + (c & 1), j + ((c >> 1) & 1), k + (c >> 2));
+                T.p[n].x = pv[3 * vq + 0];
+                T.p[n].y = pv[3 * vq + 1];
+                T.p[n].z = pv[3 * vq + 2];
+                const double pt[3] = {T.p[n].x, T.p[n].y, T.p[n].z};
+                pt2index(m, ind[n], pt, i, j, k, &o);
+            }
+            if (o) atomicOr(ovf, 1);
+            int lo[3];
+            unsigned bits = 0;
+            for (int a = 0; a < 3; ++a) {
+                const int mn = min(min(ind[0][a], ind[1][a]), min(ind[2][a], ind[3][a]));
+                const int mx = max(max(ind[0][a], ind[1][a]), max(ind[2][a], ind[3][a]));
+                lo[a] = mn;
+                if (mx - mn > 1) general = 1;
+            }
+            if (!general) {
+                for (int n = 0; n < 4; ++n)
+                    for (int a = 0; a < 3; ++a) bits |= (unsigned)(ind[n][a] - lo[a]) << (3 * n + a);
+                T.bits = bits;
+                cut_tet_fast(m, nx, ny, nz, D, T, lo[0], lo[1], lo[2], vol, liq);
+            }
+        }
+        s_vol[lane] = vol;
+        s_liq[lane] = liq;
+        s_gen[lane] = general;
+        __syncthreads();
+        if (active && t == 0) {
+            const int gen = s_gen[lane] | s_gen[lane + 1] | s_gen[lane + 2] | s_gen[lane + 3] | s_gen[lane + 4];
+            if (gen) {
+                defer[atomicAdd(ndefer, 1)] = q;
+            } else {
+                const double ts = ((i + j + k) % 2) == 0 ? 1.0 : -1.0;
+                double V = 0.0, L = 0.0;
+                for (int r = 0; r < 5; ++r) { V += ts * s_vol[lane + r]; L += ts * s_liq[lane + r]; }
+                VF[m.vof(i, j, k)] = L / V;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------------------
 // Initial conditions: VFcircle (src/Tools.jl:790-902), VFsphere / VFbubble3d (:907-922,
 // :1061-1076). For the sampled shapes the result depends only on the number of samples
 // inside, through a table of the reference's sequential sums; boxes far inside or outside
@@ -872,10 +1225,8 @@ __global__ void k_plic_vf_batch(DevMesh m, const int64_t* ijk, const double* nrm
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
         const int i = (int)ijk[3 * q], j = (int)ijk[3 * q + 1], k = (int)ijk[3 * q + 2];
         const double cnx = nrm[3 * q], cny = nrm[3 * q + 1], cnz = nrm[3 * q + 2];
-        const double cx[2] = {m.x[i], m.x[i + 1]}, cy[2] = {m.y[j], m.y[j + 1]}, cz[2] = {m.z[k], m.z[k + 1]};
-        double dc[8];
-        for (int c = 0; c < 8; ++c) dc[c] = ((cnx * cx[c & 1] + cny * cy[(c >> 1) & 1]) + cnz * cz[c >> 2]) - dist[q];
-        out[q] = plic2vf_corners(cx, cy, cz, dc, ((i + j + k) % 2) == 0, 0.0, vol);
+        out[q] = plic2vf_reg(m.x[i], m.x[i + 1], m.y[j], m.y[j + 1], m.z[k], m.z[k + 1], cnx, cny, cnz, dist[q],
+                             ((i + j + k) % 2) == 0, vol);
     }
 }
 
@@ -996,7 +1347,7 @@ int launch_normals(mist_ctx* ctx) {
     const int fam = (ctx->m.Nz > 1 ? 1 : 0) | (ctx->m.Ny > 1 ? 2 : 0) | (ctx->m.Nx > 1 ? 4 : 0);
     const int64_t ncap = (int64_t)ctx->m.Nx * ctx->m.Ny * ctx->m.Nz;
     hipEvent_t ev = prof_begin(ctx, MIST_PROF_ELVIRA);
-    LAUNCH(k_elvira, grid_for((ncap + 1) / 2, 64, 8192), 64, ctx->m, ctx->dp, (const double*)ctx->f[MIST_FIELD_VOF],
+    LAUNCH(k_elvira2, grid_for(ncap, 1, 16384), ELV_THREADS, ctx->m, ctx->dp, (const double*)ctx->f[MIST_FIELD_VOF],
            (const int*)ctx->iface_list, (const int*)(ctx->counters + 1), fam, nx, ny, nz);
     prof_end(ctx, MIST_PROF_ELVIRA, ev, ctx->counters + 1, 0);
     int st;
@@ -1024,10 +1375,17 @@ int launch_transport(mist_ctx* ctx, double dt) {
     prof_end(ctx, MIST_PROF_PROJECT, ev, nullptr, nv);
     const int64_t ncap = (int64_t)ctx->m.Nx * ctx->m.Ny * ctx->m.Nz;
     ev = prof_begin(ctx, MIST_PROF_SL_CUT);
-    LAUNCH(k_sl_cut, grid_for((ncap + 11) / 12, 64, 16384), 64, ctx->m, (const double*)ctx->pv,
+    LAUNCH(k_fill_int, 1, 64, ctx->counters + 4, 1, 0);
+    LAUNCH(k_sl_fast, grid_for((ncap + 11) / 12, 64, 16384), 64, ctx->m, (const double*)ctx->pv,
            (const int*)ctx->band_list, (const int*)(ctx->counters + 0), (const double*)ctx->f[MIST_FIELD_NX],
            (const double*)ctx->f[MIST_FIELD_NY], (const double*)ctx->f[MIST_FIELD_NZ],
-           (const double*)ctx->f[MIST_FIELD_PLIC_D], (double*)ctx->f[MIST_FIELD_VOF], ctx->counters + 2);
+           (const double*)ctx->f[MIST_FIELD_PLIC_D], (double*)ctx->f[MIST_FIELD_VOF], ctx->counters + 2,
+           ctx->iface_list, ctx->counters + 4);
+    // cells with a tet spanning three or more cells along an axis: general walk
+    LAUNCH(k_sl_cut, 256, 64, ctx->m, (const double*)ctx->pv, (const int*)ctx->iface_list,
+           (const int*)(ctx->counters + 4), (const double*)ctx->f[MIST_FIELD_NX], (const double*)ctx->f[MIST_FIELD_NY],
+           (const double*)ctx->f[MIST_FIELD_NZ], (const double*)ctx->f[MIST_FIELD_PLIC_D],
+           (double*)ctx->f[MIST_FIELD_VOF], ctx->counters + 2);
     prof_end(ctx, MIST_PROF_SL_CUT, ev, ctx->counters + 0, 0);
     int st;
     if ((st = launch_neumann(ctx, MIST_FIELD_VOF))) return st;
