@@ -189,7 +189,8 @@ def main():
                 "kernel": "k_elvira", "avg_launch_us": 1e6 * avg_s, "launches": e_n,
                 "interface_cells_per_launch": e_units / e_n, "flops_per_interface_cell": fl_cell,
                 "note": "FP64 vector peak of MI355X (78.6 TFLOP/s); no FMA by construction (-ffp-contract=off)"}
-    kernels = {k: {"ms": v[0], "launches": v[1], "avg_us": (1e3 * v[0] / v[1]) if v[1] else None}
+    kernels = {k: {"ms": v[0], "launches": v[1], "avg_us": (1e3 * v[0] / v[1]) if v[1] else None,
+                   "units_per_launch": v[2] / v[1] if v[1] else None}
                for k, v in prof.items() if v[1]}
     ctx.close()
 

@@ -51,7 +51,9 @@ enum { MIST_PROJ_RK4 = 0, MIST_PROJ_EULER = 1, MIST_PROJ_MIDPOINT = 2 };    /* p
 enum { MIST_TESS_5_TETS = 0 };                                              /* tesselation */
 enum { MIST_CASE_GENERIC = 0, MIST_CASE_ZALESAK = 1 };                      /* test_case */
 enum { MIST_MG_FP64 = 0, MIST_MG_MIXED = 1 };                               /* new: V-cycle precision */
-enum { MIST_COARSE_FACE_AVERAGE = 0, MIST_COARSE_RESTRICTED_VF = 1 };       /* new: coarse operator */
+/* new: coarse operator. MIST_COARSE_RESTRICTED_VF (coarse densities from restricted VF,
+ * src/Multigrid.jl:732) is reserved and rejected with MIST_ERR_PARAM: it is not built. */
+enum { MIST_COARSE_FACE_AVERAGE = 0, MIST_COARSE_RESTRICTED_VF = 1 };
 
 /*
  * POD mirror of the fields of `parameters` (src/Parameters.jl:3-91) that the hot path
@@ -84,7 +86,7 @@ typedef struct mist_params {
     int64_t mg_precision;   /* MIST_MG_FP64 or MIST_MG_MIXED */
     int64_t mg_nu1, mg_nu2; /* smoothing sweeps, reference 10 and 10 */
     int64_t mg_max_cycles;  /* outer cycle limit, reference 10000 */
-    int64_t mg_coarse_op;   /* MIST_COARSE_FACE_AVERAGE or MIST_COARSE_RESTRICTED_VF */
+    int64_t mg_coarse_op;   /* MIST_COARSE_FACE_AVERAGE (the only one built) */
     int64_t mg_coarse_sweeps; /* RB-GS sweeps on the coarsest level */
 } mist_params;
 
@@ -186,7 +188,8 @@ enum { MIST_PROF_ELVIRA = 0,     /* units: interface cells scored */
        MIST_PROF_PROJECT = 3,    /* units: cell vertices visited */
        MIST_PROF_MG_RBGS = 4,    /* units: cells relaxed (one colour sweep = half a level) */
        MIST_PROF_MG_RESID = 5,   /* units: cells */
-       MIST_PROF_COUNT = 6 };
+       MIST_PROF_SL_GENERAL = 6, /* units: band cells left to the general cut walk */
+       MIST_PROF_COUNT = 7 };
 int mist_profile_enable(mist_ctx* ctx, int on);   /* on != 0 also resets the totals */
 int mist_profile_read(mist_ctx* ctx, int kernel, double* total_ms, int64_t* launches, int64_t* units);
 

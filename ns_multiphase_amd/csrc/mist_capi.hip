@@ -54,6 +54,8 @@ int check_params(const mist_params* p, std::string& err) {
     if (p->normal_method != MIST_NORMAL_ELVIRA) { err = "Unknown method to copute interface normal"; return MIST_ERR_PARAM; }
     if (p->projection_method < 0 || p->projection_method > 2) { err = "Unknown projection_method in project!"; return MIST_ERR_PARAM; }
     if (!(p->CFL > 0)) { err = "CFL must be > 0"; return MIST_ERR_PARAM; }
+    if (p->mg_coarse_op != MIST_COARSE_FACE_AVERAGE) { err = "mg_coarse_op: only the face-average coarse operator is built"; return MIST_ERR_PARAM; }
+    if (p->mg_nu1 < 0 || p->mg_nu2 < 0 || p->mg_coarse_sweeps < 0) { err = "multigrid sweep counts must be >= 0"; return MIST_ERR_PARAM; }
     return MIST_OK;
 }
 
@@ -144,7 +146,7 @@ int mist_ctx_destroy(mist_ctx* ctx) {
     mist::mg_free(ctx->mg);
     for (int f = 0; f < MIST_FIELD_COUNT; ++f)
         if (ctx->f[f]) (void)hipFree(ctx->f[f]);
-    void* bufs[] = {ctx->geom, ctx->pv, ctx->band_list, ctx->iface_list, ctx->counters, ctx->red};
+    void* bufs[] = {ctx->geom, ctx->trig, ctx->pv, ctx->band_list, ctx->iface_list, ctx->counters, ctx->red};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (ctx->h_counters) (void)hipHostFree(ctx->h_counters);
@@ -215,6 +217,18 @@ int mist_ctx_create(const mist_params* pin, int device, mist_ctx** out) {
     m.xm = ctx->geom + offm - imino;
     m.ym = ctx->geom + offm + ctx->hxm.size() - imino;
     m.zm = ctx->geom + offm + ctx->hxm.size() + ctx->hym.size() - imino;
+    {
+        const int64_t G = (int64_t)g.size();
+        if (hipMalloc(&ctx->trig, 3 * G * sizeof(double)) != hipSuccess) return bail(MIST_ERR_OOM, "trig tables");
+        if (mist::launch_trig_tables(ctx, G)) return bail(MIST_ERR_HIP, "trig tables");
+        const size_t offa[6] = {off[0], off[1], off[2], offm, offm + ctx->hxm.size(),
+                                offm + ctx->hxm.size() + ctx->hym.size()};
+        for (int a = 0; a < 6; ++a) {
+            ctx->vt.s1[a] = ctx->trig + offa[a] - imino;
+            ctx->vt.s2[a] = ctx->trig + G + offa[a] - imino;
+            ctx->vt.c1[a] = ctx->trig + 2 * G + offa[a] - imino;
+        }
+    }
     mist::DevParams& dp = ctx->dp;
     dp.VFlo = p.VFlo; dp.VFhi = p.VFhi; dp.nband = (int)p.nband;
     dp.projection = (int)p.projection_method; dp.zalesak = p.test_case == MIST_CASE_ZALESAK;

@@ -35,6 +35,14 @@ struct DevMesh {
     __host__ __device__ inline int kmaxo() const { return Nz + ng; }
 };
 
+// Per-axis trig factors of the prescribed velocities, pre-offset like DevMesh's coordinate
+// pointers; index 0..2 = faces x, y, z, 3..5 = centres xm, ym, zm.
+struct VelTab {
+    const double* s1[6];  // sin(pi c)
+    const double* s2[6];  // sin(2 pi c)
+    const double* c1[6];  // cos(pi c)
+};
+
 struct DevParams {
     double VFlo, VFhi;
     int nband;
@@ -212,44 +220,6 @@ __device__ inline double compute_dist(const DevMesh& m, const DevParams& p, int 
 // order, odd cells mirror x. Returns the lexical corner index of reference vertex v.
 __device__ __forceinline__ int vert_corner(int v, bool even) { return even ? v : (v ^ 1); }
 
-// computePLIC2VF (src/VF.jl:219-278) given the 8 corner coordinates and corner distances
-// d = n.x - dist of a cell (lexical corner order). `even` selects the vertex numbering and
-// tetsign. d2case snaps the per-tet copy of d, and the snapped values are the ones
-// interpolated, as in the reference.
-__device__ inline double plic2vf_corners(const double cx[2], const double cy[2], const double cz[2],
-                                         const double dc[8], bool even, double inv_vol_unused,
-                                         double vol) {
-    const double tetsign = even ? 1.0 : -1.0;
-    double VF = 0.0;
-#pragma unroll 1
-    for (int t = 0; t < 5; ++t) {
-        const unsigned tv = TET5[t];
-        double vert[8][3];
-        double d[4];
-#pragma unroll
-        for (int n = 0; n < 4; ++n) {
-            const int c = vert_corner(nib(tv, n), even);
-            vert[n][0] = cx[c & 1]; vert[n][1] = cy[(c >> 1) & 1]; vert[n][2] = cz[c >> 2];
-            d[n] = dc[c];
-        }
-        const int cs = d2case_snap(d);
-        const int nv = CUT_NVERT[cs];
-        for (int n = 0; n < nv; ++n) {
-            const unsigned e = CUT_EDGE[cs][n];
-            const int v1 = e & 15, v2 = e >> 4;
-            const double mu = jmin(1.0, jmax(0.0, sdiv(-d[v1], d[v2] - d[v1])));
-            vert[4 + n][0] = (1.0 - mu) * vert[v1][0] + mu * vert[v2][0];
-            vert[4 + n][1] = (1.0 - mu) * vert[v1][1] + mu * vert[v2][1];
-            vert[4 + n][2] = (1.0 - mu) * vert[v1][2] + mu * vert[v2][2];
-        }
-        const int hi = CUT_NTETS[cs], lo = CUT_NNTET[cs];
-        for (int n = hi; n >= lo; --n) {
-            const unsigned q = CUT_VTET[cs][n - 1];
-            VF += (tetsign * tet_vol(vert[nib(q, 0)], vert[nib(q, 1)], vert[nib(q, 2)], vert[nib(q, 3)])) / vol;
-        }
-    }
-    return VF;
-}
 
 // ---- register-resident tet cutting -------------------------------------------------
 // The same tables as compile-time constants, so that a case-specialised cut indexes its

@@ -48,6 +48,8 @@ struct mist_ctx {
     mist::DevParams dp{};
     std::vector<double> hx, hy, hz, hxm, hym, hzm;  // host copies, index q = reference i - imino
     double* geom = nullptr;
+    double* trig = nullptr;      // sin(pi c), sin(2 pi c), cos(pi c) of the coordinates
+    mist::VelTab vt{};
     void* f[MIST_FIELD_COUNT] = {};
     int64_t flo[MIST_FIELD_COUNT][3] = {};
     int64_t fn[MIST_FIELD_COUNT][3] = {};
@@ -56,10 +58,12 @@ struct mist_ctx {
     double* pv = nullptr;        // projected cell vertices, (Nx+1)(Ny+1)(Nz+1) x 3
     int* band_list = nullptr;    // interior cells with |band| <= nband (linear cell index)
     int* iface_list = nullptr;   // interior interface cells (ELVIRA)
-    int* counters = nullptr;     // [0] band cells, [1] interface cells, [2] overflow flag, [3] max cut level
+    int* counters = nullptr;     // [0] band cells, [1] interface cells, [2] overflow flag, [4] deferred
+                                 // cells, [5] work-queue position of the cut kernel
     double* red = nullptr;       // reduction partials
     int red_blocks = 0;
     int* h_counters = nullptr;   // pinned
+    int tree_blocks = 0;         // resident blocks of the cut kernel on the device
     double* h_red = nullptr;     // pinned
     uint64_t vf_version = 1, normals_version = 0, band_version = 0;
     bool band_list_valid = false;
@@ -76,6 +80,7 @@ namespace mist {
 hipEvent_t prof_begin(mist_ctx* ctx, int kernel);
 void prof_end(mist_ctx* ctx, int kernel, hipEvent_t a, const int* device_units, int64_t host_units);
 // mist_vof.hip
+int launch_trig_tables(mist_ctx* ctx, int64_t n);
 int launch_define_velocity(mist_ctx* ctx, double t);
 int launch_maxabs_uvw(mist_ctx* ctx, double out[3]);
 int launch_neumann(mist_ctx* ctx, int field);

@@ -47,11 +47,20 @@ struct MGState {
     double* h_part = nullptr;  // pinned
     int nblocks = 0;
     int64_t nfine = 0;
+    int first_small[2] = {0, 0};              // first level solved in one block (FP64, FP32)
+    size_t small_lds[2] = {0, 0};             // its LDS bytes
+    void* small_desc[2] = {nullptr, nullptr};  // SmallLevels<T> in device memory
+    hipGraph_t graph[2] = {nullptr, nullptr};  // one defect-correction iteration
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
     std::vector<void*> allocs;
 };
 
 void mg_free(MGState* s) {
     if (!s) return;
+    for (int q = 0; q < 2; ++q) {
+        if (s->exec[q]) (void)hipGraphExecDestroy(s->exec[q]);
+        if (s->graph[q]) (void)hipGraphDestroy(s->graph[q]);
+    }
     for (void* p : s->allocs) (void)hipFree(p);
     if (s->h_part) (void)hipHostFree(s->h_part);
     delete s;
@@ -280,42 +289,172 @@ __global__ __launch_bounds__(MG_RED) void k_sum_final(const double* part, int n,
     if (threadIdx.x == 0) *out = r;
 }
 
-// Coarsest level: remove the mean of b, zero e, then `sweeps` red-black sweeps, all in one
-// work-group (the level has at most a few hundred cells).
+// ---- the coarse end of the V-cycle in one work-group --------------------------------------
+// Levels from `first` down to the coarsest run in one block of MG_SMALL_THREADS threads with
+// e, b and r of every such level in LDS (coefficients stay in global memory): the same
+// smoothing, residual, restriction and prolongation as the per-level kernels, separated by
+// block barriers instead of launches. The coarsest level removes the mean of b, zeroes e and
+// runs `coarse_sweeps` red-black sweeps.
+constexpr int MG_SMALL_THREADS = 1024;
+constexpr int MG_SMALL_MAXLEV = 12;
+constexpr size_t MG_SMALL_LDS = 120 * 1024;
+
 template <class T>
-__global__ __launch_bounds__(256) void k_coarse_solve(LevelGeom g, T* cx, T* cy, T* cz, T* e, T* b, int sweeps) {
-    __shared__ double s[256];
-    T* C[3] = {cx, cy, cz};
-    const int64_t tot = g.size();
-    double v = 0.0;
-    for (int64_t q = threadIdx.x; q < tot; q += blockDim.x) v += (double)b[q];
-    s[threadIdx.x] = v;
+struct SmallLevels {
+    int count;  // number of levels handled in the block
+    int nu1, nu2, coarse_sweeps;
+    LevelGeom g[MG_SMALL_MAXLEV];
+    T* c[MG_SMALL_MAXLEV][3];
+    int off[MG_SMALL_MAXLEV];  // LDS element offset of level l: e, then b, then r
+    const T* b_top;            // b of the first level (global, from the restriction)
+    T* e_top;                  // e of the first level (global, for the prolongation)
+};
+
+template <class T>
+__device__ __forceinline__ void blk_rbgs(const LevelGeom& g, T* const* C, T* e, const T* b, int color) {
+    const int hx = (g.n[0] + 1) / 2;
+    const int tot = hx * g.n[1] * g.n[2];
+    for (int q = threadIdx.x; q < tot; q += MG_SMALL_THREADS) {
+        const int h = q % hx, row = q / hx;
+        const int j = row % g.n[1], k = row / g.n[1];
+        const int i = 2 * h + ((color + 1 + j + k) & 1);
+        if (i >= g.n[0]) continue;
+        const int64_t cell = i + (int64_t)g.n[0] * (j + (int64_t)g.n[1] * k);
+        int64_t nb[6];
+        T cf[6];
+        stencil(g, C, i, j, k, cell, nb, cf);
+        const T S = ((((cf[0] + cf[1]) + cf[2]) + cf[3]) + cf[4]) + cf[5];
+        if (!(S > T(0))) continue;
+        const T sum = ((((cf[0] * e[nb[0]] + cf[1] * e[nb[1]]) + cf[2] * e[nb[2]]) + cf[3] * e[nb[3]]) +
+                       cf[4] * e[nb[4]]) + cf[5] * e[nb[5]];
+        e[cell] = (sum - b[cell]) / S;
+    }
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+}
+
+template <class T>
+__device__ __forceinline__ void blk_smooth(const LevelGeom& g, T* const* C, T* e, const T* b, int sweeps) {
+    for (int sw = 0; sw < sweeps; ++sw) {
+        blk_rbgs(g, C, e, b, 0);
+        blk_rbgs(g, C, e, b, 1);
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(MG_SMALL_THREADS) void k_small_vcycle(const SmallLevels<T>* P) {
+    extern __shared__ double lds_raw[];
+    T* lds = reinterpret_cast<T*>(lds_raw);
+    const int tid = threadIdx.x;
+    const int nl = P->count;
+    {
+        const int n = (int)P->g[0].size();
+        T* e = lds + P->off[0];
+        T* b = e + n;
+        for (int q = tid; q < n; q += MG_SMALL_THREADS) {
+            b[q] = P->b_top[q];
+            e[q] = T(0);
+        }
         __syncthreads();
     }
-    const T mean = (T)(s[0] / (double)tot);
-    for (int64_t q = threadIdx.x; q < tot; q += blockDim.x) { b[q] -= mean; e[q] = T(0); }
-    __syncthreads();
-    for (int sw = 0; sw < sweeps; ++sw)
-        for (int color = 0; color < 2; ++color) {
-            for (int64_t cell = threadIdx.x; cell < tot; cell += blockDim.x) {
-                const int i = (int)(cell % g.n[0]);
-                const int j = (int)((cell / g.n[0]) % g.n[1]);
-                const int k = (int)(cell / ((int64_t)g.n[0] * g.n[1]));
-                if (((i + 1) + (j + 1) + (k + 1)) % 2 != color) continue;
-                int64_t nb[6];
-                T cf[6];
-                stencil(g, C, i, j, k, cell, nb, cf);
-                const T S = ((((cf[0] + cf[1]) + cf[2]) + cf[3]) + cf[4]) + cf[5];
-                if (!(S > T(0))) continue;
-                const T sum = ((((cf[0] * e[nb[0]] + cf[1] * e[nb[1]]) + cf[2] * e[nb[2]]) + cf[3] * e[nb[3]]) +
-                               cf[4] * e[nb[4]]) + cf[5] * e[nb[5]];
-                e[cell] = (sum - b[cell]) / S;
-            }
-            __syncthreads();
+    // down: smooth, residual, restrict
+    for (int l = 0; l + 1 < nl; ++l) {
+        const LevelGeom g = P->g[l];
+        const int n = (int)g.size();
+        T* C[3] = {P->c[l][0], P->c[l][1], P->c[l][2]};
+        T* e = lds + P->off[l];
+        T* b = e + n;
+        T* r = b + n;
+        blk_smooth(g, C, e, b, P->nu1);
+        for (int cell = tid; cell < n; cell += MG_SMALL_THREADS) {
+            const int i = cell % g.n[0], j = (cell / g.n[0]) % g.n[1], k = cell / (g.n[0] * g.n[1]);
+            int64_t nb[6];
+            T cf[6];
+            stencil(g, C, i, j, k, cell, nb, cf);
+            const T pc = e[cell];
+            const T Ap = ((((cf[0] * (e[nb[0]] - pc) + cf[1] * (e[nb[1]] - pc)) + cf[2] * (e[nb[2]] - pc)) +
+                           cf[3] * (e[nb[3]] - pc)) + cf[4] * (e[nb[4]] - pc)) + cf[5] * (e[nb[5]] - pc);
+            r[cell] = b[cell] - Ap;
         }
+        __syncthreads();
+        const LevelGeom gc = P->g[l + 1];
+        const int nc = (int)gc.size();
+        T* ec = lds + P->off[l + 1];
+        T* bc = ec + nc;
+        const int f0 = gc.coarsened[0] ? 2 : 1, f1 = gc.coarsened[1] ? 2 : 1, f2 = gc.coarsened[2] ? 2 : 1;
+        const T inv = T(1) / T(f0 * f1 * f2);
+        for (int q = tid; q < nc; q += MG_SMALL_THREADS) {
+            const int I = q % gc.n[0], J = (q / gc.n[0]) % gc.n[1], K = q / (gc.n[0] * gc.n[1]);
+            T sum = T(0);
+            for (int c = 0; c < f2; ++c)
+                for (int bb = 0; bb < f1; ++bb)
+                    for (int a = 0; a < f0; ++a)
+                        sum += r[(I * f0 + a) + g.n[0] * ((J * f1 + bb) + g.n[1] * (K * f2 + c))];
+            bc[q] = sum * inv;
+            ec[q] = T(0);
+        }
+        __syncthreads();
+    }
+    // coarsest: consistent right-hand side, then sweeps from zero
+    {
+        const int l = nl - 1;
+        const LevelGeom g = P->g[l];
+        const int n = (int)g.size();
+        T* C[3] = {P->c[l][0], P->c[l][1], P->c[l][2]};
+        T* e = lds + P->off[l];
+        T* b = e + n;
+        __shared__ double red[MG_SMALL_THREADS / 64];
+        double v = 0.0;
+        for (int q = tid; q < n; q += MG_SMALL_THREADS) v += (double)b[q];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+        if ((tid & 63) == 0) red[tid >> 6] = v;
+        __syncthreads();
+        double tot = 0.0;
+        for (int w = 0; w < MG_SMALL_THREADS / 64; ++w) tot += red[w];
+        const T mean = (T)(tot / (double)n);
+        for (int q = tid; q < n; q += MG_SMALL_THREADS) {
+            b[q] -= mean;
+            e[q] = T(0);
+        }
+        __syncthreads();
+        blk_smooth(g, C, e, (const T*)b, P->coarse_sweeps);
+    }
+    // up: prolongate and smooth
+    for (int l = nl - 2; l >= 0; --l) {
+        const LevelGeom g = P->g[l], gc = P->g[l + 1];
+        const int n = (int)g.size();
+        T* C[3] = {P->c[l][0], P->c[l][1], P->c[l][2]};
+        T* e = lds + P->off[l];
+        T* b = e + n;
+        const T* ec = lds + P->off[l + 1];
+        for (int q = tid; q < n; q += MG_SMALL_THREADS) {
+            const int qq[3] = {q % g.n[0], (q / g.n[0]) % g.n[1], q / (g.n[0] * g.n[1])};
+            int lo[3], hi[3];
+            T w1[3], w2[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                if (!gc.coarsened[d]) { lo[d] = hi[d] = qq[d]; w1[d] = T(1); w2[d] = T(0); continue; }
+                const int Q = qq[d] / 2;
+                if ((qq[d] & 1) == 0) { lo[d] = Q - 1; hi[d] = Q; w1[d] = T(0.75); w2[d] = T(0.25); }
+                else { lo[d] = Q; hi[d] = Q + 1; w1[d] = T(0.25); w2[d] = T(0.75); }
+                if (lo[d] < 0) lo[d] = gc.per[d] ? gc.n[d] - 1 : 0;
+                if (hi[d] >= gc.n[d]) hi[d] = gc.per[d] ? 0 : gc.n[d] - 1;
+            }
+            auto E = [&](int a, int bb, int c) { return ec[a + gc.n[0] * (bb + gc.n[1] * c)]; };
+            const T v = w1[2] * (w1[1] * (w1[0] * E(hi[0], hi[1], hi[2]) + w2[0] * E(lo[0], hi[1], hi[2])) +
+                                 w2[1] * (w1[0] * E(hi[0], lo[1], hi[2]) + w2[0] * E(lo[0], lo[1], hi[2]))) +
+                        w2[2] * (w1[1] * (w1[0] * E(hi[0], hi[1], lo[2]) + w2[0] * E(lo[0], hi[1], lo[2])) +
+                                 w2[1] * (w1[0] * E(hi[0], lo[1], lo[2]) + w2[0] * E(lo[0], lo[1], lo[2])));
+            e[q] += v;
+        }
+        __syncthreads();
+        blk_smooth(g, C, e, (const T*)b, P->nu2);
+    }
+    {
+        const int n = (int)P->g[0].size();
+        const T* e = lds + P->off[0];
+        for (int q = tid; q < n; q += MG_SMALL_THREADS) P->e_top[q] = e[q];
+    }
 }
 
 __global__ void k_store_p(DevMesh m, LevelGeom g, const double* p, double* P) {
@@ -351,6 +490,44 @@ static int alloc_level(mist_ctx* ctx, MGState* s, const LevelGeom& g, LevelBuf<T
     return MIST_OK;
 }
 
+// The coarse end of the hierarchy that fits in MG_SMALL_LDS bytes of LDS (e, b, r per level)
+// goes to k_small_vcycle; its descriptor is copied to device memory once.
+template <class T>
+static int build_small(mist_ctx* ctx, MGState* s, std::vector<LevelBuf<T>>& L, int q) {
+    const int nlev = (int)s->g.size();
+    int first = nlev - 1;
+    size_t bytes = 3 * sizeof(T) * (size_t)s->g[first].size();
+    while (first > 0 && nlev - first < MG_SMALL_MAXLEV &&
+           bytes + 3 * sizeof(T) * (size_t)s->g[first - 1].size() <= MG_SMALL_LDS) {
+        --first;
+        bytes += 3 * sizeof(T) * (size_t)s->g[first].size();
+    }
+    if (bytes > MG_SMALL_LDS) { ctx->err = "multigrid: coarsest level too large for one work-group (raise mg_lvl)"; return MIST_ERR_PARAM; }
+    SmallLevels<T> h{};
+    h.count = nlev - first;
+    h.nu1 = (int)ctx->p.mg_nu1;
+    h.nu2 = (int)ctx->p.mg_nu2;
+    h.coarse_sweeps = (int)ctx->p.mg_coarse_sweeps;
+    int off = 0;
+    for (int l = first; l < nlev; ++l) {
+        h.g[l - first] = s->g[l];
+        for (int d = 0; d < 3; ++d) h.c[l - first][d] = L[l].c[d];
+        h.off[l - first] = off;
+        off += 3 * (int)s->g[l].size();
+    }
+    h.b_top = L[first].b;
+    h.e_top = L[first].e;
+    void* dsc = nullptr;
+    if (hipMalloc(&dsc, sizeof(h)) != hipSuccess) { ctx->err = "multigrid: device allocation failed"; return MIST_ERR_OOM; }
+    s->allocs.push_back(dsc);
+    MIST_HIP_TRY(hipMemcpy(dsc, &h, sizeof(h), hipMemcpyHostToDevice));
+    MIST_HIP_TRY(hipFuncSetAttribute((const void*)k_small_vcycle<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    s->first_small[q] = first;
+    s->small_lds[q] = bytes;
+    s->small_desc[q] = dsc;
+    return MIST_OK;
+}
+
 static int build_state(mist_ctx* ctx) {
     if (ctx->mg) return MIST_OK;
     const mist_params& p = ctx->p;
@@ -383,13 +560,16 @@ static int build_state(mist_ctx* ctx) {
         }
         s->g.push_back(c);
     }
-    if (s->g.back().size() > 4096) { mg_free(s); ctx->err = "multigrid: coarsest level too large (raise mg_lvl)"; return MIST_ERR_PARAM; }
     s->d.resize(s->g.size());
     s->s.resize(s->g.size());
     int st;
     for (size_t l = 0; l < s->g.size(); ++l) {
         if ((st = alloc_level(ctx, s, s->g[l], s->d[l]))) { mg_free(s); return st; }
         if ((st = alloc_level(ctx, s, s->g[l], s->s[l]))) { mg_free(s); return st; }
+    }
+    if ((st = build_small<double>(ctx, s, s->d, 0)) || (st = build_small<float>(ctx, s, s->s, 1))) {
+        mg_free(s);
+        return st;
     }
     s->nfine = g0.size();
     const size_t fb = (size_t)s->nfine * sizeof(double);
@@ -424,8 +604,11 @@ static int vcycle(mist_ctx* ctx, std::vector<LevelBuf<T>>& L, size_t l) {
     const int nu1 = (int)ctx->p.mg_nu1, nu2 = (int)ctx->p.mg_nu2;
     const int64_t n = g.size();
     const int64_t nh = (int64_t)((g.n[0] + 1) / 2) * g.n[1] * g.n[2];
-    if (l + 1 == L.size()) {
-        MGL(k_coarse_solve<T>, 1, 256, g, B.c[0], B.c[1], B.c[2], B.e, B.b, (int)ctx->p.mg_coarse_sweeps);
+    const int q = sizeof(T) == sizeof(double) ? 0 : 1;
+    if ((int)l == s->first_small[q]) {
+        hipLaunchKernelGGL(k_small_vcycle<T>, dim3(1), dim3(MG_SMALL_THREADS), s->small_lds[q], ctx->stream,
+                           (const SmallLevels<T>*)s->small_desc[q]);
+        MIST_HIP_TRY(hipGetLastError());
         return MIST_OK;
     }
     if (l > 0) MGL(k_zero<T>, gridn(n), 256, B.e, n);
@@ -460,31 +643,66 @@ static int setup_hierarchy(mist_ctx* ctx, std::vector<LevelBuf<T>>& L) {
     return MIST_OK;
 }
 
+// FP64 residual r = b - A p of the outer loop and max|r| into s->part[nblocks]
+static int outer_residual(mist_ctx* ctx) {
+    MGState* s = ctx->mg;
+    const int64_t n = s->nfine;
+    hipEvent_t ev = prof_begin(ctx, MIST_PROF_MG_RESID);
+    MGL(k_resid<double>, gridn(n), 256, s->g[0], s->c64[0], s->c64[1], s->c64[2], (const double*)s->p64,
+        (const double*)s->b64, s->r64);
+    prof_end(ctx, MIST_PROF_MG_RESID, ev, nullptr, n);
+    return dev_reduce<double>(ctx, s->r64, n, 1);
+}
+
+// One defect-correction iteration: e = V-cycle(r) in precision T, p += e, new residual.
+template <class T>
+static int outer_iteration(mist_ctx* ctx, std::vector<LevelBuf<T>>& L) {
+    MGState* s = ctx->mg;
+    const int64_t n = s->nfine;
+    MGL(k_start_cycle<T>, gridn(n), 256, (const double*)s->r64, L[0].b, L[0].e, n);
+    int st = vcycle(ctx, L, 0);
+    if (st) return st;
+    MGL(k_add_correction<T>, gridn(n), 256, (const T*)L[0].e, s->p64, n);
+    return outer_residual(ctx);
+}
+
 template <class T>
 static int solve_t(mist_ctx* ctx, std::vector<LevelBuf<T>>& L, double tol, int64_t* cycles, double* resid) {
     MGState* s = ctx->mg;
     int st = setup_hierarchy(ctx, L);
     if (st) return st;
-    const LevelGeom& g0 = s->g[0];
-    const int64_t n = s->nfine;
+    const int q = sizeof(T) == sizeof(double) ? 0 : 1;
     const int64_t maxc = ctx->p.mg_max_cycles > 0 ? ctx->p.mg_max_cycles : 10000;
     int64_t cyc = 0;
     double res = 0.0;
     int status = MIST_OK;
+    if ((st = outer_residual(ctx))) return st;
     for (;;) {
-        hipEvent_t ev = prof_begin(ctx, MIST_PROF_MG_RESID);
-        MGL(k_resid<double>, gridn(n), 256, g0, s->c64[0], s->c64[1], s->c64[2], (const double*)s->p64,
-            (const double*)s->b64, s->r64);
-        prof_end(ctx, MIST_PROF_MG_RESID, ev, nullptr, n);
-        if ((st = dev_reduce<double>(ctx, s->r64, n, 1))) return st;
         MIST_HIP_TRY(hipMemcpyAsync(s->h_part, s->part + s->nblocks, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         MIST_HIP_TRY(hipStreamSynchronize(ctx->stream));
         res = s->h_part[0];
         if (!(res > tol)) break;  // also stops on NaN, reported through the residual
         if (cyc >= maxc) { status = MIST_ERR_NOCONV; break; }
-        MGL(k_start_cycle<T>, gridn(n), 256, (const double*)s->r64, L[0].b, L[0].e, n);
-        if ((st = vcycle(ctx, L, 0))) return st;
-        MGL(k_add_correction<T>, gridn(n), 256, (const T*)L[0].e, s->p64, n);
+        if (ctx->prof.on) {
+            // per-kernel timing needs the kernels launched one by one
+            if ((st = outer_iteration(ctx, L))) return st;
+        } else {
+            // the ~200 launches of an iteration replayed as one graph (captured once)
+            if (!s->exec[q]) {
+                MIST_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+                st = outer_iteration(ctx, L);
+                hipGraph_t g = nullptr;
+                const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+                if (st) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return st;
+                }
+                MIST_HIP_TRY(ec);
+                s->graph[q] = g;
+                MIST_HIP_TRY(hipGraphInstantiate(&s->exec[q], g, nullptr, nullptr, 0));
+            }
+            MIST_HIP_TRY(hipGraphLaunch(s->exec[q], ctx->stream));
+        }
         ++cyc;
     }
     if (res != res) status = MIST_ERR_NOCONV;

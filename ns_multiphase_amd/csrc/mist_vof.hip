@@ -73,20 +73,53 @@ __device__ __forceinline__ void vel_fun(int kind, double T, double x, double y, 
     }
 }
 
-__global__ void k_define_velocity(DevMesh m, int kind, double T, double t, double* u, double* v, double* w,
+// sin(pi c), sin(2 pi c) and cos(pi c) of every coordinate entry, computed once per context:
+// the prescribed fields are products of these per-axis factors, so a step multiplies table
+// entries in the reference's order instead of evaluating ~24 FP64 trig calls per cell.
+__global__ void k_trig_tables(const double* g, int64_t n, double* t) {
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        const double c = g[q];
+        t[q] = sin(PI * c);
+        t[n + q] = sin((2.0 * PI) * c);
+        t[2 * n + q] = cos(PI * c);
+    }
+}
+
+// vel_fun at (x, y, z) = (faces or centres per axis: CX, CY, CZ = 1 for centres) of cell (i, j, k)
+template <int CX, int CY, int CZ>
+__device__ __forceinline__ void vel_at(int kind, double T, double t, double ct, const DevMesh& m, const VelTab& V,
+                                       int i, int j, int k, double& u, double& v, double& w) {
+    constexpr int X = CX ? 3 : 0, Y = CY ? 4 : 1, Z = CZ ? 5 : 2;
+    if (kind == MIST_VEL_DEFORMATION) {
+        const double sx = V.s1[X][i], sy = V.s1[Y][j];
+        u = ((-2.0 * (sx * sx) * V.s1[Y][j]) * V.c1[Y][j]) * ct;
+        v = ((2.0 * (sy * sy) * V.s1[X][i]) * V.c1[X][i]) * ct;
+        w = 0.0;
+    } else if (kind == MIST_VEL_DEFORMATION3D) {
+        const double sx = V.s1[X][i], sy = V.s1[Y][j], sz = V.s1[Z][k];
+        u = ((2.0 * (sx * sx) * V.s2[Y][j]) * V.s2[Z][k]) * ct;
+        v = ((-(sy * sy) * V.s2[X][i]) * V.s2[Z][k]) * ct;
+        w = ((-(sz * sz) * V.s2[X][i]) * V.s2[Y][j]) * ct;
+    } else {
+        vel_fun(kind, T, CX ? m.xm[i] : m.x[i], CY ? m.ym[j] : m.y[j], CZ ? m.zm[k] : m.z[k], t, u, v, w);
+    }
+}
+
+__global__ void k_define_velocity(DevMesh m, VelTab V, int kind, double T, double t, double* u, double* v, double* w,
                                   double* uf, double* vf, double* wf) {
     const int64_t n = (int64_t)m.cnx * m.cny * m.cnz;
+    const double ct = cos((PI * t) / T);
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
         int i, j, k;
         cell_of(m, q, i, j, k);
         double a, b, c;
-        vel_fun(kind, T, m.xm[i], m.ym[j], m.zm[k], t, a, b, c);
+        vel_at<1, 1, 1>(kind, T, t, ct, m, V, i, j, k, a, b, c);
         u[q] = a; v[q] = b; w[q] = c;
-        vel_fun(kind, T, m.x[i], m.ym[j], m.zm[k], t, a, b, c);
+        vel_at<0, 1, 1>(kind, T, t, ct, m, V, i, j, k, a, b, c);
         uf[q] = a;
-        vel_fun(kind, T, m.xm[i], m.y[j], m.zm[k], t, a, b, c);
+        vel_at<1, 0, 1>(kind, T, t, ct, m, V, i, j, k, a, b, c);
         vf[q] = b;
-        vel_fun(kind, T, m.xm[i], m.ym[j], m.z[k], t, a, b, c);
+        vel_at<1, 1, 0>(kind, T, t, ct, m, V, i, j, k, a, b, c);
         wf[q] = c;
     }
 }
@@ -317,119 +350,8 @@ __global__ void k_normals_prep(DevMesh m, DevParams p, const double* VF, double*
     }
 }
 
-// ELVIRA_calc! + check_normal! (src/ELVIRA.jl:62-137). Two interface cells per wave: lane
-// c < 27 of each half scores candidate c (family c/9: X-Y, X-Z, Y-Z; outer pair (c%9)/3,
-// inner pair c%3, the reference's loop order). The 27 liquid fractions of the 3x3x3 block and
-// the full-cell fractions of its cells sit in LDS. The best candidate is the lowest-numbered
-// one with the smallest finite error, which is what the reference's strict `<` scan keeps.
-__global__ __launch_bounds__(64) void k_elvira(DevMesh m, DevParams p, const double* VF, const int* list,
-                                               const int* count, int fam_mask, double* nxo, double* nyo,
-                                               double* nzo) {
-    __shared__ double sVF[2][27];
-    __shared__ double sFull[2][27];
-    const int half = threadIdx.x >> 5, lane = threadIdx.x & 31;
-    const int ncell = *count;
-    const double vol = (m.dx * m.dy) * m.dz;
-    for (int base = blockIdx.x * 2; base < ncell; base += gridDim.x * 2) {
-        const int cidx = base + half;
-        const bool valid = cidx < ncell;
-        int i = 1, j = 1, k = 1;
-        if (valid) interior_of(m, list[cidx], i, j, k);
-        if (valid && lane < 27) {
-            const int ii = i - 1 + lane % 3, jj = j - 1 + (lane / 3) % 3, kk = k - 1 + lane / 9;
-            sVF[half][lane] = VF[m.vof(ii, jj, kk)];
-            // fraction of a cell entirely below the plane: case 1 for all five tets
-            const double cx[2] = {m.x[ii], m.x[ii + 1]}, cy[2] = {m.y[jj], m.y[jj + 1]}, cz[2] = {m.z[kk], m.z[kk + 1]};
-            double dc[8];
-            for (int c = 0; c < 8; ++c) dc[c] = -1.0;
-            sFull[half][lane] = plic2vf_corners(cx, cy, cz, dc, ((ii + jj + kk) % 2) == 0, 0.0, vol);
-        }
-        __syncthreads();
-        double err = __builtin_huge_val();
-        double bx = 0.0, by = 0.0, bz = 0.0;
-        const int fam = lane / 9, a = (lane % 9) / 3, b = lane % 3;
-        if (valid && lane < 27 && ((fam_mask >> fam) & 1)) {
-            const double* V = sVF[half];
-            auto at = [&](int di, int dj, int dk) { return V[(di + 1) + 3 * (dj + 1) + 9 * (dk + 1)]; };
-            const int P0[3] = {-1, -1, 0}, P1[3] = {0, 1, 1};
-            double cnx, cny, cnz;
-            if (fam == 0) {  // X-Y stencils (src/ELVIRA.jl:71-78)
-                const int i1 = P0[b], i2 = P1[b], j1 = P0[a], j2 = P1[a];
-                const double dz = m.dz;
-                const double sx2 = (at(i2, 0, -1) * dz + at(i2, 0, 0) * dz) + at(i2, 0, 1) * dz;
-                const double sx1 = (at(i1, 0, -1) * dz + at(i1, 0, 0) * dz) + at(i1, 0, 1) * dz;
-                cnx = -(sx2 - sx1) / (m.xm[i + i2] - m.xm[i + i1]);
-                const double sy2 = (at(0, j2, -1) * dz + at(0, j2, 0) * dz) + at(0, j2, 1) * dz;
-                const double sy1 = (at(0, j1, -1) * dz + at(0, j1, 0) * dz) + at(0, j1, 1) * dz;
-                cny = -(sy2 - sy1) / (m.ym[j + j2] - m.ym[j + j1]);
-                cnz = at(0, 0, 1) > at(0, 0, -1) ? -1.0 : 1.0;
-            } else if (fam == 1) {  // X-Z stencils (src/ELVIRA.jl:81-88)
-                const int i1 = P0[b], i2 = P1[b], k1 = P0[a], k2 = P1[a];
-                const double dy = m.dy;
-                const double sx2 = (at(i2, -1, 0) * dy + at(i2, 0, 0) * dy) + at(i2, 1, 0) * dy;
-                const double sx1 = (at(i1, -1, 0) * dy + at(i1, 0, 0) * dy) + at(i1, 1, 0) * dy;
-                cnx = -(sx2 - sx1) / (m.xm[i + i2] - m.xm[i + i1]);
-                cny = at(0, 1, 0) > at(0, -1, 0) ? -1.0 : 1.0;
-                const double sz2 = (at(0, -1, k2) * dy + at(0, 0, k2) * dy) + at(0, 1, k2) * dy;
-                const double sz1 = (at(0, -1, k1) * dy + at(0, 0, k1) * dy) + at(0, 1, k1) * dy;
-                cnz = -(sz2 - sz1) / (m.zm[k + k2] - m.zm[k + k1]);
-            } else {  // Y-Z stencils (src/ELVIRA.jl:91-98)
-                const int j1 = P0[b], j2 = P1[b], k1 = P0[a], k2 = P1[a];
-                const double dx = m.dx;
-                cnx = at(1, 0, 0) > at(-1, 0, 0) ? -1.0 : 1.0;
-                const double sy2 = (at(-1, j2, 0) * dx + at(0, j2, 0) * dx) + at(1, j2, 0) * dx;
-                const double sy1 = (at(-1, j1, 0) * dx + at(0, j1, 0) * dx) + at(1, j1, 0) * dx;
-                cny = -(sy2 - sy1) / (m.ym[j + j2] - m.ym[j + j1]);
-                const double sz2 = (at(-1, 0, k2) * dx + at(0, 0, k2) * dx) + at(1, 0, k2) * dx;
-                const double sz1 = (at(-1, 0, k1) * dx + at(0, 0, k1) * dx) + at(1, 0, k1) * dx;
-                cnz = -(sz2 - sz1) / (m.zm[k + k2] - m.zm[k + k1]);
-            }
-            // check_normal!
-            const double norm = sqrt((cnx * cnx + cny * cny) + cnz * cnz);
-            cnx = cnx / norm; cny = cny / norm; cnz = cnz / norm;
-            const double dist = compute_dist(m, p, i, j, k, cnx, cny, cnz, at(0, 0, 0));
-            double e = 0.0;
-            for (int nb = 0; nb < 27; ++nb) {
-                const int ii = i - 1 + nb % 3, jj = j - 1 + (nb / 3) % 3, kk = k - 1 + nb / 9;
-                const double cx[2] = {m.x[ii], m.x[ii + 1]}, cy[2] = {m.y[jj], m.y[jj + 1]},
-                             cz[2] = {m.z[kk], m.z[kk + 1]};
-                double dc[8];
-                bool allneg = true, allpos = true;
-#pragma unroll
-                for (int c = 0; c < 8; ++c) {
-                    dc[c] = ((cnx * cx[c & 1] + cny * cy[(c >> 1) & 1]) + cnz * cz[c >> 2]) - dist;
-                    allneg &= dc[c] <= -EPS;
-                    allpos &= dc[c] >= EPS;
-                }
-                double vf_;
-                if (allneg) vf_ = sFull[half][nb];
-                else if (allpos) vf_ = 0.0;
-                else vf_ = plic2vf_corners(cx, cy, cz, dc, ((ii + jj + kk) % 2) == 0, 0.0, vol);
-                const double df = V[nb] - vf_;
-                e += df * df;
-            }
-            err = e;
-            bx = cnx; by = cny; bz = cnz;
-        }
-        // argmin over the 32 lanes of this half: smallest finite error, lowest lane on ties
-        const bool ok = err < __builtin_huge_val();
-        double be = ok ? err : __builtin_huge_val();
-        int bl = ok ? lane : 64;
-        for (int off = 16; off > 0; off >>= 1) {
-            const double oe = __shfl_xor(be, off, 32);
-            const int ol = __shfl_xor(bl, off, 32);
-            if (ol < 64 && (bl == 64 || oe < be || (oe == be && ol < bl))) { be = oe; bl = ol; }
-        }
-        const double wx = __shfl(bx, bl & 31, 32), wy = __shfl(by, bl & 31, 32), wz = __shfl(bz, bl & 31, 32);
-        if (valid && lane == 0 && bl < 64) {
-            const int64_t c = m.cell(i, j, k);
-            nxo[c] = wx; nyo[c] = wy; nzo[c] = wz;
-        }
-        __syncthreads();
-    }
-}
-
-// ELVIRA with one interface cell per work-group of 256 threads. Threads 0..26 load the
+// ELVIRA_calc! + check_normal! (src/ELVIRA.jl:62-137) with one interface cell per work-group
+// of 256 threads. Threads 0..26 load the
 // 3x3x3 liquid fractions and build the 27 candidate normals and their PLIC constants once
 // (computeDist); then half-wave h scores every candidate on neighbours h, h+8, h+16, h+24,
 // writing (VF - VF_)^2 to LDS; the first half-wave sums each candidate's 27 terms in the
@@ -437,7 +359,7 @@ __global__ __launch_bounds__(64) void k_elvira(DevMesh m, DevParams p, const dou
 // candidate plane misses by a wide margin take the exact all-liquid / all-gas value.
 constexpr int ELV_THREADS = 256;
 
-__global__ __launch_bounds__(ELV_THREADS) void k_elvira2(DevMesh m, DevParams p, const double* VF, const int* list,
+__global__ __launch_bounds__(ELV_THREADS) void k_elvira(DevMesh m, DevParams p, const double* VF, const int* list,
                                                          const int* count, int fam_mask, double* nxo, double* nyo,
                                                          double* nzo) {
     __shared__ double sVF[27];
@@ -821,229 +743,511 @@ __global__ __launch_bounds__(64) void k_sl_cut(DevMesh m, const double* pv, cons
 }
 
 // ------------------------------------------------------------------------------------
-// cutTet for tets whose vertices lie in at most two cells along each axis (all but a few
-// pre-image tets at CFL <= 1): then the reference recursion is at most one x cut, one y cut
-// and one z cut before the PLIC leaf, and every vertex index is lo or lo+1 per axis. The
-// three levels are nested loops over sub-tets with their vertices in registers; a vertex's
-// cell is three bits relative to lo. A cut vertex's index is the pt2index! walk clamped
-// between the indices of its edge ends (src/VFgeom.jl:853-862), which for two candidate
-// cells reduces to one comparison with the plane between them.
+// Band-cell transport, one block per cell: cutTet (src/VFgeom.jl:713-900) of the five
+// pre-image tets of a cell (src/Transport.jl:108-150) as a breadth-first walk of the cut tree.
+//
+// All tets of a cell are cut level by level: the x-planes while any node spans two or more
+// x-cells, then y, then z while a node spans three or more z-cells. Each level is one pass of
+// the block over the node list: a thread cuts one node by the plane at its largest index
+// (src/VFgeom.jl:738-756), appends its new vertices to the cell's vertex pool and its child
+// tets to the next list, at offsets from block prefix sums. The last z-plane (a node spanning
+// two z-cells) is cut again by each thread that evaluates one of its children, so the leaves
+// (the PLIC cuts, src/VFgeom.jl:757-830) run one per thread with nothing stored per leaf.
+//
+// Cell indices are kept as 2-bit offsets from the smallest index `lo` of the projected
+// corners (6 bits per vertex: x, y, z). The index of a cut vertex is the pt2index! walk from
+// the cell of the vertex numbered like it, clamped between the indices of its edge ends
+// (src/VFgeom.jl:853-862); the walk is bounded by the corners' index box, which the clamp
+// makes exact.
+//
+// The liquid and total volumes of the cell are the sums of the leaf volumes, added per thread
+// and then over the block in a fixed order. When every cell that the corners' index box covers
+// lies entirely on one side of its PLIC plane, every leaf is all-liquid (or all-gas), the
+// reference's two sums are identical (or the liquid one is zero) and the new VF is exactly
+// 1 (or 0): such cells are written without cutting.
 // ------------------------------------------------------------------------------------
-struct FastTet {
-    V3 p[4];
-    unsigned bits;  // 3 bits per vertex (x, y, z offset from lo)
+constexpr int TREE_PV = 256;      // vertex pool per cell: 8 corners + mesh-cut vertices
+constexpr int TREE_NL = 384;      // nodes per list
+constexpr int TREE_NLEAF = 1152;  // leaves per cell
+constexpr int TREE_THREADS = 64;  // threads per cell
+
+struct TreeNode {
+    unsigned w0, w1;  // pool ids of vertices 0,1 and 2,3 (16 bits each)
+    unsigned bits;    // cell offset from lo of vertex n along axis q at bit 6n + 2q
 };
 
-struct FastLevel {
-    V3 q[8];
-    unsigned bits1;  // side-1 bits of the 8 level vertices
-    int cs, nchild, axis;
+template <int NT>
+struct TreeShared {
+    double px[TREE_PV], py[TREE_PV], pz[TREE_PV];
+    TreeNode node[2][TREE_NL];
+    unsigned short loff[TREE_NL];      // first leaf of each node of the last list
+    unsigned short lnode[TREE_NLEAF];  // node of each leaf
+    double plane[3][5];                // faces lo+0 .. lo+span+1 per axis
+    double nb[64][4];                  // nx, ny, nz, D of the cells of the index box
+    int ci[8][3];                      // cell index of each projected corner
+    int next;                          // cell taken from the work queue
+    int scan[NT / 64];
+    double red[2][NT / 64];
 };
 
-__device__ __forceinline__ V3 sel8(const V3 (&q)[8], int v) {
-    V3 r = q[0];
-#pragma unroll
-    for (int k = 1; k < 8; ++k)
-        if (v == k) r = q[k];
+__device__ __forceinline__ unsigned tn_id(const TreeNode& N, int n) {
+    return ((n < 2 ? N.w0 : N.w1) >> ((n & 1) * 16)) & 0xFFFFu;
+}
+__device__ __forceinline__ unsigned boff(unsigned bits, int n, int q) { return (bits >> (6 * n + 2 * q)) & 3u; }
+// a[n] for a register array and a runtime n. The empty asm makes the four values opaque so
+// the selects stay selects of registers; otherwise the optimizer turns them into an indexed
+// load, which puts the array in scratch memory.
+__device__ __forceinline__ void opaque(double& v) { asm("" : "+v"(v)); }
+__device__ __forceinline__ void opaque(unsigned& v) { asm("" : "+v"(v)); }
+__device__ __forceinline__ void opaque(V3& v) {
+    opaque(v.x);
+    opaque(v.y);
+    opaque(v.z);
+}
+template <class T>
+__device__ __forceinline__ T sel4(const T (&a)[4], int n) {
+    T a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3];
+    opaque(a0);
+    opaque(a1);
+    opaque(a2);
+    opaque(a3);
+    T r = a0;
+    if (n == 1) r = a1;
+    if (n == 2) r = a2;
+    if (n == 3) r = a3;
     return r;
 }
-__device__ __forceinline__ double comp(const V3& a, int axis) { return axis == 0 ? a.x : (axis == 1 ? a.y : a.z); }
-
-// True when the four vertices do not share their cell index along `axis`.
-__device__ __forceinline__ bool spans(unsigned bits, int axis) {
-    const unsigned b0 = (bits >> axis) & 1u, b1 = (bits >> (3 + axis)) & 1u, b2 = (bits >> (6 + axis)) & 1u,
-                   b3 = (bits >> (9 + axis)) & 1u;
-    return (b0 | b1 | b2 | b3) != (b0 & b1 & b2 & b3);
+template <int NT>
+__device__ __forceinline__ V3 pool_v(const TreeShared<NT>& S, unsigned id) {
+    return V3{S.px[id], S.py[id], S.pz[id]};
+}
+template <int NT>
+__device__ __forceinline__ double pool_c(const TreeShared<NT>& S, unsigned id, int axis) {
+    return axis == 0 ? S.px[id] : (axis == 1 ? S.py[id] : S.pz[id]);
 }
 
-// Mesh-plane cut of node T along `axis` at the plane between lo and lo+1 (cut_ind = lo+1).
-__device__ __forceinline__ void fast_cut(const FastTet& T, int axis, const double planes[3], FastLevel& F) {
-    const double P = planes[axis];
-    double d[4], dd[4];
+// Exclusive prefix sum over the block; *total receives the block sum.
+template <int NT>
+__device__ __forceinline__ int block_scan(int v, int& total, int* tmp) {
+    const int lane = threadIdx.x & 63;
+    int x = v;
 #pragma unroll
-    for (int n = 0; n < 4; ++n) { d[n] = comp(T.p[n], axis) - P; dd[n] = d[n]; }
-    const int cs = d2case_snap(dd);  // the reference snaps a copy
-    F.cs = cs;
-    F.axis = axis;
-    F.nchild = CUT_NTETS[cs];
-    // original vertices: side-1 index along the axis is min(ind, cut_ind - 1) = lo
-    unsigned b1 = T.bits & ~((1u << axis) | (1u << (3 + axis)) | (1u << (6 + axis)) | (1u << (9 + axis)));
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+    }
+    if constexpr (NT == 64) {
+        total = __shfl(x, 63);
+        return x - v;
+    } else {
+        const int w = threadIdx.x >> 6;
+        if (lane == 63) tmp[w] = x;
+        __syncthreads();
+        int base = 0, tot = 0;
 #pragma unroll
-    for (int n = 0; n < 4; ++n) F.q[n] = T.p[n];
-    const int nv = CUT_NVERT[cs];
-#pragma unroll
-    for (int n = 0; n < 4; ++n) {
-        if (n < nv) {
-            const unsigned e = CUT_EDGE[cs][n];
-            const int v1 = e & 15, v2 = e >> 4;
-            double d1 = d[0], d2 = d[0];
-#pragma unroll
-            for (int k = 1; k < 4; ++k) { if (v1 == k) d1 = d[k]; if (v2 == k) d2 = d[k]; }
-            V3 a = T.p[0], b = T.p[0];
-#pragma unroll
-            for (int k = 1; k < 4; ++k) { if (v1 == k) a = T.p[k]; if (v2 == k) b = T.p[k]; }
-            const double mu = jmin(1.0, jmax(0.0, sdiv(-d1, d2 - d1)));
-            const V3 c = lerp3(a, b, mu);
-            F.q[4 + n] = c;
-            // index of the cut vertex: walk from vertex n's side-1 index, clamp to [v1, v2]
-            const unsigned s = (b1 >> (3 * n)) & 7u;
-            const unsigned e1 = (b1 >> (3 * v1)) & 7u, e2 = (b1 >> (3 * v2)) & 7u;
-            unsigned nb = 0;
-#pragma unroll
-            for (int ax = 0; ax < 3; ++ax) {
-                const unsigned u1 = (e1 >> ax) & 1u, u2 = (e2 >> ax) & 1u;
-                unsigned bit;
-                if (ax == axis) bit = 0u;  // side 1: cut_ind - 1
-                else if (u1 == u2) bit = u1;
-                else {
-                    const double pc = comp(c, ax), pl = planes[ax];
-                    bit = ((s >> ax) & 1u) ? (pc < pl - EPS ? 0u : 1u) : (pc > pl + EPS ? 1u : 0u);
-                }
-                nb |= bit << ax;
-            }
-            b1 |= nb << (3 * (4 + n));
+        for (int q = 0; q < NT / 64; ++q) {
+            const int t = tmp[q];
+            base += q < w ? t : 0;
+            tot += t;
         }
+        __syncthreads();
+        total = tot;
+        return base + x - v;
     }
-    F.bits1 = b1;
 }
 
-__device__ __forceinline__ FastTet fast_child(const FastLevel& F, int n) {
-    FastTet T;
-    const unsigned q = CUT_VTET[F.cs][n];
-    const bool side2 = (CUT_SIDE2[F.cs] >> n) & 1;
-    unsigned bits = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int v = nib(q, k);
-        T.p[k] = sel8(F.q, v);
-        unsigned b = (F.bits1 >> (3 * v)) & 7u;
-        if (side2) b |= 1u << F.axis;
-        bits |= b << (3 * k);
-    }
-    T.bits = bits;
-    return T;
+// pt2index! walk (src/Tools.jl:424-434) of coordinate c from offset s over the faces P of the
+// index box [0, span], clamped to the offsets e1, e2 of the edge ends.
+__device__ __forceinline__ unsigned walk_clamp(double c, int s, int e1, int e2, const double* P, int span) {
+    int I = s;
+    while (I < span && c > P[I + 1] + EPS) ++I;
+    while (I > 0 && c < P[I] - EPS) --I;
+    return (unsigned)min(max(I, min(e1, e2)), max(e1, e2));
 }
 
-__device__ __forceinline__ void fast_leaf(const DevMesh& m, const double* nx, const double* ny, const double* nz,
-                                          const double* D, const FastTet& T, int lo0, int lo1, int lo2, double& vol,
-                                          double& liq) {
-    const unsigned b = T.bits & 7u;
-    const int64_t c = m.cell(lo0 + (b & 1u), lo1 + ((b >> 1) & 1u), lo2 + ((b >> 2) & 1u));
-    const double n0 = nx[c], n1 = ny[c], n2 = nz[c], Dc = D[c];
-    double d[4], dd[4];
-#pragma unroll
-    for (int n = 0; n < 4; ++n) {
-        d[n] = ((n0 * T.p[n].x + n1 * T.p[n].y) + n2 * T.p[n].z) - Dc;
-        dd[n] = d[n];
-    }
-    const int cs = d2case_snap(dd);
-    double V = 0.0, L = 0.0;
-    cut_dispatch<true>(cs, T.p, d, [&](double tv) { V += tv; L += tv; }, [&](double tv) { V += tv; });
-    vol = V;
-    liq = L;
-}
+// One level of mesh-plane cuts along `axis` (src/VFgeom.jl:832-899) of every node of list
+// `cur` spanning at least `minspan` cells along it; others pass unchanged. Returns false when
+// the pool or the next list would overflow.
+struct Span3 {
+    int s[3];
+};
 
-// Volumes of the tet after the x, y, z and PLIC cuts, summed as the recursion nests them.
-__device__ void cut_tet_fast(const DevMesh& m, const double* nx, const double* ny, const double* nz, const double* D,
-                             const FastTet& root, int lo0, int lo1, int lo2, double& vol, double& liq) {
-    const double planes[3] = {m.x[lo0 + 1], m.y[lo1 + 1], m.z[lo2 + 1]};
-    FastLevel FX, FY, FZ;
-    const bool cx = spans(root.bits, 0);
-    if (cx) fast_cut(root, 0, planes, FX);
-    const int nxc = cx ? FX.nchild : 1;
-    double vX = 0.0, lX = 0.0;
-    for (int a = 0; a < nxc; ++a) {
-        const FastTet TY = cx ? fast_child(FX, a) : root;
-        const bool cy = spans(TY.bits, 1);
-        if (cy) fast_cut(TY, 1, planes, FY);
-        const int nyc = cy ? FY.nchild : 1;
-        double vY = 0.0, lY = 0.0;
-        for (int b = 0; b < nyc; ++b) {
-            const FastTet TZ = cy ? fast_child(FY, b) : TY;
-            const bool cz = spans(TZ.bits, 2);
-            if (cz) fast_cut(TZ, 2, planes, FZ);
-            const int nzc = cz ? FZ.nchild : 1;
-            double vZ = 0.0, lZ = 0.0;
-            for (int c = 0; c < nzc; ++c) {
-                const FastTet TL = cz ? fast_child(FZ, c) : TZ;
-                double v, l;
-                fast_leaf(m, nx, ny, nz, D, TL, lo0, lo1, lo2, v, l);
-                if (cz) { vZ += v; lZ += l; } else { vZ = v; lZ = l; }
-            }
-            if (cy) { vY += vZ; lY += lZ; } else { vY = vZ; lY = lZ; }
-        }
-        if (cx) { vX += vY; lX += lY; } else { vX = vY; lX = lY; }
-    }
-    vol = vX;
-    liq = lX;
-}
-
-// SL volume fraction of band cells: 12 cells per wave, one lane per (cell, tet). Cells with a
-// tet spanning more than two cells along an axis go to the general kernel (k_sl_cut) through
-// `defer`.
-__global__ __launch_bounds__(64) void k_sl_fast(DevMesh m, const double* pv, const int* list, const int* count,
-                                                const double* nx, const double* ny, const double* nz,
-                                                const double* D, double* VF, int* ovf, int* defer,
-                                                int* ndefer) {
-    __shared__ double s_vol[64], s_liq[64];
-    __shared__ int s_gen[64];
-    const int lane = threadIdx.x;
-    const int cl = lane / 5, t = lane - 5 * cl;
-    const int nb = *count;
-    for (int base = blockIdx.x * 12; base < nb; base += gridDim.x * 12) {
-        const int cidx = base + cl;
-        const bool active = lane < 60 && cidx < nb;
-        double vol = 0.0, liq = 0.0;
-        int general = 0;
-        int i = 1, j = 1, k = 1, q = 0;
+template <int NT>
+__device__ bool tree_level(TreeShared<NT>& S, int axis, int minspan, int cur, int& n, int& npool, Span3 span) {
+    int nout = 0;
+    for (int b = 0; b < n; b += NT) {
+        const int e = b + (int)threadIdx.x;
+        const bool active = e < n;
+        TreeNode N{0u, 0u, 0u};
+        int mx = 0, cs = 0, nch = 0, nv = 0;
+        bool cut = false;
+        double d[4] = {0.0, 0.0, 0.0, 0.0};
         if (active) {
-            q = list[cidx];
-            interior_of(m, q, i, j, k);
-            const bool even = ((i + j + k) % 2) == 0;
-            const unsigned tv = TET5[t];
-            FastTet T;
-            int ind[4][3];
-            int o = 0;
+            N = S.node[cur][e];
+            const int o0 = boff(N.bits, 0, axis), o1 = boff(N.bits, 1, axis), o2 = boff(N.bits, 2, axis),
+                      o3 = boff(N.bits, 3, axis);
+            mx = max(max(o0, o1), max(o2, o3));
+            const int mn = min(min(o0, o1), min(o2, o3));
+            cut = mx - mn >= minspan;
+            if (cut) {
+                const double P = S.plane[axis][mx];
+                double dd[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    d[q] = pool_c(S, tn_id(N, q), axis) - P;
+                    dd[q] = d[q];
+                }
+                cs = d2case_snap(dd);  // the reference snaps a copy
+                nch = CUT_NTETS[cs];
+                nv = CUT_NVERT[cs];
+            } else {
+                nch = 1;
+            }
+        }
+        int tc, tv;
+        const int oc = block_scan<NT>(nch, tc, S.scan);
+        const int ov = block_scan<NT>(nv, tv, S.scan);
+        if (nout + tc > TREE_NL || npool + tv > TREE_PV) return false;
+        if (active && !cut) S.node[cur ^ 1][nout + oc] = N;
+        if (cut) {
+            const int vb = npool + ov;
+            unsigned cb[4] = {0u, 0u, 0u, 0u};  // side-1 offsets (6 bits) of the cut vertices
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (q < nv) {
+                    const unsigned eg = CUT_EDGE[cs][q];
+                    const int v1 = eg & 15, v2 = eg >> 4;
+                    const double d1 = sel4(d, v1), d2 = sel4(d, v2);
+                    const double mu = jmin(1.0, jmax(0.0, sdiv(-d1, d2 - d1)));
+                    const V3 c = lerp3(pool_v(S, tn_id(N, v1)), pool_v(S, tn_id(N, v2)), mu);
+                    S.px[vb + q] = c.x;
+                    S.py[vb + q] = c.y;
+                    S.pz[vb + q] = c.z;
+                    unsigned bb = 0;
+#pragma unroll
+                    for (int ax = 0; ax < 3; ++ax) {
+                        unsigned o;
+                        if (ax == axis) {
+                            o = (unsigned)(mx - 1);
+                        } else {
+                            const int e1 = boff(N.bits, v1, ax), e2 = boff(N.bits, v2, ax);
+                            o = e1 == e2 ? (unsigned)e1
+                                         : walk_clamp(ax == 0 ? c.x : (ax == 1 ? c.y : c.z), boff(N.bits, q, ax),
+                                                      e1, e2, S.plane[ax], span.s[ax]);
+                        }
+                        bb |= o << (2 * ax);
+                    }
+                    cb[q] = bb;
+                }
+            }
+            const unsigned amask = 3u << (2 * axis);
+            const unsigned side2 = CUT_SIDE2[cs];
+            for (int c = 0; c < nch; ++c) {
+                const unsigned qv = CUT_VTET[cs][c];
+                const bool s2 = (side2 >> c) & 1;
+                unsigned ids[4], bits = 0;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int lv = nib(qv, k);
+                    unsigned ob, id;
+                    if (lv < 4) {
+                        id = tn_id(N, lv);
+                        ob = (N.bits >> (6 * lv)) & 63u;
+                        const unsigned oa = (ob >> (2 * axis)) & 3u;
+                        const unsigned na = s2 ? (unsigned)mx : min(oa, (unsigned)(mx - 1));
+                        ob = (ob & ~amask) | (na << (2 * axis));
+                    } else {
+                        id = (unsigned)(vb + lv - 4);
+                        ob = sel4(cb, lv - 4);
+                        if (s2) ob = (ob & ~amask) | ((unsigned)mx << (2 * axis));
+                    }
+                    ids[k] = id;
+                    bits |= ob << (6 * k);
+                }
+                TreeNode C;
+                C.w0 = ids[0] | (ids[1] << 16);
+                C.w1 = ids[2] | (ids[3] << 16);
+                C.bits = bits;
+                S.node[cur ^ 1][nout + oc + c] = C;
+            }
+        }
+        nout += tc;
+        npool += tv;
+    }
+    n = nout;
+    __syncthreads();
+    return true;
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void k_sl_tree(DevMesh m, const double* pv, const int* list, const int* count,
+                                                const double* nx, const double* ny, const double* nz,
+                                                const double* D, double* VF, int* ovf, int* defer, int* ndefer,
+                                                int* queue) {
+    __shared__ TreeShared<NT> S;
+    const int tid = threadIdx.x;
+    const int nb = *count;
+    for (;;) {
+        if (tid == 0) S.next = atomicAdd(queue, 1);
+        __syncthreads();
+        const int cidx = S.next;
+        if (cidx >= nb) break;
+        const int q = list[cidx];
+        int i, j, k;
+        interior_of(m, q, i, j, k);
+        const bool even = ((i + j + k) % 2) == 0;
+        // ---- projected corners and their cells (pt2index! from the cell itself) ----------
+        if (tid < 8) {
+            const int64_t vq = vtx_id(m, i + (tid & 1), j + ((tid >> 1) & 1), k + (tid >> 2));
+            const double pt[3] = {pv[3 * vq + 0], pv[3 * vq + 1], pv[3 * vq + 2]};
+            S.px[tid] = pt[0];
+            S.py[tid] = pt[1];
+            S.pz[tid] = pt[2];
+            int I[3], o = 0;
+            pt2index(m, I, pt, i, j, k, &o);
+            if (o) atomicOr(ovf, 1);
+            S.ci[tid][0] = I[0];
+            S.ci[tid][1] = I[1];
+            S.ci[tid][2] = I[2];
+        }
+        __syncthreads();
+        int lo[3], span[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            int mn = S.ci[0][a], mx = S.ci[0][a];
+#pragma unroll
+            for (int c = 1; c < 8; ++c) {
+                mn = min(mn, S.ci[c][a]);
+                mx = max(mx, S.ci[c][a]);
+            }
+            lo[a] = mn;
+            span[a] = mx - mn;
+        }
+        if (span[0] > 3 || span[1] > 3 || span[2] > 3) {  // beyond 2-bit offsets: general walk
+            if (tid == 0) defer[atomicAdd(ndefer, 1)] = q;
+            __syncthreads();
+            continue;
+        }
+        if (tid < 15) {
+            const int a = tid / 5, off = tid - 5 * a;
+            const int la = a == 0 ? lo[0] : (a == 1 ? lo[1] : lo[2]);
+            const int sa = a == 0 ? span[0] : (a == 1 ? span[1] : span[2]);
+            if (off <= sa + 1) S.plane[a][off] = (a == 0 ? m.x : (a == 1 ? m.y : m.z))[la + off];
+        }
+        __syncthreads();
+        // ---- the cells of the index box: PLIC planes and which side they are on ----------
+        const int bx = span[0] + 1, by = span[1] + 1, bz = span[2] + 1;
+        const int ncell = bx * by * bz;
+        int cls = 0;
+        for (int l = tid; l < ncell; l += NT) {
+            const int ox = l % bx, oy = (l / bx) % by, oz = l / (bx * by);
+            const int64_t c = m.cell(lo[0] + ox, lo[1] + oy, lo[2] + oz);
+            const double n0 = nx[c], n1 = ny[c], n2 = nz[c], Dc = D[c];
+            S.nb[l][0] = n0;
+            S.nb[l][1] = n1;
+            S.nb[l][2] = n2;
+            S.nb[l][3] = Dc;
+            const double x0 = S.plane[0][ox], x1 = S.plane[0][ox + 1], y0 = S.plane[1][oy], y1 = S.plane[1][oy + 1],
+                         z0 = S.plane[2][oz], z1 = S.plane[2][oz + 1];
+            const double a0 = n0 * x0, a1 = n0 * x1, b0 = n1 * y0, b1 = n1 * y1, c0 = n2 * z0, c1 = n2 * z1;
+            const double dmax = ((fmax(a0, a1) + fmax(b0, b1)) + fmax(c0, c1)) - Dc;
+            const double dmin = ((fmin(a0, a1) + fmin(b0, b1)) + fmin(c0, c1)) - Dc;
+            const double mag = fabs(n0) * (fmax(fabs(x0), fabs(x1)) + (x1 - x0)) +
+                               fabs(n1) * (fmax(fabs(y0), fabs(y1)) + (y1 - y0)) +
+                               fabs(n2) * (fmax(fabs(z0), fabs(z1)) + (z1 - z0)) + fabs(Dc);
+            const double tau = 1e-9 * mag;
+            cls = dmax < -tau ? 1 : (dmin > tau ? -1 : 0);
+        }
+        const int all_liq = __syncthreads_and(tid >= ncell || cls == 1);
+        const int all_gas = __syncthreads_and(tid >= ncell || cls == -1);
+        if (all_liq || all_gas) {
+            if (tid == 0) VF[m.vof(i, j, k)] = all_liq ? 1.0 : 0.0;
+            __syncthreads();
+            continue;
+        }
+        // ---- the five tets (src/VFgeom.jl:235-256) ---------------------------------------
+        if (tid < 5) {
+            const unsigned tv = TET5[tid];
+            unsigned ids[4], bits = 0;
+#pragma unroll
             for (int n = 0; n < 4; ++n) {
                 const int c = vert_corner(nib(tv, n), even);
-                const int64_t vq = vtx_id(m, i + (c & 1), j + ((c >> 1) & 1), k + (c >> 2));
-                T.p[n].x = pv[3 * vq + 0];
-                T.p[n].y = pv[3 * vq + 1];
-                T.p[n].z = pv[3 * vq + 2];
-                const double pt[3] = {T.p[n].x, T.p[n].y, T.p[n].z};
-                pt2index(m, ind[n], pt, i, j, k, &o);
+                ids[n] = (unsigned)c;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) bits |= (unsigned)(S.ci[c][a] - lo[a]) << (6 * n + 2 * a);
             }
-            if (o) atomicOr(ovf, 1);
-            int lo[3];
-            unsigned bits = 0;
-            for (int a = 0; a < 3; ++a) {
-                const int mn = min(min(ind[0][a], ind[1][a]), min(ind[2][a], ind[3][a]));
-                const int mx = max(max(ind[0][a], ind[1][a]), max(ind[2][a], ind[3][a]));
-                lo[a] = mn;
-                if (mx - mn > 1) general = 1;
-            }
-            if (!general) {
-                for (int n = 0; n < 4; ++n)
-                    for (int a = 0; a < 3; ++a) bits |= (unsigned)(ind[n][a] - lo[a]) << (3 * n + a);
-                T.bits = bits;
-                cut_tet_fast(m, nx, ny, nz, D, T, lo[0], lo[1], lo[2], vol, liq);
-            }
+            S.node[0][tid] = TreeNode{ids[0] | (ids[1] << 16), ids[2] | (ids[3] << 16), bits};
         }
-        s_vol[lane] = vol;
-        s_liq[lane] = liq;
-        s_gen[lane] = general;
         __syncthreads();
-        if (active && t == 0) {
-            const int gen = s_gen[lane] | s_gen[lane + 1] | s_gen[lane + 2] | s_gen[lane + 3] | s_gen[lane + 4];
-            if (gen) {
-                defer[atomicAdd(ndefer, 1)] = q;
-            } else {
-                const double ts = ((i + j + k) % 2) == 0 ? 1.0 : -1.0;
-                double V = 0.0, L = 0.0;
-                for (int r = 0; r < 5; ++r) { V += ts * s_vol[lane + r]; L += ts * s_liq[lane + r]; }
-                VF[m.vof(i, j, k)] = L / V;
+        // ---- mesh-plane levels -------------------------------------------------------------
+        int n = 5, npool = 8, cur = 0;
+        bool ok = true;
+        for (int axis = 0; axis < 3 && ok; ++axis) {
+            const int minspan = axis == 2 ? 2 : 1;
+            for (int rep = 0; rep < 3 && ok; ++rep) {
+                int any = 0;
+                for (int e = tid; e < n; e += NT) {
+                    const unsigned bits = S.node[cur][e].bits;
+                    const int o0 = boff(bits, 0, axis), o1 = boff(bits, 1, axis), o2 = boff(bits, 2, axis),
+                              o3 = boff(bits, 3, axis);
+                    any |= max(max(o0, o1), max(o2, o3)) - min(min(o0, o1), min(o2, o3)) >= minspan;
+                }
+                if (!__syncthreads_or(any)) break;
+                ok = tree_level<NT>(S, axis, minspan, cur, n, npool, Span3{{span[0], span[1], span[2]}});
+                cur ^= 1;
             }
         }
+        // ---- leaves per node (the last z-plane is cut again by each leaf) ------------------
+        int nleaf = 0;
+        for (int b = 0; b < n && ok; b += NT) {
+            const int e = b + tid;
+            int cnt = 0;
+            if (e < n) {
+                const TreeNode N = S.node[cur][e];
+                const int o0 = boff(N.bits, 0, 2), o1 = boff(N.bits, 1, 2), o2 = boff(N.bits, 2, 2),
+                          o3 = boff(N.bits, 3, 2);
+                const int mx = max(max(o0, o1), max(o2, o3));
+                if (mx > min(min(o0, o1), min(o2, o3))) {
+                    const double P = S.plane[2][mx];
+                    double dd[4];
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) dd[v] = S.pz[tn_id(N, v)] - P;
+                    cnt = CUT_NTETS[d2case_snap(dd)];
+                } else {
+                    cnt = 1;
+                }
+            }
+            int tot;
+            const int o = block_scan<NT>(cnt, tot, S.scan);
+            if (nleaf + tot > TREE_NLEAF) {
+                ok = false;
+                break;
+            }
+            if (e < n) {
+                S.loff[e] = (unsigned short)(nleaf + o);
+                for (int c = 0; c < cnt; ++c) S.lnode[nleaf + o + c] = (unsigned short)e;
+            }
+            nleaf += tot;
+        }
+        if (!ok) {  // pool or lists too small for this cell: general walk
+            if (tid == 0) defer[atomicAdd(ndefer, 1)] = q;
+            __syncthreads();
+            continue;
+        }
+        __syncthreads();
+        // ---- leaves: last z cut and the PLIC cut (src/VFgeom.jl:757-830) ------------------
+        double Vs = 0.0, Ls = 0.0;
+        for (int L = tid; L < nleaf; L += NT) {
+            const int e = S.lnode[L];
+            const int c = L - S.loff[e];
+            const TreeNode N = S.node[cur][e];
+            V3 Pn[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Pn[v] = pool_v(S, tn_id(N, v));
+            const int o0 = boff(N.bits, 0, 2), o1 = boff(N.bits, 1, 2), o2 = boff(N.bits, 2, 2),
+                      o3 = boff(N.bits, 3, 2);
+            const int mx = max(max(o0, o1), max(o2, o3));
+            V3 Q[4];
+            unsigned ox, oy, oz;
+            if (mx > min(min(o0, o1), min(o2, o3))) {
+                const double P = S.plane[2][mx];
+                double d[4], dd[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    d[v] = Pn[v].z - P;
+                    dd[v] = d[v];
+                }
+                const int cs = d2case_snap(dd);
+                const int nv = CUT_NVERT[cs];
+                V3 Cv[4];
+                unsigned cb[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    Cv[v] = Pn[0];
+                    if (v < nv) {
+                        const unsigned eg = CUT_EDGE[cs][v];
+                        const int v1 = eg & 15, v2 = eg >> 4;
+                        const double mu = jmin(1.0, jmax(0.0, sdiv(-sel4(d, v1), sel4(d, v2) - sel4(d, v1))));
+                        Cv[v] = lerp3(sel4(Pn, v1), sel4(Pn, v2), mu);
+                        unsigned bb = (unsigned)(mx - 1) << 4;
+#pragma unroll
+                        for (int ax = 0; ax < 2; ++ax) {
+                            const int e1 = boff(N.bits, v1, ax), e2 = boff(N.bits, v2, ax);
+                            const unsigned oo =
+                                e1 == e2 ? (unsigned)e1
+                                         : walk_clamp(ax == 0 ? Cv[v].x : Cv[v].y, boff(N.bits, v, ax), e1, e2,
+                                                      S.plane[ax], span[ax]);
+                            bb |= oo << (2 * ax);
+                        }
+                        cb[v] = bb;
+                    }
+                }
+                const unsigned qv = CUT_VTET[cs][c];
+                const bool s2 = (CUT_SIDE2[cs] >> c) & 1;
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    const int lv = nib(qv, kk);
+                    Q[kk] = lv < 4 ? sel4(Pn, lv) : sel4(Cv, lv - 4);
+                }
+                const int lv0 = nib(qv, 0);
+                unsigned ob;
+                if (lv0 < 4) {
+                    ob = (N.bits >> (6 * lv0)) & 63u;
+                    const unsigned oa = ob >> 4;
+                    ob = (ob & 15u) | ((s2 ? (unsigned)mx : min(oa, (unsigned)(mx - 1))) << 4);
+                } else {
+                    ob = sel4(cb, lv0 - 4);
+                    if (s2) ob = (ob & 15u) | ((unsigned)mx << 4);
+                }
+                ox = ob & 3u;
+                oy = (ob >> 2) & 3u;
+                oz = (ob >> 4) & 3u;
+            } else {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) Q[v] = Pn[v];
+                ox = boff(N.bits, 0, 0);
+                oy = boff(N.bits, 0, 1);
+                oz = boff(N.bits, 0, 2);
+            }
+            const int l = (int)ox + bx * ((int)oy + by * (int)oz);
+            const double n0 = S.nb[l][0], n1 = S.nb[l][1], n2 = S.nb[l][2], Dc = S.nb[l][3];
+            double d[4], dd[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                d[v] = ((n0 * Q[v].x + n1 * Q[v].y) + n2 * Q[v].z) - Dc;
+                dd[v] = d[v];
+            }
+            const int cs = d2case_snap(dd);
+            if (cs == 0 || cs == 15) {  // one sub-tet, the leaf itself (vertex order 1,2,3,4)
+                const double tv = tet_vol3(Q[0], Q[1], Q[2], Q[3]);
+                Vs += tv;
+                if (cs == 0) Ls += tv;
+            } else {
+                cut_dispatch<true>(cs, Q, d, [&](double tv) { Vs += tv; Ls += tv; }, [&](double tv) { Vs += tv; });
+            }
+        }
+        // ---- block sums in a fixed order ----------------------------------------------------
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            Vs += __shfl_xor(Vs, off);
+            Ls += __shfl_xor(Ls, off);
+        }
+        if constexpr (NT > 64) {
+            if ((tid & 63) == 0) {
+                S.red[0][tid >> 6] = Vs;
+                S.red[1][tid >> 6] = Ls;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                Vs = S.red[0][0];
+                Ls = S.red[1][0];
+                for (int w = 1; w < NT / 64; ++w) {
+                    Vs += S.red[0][w];
+                    Ls += S.red[1][w];
+                }
+            }
+        }
+        if (tid == 0) VF[m.vof(i, j, k)] = Ls / Vs;
         __syncthreads();
     }
 }
@@ -1252,9 +1456,14 @@ static Ext3 ext_of(mist_ctx* ctx, int field) {
         MIST_HIP_TRY(hipGetLastError());                                                 \
     } while (0)
 
+int launch_trig_tables(mist_ctx* ctx, int64_t n) {
+    LAUNCH(k_trig_tables, grid_for(n, 256), 256, (const double*)ctx->geom, n, ctx->trig);
+    return MIST_OK;
+}
+
 int launch_define_velocity(mist_ctx* ctx, double t) {
     const int64_t n = (int64_t)ctx->m.cnx * ctx->m.cny * ctx->m.cnz;
-    LAUNCH(k_define_velocity, grid_for(n, 256), 256, ctx->m, (int)ctx->p.vf_velocity, ctx->p.vel_period, t,
+    LAUNCH(k_define_velocity, grid_for(n, 256), 256, ctx->m, ctx->vt, (int)ctx->p.vf_velocity, ctx->p.vel_period, t,
            (double*)ctx->f[MIST_FIELD_U], (double*)ctx->f[MIST_FIELD_V], (double*)ctx->f[MIST_FIELD_W],
            (double*)ctx->f[MIST_FIELD_UF], (double*)ctx->f[MIST_FIELD_VF], (double*)ctx->f[MIST_FIELD_WF]);
     return MIST_OK;
@@ -1347,7 +1556,7 @@ int launch_normals(mist_ctx* ctx) {
     const int fam = (ctx->m.Nz > 1 ? 1 : 0) | (ctx->m.Ny > 1 ? 2 : 0) | (ctx->m.Nx > 1 ? 4 : 0);
     const int64_t ncap = (int64_t)ctx->m.Nx * ctx->m.Ny * ctx->m.Nz;
     hipEvent_t ev = prof_begin(ctx, MIST_PROF_ELVIRA);
-    LAUNCH(k_elvira2, grid_for(ncap, 1, 16384), ELV_THREADS, ctx->m, ctx->dp, (const double*)ctx->f[MIST_FIELD_VOF],
+    LAUNCH(k_elvira, grid_for(ncap, 1, 16384), ELV_THREADS, ctx->m, ctx->dp, (const double*)ctx->f[MIST_FIELD_VOF],
            (const int*)ctx->iface_list, (const int*)(ctx->counters + 1), fam, nx, ny, nz);
     prof_end(ctx, MIST_PROF_ELVIRA, ev, ctx->counters + 1, 0);
     int st;
@@ -1375,18 +1584,28 @@ int launch_transport(mist_ctx* ctx, double dt) {
     prof_end(ctx, MIST_PROF_PROJECT, ev, nullptr, nv);
     const int64_t ncap = (int64_t)ctx->m.Nx * ctx->m.Ny * ctx->m.Nz;
     ev = prof_begin(ctx, MIST_PROF_SL_CUT);
-    LAUNCH(k_fill_int, 1, 64, ctx->counters + 4, 1, 0);
-    LAUNCH(k_sl_fast, grid_for((ncap + 11) / 12, 64, 16384), 64, ctx->m, (const double*)ctx->pv,
+    LAUNCH(k_fill_int, 1, 64, ctx->counters + 4, 2, 0);  // deferred cells, work-queue position
+    if (!ctx->tree_blocks) {
+        int per = 0, dev = 0, cus = 0;
+        MIST_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_sl_tree<TREE_THREADS>, TREE_THREADS, 0));
+        MIST_HIP_TRY(hipGetDevice(&dev));
+        MIST_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        ctx->tree_blocks = std::max(1, per) * std::max(1, cus);
+    }
+    LAUNCH(k_sl_tree<TREE_THREADS>, ctx->tree_blocks, TREE_THREADS, ctx->m, (const double*)ctx->pv,
            (const int*)ctx->band_list, (const int*)(ctx->counters + 0), (const double*)ctx->f[MIST_FIELD_NX],
            (const double*)ctx->f[MIST_FIELD_NY], (const double*)ctx->f[MIST_FIELD_NZ],
            (const double*)ctx->f[MIST_FIELD_PLIC_D], (double*)ctx->f[MIST_FIELD_VOF], ctx->counters + 2,
-           ctx->iface_list, ctx->counters + 4);
-    // cells with a tet spanning three or more cells along an axis: general walk
+           ctx->iface_list, ctx->counters + 4, ctx->counters + 5);
+    prof_end(ctx, MIST_PROF_SL_CUT, ev, ctx->counters + 0, 0);
+    // cells whose corners span more than four cells along an axis, or whose cut tree
+    // outgrows the block's pool: general walk
+    ev = prof_begin(ctx, MIST_PROF_SL_GENERAL);
     LAUNCH(k_sl_cut, 256, 64, ctx->m, (const double*)ctx->pv, (const int*)ctx->iface_list,
            (const int*)(ctx->counters + 4), (const double*)ctx->f[MIST_FIELD_NX], (const double*)ctx->f[MIST_FIELD_NY],
            (const double*)ctx->f[MIST_FIELD_NZ], (const double*)ctx->f[MIST_FIELD_PLIC_D],
            (double*)ctx->f[MIST_FIELD_VOF], ctx->counters + 2);
-    prof_end(ctx, MIST_PROF_SL_CUT, ev, ctx->counters + 0, 0);
+    prof_end(ctx, MIST_PROF_SL_GENERAL, ev, ctx->counters + 4, 0);
     int st;
     if ((st = launch_neumann(ctx, MIST_FIELD_VOF))) return st;
     return launch_borders(ctx, MIST_FIELD_VOF);

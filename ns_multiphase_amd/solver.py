@@ -226,7 +226,7 @@ class Context:
         self._check(lib().mist_plic_vf_batch(self.h, _ip(ijk), _dp(nrm), _dp(dist), _dp(out), len(dist)))
         return out
 
-    PROFILE_KERNELS = ("elvira", "sl_cut", "plic", "project", "mg_rbgs", "mg_resid")
+    PROFILE_KERNELS = ("elvira", "sl_cut", "plic", "project", "mg_rbgs", "mg_resid", "sl_general")
 
     def profile(self, on=True):
         self._check(lib().mist_profile_enable(self.h, 1 if on else 0))

@@ -57,3 +57,25 @@ def test_context_without_device_fails_cleanly():
         pytest.skip("a GPU is present")
     with pytest.raises(solver.MistError, match="no HIP device"):
         solver.Context(deformation3d(8)[0])
+
+
+@pytest.mark.parametrize("field,value,message", [
+    ("tesselation", 7, b"incompatible tesselation called"),
+    ("normal_method", 3, b"Unknown method to copute interface normal"),
+    ("projection_method", 5, b"Unknown projection_method in project!"),
+    ("mg_coarse_op", 1, b"only the face-average coarse operator is built"),
+    ("mg_nu1", -1, b"sweep counts must be >= 0"),
+    ("CFL", 0.0, b"CFL must be > 0"),
+    ("Nx", 0, b"Nx, Ny, Nz must be >= 1"),
+])
+def test_parameter_errors_come_before_the_device(field, value, message):
+    # parameters are checked before any HIP call, so the reference's error text comes back
+    # with MIST_ERR_PARAM (1) on any machine, like the reference's error(...) at set-up
+    from ns_multiphase_amd import solver
+    from ns_multiphase_amd.workloads import deformation3d
+    cp = deformation3d(8)[0].to_c()
+    setattr(cp, field, value)
+    h = ctypes.c_void_p()
+    st = solver.lib().mist_ctx_create(ctypes.byref(cp), 0, ctypes.byref(h))
+    assert st == 1
+    assert message in solver.lib().mist_last_error(None)

@@ -203,6 +203,30 @@ def test_pressure_fp64_vs_mixed_same_residual(mist):
     assert np.abs(res["fp64"][2] - res["mixed"][2]).max() <= 1e-6
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_pressure_graph_replays_match_eager(mist, precision):
+    # With profiling on, every kernel of an iteration is launched one by one; with it off,
+    # the iteration is captured once and replayed as a HIP graph (about 9 replays per solve,
+    # each followed by a host synchronisation). Both must give the same bits, on two solves
+    # in a row.
+    N = 32
+    dt = 1e-3
+    p = W.mms_params(N)
+    eager = mist.Context(p)
+    W.mms_setup(eager, dt)
+    P0 = eager.get("P")
+    eager.profile(True)
+    ce, re_ = eager.pressure_solve(dt, 1e-10, precision)
+    Pe = eager.get("P")
+    g = mist.Context(p)
+    W.mms_setup(g, dt)
+    for _ in range(2):
+        g.set("P", P0)
+        cg, rg = g.pressure_solve(dt, 1e-10, precision)
+        assert (cg, rg) == (ce, re_)
+        assert np.array_equal(g.get("P"), Pe)
+
+
 def test_error_paths(mist):
     ctx = mist.Context(ci_param())
     with pytest.raises(mist.MistError, match="size mismatch"):
