@@ -57,13 +57,16 @@ struct mist_ctx {
     // work buffers
     double* pv = nullptr;        // projected cell vertices, (Nx+1)(Ny+1)(Nz+1) x 3
     int* band_list = nullptr;    // interior cells with |band| <= nband (linear cell index)
-    int* iface_list = nullptr;   // interior interface cells (ELVIRA)
-    int* counters = nullptr;     // [0] band cells, [1] interface cells, [2] overflow flag, [4] deferred
-                                 // cells, [5] work-queue position of the cut kernel
+    int* iface_list = nullptr;   // interior interface cells (ELVIRA); cells the main cut pass defers
+    int* defer_list = nullptr;   // cells the wide cut pass defers to the general walk
+    int* counters = nullptr;     // [0] band cells, [1] interface cells, [2] overflow flag, [4] cells
+                                 // deferred by the main cut pass, [5] its work queue, [6] cells
+                                 // deferred by the wide pass, [7] its work queue
     double* red = nullptr;       // reduction partials
     int red_blocks = 0;
     int* h_counters = nullptr;   // pinned
-    int tree_blocks = 0;         // resident blocks of the cut kernel on the device
+    int tree_blocks = 0;         // resident blocks of the main cut pass on the device
+    int tree_cus = 0;            // compute units of the device
     double* h_red = nullptr;     // pinned
     uint64_t vf_version = 1, normals_version = 0, band_version = 0;
     bool band_list_valid = false;

@@ -766,28 +766,36 @@ __global__ __launch_bounds__(64) void k_sl_cut(DevMesh m, const double* pv, cons
 // reference's two sums are identical (or the liquid one is zero) and the new VF is exactly
 // 1 (or 0): such cells are written without cutting.
 // ------------------------------------------------------------------------------------
-constexpr int TREE_PV = 256;      // vertex pool per cell: 8 corners + mesh-cut vertices
-constexpr int TREE_NL = 384;      // nodes per list
-constexpr int TREE_NLEAF = 1152;  // leaves per cell
-constexpr int TREE_THREADS = 64;  // threads per cell
+// Threads per cell and capacities: vertex pool (8 corners + mesh-cut vertices), nodes per
+// list, leaves per cell.
+template <int NT_, int PV_, int NL_, int NLEAF_>
+struct TreeCfg {
+    static constexpr int NT = NT_, PV = PV_, NL = NL_, NLEAF = NLEAF_;
+};
+// Sized from the oracle's counts on the 3D deformation (99.9% of cells need at most ~320
+// pool vertices and ~230 nodes after the y cuts): the main pass takes almost all cells at
+// ~21 KB of LDS per block; the few that overflow it go to a second pass with 256 threads
+// and ~90 KB.
+using TreeMain = TreeCfg<64, 256, 384, 1152>;
+using TreeWide = TreeCfg<256, 1536, 1536, 6144>;
 
 struct TreeNode {
     unsigned w0, w1;  // pool ids of vertices 0,1 and 2,3 (16 bits each)
     unsigned bits;    // cell offset from lo of vertex n along axis q at bit 6n + 2q
 };
 
-template <int NT>
+template <class C>
 struct TreeShared {
-    double px[TREE_PV], py[TREE_PV], pz[TREE_PV];
-    TreeNode node[2][TREE_NL];
-    unsigned short loff[TREE_NL];      // first leaf of each node of the last list
-    unsigned short lnode[TREE_NLEAF];  // node of each leaf
+    double px[C::PV], py[C::PV], pz[C::PV];
+    TreeNode node[2][C::NL];
+    unsigned short loff[C::NL];      // first leaf of each node of the last list
+    unsigned short lnode[C::NLEAF];  // node of each leaf
     double plane[3][5];                // faces lo+0 .. lo+span+1 per axis
     double nb[64][4];                  // nx, ny, nz, D of the cells of the index box
     int ci[8][3];                      // cell index of each projected corner
     int next;                          // cell taken from the work queue
-    int scan[NT / 64];
-    double red[2][NT / 64];
+    int scan[C::NT / 64];
+    double red[2][C::NT / 64];
 };
 
 __device__ __forceinline__ unsigned tn_id(const TreeNode& N, int n) {
@@ -817,12 +825,12 @@ __device__ __forceinline__ T sel4(const T (&a)[4], int n) {
     if (n == 3) r = a3;
     return r;
 }
-template <int NT>
-__device__ __forceinline__ V3 pool_v(const TreeShared<NT>& S, unsigned id) {
+template <class C>
+__device__ __forceinline__ V3 pool_v(const TreeShared<C>& S, unsigned id) {
     return V3{S.px[id], S.py[id], S.pz[id]};
 }
-template <int NT>
-__device__ __forceinline__ double pool_c(const TreeShared<NT>& S, unsigned id, int axis) {
+template <class C>
+__device__ __forceinline__ double pool_c(const TreeShared<C>& S, unsigned id, int axis) {
     return axis == 0 ? S.px[id] : (axis == 1 ? S.py[id] : S.pz[id]);
 }
 
@@ -872,8 +880,9 @@ struct Span3 {
     int s[3];
 };
 
-template <int NT>
-__device__ bool tree_level(TreeShared<NT>& S, int axis, int minspan, int cur, int& n, int& npool, Span3 span) {
+template <class C>
+__device__ bool tree_level(TreeShared<C>& S, int axis, int minspan, int cur, int& n, int& npool, Span3 span) {
+    constexpr int NT = C::NT;
     int nout = 0;
     for (int b = 0; b < n; b += NT) {
         const int e = b + (int)threadIdx.x;
@@ -907,7 +916,7 @@ __device__ bool tree_level(TreeShared<NT>& S, int axis, int minspan, int cur, in
         int tc, tv;
         const int oc = block_scan<NT>(nch, tc, S.scan);
         const int ov = block_scan<NT>(nv, tv, S.scan);
-        if (nout + tc > TREE_NL || npool + tv > TREE_PV) return false;
+        if (nout + tc > C::NL || npool + tv > C::PV) return false;
         if (active && !cut) S.node[cur ^ 1][nout + oc] = N;
         if (cut) {
             const int vb = npool + ov;
@@ -964,11 +973,11 @@ __device__ bool tree_level(TreeShared<NT>& S, int axis, int minspan, int cur, in
                     ids[k] = id;
                     bits |= ob << (6 * k);
                 }
-                TreeNode C;
-                C.w0 = ids[0] | (ids[1] << 16);
-                C.w1 = ids[2] | (ids[3] << 16);
-                C.bits = bits;
-                S.node[cur ^ 1][nout + oc + c] = C;
+                TreeNode ch;
+                ch.w0 = ids[0] | (ids[1] << 16);
+                ch.w1 = ids[2] | (ids[3] << 16);
+                ch.bits = bits;
+                S.node[cur ^ 1][nout + oc + c] = ch;
             }
         }
         nout += tc;
@@ -979,12 +988,13 @@ __device__ bool tree_level(TreeShared<NT>& S, int axis, int minspan, int cur, in
     return true;
 }
 
-template <int NT>
-__global__ __launch_bounds__(NT) void k_sl_tree(DevMesh m, const double* pv, const int* list, const int* count,
+template <class C>
+__global__ __launch_bounds__(C::NT) void k_sl_tree(DevMesh m, const double* pv, const int* list, const int* count,
                                                 const double* nx, const double* ny, const double* nz,
                                                 const double* D, double* VF, int* ovf, int* defer, int* ndefer,
                                                 int* queue) {
-    __shared__ TreeShared<NT> S;
+    constexpr int NT = C::NT;
+    __shared__ TreeShared<C> S;
     const int tid = threadIdx.x;
     const int nb = *count;
     for (;;) {
@@ -1093,7 +1103,7 @@ __global__ __launch_bounds__(NT) void k_sl_tree(DevMesh m, const double* pv, con
                     any |= max(max(o0, o1), max(o2, o3)) - min(min(o0, o1), min(o2, o3)) >= minspan;
                 }
                 if (!__syncthreads_or(any)) break;
-                ok = tree_level<NT>(S, axis, minspan, cur, n, npool, Span3{{span[0], span[1], span[2]}});
+                ok = tree_level<C>(S, axis, minspan, cur, n, npool, Span3{{span[0], span[1], span[2]}});
                 cur ^= 1;
             }
         }
@@ -1119,7 +1129,7 @@ __global__ __launch_bounds__(NT) void k_sl_tree(DevMesh m, const double* pv, con
             }
             int tot;
             const int o = block_scan<NT>(cnt, tot, S.scan);
-            if (nleaf + tot > TREE_NLEAF) {
+            if (nleaf + tot > C::NLEAF) {
                 ok = false;
                 break;
             }
@@ -1584,27 +1594,32 @@ int launch_transport(mist_ctx* ctx, double dt) {
     prof_end(ctx, MIST_PROF_PROJECT, ev, nullptr, nv);
     const int64_t ncap = (int64_t)ctx->m.Nx * ctx->m.Ny * ctx->m.Nz;
     ev = prof_begin(ctx, MIST_PROF_SL_CUT);
-    LAUNCH(k_fill_int, 1, 64, ctx->counters + 4, 2, 0);  // deferred cells, work-queue position
+    LAUNCH(k_fill_int, 1, 64, ctx->counters + 4, 4, 0);  // deferred cells and work queues
     if (!ctx->tree_blocks) {
         int per = 0, dev = 0, cus = 0;
-        MIST_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_sl_tree<TREE_THREADS>, TREE_THREADS, 0));
+        MIST_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_sl_tree<TreeMain>, TreeMain::NT, 0));
         MIST_HIP_TRY(hipGetDevice(&dev));
         MIST_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        ctx->tree_blocks = std::max(1, per) * std::max(1, cus);
+        ctx->tree_cus = std::max(1, cus);
+        ctx->tree_blocks = std::max(1, per) * ctx->tree_cus;
     }
-    LAUNCH(k_sl_tree<TREE_THREADS>, ctx->tree_blocks, TREE_THREADS, ctx->m, (const double*)ctx->pv,
-           (const int*)ctx->band_list, (const int*)(ctx->counters + 0), (const double*)ctx->f[MIST_FIELD_NX],
-           (const double*)ctx->f[MIST_FIELD_NY], (const double*)ctx->f[MIST_FIELD_NZ],
-           (const double*)ctx->f[MIST_FIELD_PLIC_D], (double*)ctx->f[MIST_FIELD_VOF], ctx->counters + 2,
+    const double* nx = (const double*)ctx->f[MIST_FIELD_NX];
+    const double* ny = (const double*)ctx->f[MIST_FIELD_NY];
+    const double* nz = (const double*)ctx->f[MIST_FIELD_NZ];
+    const double* D = (const double*)ctx->f[MIST_FIELD_PLIC_D];
+    double* VF = (double*)ctx->f[MIST_FIELD_VOF];
+    LAUNCH(k_sl_tree<TreeMain>, ctx->tree_blocks, TreeMain::NT, ctx->m, (const double*)ctx->pv,
+           (const int*)ctx->band_list, (const int*)(ctx->counters + 0), nx, ny, nz, D, VF, ctx->counters + 2,
            ctx->iface_list, ctx->counters + 4, ctx->counters + 5);
     prof_end(ctx, MIST_PROF_SL_CUT, ev, ctx->counters + 0, 0);
-    // cells whose corners span more than four cells along an axis, or whose cut tree
-    // outgrows the block's pool: general walk
+    // cells whose cut tree outgrows the main pass: the same walk with larger capacities, then
+    // the general walk for cells whose corners span more than four cells along an axis
     ev = prof_begin(ctx, MIST_PROF_SL_GENERAL);
-    LAUNCH(k_sl_cut, 256, 64, ctx->m, (const double*)ctx->pv, (const int*)ctx->iface_list,
-           (const int*)(ctx->counters + 4), (const double*)ctx->f[MIST_FIELD_NX], (const double*)ctx->f[MIST_FIELD_NY],
-           (const double*)ctx->f[MIST_FIELD_NZ], (const double*)ctx->f[MIST_FIELD_PLIC_D],
-           (double*)ctx->f[MIST_FIELD_VOF], ctx->counters + 2);
+    LAUNCH(k_sl_tree<TreeWide>, ctx->tree_cus, TreeWide::NT, ctx->m, (const double*)ctx->pv,
+           (const int*)ctx->iface_list, (const int*)(ctx->counters + 4), nx, ny, nz, D, VF, ctx->counters + 2,
+           ctx->defer_list, ctx->counters + 6, ctx->counters + 7);
+    LAUNCH(k_sl_cut, 256, 64, ctx->m, (const double*)ctx->pv, (const int*)ctx->defer_list,
+           (const int*)(ctx->counters + 6), nx, ny, nz, D, VF, ctx->counters + 2);
     prof_end(ctx, MIST_PROF_SL_GENERAL, ev, ctx->counters + 4, 0);
     int st;
     if ((st = launch_neumann(ctx, MIST_FIELD_VOF))) return st;
