@@ -310,33 +310,103 @@ struct SmallLevels {
     T* e_top;                  // e of the first level (global, for the prolongation)
 };
 
+// The cells of one colour that a thread relaxes on a level of at most 2 * MG_SMALL_THREADS
+// cells per colour, with their stencils: loaded once per level visit, then every sweep reads
+// only the neighbours' e from LDS.
 template <class T>
-__device__ __forceinline__ void blk_rbgs(const LevelGeom& g, T* const* C, T* e, const T* b, int color) {
+struct CellSet {
+    int cell[2];
+    int nb[2][6];
+    T cf[2][6];
+    T S[2], b[2];
+    bool on[2];
+};
+
+template <class T>
+__device__ __forceinline__ void load_cells(const LevelGeom& g, T* const* C, const T* b, int color, int first,
+                                           int stride, CellSet<T>& cs) {
     const int hx = (g.n[0] + 1) / 2;
     const int tot = hx * g.n[1] * g.n[2];
-    for (int q = threadIdx.x; q < tot; q += MG_SMALL_THREADS) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const int q = first + s * stride;
+        cs.on[s] = false;
+        cs.cell[s] = 0;
+        cs.S[s] = T(1);
+        cs.b[s] = T(0);
+#pragma unroll
+        for (int f = 0; f < 6; ++f) { cs.nb[s][f] = 0; cs.cf[s][f] = T(0); }
+        if (q >= tot) continue;
         const int h = q % hx, row = q / hx;
         const int j = row % g.n[1], k = row / g.n[1];
         const int i = 2 * h + ((color + 1 + j + k) & 1);
         if (i >= g.n[0]) continue;
-        const int64_t cell = i + (int64_t)g.n[0] * (j + (int64_t)g.n[1] * k);
+        const int cell = i + g.n[0] * (j + g.n[1] * k);
         int64_t nb[6];
         T cf[6];
         stencil(g, C, i, j, k, cell, nb, cf);
         const T S = ((((cf[0] + cf[1]) + cf[2]) + cf[3]) + cf[4]) + cf[5];
         if (!(S > T(0))) continue;
-        const T sum = ((((cf[0] * e[nb[0]] + cf[1] * e[nb[1]]) + cf[2] * e[nb[2]]) + cf[3] * e[nb[3]]) +
-                       cf[4] * e[nb[4]]) + cf[5] * e[nb[5]];
-        e[cell] = (sum - b[cell]) / S;
+        cs.on[s] = true;
+        cs.cell[s] = cell;
+        cs.S[s] = S;
+        cs.b[s] = b[cell];
+#pragma unroll
+        for (int f = 0; f < 6; ++f) { cs.nb[s][f] = (int)nb[f]; cs.cf[s][f] = cf[f]; }
     }
-    __syncthreads();
 }
 
 template <class T>
+__device__ __forceinline__ void relax_cells(T* e, const CellSet<T>& cs) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        if (!cs.on[s]) continue;
+        const T* cf = cs.cf[s];
+        const int* nb = cs.nb[s];
+        const T sum = ((((cf[0] * e[nb[0]] + cf[1] * e[nb[1]]) + cf[2] * e[nb[2]]) + cf[3] * e[nb[3]]) +
+                       cf[4] * e[nb[4]]) + cf[5] * e[nb[5]];
+        e[cs.cell[s]] = (sum - cs.b[s]) / cs.S[s];
+    }
+}
+
+// Orders the LDS accesses of one wave (its lanes run in lockstep; the fences keep the
+// compiler from moving loads above the previous phase's stores).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// `sweeps` red-black sweeps of a level in LDS. Levels of at most 64 cells run on wave 0 alone
+// with wave-level ordering; larger ones (at most 2 * MG_SMALL_THREADS cells per colour) on the
+// whole block with a barrier per colour.
+template <class T>
 __device__ __forceinline__ void blk_smooth(const LevelGeom& g, T* const* C, T* e, const T* b, int sweeps) {
+    if (sweeps <= 0) return;
+    const int tid = threadIdx.x;
+    if (g.size() <= 64) {
+        if (tid < 64) {
+            CellSet<T> red, black;
+            load_cells(g, C, b, 0, tid, 64, red);
+            load_cells(g, C, b, 1, tid, 64, black);
+            for (int sw = 0; sw < sweeps; ++sw) {
+                relax_cells(e, red);
+                wave_sync();
+                relax_cells(e, black);
+                wave_sync();
+            }
+        }
+        __syncthreads();
+        return;
+    }
+    CellSet<T> red, black;
+    load_cells(g, C, b, 0, tid, MG_SMALL_THREADS, red);
+    load_cells(g, C, b, 1, tid, MG_SMALL_THREADS, black);
     for (int sw = 0; sw < sweeps; ++sw) {
-        blk_rbgs(g, C, e, b, 0);
-        blk_rbgs(g, C, e, b, 1);
+        relax_cells(e, red);
+        __syncthreads();
+        relax_cells(e, black);
+        __syncthreads();
     }
 }
 
@@ -497,12 +567,13 @@ static int build_small(mist_ctx* ctx, MGState* s, std::vector<LevelBuf<T>>& L, i
     const int nlev = (int)s->g.size();
     int first = nlev - 1;
     size_t bytes = 3 * sizeof(T) * (size_t)s->g[first].size();
-    while (first > 0 && nlev - first < MG_SMALL_MAXLEV &&
+    const int64_t maxcells = 4 * MG_SMALL_THREADS;  // two cells per thread and colour
+    while (first > 0 && nlev - first < MG_SMALL_MAXLEV && s->g[first - 1].size() <= maxcells &&
            bytes + 3 * sizeof(T) * (size_t)s->g[first - 1].size() <= MG_SMALL_LDS) {
         --first;
         bytes += 3 * sizeof(T) * (size_t)s->g[first].size();
     }
-    if (bytes > MG_SMALL_LDS) { ctx->err = "multigrid: coarsest level too large for one work-group (raise mg_lvl)"; return MIST_ERR_PARAM; }
+    if (bytes > MG_SMALL_LDS || s->g[first].size() > maxcells) { ctx->err = "multigrid: coarsest level too large for one work-group (raise mg_lvl)"; return MIST_ERR_PARAM; }
     SmallLevels<T> h{};
     h.count = nlev - first;
     h.nu1 = (int)ctx->p.mg_nu1;
