@@ -206,7 +206,7 @@ def main():
                      "linf_err_vs_exact": r["linf_err"]}
             if rb_n:
                 avg = rb_ms / rb_n / 1e3
-                bytes_launch = 4.5 * s * rb_units / rb_n
+                bytes_launch = 6.0 * s * rb_units / rb_n  # p read + write, b, 3 face coefficients
                 entry["rbgs_fine"] = {"avg_launch_us": 1e6 * avg, "achieved_GBs": bytes_launch / avg / 1e9,
                                       "frac_hbm": bytes_launch / avg / 1e9 / PEAK_HBM_GBS,
                                       "bytes_per_launch": bytes_launch}

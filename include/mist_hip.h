@@ -186,7 +186,7 @@ enum { MIST_PROF_ELVIRA = 0,     /* units: interface cells scored */
        MIST_PROF_SL_CUT = 1,     /* units: band cells transported */
        MIST_PROF_PLIC = 2,       /* units: cells (incl. ghosts) */
        MIST_PROF_PROJECT = 3,    /* units: cell vertices visited */
-       MIST_PROF_MG_RBGS = 4,    /* units: cells relaxed (one colour sweep = half a level) */
+       MIST_PROF_MG_RBGS = 4,    /* units: cells of the finest level per red-black sweep */
        MIST_PROF_MG_RESID = 5,   /* units: cells */
        MIST_PROF_SL_GENERAL = 6, /* units: band cells left to the general cut walk */
        MIST_PROF_COUNT = 7 };

@@ -11,7 +11,10 @@
 // max|b - A p| <= tol; the solution is shifted to zero mean.
 //
 // Storage: interior cells only (n_x n_y n_z per level, x fastest); neighbours across a wall
-// have a zero coefficient and read the cell itself, so no ghost layers are needed.
+// have a zero coefficient and read the cell itself, so no ghost layers are needed. The levels
+// relaxed by per-level launches store each x-row as its red cells then its black cells, so a
+// one-colour sweep reads and writes unit-stride vectors; the coarse end (one work-group, in
+// LDS) and the FP64 outer arrays use the plain order.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -24,8 +27,31 @@ struct LevelGeom {
     int n[3];
     int per[3];
     int coarsened[3];  // coarser than the level above along axis d
+    int split;         // rows stored red half then black half (n[0] even)
     __host__ __device__ int64_t size() const { return (int64_t)n[0] * n[1] * n[2]; }
 };
+
+// storage index of cell (i, j, k); red = (i+1)+(j+1)+(k+1) even = i+j+k odd (second half)
+__host__ __device__ __forceinline__ int64_t lidx(const LevelGeom& g, int i, int j, int k) {
+    const int64_t row = (int64_t)g.n[0] * (j + (int64_t)g.n[1] * k);
+    if (!g.split) return row + i;
+    return row + (((i + j + k) & 1) ? (g.n[0] >> 1) : 0) + (i >> 1);
+}
+
+// cell (i, j, k) stored at index q
+__device__ __forceinline__ void lcell(const LevelGeom& g, int64_t q, int& i, int& j, int& k) {
+    const int64_t row = q / g.n[0];
+    const int r = (int)(q - row * g.n[0]);
+    j = (int)(row % g.n[1]);
+    k = (int)(row / g.n[1]);
+    if (!g.split) {
+        i = r;
+        return;
+    }
+    const int hn = g.n[0] >> 1;
+    const int half = r >= hn ? 1 : 0;
+    i = 2 * (r - half * hn) + ((half + j + k) & 1);
+}
 
 template <class T>
 struct LevelBuf {
@@ -36,7 +62,8 @@ struct LevelBuf {
 };
 
 struct MGState {
-    std::vector<LevelGeom> g;
+    std::vector<LevelGeom> g;         // plain order (the FP64 outer arrays use g[0])
+    std::vector<LevelGeom> gq[2];     // storage of the FP64 and FP32 hierarchies
     std::vector<LevelBuf<double>> d;  // FP64 hierarchy
     std::vector<LevelBuf<float>> s;   // FP32 hierarchy
     double* c64[3] = {nullptr, nullptr, nullptr};  // fine FP64 operator
@@ -67,35 +94,40 @@ void mg_free(MGState* s) {
 }
 
 // ---- kernels ---------------------------------------------------------------------------
-struct Nb {
-    int64_t id[6];
-};
-
-template <class T>
+// Neighbours (storage indices) and face coefficients of cell (i, j, k): faces x-, x+, y-, y+,
+// z-, z+; C[d][c] is the face between c and its lower neighbour along d. A neighbour across a
+// wall is the cell itself with coefficient 0; across a periodic side, the wrapped cell.
+template <class T, bool MAY_SPLIT = true>
 __device__ __forceinline__ void stencil(const LevelGeom& g, T* const* C, int i, int j, int k, int64_t cell, int64_t nb[6],
                                        T cf[6]) {
+    auto lidx = [&](int a, int b, int c) -> int64_t {
+        if constexpr (MAY_SPLIT) return ::mist::lidx(g, a, b, c);
+        return a + (int64_t)g.n[0] * (b + (int64_t)g.n[1] * c);
+    };
     const int q[3] = {i, j, k};
-    const int64_t st[3] = {1, (int64_t)g.n[0], (int64_t)g.n[0] * g.n[1]};
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         const int nd = g.n[d];
+        int up[3] = {i, j, k}, dn[3] = {i, j, k};
         cf[2 * d] = C[d][cell];
-        if (q[d] + 1 < nd) {
-            nb[2 * d + 1] = cell + st[d];
-            cf[2 * d + 1] = C[d][cell + st[d]];
-        } else if (g.per[d]) {
-            nb[2 * d + 1] = cell - (int64_t)(nd - 1) * st[d];
+        if (q[d] + 1 < nd || g.per[d]) {
+            up[d] = q[d] + 1 < nd ? q[d] + 1 : 0;
+            nb[2 * d + 1] = lidx(up[0], up[1], up[2]);
             cf[2 * d + 1] = C[d][nb[2 * d + 1]];
         } else {
             nb[2 * d + 1] = cell;
             cf[2 * d + 1] = T(0);
         }
-        if (q[d] > 0) nb[2 * d] = cell - st[d];
-        else if (g.per[d]) nb[2 * d] = cell + (int64_t)(nd - 1) * st[d];
-        else nb[2 * d] = cell;
+        if (q[d] > 0 || g.per[d]) {
+            dn[d] = q[d] > 0 ? q[d] - 1 : nd - 1;
+            nb[2 * d] = lidx(dn[0], dn[1], dn[2]);
+        } else {
+            nb[2 * d] = cell;
+        }
     }
 }
 
+// One colour of a red-black Gauss-Seidel sweep (rbgs_update!, src/Pressure.jl:1096-1133)
 template <class T>
 __global__ void k_rbgs(LevelGeom g, T* cx, T* cy, T* cz, T* e, const T* b, int color) {
     T* C[3] = {cx, cy, cz};
@@ -108,7 +140,7 @@ __global__ void k_rbgs(LevelGeom g, T* cx, T* cy, T* cz, T* e, const T* b, int c
         const int k = (int)(row / g.n[1]);
         const int i = 2 * h + ((color + 1 + j + k) & 1);  // ((i+1)+(j+1)+(k+1)) % 2 == color
         if (i >= g.n[0]) continue;
-        const int64_t cell = i + (int64_t)g.n[0] * (j + (int64_t)g.n[1] * k);
+        const int64_t cell = lidx(g, i, j, k);
         int64_t nb[6];
         T cf[6];
         stencil(g, C, i, j, k, cell, nb, cf);
@@ -120,15 +152,53 @@ __global__ void k_rbgs(LevelGeom g, T* cx, T* cy, T* cz, T* e, const T* b, int c
     }
 }
 
+// k_rbgs on a split level: thread (h, j, k) relaxes the cell of `color` at position h of row
+// (j, k), i = 2h + p. Its x-neighbours sit at h-1+p and h+p of the other half of the same row,
+// its y- and z-neighbours at h of the other half of the adjacent rows, so every load and the
+// store are unit-stride across the wave; no index divisions.
+template <class T>
+__global__ __launch_bounds__(256) void k_rbgs_split(LevelGeom g, const T* __restrict__ cx, const T* __restrict__ cy,
+                                                    const T* __restrict__ cz, T* e, const T* __restrict__ b,
+                                                    int color) {
+    const int hn = g.n[0] >> 1;
+    const int h = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int k = blockIdx.z;
+    if (h >= hn || j >= g.n[1]) return;
+    const int p = (color + 1 + j + k) & 1;
+    const int i = 2 * h + p;
+    const int64_t sy = g.n[0], sz = (int64_t)g.n[0] * g.n[1];
+    const int64_t row = sy * j + sz * k;
+    const int own_off = color == 0 ? hn : 0, oth_off = hn - own_off;
+    const int64_t cell = row + own_off + h;
+    const int64_t oth = row + oth_off;
+    // neighbours (storage index) and whether the face is open (not a wall)
+    int64_t nxm = cell, nxp = cell, nym = cell, nyp = cell, nzm = cell, nzp = cell;
+    bool oxm = true, oxp = true, oym = true, oyp = true, ozm = true, ozp = true;
+    if (i > 0) nxm = oth + h - 1 + p; else if (g.per[0]) nxm = oth + hn - 1; else oxm = false;
+    if (i < g.n[0] - 1) nxp = oth + h + p; else if (g.per[0]) nxp = oth; else oxp = false;
+    if (j > 0) nym = oth - sy + h; else if (g.per[1]) nym = oth + sy * (g.n[1] - 1) + h; else oym = false;
+    if (j < g.n[1] - 1) nyp = oth + sy + h; else if (g.per[1]) nyp = oth - sy * (g.n[1] - 1) + h; else oyp = false;
+    if (k > 0) nzm = oth - sz + h; else if (g.per[2]) nzm = oth + sz * (g.n[2] - 1) + h; else ozm = false;
+    if (k < g.n[2] - 1) nzp = oth + sz + h; else if (g.per[2]) nzp = oth - sz * (g.n[2] - 1) + h; else ozp = false;
+    (void)oxm; (void)oym; (void)ozm;
+    const T c0 = cx[cell], c1 = oxp ? cx[nxp] : T(0);
+    const T c2 = cy[cell], c3 = oyp ? cy[nyp] : T(0);
+    const T c4 = cz[cell], c5 = ozp ? cz[nzp] : T(0);
+    const T S = ((((c0 + c1) + c2) + c3) + c4) + c5;
+    if (!(S > T(0))) return;
+    const T sum = ((((c0 * e[nxm] + c1 * e[nxp]) + c2 * e[nym]) + c3 * e[nyp]) + c4 * e[nzm]) + c5 * e[nzp];
+    e[cell] = (sum - b[cell]) / S;
+}
+
 template <class T>
 __global__ void k_resid(LevelGeom g, T* cx, T* cy, T* cz, const T* e, const T* b, T* r) {
     T* C[3] = {cx, cy, cz};
     const int64_t tot = g.size();
     for (int64_t cell = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; cell < tot;
          cell += (int64_t)gridDim.x * blockDim.x) {
-        const int i = (int)(cell % g.n[0]);
-        const int j = (int)((cell / g.n[0]) % g.n[1]);
-        const int k = (int)(cell / ((int64_t)g.n[0] * g.n[1]));
+        int i, j, k;
+        lcell(g, cell, i, j, k);
         int64_t nb[6];
         T cf[6];
         stencil(g, C, i, j, k, cell, nb, cf);
@@ -145,14 +215,12 @@ __global__ void k_restrict(LevelGeom F, LevelGeom C, const T* rf, T* bc) {
     const T inv = T(1) / T(f0 * f1 * f2);
     const int64_t tot = C.size();
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
-        const int I = (int)(q % C.n[0]);
-        const int J = (int)((q / C.n[0]) % C.n[1]);
-        const int K = (int)(q / ((int64_t)C.n[0] * C.n[1]));
+        int I, J, K;
+        lcell(C, q, I, J, K);
         T s = T(0);
         for (int c = 0; c < f2; ++c)
             for (int b = 0; b < f1; ++b)
-                for (int a = 0; a < f0; ++a)
-                    s += rf[(I * f0 + a) + (int64_t)F.n[0] * ((J * f1 + b) + (int64_t)F.n[1] * (K * f2 + c))];
+                for (int a = 0; a < f0; ++a) s += rf[lidx(F, I * f0 + a, J * f1 + b, K * f2 + c)];
         bc[q] = s * inv;
     }
 }
@@ -161,7 +229,8 @@ template <class T>
 __global__ void k_prolong_add(LevelGeom C, LevelGeom F, const T* ec, T* ef) {
     const int64_t tot = F.size();
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
-        const int qq[3] = {(int)(q % F.n[0]), (int)((q / F.n[0]) % F.n[1]), (int)(q / ((int64_t)F.n[0] * F.n[1]))};
+        int qq[3];
+        lcell(F, q, qq[0], qq[1], qq[2]);
         int lo[3], hi[3];
         T w1[3], w2[3];
 #pragma unroll
@@ -173,7 +242,7 @@ __global__ void k_prolong_add(LevelGeom C, LevelGeom F, const T* ec, T* ef) {
             if (lo[d] < 0) lo[d] = C.per[d] ? C.n[d] - 1 : 0;
             if (hi[d] >= C.n[d]) hi[d] = C.per[d] ? 0 : C.n[d] - 1;
         }
-        auto E = [&](int a, int b, int c) { return ec[a + (int64_t)C.n[0] * (b + (int64_t)C.n[1] * c)]; };
+        auto E = [&](int a, int b, int c) { return ec[lidx(C, a, b, c)]; };
         const T v = w1[2] * (w1[1] * (w1[0] * E(hi[0], hi[1], hi[2]) + w2[0] * E(lo[0], hi[1], hi[2])) +
                              w2[1] * (w1[0] * E(hi[0], lo[1], hi[2]) + w2[0] * E(lo[0], lo[1], hi[2]))) +
                     w2[2] * (w1[1] * (w1[0] * E(hi[0], hi[1], lo[2]) + w2[0] * E(lo[0], hi[1], lo[2])) +
@@ -189,7 +258,8 @@ __global__ void k_coarsen_coef(LevelGeom F, LevelGeom C, int d, const T* cf, T* 
     const T scale = T(1) / T(f[d] * f[d]) / T(f[d1] * f[d2]);
     const int64_t tot = C.size();
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
-        const int Q[3] = {(int)(q % C.n[0]), (int)((q / C.n[0]) % C.n[1]), (int)(q / ((int64_t)C.n[0] * C.n[1]))};
+        int Q[3];
+        lcell(C, q, Q[0], Q[1], Q[2]);
         T s = T(0);
         for (int b = 0; b < f[d2]; ++b)
             for (int a = 0; a < f[d1]; ++a) {
@@ -197,7 +267,7 @@ __global__ void k_coarsen_coef(LevelGeom F, LevelGeom C, int d, const T* cf, T* 
                 x[d] = Q[d] * f[d];
                 x[d1] = Q[d1] * f[d1] + a;
                 x[d2] = Q[d2] * f[d2] + b;
-                s += cf[x[0] + (int64_t)F.n[0] * (x[1] + (int64_t)F.n[1] * x[2])];
+                s += cf[lidx(F, x[0], x[1], x[2])];
             }
         cc[q] = s * scale;
     }
@@ -225,25 +295,38 @@ __global__ void k_fine_setup(DevMesh m, LevelGeom g, double dt, const double* de
     }
 }
 
-template <class Ts, class Td>
-__global__ void k_convert(const Ts* a, Td* b, int64_t n) {
-    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
-        b[q] = (Td)a[q];
+// level storage of g from a plain-order FP64 array
+template <class T>
+__global__ void k_to_level(LevelGeom g, const double* a, T* b) {
+    const int64_t n = g.size();
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        int i, j, k;
+        lcell(g, q, i, j, k);
+        b[q] = (T)a[i + (int64_t)g.n[0] * (j + (int64_t)g.n[1] * k)];
+    }
 }
 
-// b_T = (T) r64, e_T = 0
+// b_T = (T) r64, e_T = 0 (r64 in plain order, b and e in the storage of g)
 template <class T>
-__global__ void k_start_cycle(const double* r64, T* b, T* e, int64_t n) {
+__global__ void k_start_cycle(LevelGeom g, const double* r64, T* b, T* e) {
+    const int64_t n = g.size();
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
-        b[q] = (T)r64[q];
+        int i, j, k;
+        lcell(g, q, i, j, k);
+        b[q] = (T)r64[i + (int64_t)g.n[0] * (j + (int64_t)g.n[1] * k)];
         e[q] = T(0);
     }
 }
 
+// p64 += e (p64 in plain order, e in the storage of g)
 template <class T>
-__global__ void k_add_correction(const T* e, double* p, int64_t n) {
-    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
-        p[q] += (double)e[q];
+__global__ void k_add_correction(LevelGeom g, const T* e, double* p) {
+    const int64_t n = g.size();
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        int i, j, k;
+        lcell(g, q, i, j, k);
+        p[i + (int64_t)g.n[0] * (j + (int64_t)g.n[1] * k)] += (double)e[q];
+    }
 }
 
 template <class T>
@@ -344,7 +427,7 @@ __device__ __forceinline__ void load_cells(const LevelGeom& g, T* const* C, cons
         const int cell = i + g.n[0] * (j + g.n[1] * k);
         int64_t nb[6];
         T cf[6];
-        stencil(g, C, i, j, k, cell, nb, cf);
+        stencil<T, false>(g, C, i, j, k, cell, nb, cf);
         const T S = ((((cf[0] + cf[1]) + cf[2]) + cf[3]) + cf[4]) + cf[5];
         if (!(S > T(0))) continue;
         cs.on[s] = true;
@@ -439,7 +522,7 @@ __global__ __launch_bounds__(MG_SMALL_THREADS) void k_small_vcycle(const SmallLe
             const int i = cell % g.n[0], j = (cell / g.n[0]) % g.n[1], k = cell / (g.n[0] * g.n[1]);
             int64_t nb[6];
             T cf[6];
-            stencil(g, C, i, j, k, cell, nb, cf);
+            stencil<T, false>(g, C, i, j, k, cell, nb, cf);
             const T pc = e[cell];
             const T Ap = ((((cf[0] * (e[nb[0]] - pc) + cf[1] * (e[nb[1]] - pc)) + cf[2] * (e[nb[2]] - pc)) +
                            cf[3] * (e[nb[3]] - pc)) + cf[4] * (e[nb[4]] - pc)) + cf[5] * (e[nb[5]] - pc);
@@ -642,6 +725,10 @@ static int build_state(mist_ctx* ctx) {
         mg_free(s);
         return st;
     }
+    for (int q = 0; q < 2; ++q) {
+        s->gq[q] = s->g;
+        for (int l = 0; l < s->first_small[q]; ++l) s->gq[q][l].split = s->g[l].n[0] % 2 == 0;
+    }
     s->nfine = g0.size();
     const size_t fb = (size_t)s->nfine * sizeof(double);
     double** f64[6] = {&s->c64[0], &s->c64[1], &s->c64[2], &s->p64, &s->b64, &s->r64};
@@ -667,15 +754,35 @@ static int dev_reduce(mist_ctx* ctx, const T* a, int64_t n, int maxabs) {
     return MIST_OK;
 }
 
+// `sweeps` red-black sweeps on level g (one launch per colour, red first)
+template <class T>
+static int smooth(mist_ctx* ctx, const LevelGeom& g, LevelBuf<T>& B, int sweeps, bool prof) {
+    const int64_t nh = (int64_t)((g.n[0] + 1) / 2) * g.n[1] * g.n[2];
+    const dim3 grid((g.n[0] / 2 + 63) / 64, (g.n[1] + 3) / 4, g.n[2]);
+    for (int sw = 0; sw < sweeps; ++sw) {
+        hipEvent_t ev = prof ? prof_begin(ctx, MIST_PROF_MG_RBGS) : nullptr;
+        for (int c = 0; c < 2; ++c) {
+            if (g.split) {
+                hipLaunchKernelGGL(k_rbgs_split<T>, grid, dim3(64, 4), 0, ctx->stream, g, (const T*)B.c[0],
+                                   (const T*)B.c[1], (const T*)B.c[2], B.e, (const T*)B.b, c);
+                MIST_HIP_TRY(hipGetLastError());
+            } else {
+                MGL(k_rbgs<T>, gridn(nh), 256, g, B.c[0], B.c[1], B.c[2], B.e, (const T*)B.b, c);
+            }
+        }
+        prof_end(ctx, MIST_PROF_MG_RBGS, ev, nullptr, g.size());
+    }
+    return MIST_OK;
+}
+
 template <class T>
 static int vcycle(mist_ctx* ctx, std::vector<LevelBuf<T>>& L, size_t l) {
     MGState* s = ctx->mg;
-    const LevelGeom& g = s->g[l];
+    const int q = sizeof(T) == sizeof(double) ? 0 : 1;
+    const LevelGeom& g = s->gq[q][l];
     LevelBuf<T>& B = L[l];
     const int nu1 = (int)ctx->p.mg_nu1, nu2 = (int)ctx->p.mg_nu2;
     const int64_t n = g.size();
-    const int64_t nh = (int64_t)((g.n[0] + 1) / 2) * g.n[1] * g.n[2];
-    const int q = sizeof(T) == sizeof(double) ? 0 : 1;
     if ((int)l == s->first_small[q]) {
         hipLaunchKernelGGL(k_small_vcycle<T>, dim3(1), dim3(MG_SMALL_THREADS), s->small_lds[q], ctx->stream,
                            (const SmallLevels<T>*)s->small_desc[q]);
@@ -683,34 +790,24 @@ static int vcycle(mist_ctx* ctx, std::vector<LevelBuf<T>>& L, size_t l) {
         return MIST_OK;
     }
     if (l > 0) MGL(k_zero<T>, gridn(n), 256, B.e, n);
-    for (int sw = 0; sw < nu1; ++sw)
-        for (int c = 0; c < 2; ++c) {
-            hipEvent_t ev = l == 0 ? prof_begin(ctx, MIST_PROF_MG_RBGS) : nullptr;
-            MGL(k_rbgs<T>, gridn(nh), 256, g, B.c[0], B.c[1], B.c[2], B.e, (const T*)B.b, c);
-            prof_end(ctx, MIST_PROF_MG_RBGS, ev, nullptr, n);
-        }
+    int st;
+    if ((st = smooth(ctx, g, B, nu1, l == 0))) return st;
     MGL(k_resid<T>, gridn(n), 256, g, B.c[0], B.c[1], B.c[2], (const T*)B.e, (const T*)B.b, B.r);
-    const LevelGeom& gc = s->g[l + 1];
+    const LevelGeom& gc = s->gq[q][l + 1];
     MGL(k_restrict<T>, gridn(gc.size()), 256, g, gc, (const T*)B.r, L[l + 1].b);
-    int st = vcycle(ctx, L, l + 1);
-    if (st) return st;
+    if ((st = vcycle(ctx, L, l + 1))) return st;
     MGL(k_prolong_add<T>, gridn(n), 256, gc, g, (const T*)L[l + 1].e, B.e);
-    for (int sw = 0; sw < nu2; ++sw)
-        for (int c = 0; c < 2; ++c) {
-            hipEvent_t ev = l == 0 ? prof_begin(ctx, MIST_PROF_MG_RBGS) : nullptr;
-            MGL(k_rbgs<T>, gridn(nh), 256, g, B.c[0], B.c[1], B.c[2], B.e, (const T*)B.b, c);
-            prof_end(ctx, MIST_PROF_MG_RBGS, ev, nullptr, n);
-        }
-    return MIST_OK;
+    return smooth(ctx, g, B, nu2, l == 0);
 }
 
 template <class T>
 static int setup_hierarchy(mist_ctx* ctx, std::vector<LevelBuf<T>>& L) {
     MGState* s = ctx->mg;
-    for (int d = 0; d < 3; ++d) MGL((k_convert<double, T>), gridn(s->nfine), 256, (const double*)s->c64[d], L[0].c[d], s->nfine);
+    const std::vector<LevelGeom>& G = s->gq[sizeof(T) == sizeof(double) ? 0 : 1];
+    for (int d = 0; d < 3; ++d) MGL(k_to_level<T>, gridn(s->nfine), 256, G[0], (const double*)s->c64[d], L[0].c[d]);
     for (size_t l = 1; l < L.size(); ++l)
         for (int d = 0; d < 3; ++d)
-            MGL(k_coarsen_coef<T>, gridn(s->g[l].size()), 256, s->g[l - 1], s->g[l], d, (const T*)L[l - 1].c[d], L[l].c[d]);
+            MGL(k_coarsen_coef<T>, gridn(G[l].size()), 256, G[l - 1], G[l], d, (const T*)L[l - 1].c[d], L[l].c[d]);
     return MIST_OK;
 }
 
@@ -730,10 +827,11 @@ template <class T>
 static int outer_iteration(mist_ctx* ctx, std::vector<LevelBuf<T>>& L) {
     MGState* s = ctx->mg;
     const int64_t n = s->nfine;
-    MGL(k_start_cycle<T>, gridn(n), 256, (const double*)s->r64, L[0].b, L[0].e, n);
+    const LevelGeom& g0 = s->gq[sizeof(T) == sizeof(double) ? 0 : 1][0];
+    MGL(k_start_cycle<T>, gridn(n), 256, g0, (const double*)s->r64, L[0].b, L[0].e);
     int st = vcycle(ctx, L, 0);
     if (st) return st;
-    MGL(k_add_correction<T>, gridn(n), 256, (const T*)L[0].e, s->p64, n);
+    MGL(k_add_correction<T>, gridn(n), 256, g0, (const T*)L[0].e, s->p64);
     return outer_residual(ctx);
 }
 
