@@ -351,24 +351,39 @@ __global__ void k_normals_prep(DevMesh m, DevParams p, const double* VF, double*
 }
 
 // ELVIRA_calc! + check_normal! (src/ELVIRA.jl:62-137) with one interface cell per work-group
-// of 256 threads. Threads 0..26 load the
-// 3x3x3 liquid fractions and build the 27 candidate normals and their PLIC constants once
-// (computeDist); then half-wave h scores every candidate on neighbours h, h+8, h+16, h+24,
-// writing (VF - VF_)^2 to LDS; the first half-wave sums each candidate's 27 terms in the
-// reference's neighbour order and takes the lowest-numbered minimum. Neighbours that every
-// candidate plane misses by a wide margin take the exact all-liquid / all-gas value.
+// of 256 threads. Threads 0..26 load the 3x3x3 liquid fractions and build the 27 candidate
+// normals and their PLIC constants once (computeDist). Each candidate's error is the sum over
+// the 27 neighbours, in the reference's order, of (VF - computePLIC2VF)^2, where
+// computePLIC2VF adds, tet by tet, the liquid sub-tet volumes of the cut (src/VF.jl:219-278).
+//
+// The neighbours are scored a z-plane (9 neighbours, 243 candidate-neighbour pairs) at a
+// time. (A) one thread per pair finds the d2case case of its five tets; a plane missing the
+// cell by a wide margin gives the exact all-gas / all-liquid value at once, and an all-liquid
+// tet's single sub-tet is its whole volume, computed once per neighbour. (B) the other tets
+// are bucketed by case, so (C) the threads that cut them run mostly the same case in a wave.
+// Every sub-tet term lands in its own slot; (D) one thread per pair adds the terms in the
+// reference's tet and sub-tet order, and one thread per candidate adds the squared errors in
+// neighbour order. The best candidate is the lowest-numbered one with the smallest finite
+// error, which is what the reference's strict `<` scan keeps.
 constexpr int ELV_THREADS = 256;
+constexpr int ELV_PAIRS = 27 * 9;  // candidate-neighbour pairs of one z-plane of neighbours
 
 __global__ __launch_bounds__(ELV_THREADS) void k_elvira(DevMesh m, DevParams p, const double* VF, const int* list,
-                                                         const int* count, int fam_mask, double* nxo, double* nyo,
-                                                         double* nzo) {
+                                                        const int* count, int fam_mask, double* nxo, double* nyo,
+                                                        double* nzo) {
     __shared__ double sVF[27];
-    __shared__ double sFull[27];
     __shared__ double sC[27][4];
     __shared__ int sOk[27];
-    __shared__ double sE[27][28];
+    __shared__ double sT[27][5];      // all-liquid term (ts * vol(tet)) / cellvol per neighbour and tet
+    __shared__ double sFull[27];      // computePLIC2VF of an all-liquid neighbour
+    __shared__ double sTerm[ELV_PAIRS][5][3];
+    __shared__ unsigned char sCase[ELV_PAIRS][5];
+    __shared__ unsigned char sKind[ELV_PAIRS];  // 0: cut tets, 1: all gas, 2: all liquid
+    __shared__ unsigned short sItem[ELV_PAIRS * 5];
+    __shared__ int sCount[16], sBase[16];
+    __shared__ double sE[ELV_PAIRS];
+    __shared__ double sErr[27];
     const int tid = threadIdx.x;
-    const int half = tid >> 5, lane = tid & 31;
     const int ncell = *count;
     const double cellvol = (m.dx * m.dy) * m.dz;
     for (int cidx = blockIdx.x; cidx < ncell; cidx += gridDim.x) {
@@ -377,10 +392,22 @@ __global__ __launch_bounds__(ELV_THREADS) void k_elvira(DevMesh m, DevParams p, 
         if (tid < 27) {
             const int ii = i - 1 + tid % 3, jj = j - 1 + (tid / 3) % 3, kk = k - 1 + tid / 9;
             sVF[tid] = VF[m.vof(ii, jj, kk)];
-            sFull[tid] = plic2vf_full(m.x[ii], m.x[ii + 1], m.y[jj], m.y[jj + 1], m.z[kk], m.z[kk + 1],
-                                      ((ii + jj + kk) % 2) == 0, cellvol);
+            sErr[tid] = 0.0;
+        }
+        if (tid < 27 * 5) {
+            const int nb = tid / 5, t = tid % 5;
+            const int ii = i - 1 + nb % 3, jj = j - 1 + (nb / 3) % 3, kk = k - 1 + nb / 9;
+            const bool even = ((ii + jj + kk) % 2) == 0;
+            V3 P[4];
+            tet5_vertices(t, even, m.x[ii], m.x[ii + 1], m.y[jj], m.y[jj + 1], m.z[kk], m.z[kk + 1], P);
+            sT[nb][t] = ((even ? 1.0 : -1.0) * tet_vol3(P[0], P[1], P[2], P[3])) / cellvol;
         }
         __syncthreads();
+        if (tid < 27) {
+            double f = 0.0;
+            for (int t = 0; t < 5; ++t) f += sT[tid][t];
+            sFull[tid] = f;
+        }
         if (tid < 27) {
             const int fam = tid / 9, a = (tid % 9) / 3, b = tid % 3;
             int ok = (fam_mask >> fam) & 1;
@@ -429,36 +456,112 @@ __global__ __launch_bounds__(ELV_THREADS) void k_elvira(DevMesh m, DevParams p, 
             sOk[tid] = ok;
         }
         __syncthreads();
-        if (lane < 27 && sOk[lane]) {
-            const double cnx = sC[lane][0], cny = sC[lane][1], cnz = sC[lane][2], dist = sC[lane][3];
-            const double hx = fabs(cnx) * m.dx, hy = fabs(cny) * m.dy, hz = fabs(cnz) * m.dz;
-            for (int nb = half; nb < 27; nb += 8) {
+        for (int plane = 0; plane < 3; ++plane) {
+            if (tid < 16) sCount[tid] = 0;
+            __syncthreads();
+            // (A) classify the five tets of each pair
+            if (tid < ELV_PAIRS) {
+                const int c = tid % 27, nb = 9 * plane + tid / 27;
+                int kind = 1;
+                if (sOk[c]) {
+                    const double cnx = sC[c][0], cny = sC[c][1], cnz = sC[c][2], dist = sC[c][3];
+                    const int ii = i - 1 + nb % 3, jj = j - 1 + (nb / 3) % 3, kk = k - 1 + nb / 9;
+                    const double hx = fabs(cnx) * m.dx, hy = fabs(cny) * m.dy, hz = fabs(cnz) * m.dz;
+                    const double dcen = ((cnx * m.xm[ii] + cny * m.ym[jj]) + cnz * m.zm[kk]) - dist;
+                    const double hext = 0.5 * ((hx + hy) + hz);
+                    const double margin =
+                        1e-9 * (((1.0 + fabs(dist)) + fabs(m.xm[ii])) + (fabs(m.ym[jj]) + fabs(m.zm[kk])));
+                    if (dcen - hext > margin) {
+                        kind = 1;
+                    } else if (dcen + hext < -margin) {
+                        kind = 2;
+                    } else {
+                        kind = 0;
+                        const bool even = ((ii + jj + kk) % 2) == 0;
+                        for (int t = 0; t < 5; ++t) {
+                            V3 P[4];
+                            tet5_vertices(t, even, m.x[ii], m.x[ii + 1], m.y[jj], m.y[jj + 1], m.z[kk], m.z[kk + 1], P);
+                            double d[4];
+#pragma unroll
+                            for (int n = 0; n < 4; ++n) d[n] = ((cnx * P[n].x + cny * P[n].y) + cnz * P[n].z) - dist;
+                            const int cs = d2case_snap(d);
+                            sCase[tid][t] = (unsigned char)cs;
+                            if (cs == 0) sTerm[tid][t][0] = sT[nb][t];
+                            else if (cs != 15) atomicAdd(&sCount[cs], 1);
+                        }
+                    }
+                }
+                sKind[tid] = (unsigned char)kind;
+            }
+            __syncthreads();
+            // (B) bucket the cut tets by case
+            if (tid == 0) {
+                int b = 0;
+                for (int q = 0; q < 16; ++q) {
+                    sBase[q] = b;
+                    b += sCount[q];
+                }
+                sCount[0] = b;  // total; sCount[1..14] become the fill cursors
+                for (int q = 1; q < 15; ++q) sCount[q] = sBase[q];
+            }
+            __syncthreads();
+            const int nitem = sCount[0];
+            if (tid < ELV_PAIRS && sKind[tid] == 0) {
+                for (int t = 0; t < 5; ++t) {
+                    const int cs = sCase[tid][t];
+                    if (cs != 0 && cs != 15) sItem[atomicAdd(&sCount[cs], 1)] = (unsigned short)(tid * 5 + t);
+                }
+            }
+            __syncthreads();
+            // (C) cut the bucketed tets
+            for (int q = tid; q < nitem; q += ELV_THREADS) {
+                const int item = sItem[q];
+                const int pr = item / 5, t = item - 5 * pr;
+                const int c = pr % 27, nb = 9 * plane + pr / 27;
+                const double cnx = sC[c][0], cny = sC[c][1], cnz = sC[c][2], dist = sC[c][3];
                 const int ii = i - 1 + nb % 3, jj = j - 1 + (nb / 3) % 3, kk = k - 1 + nb / 9;
-                const double dcen = ((cnx * m.xm[ii] + cny * m.ym[jj]) + cnz * m.zm[kk]) - dist;
-                const double hext = 0.5 * ((hx + hy) + hz);
-                const double margin =
-                    1e-9 * (((1.0 + fabs(dist)) + fabs(m.xm[ii])) + (fabs(m.ym[jj]) + fabs(m.zm[kk])));
-                double vf_;
-                if (dcen - hext > margin) {
-                    vf_ = 0.0;
-                } else if (dcen + hext < -margin) {
+                const bool even = ((ii + jj + kk) % 2) == 0;
+                const double ts = even ? 1.0 : -1.0;
+                V3 P[4];
+                tet5_vertices(t, even, m.x[ii], m.x[ii + 1], m.y[jj], m.y[jj + 1], m.z[kk], m.z[kk + 1], P);
+                double d[4];
+#pragma unroll
+                for (int n = 0; n < 4; ++n) d[n] = ((cnx * P[n].x + cny * P[n].y) + cnz * P[n].z) - dist;
+                const int cs = d2case_snap(d);
+                int w = 0;
+                cut_dispatch<false>(
+                    cs, P, d, [&](double v) { sTerm[pr][t][w++] = (ts * v) / cellvol; }, [](double) {});
+            }
+            __syncthreads();
+            // (D) computePLIC2VF of each pair in the reference's order, then the errors
+            if (tid < ELV_PAIRS) {
+                const int nb = 9 * plane + tid / 27;
+                const int kind = sKind[tid];
+                double vf_ = 0.0;
+                if (kind == 2) {
                     vf_ = sFull[nb];
-                } else {
-                    vf_ = plic2vf_reg(m.x[ii], m.x[ii + 1], m.y[jj], m.y[jj + 1], m.z[kk], m.z[kk + 1], cnx, cny,
-                                      cnz, dist, ((ii + jj + kk) % 2) == 0, cellvol);
+                } else if (kind == 0) {
+                    for (int t = 0; t < 5; ++t) {
+                        const int cs = sCase[tid][t];
+                        const int nt = CUT_NTETS[cs] - CUT_NNTET[cs] + 1;
+                        for (int w = 0; w < nt; ++w) vf_ += sTerm[tid][t][w];
+                    }
                 }
                 const double df = sVF[nb] - vf_;
-                sE[lane][nb] = df * df;
+                sE[tid] = df * df;
             }
+            __syncthreads();
+            if (tid < 27) {
+                double e = sErr[tid];
+                for (int q = 0; q < 9; ++q) e += sE[q * 27 + tid];
+                sErr[tid] = e;
+            }
+            __syncthreads();
         }
-        __syncthreads();
         if (tid < 32) {
+            const int lane = tid;
             double err = __builtin_huge_val();
-            if (lane < 27 && sOk[lane]) {
-                double e = 0.0;
-                for (int nb = 0; nb < 27; ++nb) e += sE[lane][nb];
-                err = e;
-            }
+            if (lane < 27 && sOk[lane]) err = sErr[lane];
             const bool ok = err < __builtin_huge_val();
             double be = ok ? err : __builtin_huge_val();
             int bl = ok ? lane : 64;
