@@ -368,7 +368,7 @@ __global__ void k_normals_prep(DevMesh m, DevParams p, const double* VF, double*
 constexpr int ELV_THREADS = 256;
 constexpr int ELV_PAIRS = 27 * 9;  // candidate-neighbour pairs of one z-plane of neighbours
 
-__global__ __launch_bounds__(ELV_THREADS) void k_elvira(DevMesh m, DevParams p, const double* VF, const int* list,
+__global__ __launch_bounds__(ELV_THREADS, 4) void k_elvira(DevMesh m, DevParams p, const double* VF, const int* list,
                                                         const int* count, int fam_mask, double* nxo, double* nyo,
                                                         double* nzo) {
     __shared__ double sVF[27];
