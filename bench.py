@@ -31,24 +31,51 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def elvira_flops_per_cell(N):
-    """Counted FP64 operations per interface cell of one ELVIRA pass, from the oracle's
-    operation counter on the workload's initial state at a reduced size."""
+def counted_flops(N):
+    """Counted FP64 operations (+ - * / = 1 each) of the oracle, on one thread, over one step of
+    the workload from its initial state at a reduced size: per interface cell of one ELVIRA
+    pass, and per band cell of the SL cut (mesh-plane and PLIC cuts of cutTet, sub-tet
+    volumes, the vol / vLiq sums)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     from ns_multiphase_amd import workloads as W
-    import numpy as np
     param, ic = W.deformation3d(N)
     o = pyoracle.OracleContext(param)
     o.init_vof(*ic)
-    o.neumann("VF")
-    pyoracle.count_flops(True)
-    o.compute_normal()
-    fl, sp = pyoracle.flops()
-    pyoracle.count_flops(False)
+    o.vof_init()
+    threads = pyoracle.get_threads()
+    pyoracle.set_threads(1)  # the counters are per thread
+
+    def count(f):
+        pyoracle.flops()
+        pyoracle.count_flops(True)
+        start = pyoracle.flops()[0]
+        f()
+        n = pyoracle.flops()[0] - start
+        pyoracle.count_flops(False)
+        return n
+
     VF = W.interior(o.mesh, o.get("VF"), vof=True)
     nif = int(((VF >= param.VFlo) & (VF <= param.VFhi)).sum())
-    return fl / max(nif, 1), sp / max(nif, 1), nif
+    band = W.interior(o.mesh, o.get("band"))
+    nband = int((abs(band) <= param.nband).sum())
+    dt = o.compute_dt()
+    o.define_velocity(dt)
+    f_elv0 = count(o.compute_normal)          # the transport's first ELVIRA, same state
+    f_tr = count(lambda: o.transport(dt, dt))
+    f_elv1 = count(o.compute_normal)          # its last ELVIRA and PLIC, same state
+    f_plic = count(o.compute_plic)
+    pyoracle.set_threads(threads)
+    return {"elvira_per_interface_cell": f_elv0 / max(nif, 1),
+            "cut_per_band_cell": (f_tr - f_elv0 - f_elv1 - f_plic) / max(nband, 1),
+            "N": N, "interface_cells": nif, "band_cells": nband}
+
+
+# HBM bytes per unit (interface cell for k_elvira, band cell for the cut pass) from
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per dispatch on 20 steps of this workload, FETCH_SIZE
+# scaled by 2.0 and WRITE_SIZE by 1.0 (what the counters read for a plain 256 MiB device copy);
+# see profiles/pmc_traffic_deformation3d_128.md.
+TRAFFIC_PER_UNIT = {"k_elvira": 669.0, "k_sl_tree": 1006.0}
 
 
 def run_vof(args, torch):
@@ -92,7 +119,7 @@ def cpu_baseline(args, param, ic):
     threads = min(threads, 16)
     pyoracle.set_threads(threads)
     nsteps_period = int(round(param.tFinal / param.max_dt))
-    samples = [0, nsteps_period // 4, nsteps_period // 2]
+    samples = list(range(0, nsteps_period, max(1, nsteps_period // 64)))  # 64 steps across the period
     g = solver.Context(param)
     g.init_vof(*ic)
     g.vof_init()
@@ -104,18 +131,19 @@ def cpu_baseline(args, param, ic):
         while done < s:
             t, _ = g.vof_step(t, param.tFinal)
             done += 1
-        for name in ("VF", "nx", "ny", "nz", "D", "band", "u", "v", "w"):
+        for name in ("VF", "band", "u", "v", "w"):  # the step recomputes normals, PLIC and velocity
             o.set(name, g.get(name))
         t0 = time.perf_counter()
         o.vof_step(t, param.tFinal)
         cpu_time += time.perf_counter() - t0
-        log(f"  cpu sample at step {s}: {time.perf_counter() - t0:.2f} s")
+    log(f"  cpu: {len(samples)} oracle steps in {cpu_time:.1f} s")
     g.close()
     cells = param.Nx * param.Ny * param.Nz
     return {"value": cells * len(samples) / cpu_time, "unit": "cell-updates/s", "cores": threads,
             "kind": "port",
-            "sample": f"one oracle step (C++ FP64 restatement, OpenMP) from the device state at steps "
-                      f"{samples} of the {param.Nx}^3 deformation, {cpu_time:.1f} s total"}
+            "sample": f"{len(samples)} oracle steps (C++ FP64 restatement, OpenMP), each from the device "
+                      f"state at steps 0, {samples[1] if len(samples) > 1 else 0}, ... of the {param.Nx}^3 "
+                      f"deformation period, {cpu_time:.1f} s of CPU time"}
 
 
 def run_mg(args, torch):
@@ -176,19 +204,23 @@ def main():
     value = cells * steps / elapsed
     log(f"[bench] {steps} steps in {elapsed:.3f} s -> {value:.3e} cell-updates/s (t_end={tend:.4f})")
 
-    # roofline of the dominant kernel (ELVIRA, FP64)
-    fl_cell, sp_cell, _ = elvira_flops_per_cell(min(args.N, 64))
-    e_ms, e_n, e_units = prof["elvira"]
-    roof = None
-    if e_n > 0 and e_ms > 0:
-        flops_per_launch = fl_cell * e_units / e_n
-        avg_s = e_ms / e_n / 1e3
-        achieved = flops_per_launch / avg_s / 1e12
-        roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                "kernel": "k_elvira", "avg_launch_us": 1e6 * avg_s, "launches": e_n,
-                "interface_cells_per_launch": e_units / e_n, "flops_per_interface_cell": fl_cell,
-                "note": "FP64 vector peak of MI355X (78.6 TFLOP/s); no FMA by construction (-ffp-contract=off)"}
+    # rooflines of the two FP64 kernels of the step; the dominant one is `roofline`
+    fc = counted_flops(min(args.N, 64))
+    roofs = {}
+    for key, kern, per, unit_name in (("elvira", "k_elvira", fc["elvira_per_interface_cell"], "interface_cells"),
+                                      ("sl_cut", "k_sl_tree", fc["cut_per_band_cell"], "band_cells")):
+        ms, n, units = prof[key]
+        if n <= 0 or ms <= 0:
+            continue
+        avg_s = ms / n / 1e3
+        achieved = per * units / n / avg_s / 1e12
+        roofs[key] = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": TRAFFIC_PER_UNIT[kern] * units / n,
+                      "kernel": kern, "avg_launch_us": 1e6 * avg_s, "launches": n, "total_ms": ms,
+                      f"{unit_name}_per_launch": units / n, f"flops_per_{unit_name[:-1]}": per,
+                      "note": "FP64 vector peak of MI355X (78.6 TFLOP/s, FMA = 2); the kernels form no FMA "
+                              "(-ffp-contract=off); flops counted by the oracle at N=%d" % fc["N"]}
+    roof = max(roofs.values(), key=lambda r: r["total_ms"]) if roofs else None
     kernels = {k: {"ms": v[0], "launches": v[1], "avg_us": (1e3 * v[0] / v[1]) if v[1] else None,
                    "units_per_launch": v[2] / v[1] if v[1] else None}
                for k, v in prof.items() if v[1]}
@@ -229,6 +261,7 @@ def main():
                    "steps_timed_from_t0": steps, "volume_rel_change": (V1 - V0) / V0,
                    "mg_workload": f"variable-density MMS {args.mg_N}^3, rho 1000/1, tol 1e-10"},
         "roofline": roof, "cpu_baseline": cpu, "mg": mg, "kernels": kernels,
+        "rooflines": roofs,
     }
     print(json.dumps(line), flush=True)
 

@@ -58,10 +58,11 @@ struct mist_ctx {
     double* pv = nullptr;        // projected cell vertices, (Nx+1)(Ny+1)(Nz+1) x 3
     int* band_list = nullptr;    // interior cells with |band| <= nband (linear cell index)
     int* iface_list = nullptr;   // interior interface cells (ELVIRA); cells the main cut pass defers
-    int* defer_list = nullptr;   // cells the wide cut pass defers to the general walk
+    int* defer_list = nullptr;   // cells for the general cut walk
+    void* cut_list = nullptr;    // CutCell entries of the band cells to cut
     int* counters = nullptr;     // [0] band cells, [1] interface cells, [2] overflow flag, [4] cells
-                                 // deferred by the main cut pass, [5] its work queue, [6] cells
-                                 // deferred by the wide pass, [7] its work queue
+                                 // deferred by the main cut pass, [5] its work queue, [6] cells for
+                                 // the general walk, [7] the wide pass's work queue, [8] cells to cut
     double* red = nullptr;       // reduction partials
     int red_blocks = 0;
     int* h_counters = nullptr;   // pinned

@@ -895,7 +895,7 @@ struct TreeShared {
     unsigned short lnode[C::NLEAF];  // node of each leaf
     double plane[3][5];                // faces lo+0 .. lo+span+1 per axis
     double nb[64][4];                  // nx, ny, nz, D of the cells of the index box
-    int ci[8][3];                      // cell index of each projected corner
+    signed char cls[64];               // +1 / -1: the cell lies on the liquid / gas side of its plane
     int next;                          // cell taken from the work queue
     int scan[C::NT / 64];
     double red[2][C::NT / 64];
@@ -1091,55 +1091,148 @@ __device__ bool tree_level(TreeShared<C>& S, int axis, int minspan, int cur, int
     return true;
 }
 
+// Side of cell `c` of the index box relative to its PLIC plane over the cell's extent
+// [x0,x1]x[y0,y1]x[z0,z1]: +1 when every point gives n.x - D < 0 (liquid) by a margin far
+// above rounding, -1 when every point gives > 0 (gas), 0 otherwise.
+__device__ __forceinline__ int box_class(double n0, double n1, double n2, double Dc, double x0, double x1, double y0,
+                                         double y1, double z0, double z1) {
+    const double a0 = n0 * x0, a1 = n0 * x1, b0 = n1 * y0, b1 = n1 * y1, c0 = n2 * z0, c1 = n2 * z1;
+    const double dmax = ((fmax(a0, a1) + fmax(b0, b1)) + fmax(c0, c1)) - Dc;
+    const double dmin = ((fmin(a0, a1) + fmin(b0, b1)) + fmin(c0, c1)) - Dc;
+    const double mag = fabs(n0) * (fmax(fabs(x0), fabs(x1)) + (x1 - x0)) +
+                       fabs(n1) * (fmax(fabs(y0), fabs(y1)) + (y1 - y0)) +
+                       fabs(n2) * (fmax(fabs(z0), fabs(z1)) + (z1 - z0)) + fabs(Dc);
+    const double tau = 1e-9 * mag;
+    return dmax < -tau ? 1 : (dmin > tau ? -1 : 0);
+}
+
+// Slot `base + rank` of a wave-aggregated append of the lanes with `flag` to *count.
+__device__ __forceinline__ int wave_slot(bool flag, int* count) {
+    const unsigned long long mask = __ballot(flag);
+    if (mask == 0) return -1;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(count, __popcll(mask));
+    base = __shfl(base, leader);
+    return flag ? base + __popcll(mask & ((1ull << lane) - 1ull)) : -1;
+}
+
+// A band cell to cut: interior id, smallest cell index `lo` of its projected corners, and the
+// 6-bit offsets (x, y, z: 2 bits each) of corner c from lo at bits 6c.
+struct CutCell {
+    int q, lo0, lo1, lo2;
+    unsigned long long offs;
+};
+static_assert(sizeof(CutCell) == 24, "the context allocates 24 bytes per cut-list entry");
+
+// Corners, index box and bulk test of every band cell, one thread per cell (the set-up of the
+// cut walk, kept out of the work-group kernel where its chain of dependent loads would stall a
+// whole block per cell). Corner cells come from pt2index! started at the cell itself
+// (src/Transport.jl:128-131). When every cell of the box lies on one side of its PLIC plane,
+// each leaf of the reference's cutTet is all-liquid (or all-gas), its vol and vLiq sums receive
+// the same additions (or vLiq none) and the new VF is exactly 1 (or 0): it is written here. The
+// other cells go to the cut list; those whose corners span more than 4 cells along an axis to
+// the general walk.
+__global__ __launch_bounds__(256) void k_sl_classify(DevMesh m, const double* pv, const int* list, const int* count,
+                                                     const double* nx, const double* ny, const double* nz,
+                                                     const double* D, double* VF, int* ovf, CutCell* cut,
+                                                     int* ncut, int* general, int* ngeneral) {
+    const int nb = *count;
+    const int nloop = (nb + 63) & ~63;  // whole waves in the loop (the appends use ballots)
+    for (int cidx = blockIdx.x * blockDim.x + threadIdx.x; cidx < nloop; cidx += gridDim.x * blockDim.x) {
+        const bool valid = cidx < nb;
+        int q = 0, i = 1, j = 1, k = 1;
+        int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        unsigned long long offs = 0;
+        int kind = 0;  // 0: none, 1: cut list, 2: general walk
+        if (valid) {
+            q = list[cidx];
+            interior_of(m, q, i, j, k);
+            int I[8][3];
+            int o = 0;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const int64_t vq = vtx_id(m, i + (c & 1), j + ((c >> 1) & 1), k + (c >> 2));
+                const double pt[3] = {pv[3 * vq + 0], pv[3 * vq + 1], pv[3 * vq + 2]};
+                pt2index(m, I[c], pt, i, j, k, &o);
+            }
+            if (o) atomicOr(ovf, 1);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = I[0][a];
+                hi[a] = I[0][a];
+#pragma unroll
+                for (int c = 1; c < 8; ++c) {
+                    lo[a] = min(lo[a], I[c][a]);
+                    hi[a] = max(hi[a], I[c][a]);
+                }
+            }
+            if (hi[0] - lo[0] > 3 || hi[1] - lo[1] > 3 || hi[2] - lo[2] > 3) {
+                kind = 2;
+            } else {
+#pragma unroll
+                for (int c = 0; c < 8; ++c)
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) offs |= (unsigned long long)(I[c][a] - lo[a]) << (6 * c + 2 * a);
+                bool liq = true, gas = true;
+                for (int cz = lo[2]; cz <= hi[2] && (liq || gas); ++cz)
+                    for (int cy = lo[1]; cy <= hi[1] && (liq || gas); ++cy)
+                        for (int cx = lo[0]; cx <= hi[0] && (liq || gas); ++cx) {
+                            const int64_t c = m.cell(cx, cy, cz);
+                            const int cl = box_class(nx[c], ny[c], nz[c], D[c], m.x[cx], m.x[cx + 1], m.y[cy],
+                                                     m.y[cy + 1], m.z[cz], m.z[cz + 1]);
+                            liq = liq && cl == 1;
+                            gas = gas && cl == -1;
+                        }
+                if (liq || gas) VF[m.vof(i, j, k)] = liq ? 1.0 : 0.0;
+                else kind = 1;
+            }
+        }
+        const int sc = wave_slot(kind == 1, ncut);
+        if (sc >= 0) cut[sc] = CutCell{q, lo[0], lo[1], lo[2], offs};
+        const int sg = wave_slot(kind == 2, ngeneral);
+        if (sg >= 0) general[sg] = q;
+    }
+}
+
 template <class C>
-__global__ __launch_bounds__(C::NT) void k_sl_tree(DevMesh m, const double* pv, const int* list, const int* count,
-                                                const double* nx, const double* ny, const double* nz,
-                                                const double* D, double* VF, int* ovf, int* defer, int* ndefer,
-                                                int* queue) {
+__global__ __launch_bounds__(C::NT) void k_sl_tree(DevMesh m, const double* pv, const CutCell* cut, const int* idx,
+                                                const int* count, const double* nx, const double* ny,
+                                                const double* nz, const double* D, double* VF, int* defer,
+                                                int* ndefer, int defer_cell, int* queue) {
     constexpr int NT = C::NT;
     __shared__ TreeShared<C> S;
     const int tid = threadIdx.x;
     const int nb = *count;
-    for (;;) {
-        if (tid == 0) S.next = atomicAdd(queue, 1);
-        __syncthreads();
-        const int cidx = S.next;
-        if (cidx >= nb) break;
-        const int q = list[cidx];
+    // the next cell is claimed while the current one is cut (the atomic's latency is hidden)
+    if (tid == 0) S.next = atomicAdd(queue, 1);
+    __syncthreads();
+    int cidx = S.next;
+    while (cidx < nb) {
+        __syncthreads();  // every thread has read S.next
+        int claim = 0;
+        if (tid == 0) claim = atomicAdd(queue, 1);
+        const int entry = idx ? idx[cidx] : cidx;
+        const CutCell cc = cut[entry];
+        const int q = cc.q;
         int i, j, k;
         interior_of(m, q, i, j, k);
         const bool even = ((i + j + k) % 2) == 0;
-        // ---- projected corners and their cells (pt2index! from the cell itself) ----------
+        const int lo[3] = {cc.lo0, cc.lo1, cc.lo2};
+        int span[3] = {0, 0, 0};
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) span[a] = max(span[a], (int)((cc.offs >> (6 * c + 2 * a)) & 3ull));
+        const int bx = span[0] + 1, by = span[1] + 1, bz = span[2] + 1;
+        const int ncell = bx * by * bz;
+        // ---- projected corners, mesh planes and the box cells' PLIC planes, all at once ------
         if (tid < 8) {
             const int64_t vq = vtx_id(m, i + (tid & 1), j + ((tid >> 1) & 1), k + (tid >> 2));
-            const double pt[3] = {pv[3 * vq + 0], pv[3 * vq + 1], pv[3 * vq + 2]};
-            S.px[tid] = pt[0];
-            S.py[tid] = pt[1];
-            S.pz[tid] = pt[2];
-            int I[3], o = 0;
-            pt2index(m, I, pt, i, j, k, &o);
-            if (o) atomicOr(ovf, 1);
-            S.ci[tid][0] = I[0];
-            S.ci[tid][1] = I[1];
-            S.ci[tid][2] = I[2];
-        }
-        __syncthreads();
-        int lo[3], span[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            int mn = S.ci[0][a], mx = S.ci[0][a];
-#pragma unroll
-            for (int c = 1; c < 8; ++c) {
-                mn = min(mn, S.ci[c][a]);
-                mx = max(mx, S.ci[c][a]);
-            }
-            lo[a] = mn;
-            span[a] = mx - mn;
-        }
-        if (span[0] > 3 || span[1] > 3 || span[2] > 3) {  // beyond 2-bit offsets: general walk
-            if (tid == 0) defer[atomicAdd(ndefer, 1)] = q;
-            __syncthreads();
-            continue;
+            S.px[tid] = pv[3 * vq + 0];
+            S.py[tid] = pv[3 * vq + 1];
+            S.pz[tid] = pv[3 * vq + 2];
         }
         if (tid < 15) {
             const int a = tid / 5, off = tid - 5 * a;
@@ -1147,11 +1240,6 @@ __global__ __launch_bounds__(C::NT) void k_sl_tree(DevMesh m, const double* pv, 
             const int sa = a == 0 ? span[0] : (a == 1 ? span[1] : span[2]);
             if (off <= sa + 1) S.plane[a][off] = (a == 0 ? m.x : (a == 1 ? m.y : m.z))[la + off];
         }
-        __syncthreads();
-        // ---- the cells of the index box: PLIC planes and which side they are on ----------
-        const int bx = span[0] + 1, by = span[1] + 1, bz = span[2] + 1;
-        const int ncell = bx * by * bz;
-        int cls = 0;
         for (int l = tid; l < ncell; l += NT) {
             const int ox = l % bx, oy = (l / bx) % by, oz = l / (bx * by);
             const int64_t c = m.cell(lo[0] + ox, lo[1] + oy, lo[2] + oz);
@@ -1160,23 +1248,8 @@ __global__ __launch_bounds__(C::NT) void k_sl_tree(DevMesh m, const double* pv, 
             S.nb[l][1] = n1;
             S.nb[l][2] = n2;
             S.nb[l][3] = Dc;
-            const double x0 = S.plane[0][ox], x1 = S.plane[0][ox + 1], y0 = S.plane[1][oy], y1 = S.plane[1][oy + 1],
-                         z0 = S.plane[2][oz], z1 = S.plane[2][oz + 1];
-            const double a0 = n0 * x0, a1 = n0 * x1, b0 = n1 * y0, b1 = n1 * y1, c0 = n2 * z0, c1 = n2 * z1;
-            const double dmax = ((fmax(a0, a1) + fmax(b0, b1)) + fmax(c0, c1)) - Dc;
-            const double dmin = ((fmin(a0, a1) + fmin(b0, b1)) + fmin(c0, c1)) - Dc;
-            const double mag = fabs(n0) * (fmax(fabs(x0), fabs(x1)) + (x1 - x0)) +
-                               fabs(n1) * (fmax(fabs(y0), fabs(y1)) + (y1 - y0)) +
-                               fabs(n2) * (fmax(fabs(z0), fabs(z1)) + (z1 - z0)) + fabs(Dc);
-            const double tau = 1e-9 * mag;
-            cls = dmax < -tau ? 1 : (dmin > tau ? -1 : 0);
-        }
-        const int all_liq = __syncthreads_and(tid >= ncell || cls == 1);
-        const int all_gas = __syncthreads_and(tid >= ncell || cls == -1);
-        if (all_liq || all_gas) {
-            if (tid == 0) VF[m.vof(i, j, k)] = all_liq ? 1.0 : 0.0;
-            __syncthreads();
-            continue;
+            S.cls[l] = (signed char)box_class(n0, n1, n2, Dc, m.x[lo[0] + ox], m.x[lo[0] + ox + 1], m.y[lo[1] + oy],
+                                              m.y[lo[1] + oy + 1], m.z[lo[2] + oz], m.z[lo[2] + oz + 1]);
         }
         // ---- the five tets (src/VFgeom.jl:235-256) ---------------------------------------
         if (tid < 5) {
@@ -1186,8 +1259,7 @@ __global__ __launch_bounds__(C::NT) void k_sl_tree(DevMesh m, const double* pv, 
             for (int n = 0; n < 4; ++n) {
                 const int c = vert_corner(nib(tv, n), even);
                 ids[n] = (unsigned)c;
-#pragma unroll
-                for (int a = 0; a < 3; ++a) bits |= (unsigned)(S.ci[c][a] - lo[a]) << (6 * n + 2 * a);
+                bits |= (unsigned)((cc.offs >> (6 * c)) & 63ull) << (6 * n);
             }
             S.node[0][tid] = TreeNode{ids[0] | (ids[1] << 16), ids[2] | (ids[3] << 16), bits};
         }
@@ -1242,126 +1314,118 @@ __global__ __launch_bounds__(C::NT) void k_sl_tree(DevMesh m, const double* pv, 
             }
             nleaf += tot;
         }
-        if (!ok) {  // pool or lists too small for this cell: general walk
-            if (tid == 0) defer[atomicAdd(ndefer, 1)] = q;
+        if (!ok) {  // pool or lists too small for this cell: the next pass
+            if (tid == 0) defer[atomicAdd(ndefer, 1)] = defer_cell ? q : entry;
+        } else {
             __syncthreads();
-            continue;
-        }
-        __syncthreads();
-        // ---- leaves: last z cut and the PLIC cut (src/VFgeom.jl:757-830) ------------------
-        double Vs = 0.0, Ls = 0.0;
-        for (int L = tid; L < nleaf; L += NT) {
-            const int e = S.lnode[L];
-            const int c = L - S.loff[e];
-            const TreeNode N = S.node[cur][e];
-            V3 Pn[4];
+            // ---- leaves: last z cut and the PLIC cut (src/VFgeom.jl:757-830) ------------------
+            double Vs = 0.0, Ls = 0.0;
+            for (int L = tid; L < nleaf; L += NT) {
+                const int e = S.lnode[L];
+                const int c = L - S.loff[e];
+                const TreeNode N = S.node[cur][e];
+                const int o0 = boff(N.bits, 0, 2), o1 = boff(N.bits, 1, 2), o2 = boff(N.bits, 2, 2),
+                          o3 = boff(N.bits, 3, 2);
+                const int mx = max(max(o0, o1), max(o2, o3));
+                V3 Q[4];
+                unsigned ob;  // cell offsets (x, y, z: 2 bits each) of the leaf's first vertex
+                if (mx > min(min(o0, o1), min(o2, o3))) {
+                    // child c of the node's last z cut, its vertices straight from the pool: an
+                    // original vertex is the edge (v, v) at mu = 0, which lerp3 returns exactly
+                    const double P = S.plane[2][mx];
+                    double dd[4];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) Pn[v] = pool_v(S, tn_id(N, v));
-            const int o0 = boff(N.bits, 0, 2), o1 = boff(N.bits, 1, 2), o2 = boff(N.bits, 2, 2),
-                      o3 = boff(N.bits, 3, 2);
-            const int mx = max(max(o0, o1), max(o2, o3));
-            V3 Q[4];
-            unsigned ox, oy, oz;
-            if (mx > min(min(o0, o1), min(o2, o3))) {
-                const double P = S.plane[2][mx];
+                    for (int v = 0; v < 4; ++v) dd[v] = S.pz[tn_id(N, v)] - P;
+                    const int cs = d2case_snap(dd);
+                    const unsigned qv = CUT_VTET[cs][c];
+                    const bool s2 = (CUT_SIDE2[cs] >> c) & 1;
+                    int e1v = 0, e2v = 0;  // edge of the first vertex
+#pragma unroll
+                    for (int kk = 0; kk < 4; ++kk) {
+                        const int lv = nib(qv, kk);
+                        const bool cutv = lv >= 4;
+                        const unsigned eg = CUT_EDGE[cs][cutv ? lv - 4 : 0];
+                        const int v1 = cutv ? (int)(eg & 15) : lv, v2 = cutv ? (int)(eg >> 4) : lv;
+                        const V3 A = pool_v(S, tn_id(N, v1)), B = pool_v(S, tn_id(N, v2));
+                        const double d1 = A.z - P, d2 = B.z - P;
+                        const double mu = cutv ? jmin(1.0, jmax(0.0, sdiv(-d1, d2 - d1))) : 0.0;
+                        Q[kk] = lerp3(A, B, mu);
+                        if (kk == 0) { e1v = v1; e2v = v2; }
+                    }
+                    // cell of the leaf: that of its first vertex (side-1/2 index along z, the walk
+                    // and clamp of src/VFgeom.jl:853-862 along x and y for a cut vertex)
+                    const int lv0 = nib(qv, 0);
+                    if (lv0 < 4) {
+                        ob = (N.bits >> (6 * lv0)) & 63u;
+                        const unsigned oa = ob >> 4;
+                        ob = (ob & 15u) | ((s2 ? (unsigned)mx : min(oa, (unsigned)(mx - 1))) << 4);
+                    } else {
+                        ob = (unsigned)(s2 ? mx : mx - 1) << 4;
+#pragma unroll
+                        for (int ax = 0; ax < 2; ++ax) {
+                            const int ea = boff(N.bits, e1v, ax), eb = boff(N.bits, e2v, ax);
+                            const unsigned oo = ea == eb ? (unsigned)ea
+                                                         : walk_clamp(ax == 0 ? Q[0].x : Q[0].y, boff(N.bits, lv0 - 4, ax),
+                                                                      ea, eb, S.plane[ax], span[ax]);
+                            ob |= oo << (2 * ax);
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) Q[v] = pool_v(S, tn_id(N, v));
+                    ob = N.bits & 63u;
+                }
+                const unsigned ox = ob & 3u, oy = (ob >> 2) & 3u, oz = (ob >> 4) & 3u;
+                const int l = (int)ox + bx * ((int)oy + by * (int)oz);
+                const int lcls = S.cls[l];
+                if (lcls != 0) {  // every vertex strictly on one side: the d2case case is 1 or 16
+                    const double tv = tet_vol3(Q[0], Q[1], Q[2], Q[3]);
+                    Vs += tv;
+                    if (lcls > 0) Ls += tv;
+                    continue;
+                }
+                const double n0 = S.nb[l][0], n1 = S.nb[l][1], n2 = S.nb[l][2], Dc = S.nb[l][3];
                 double d[4], dd[4];
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    d[v] = Pn[v].z - P;
+                    d[v] = ((n0 * Q[v].x + n1 * Q[v].y) + n2 * Q[v].z) - Dc;
                     dd[v] = d[v];
                 }
                 const int cs = d2case_snap(dd);
-                const int nv = CUT_NVERT[cs];
-                V3 Cv[4];
-                unsigned cb[4] = {0u, 0u, 0u, 0u};
+                if (cs == 0 || cs == 15) {  // one sub-tet, the leaf itself (vertex order 1,2,3,4)
+                    const double tv = tet_vol3(Q[0], Q[1], Q[2], Q[3]);
+                    Vs += tv;
+                    if (cs == 0) Ls += tv;
+                } else {
+                    cut_dispatch<true>(cs, Q, d, [&](double tv) { Vs += tv; Ls += tv; }, [&](double tv) { Vs += tv; });
+                }
+            }
+            // ---- block sums in a fixed order ----------------------------------------------------
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    Cv[v] = Pn[0];
-                    if (v < nv) {
-                        const unsigned eg = CUT_EDGE[cs][v];
-                        const int v1 = eg & 15, v2 = eg >> 4;
-                        const double mu = jmin(1.0, jmax(0.0, sdiv(-sel4(d, v1), sel4(d, v2) - sel4(d, v1))));
-                        Cv[v] = lerp3(sel4(Pn, v1), sel4(Pn, v2), mu);
-                        unsigned bb = (unsigned)(mx - 1) << 4;
-#pragma unroll
-                        for (int ax = 0; ax < 2; ++ax) {
-                            const int e1 = boff(N.bits, v1, ax), e2 = boff(N.bits, v2, ax);
-                            const unsigned oo =
-                                e1 == e2 ? (unsigned)e1
-                                         : walk_clamp(ax == 0 ? Cv[v].x : Cv[v].y, boff(N.bits, v, ax), e1, e2,
-                                                      S.plane[ax], span[ax]);
-                            bb |= oo << (2 * ax);
-                        }
-                        cb[v] = bb;
+            for (int off = 32; off >= 1; off >>= 1) {
+                Vs += __shfl_xor(Vs, off);
+                Ls += __shfl_xor(Ls, off);
+            }
+            if constexpr (NT > 64) {
+                if ((tid & 63) == 0) {
+                    S.red[0][tid >> 6] = Vs;
+                    S.red[1][tid >> 6] = Ls;
+                }
+                __syncthreads();
+                if (tid == 0) {
+                    Vs = S.red[0][0];
+                    Ls = S.red[1][0];
+                    for (int w = 1; w < NT / 64; ++w) {
+                        Vs += S.red[0][w];
+                        Ls += S.red[1][w];
                     }
                 }
-                const unsigned qv = CUT_VTET[cs][c];
-                const bool s2 = (CUT_SIDE2[cs] >> c) & 1;
-#pragma unroll
-                for (int kk = 0; kk < 4; ++kk) {
-                    const int lv = nib(qv, kk);
-                    Q[kk] = lv < 4 ? sel4(Pn, lv) : sel4(Cv, lv - 4);
-                }
-                const int lv0 = nib(qv, 0);
-                unsigned ob;
-                if (lv0 < 4) {
-                    ob = (N.bits >> (6 * lv0)) & 63u;
-                    const unsigned oa = ob >> 4;
-                    ob = (ob & 15u) | ((s2 ? (unsigned)mx : min(oa, (unsigned)(mx - 1))) << 4);
-                } else {
-                    ob = sel4(cb, lv0 - 4);
-                    if (s2) ob = (ob & 15u) | ((unsigned)mx << 4);
-                }
-                ox = ob & 3u;
-                oy = (ob >> 2) & 3u;
-                oz = (ob >> 4) & 3u;
-            } else {
-#pragma unroll
-                for (int v = 0; v < 4; ++v) Q[v] = Pn[v];
-                ox = boff(N.bits, 0, 0);
-                oy = boff(N.bits, 0, 1);
-                oz = boff(N.bits, 0, 2);
             }
-            const int l = (int)ox + bx * ((int)oy + by * (int)oz);
-            const double n0 = S.nb[l][0], n1 = S.nb[l][1], n2 = S.nb[l][2], Dc = S.nb[l][3];
-            double d[4], dd[4];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                d[v] = ((n0 * Q[v].x + n1 * Q[v].y) + n2 * Q[v].z) - Dc;
-                dd[v] = d[v];
-            }
-            const int cs = d2case_snap(dd);
-            if (cs == 0 || cs == 15) {  // one sub-tet, the leaf itself (vertex order 1,2,3,4)
-                const double tv = tet_vol3(Q[0], Q[1], Q[2], Q[3]);
-                Vs += tv;
-                if (cs == 0) Ls += tv;
-            } else {
-                cut_dispatch<true>(cs, Q, d, [&](double tv) { Vs += tv; Ls += tv; }, [&](double tv) { Vs += tv; });
-            }
+            if (tid == 0) VF[m.vof(i, j, k)] = Ls / Vs;
         }
-        // ---- block sums in a fixed order ----------------------------------------------------
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-            Vs += __shfl_xor(Vs, off);
-            Ls += __shfl_xor(Ls, off);
-        }
-        if constexpr (NT > 64) {
-            if ((tid & 63) == 0) {
-                S.red[0][tid >> 6] = Vs;
-                S.red[1][tid >> 6] = Ls;
-            }
-            __syncthreads();
-            if (tid == 0) {
-                Vs = S.red[0][0];
-                Ls = S.red[1][0];
-                for (int w = 1; w < NT / 64; ++w) {
-                    Vs += S.red[0][w];
-                    Ls += S.red[1][w];
-                }
-            }
-        }
-        if (tid == 0) VF[m.vof(i, j, k)] = Ls / Vs;
+        if (tid == 0) S.next = claim;
         __syncthreads();
+        cidx = S.next;
     }
 }
 
@@ -1697,7 +1761,7 @@ int launch_transport(mist_ctx* ctx, double dt) {
     prof_end(ctx, MIST_PROF_PROJECT, ev, nullptr, nv);
     const int64_t ncap = (int64_t)ctx->m.Nx * ctx->m.Ny * ctx->m.Nz;
     ev = prof_begin(ctx, MIST_PROF_SL_CUT);
-    LAUNCH(k_fill_int, 1, 64, ctx->counters + 4, 4, 0);  // deferred cells and work queues
+    LAUNCH(k_fill_int, 1, 64, ctx->counters + 4, 5, 0);  // deferred cells, work queues, cut-list size
     if (!ctx->tree_blocks) {
         int per = 0, dev = 0, cus = 0;
         MIST_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_sl_tree<TreeMain>, TreeMain::NT, 0));
@@ -1711,19 +1775,24 @@ int launch_transport(mist_ctx* ctx, double dt) {
     const double* nz = (const double*)ctx->f[MIST_FIELD_NZ];
     const double* D = (const double*)ctx->f[MIST_FIELD_PLIC_D];
     double* VF = (double*)ctx->f[MIST_FIELD_VOF];
-    LAUNCH(k_sl_tree<TreeMain>, ctx->tree_blocks, TreeMain::NT, ctx->m, (const double*)ctx->pv,
-           (const int*)ctx->band_list, (const int*)(ctx->counters + 0), nx, ny, nz, D, VF, ctx->counters + 2,
-           ctx->iface_list, ctx->counters + 4, ctx->counters + 5);
+    CutCell* cut = (CutCell*)ctx->cut_list;
+    // bulk cells, index boxes; cells to cut go to the cut list
+    LAUNCH(k_sl_classify, grid_for(ncap, 256, 8192), 256, ctx->m, (const double*)ctx->pv, (const int*)ctx->band_list,
+           (const int*)(ctx->counters + 0), nx, ny, nz, D, VF, ctx->counters + 2, cut, ctx->counters + 8,
+           ctx->defer_list, ctx->counters + 6);
+    LAUNCH(k_sl_tree<TreeMain>, ctx->tree_blocks, TreeMain::NT, ctx->m, (const double*)ctx->pv, (const CutCell*)cut,
+           (const int*)nullptr, (const int*)(ctx->counters + 8), nx, ny, nz, D, VF, ctx->iface_list,
+           ctx->counters + 4, 0, ctx->counters + 5);
     prof_end(ctx, MIST_PROF_SL_CUT, ev, ctx->counters + 0, 0);
-    // cells whose cut tree outgrows the main pass: the same walk with larger capacities, then
-    // the general walk for cells whose corners span more than four cells along an axis
+    // cells whose cut tree outgrows the main pass: the same walk with larger capacities; then
+    // the general walk for those and for cells whose corners span more than four cells
     ev = prof_begin(ctx, MIST_PROF_SL_GENERAL);
-    LAUNCH(k_sl_tree<TreeWide>, ctx->tree_cus, TreeWide::NT, ctx->m, (const double*)ctx->pv,
-           (const int*)ctx->iface_list, (const int*)(ctx->counters + 4), nx, ny, nz, D, VF, ctx->counters + 2,
-           ctx->defer_list, ctx->counters + 6, ctx->counters + 7);
-    LAUNCH(k_sl_cut, 256, 64, ctx->m, (const double*)ctx->pv, (const int*)ctx->defer_list,
+    LAUNCH(k_sl_tree<TreeWide>, 32, TreeWide::NT, ctx->m, (const double*)ctx->pv, (const CutCell*)cut,
+           (const int*)ctx->iface_list, (const int*)(ctx->counters + 4), nx, ny, nz, D, VF, ctx->defer_list,
+           ctx->counters + 6, 1, ctx->counters + 7);
+    LAUNCH(k_sl_cut, 32, 64, ctx->m, (const double*)ctx->pv, (const int*)ctx->defer_list,
            (const int*)(ctx->counters + 6), nx, ny, nz, D, VF, ctx->counters + 2);
-    prof_end(ctx, MIST_PROF_SL_GENERAL, ev, ctx->counters + 4, 0);
+    prof_end(ctx, MIST_PROF_SL_GENERAL, ev, ctx->counters + 6, 0);
     int st;
     if ((st = launch_neumann(ctx, MIST_FIELD_VOF))) return st;
     return launch_borders(ctx, MIST_FIELD_VOF);

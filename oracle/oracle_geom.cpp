@@ -508,6 +508,7 @@ CutSums cut_tet(Ctx& c, double tet[4][3], int64_t ind[4][3], bool xdone, bool yd
         break;
     }
     double(*vert)[3] = s.vert[lvl];
+    if (dir >= 0) FL(4);  // plane distances of the chosen mesh cut
     if (dir < 0) {
         // Leaf: cut by the PLIC plane of cell ind[:,1] (src/VFgeom.jl:757-830).
         CutSums r;
@@ -517,12 +518,16 @@ CutSums cut_tet(Ctx& c, double tet[4][3], int64_t ind[4][3], bool xdone, bool yd
         const int64_t i = ind[0][0], j = ind[0][1], k = ind[0][2];
         const double nx = c.nx(i, j, k), ny = c.ny(i, j, k), nz = c.nz(i, j, k), D = c.D(i, j, k);
         for (int n = 0; n < 4; ++n) d[n] = ((nx * vert[n][0] + ny * vert[n][1]) + nz * vert[n][2]) - D;
+        FL(20);
         const int cs = d2case_copy(d);
         for (int n = 0; n < CUT_NVERT[cs]; ++n) {
             const int v1 = CUT_V1[cs][n] - 1, v2 = CUT_V2[cs][n] - 1;
             const double mu = jmin(1.0, jmax(0.0, sdiv(-d[v1], d[v2] - d[v1])));
             for (int q = 0; q < 3; ++q) vert[4 + n][q] = (1.0 - mu) * vert[v1][q] + mu * vert[v2][q];
+            FL(13);
         }
+        // counted: vol and vLiq; the momentum sums (not needed by the VOF path) are not
+        FL(2 * (CUT_NTETS[cs] - CUT_NNTET[cs] + 1) + (CUT_NNTET[cs] - 1));
         const double uu = c.u(i, j, k), vv = c.v(i, j, k), ww = c.w(i, j, k);
         const double rl = c.p.rho_liq, rg = c.p.rho_gas;
         for (int n = CUT_NTETS[cs]; n >= CUT_NNTET[cs]; --n) {
@@ -560,6 +565,7 @@ CutSums cut_tet(Ctx& c, double tet[4][3], int64_t ind[4][3], bool xdone, bool yd
         const int v1 = CUT_V1[cs][n] - 1, v2 = CUT_V2[cs][n] - 1;
         const double mu = jmin(1.0, jmax(0.0, sdiv(-d[v1], d[v2] - d[v1])));
         for (int q = 0; q < 3; ++q) vert[4 + n][q] = (1.0 - mu) * vert[v1][q] + mu * vert[v2][q];
+        FL(13);
         // the walk starts at the side-1 index of the vertex numbered like this cut vertex
         if (!pt2index(c, vi1[4 + n], vert[4 + n], vi1[n][0], vi1[n][1], vi1[n][2])) {
             c.status = MIST_ERR_OVERFLOW;

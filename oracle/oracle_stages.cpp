@@ -162,6 +162,7 @@ void compute_normal(Ctx& c) {
     std::fill(c.ny.a.begin(), c.ny.a.end(), 0.0);
     std::fill(c.nz.a.begin(), c.nz.a.end(), 0.0);
     const double VFlo = c.p.VFlo, VFhi = c.p.VFhi;
+    const FlopCount before = t_flops;  // the operation counter of the calling thread
     uint64_t fl = 0, fs = 0;
 #pragma omp parallel for collapse(2) schedule(dynamic, 1) reduction(+ : fl, fs)
     for (int64_t k = m.kmin_; k <= m.kmax_; ++k)
@@ -192,8 +193,8 @@ void compute_normal(Ctx& c) {
             fl += t_flops.flops;
             fs += t_flops.special;
         }
-    t_flops.flops = fl;
-    t_flops.special = fs;
+    t_flops.flops = before.flops + fl;
+    t_flops.special = before.special + fs;
     update_borders(c, c.nx);
     update_borders(c, c.ny);
     update_borders(c, c.nz);
@@ -220,9 +221,12 @@ int transport_vof(Ctx& c, double dt) {
     Arr3<double> VFnew;
     VFnew.init(m.cell_ext(), 0.0);
     int64_t maxlvl = 0;
-#pragma omp parallel reduction(max : maxlvl)
+    const FlopCount before = t_flops;  // the operation counter of the calling thread
+    uint64_t fl = 0, fs = 0;
+#pragma omp parallel reduction(max : maxlvl) reduction(+ : fl, fs)
     {
         std::unique_ptr<CutScratch, void (*)(CutScratch*)> s(new_cut_scratch(), delete_cut_scratch);
+        t_flops = FlopCount{};
 #pragma omp for collapse(2) schedule(dynamic, 1)
         for (int64_t k = m.kmin_; k <= m.kmax_; ++k)
             for (int64_t j = m.jmin_; j <= m.jmax_; ++j)
@@ -248,7 +252,11 @@ int transport_vof(Ctx& c, double dt) {
                         VFnew(i, j, k) = c.VF(i, j, k);
                     }
                 }
+        fl += t_flops.flops;
+        fs += t_flops.special;
     }
+    t_flops.flops = before.flops + fl;
+    t_flops.special = before.special + fs;
     c.max_cut_level = maxlvl;
     for (int64_t k = m.kmin_; k <= m.kmax_; ++k)
         for (int64_t j = m.jmin_; j <= m.jmax_; ++j)
