@@ -102,7 +102,12 @@ enum {
     MIST_FIELD_DIVG = 13,   /* divergence diagnostic */
     MIST_FIELD_DENX = 14, MIST_FIELD_DENY = 15, MIST_FIELD_DENZ = 16, /* face density */
     MIST_FIELD_RHS = 17,    /* pressure right-hand side b (interior cells are used) */
-    MIST_FIELD_COUNT = 18
+    MIST_FIELD_US = 18, MIST_FIELD_VS = 19, MIST_FIELD_WS = 20, /* predicted cell velocity */
+    MIST_FIELD_VISCX = 21, MIST_FIELD_VISCY = 22, MIST_FIELD_VISCZ = 23, /* face viscosity */
+    MIST_FIELD_CURV = 24,   /* Curve: interface curvature (height function) */
+    MIST_FIELD_SFX = 25, MIST_FIELD_SFY = 26, MIST_FIELD_SFZ = 27, /* face surface force */
+    MIST_FIELD_MASK = 28,   /* mask: int16, bit d set = mask[i,j,k][d+1] of the reference */
+    MIST_FIELD_COUNT = 29
 };
 
 enum { MIST_REDUCE_SUM = 0, MIST_REDUCE_MAX = 1, MIST_REDUCE_MIN = 2, MIST_REDUCE_MAXABS = 3 };

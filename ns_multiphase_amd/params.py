@@ -28,8 +28,11 @@ COARSE_OPS = {"face-average": 0, "restricted-vf": 1}
 FIELD = {
     "VF": 0, "nx": 1, "ny": 2, "nz": 3, "D": 4, "band": 5, "u": 6, "v": 7, "w": 8,
     "uf": 9, "vf": 10, "wf": 11, "P": 12, "divg": 13, "denx": 14, "deny": 15, "denz": 16,
-    "RHS": 17,
+    "RHS": 17, "us": 18, "vs": 19, "ws": 20, "viscx": 21, "viscy": 22, "viscz": 23,
+    "Curve": 24, "sfx": 25, "sfy": 26, "sfz": 27, "mask": 28,
 }
+# wall sides of the boundary-condition descriptor: x-, x+, y-, y+, z-, z+
+SIDES = {"xm": 0, "xp": 1, "ym": 2, "yp": 3, "zm": 4, "zp": 5}
 REDUCE = {"sum": 0, "max": 1, "min": 2, "maxabs": 3}
 IC_SHAPES = {"circle": 0, "sphere": 1, "bubble3d": 2}
 

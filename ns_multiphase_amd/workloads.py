@@ -5,6 +5,11 @@
   5 tets, plain SL transport, dt = max_dt = 0.5/N.
 * ``deformation3d``: 3D deformation in the unit cube (examples/example5_deformation3D.jl:
   sphere r=0.15 at (0.35, 0.35, 0.35), walls), period T = 3, CFL 0.5, dt = max_dt = 0.25/N.
+* ``rising_bubble``: gas bubble r = 0.125 at (0.5, 0.5, 0.3) in liquid filling the unit cube
+  (VFbubble3d), no-slip walls (the reflect BC of examples/example_viscous_bubble3D.jl:96-146
+  with zero wall velocity), grav_z = +9.81 (downward), rho 1000/1, mu 1e-3/1e-5,
+  sigma 0.07, max_dt = 1e-3, CFL 0.5, finite-difference pressure scheme (transport with
+  correction tets), multigrid down to 4 cells per side.
 * ``mms_setup``: variable-density pressure problem after test/pressure_var_den_MMS.jl in 3D:
   exact p = cos(2 pi x) cos(2 pi y) cos(2 pi z), interface z = 0.5 + 0.05 sin(2 pi x) sin(2 pi y)
   with the heavy phase below, face velocities uf = dt/rho_f dp/dx at the faces
@@ -40,6 +45,17 @@ def deformation3d(N: int = 128, T: float = 3.0, **kw) -> tuple[parameters, tuple
                 pressure_scheme="semi-lagrangian", projection_method="RK4", iter_type="standard")
     base.update(kw)
     return parameters(**base), ("sphere", 0.15, 0.35, 0.35, 0.35)
+
+
+def rising_bubble(N: int = 128, **kw) -> tuple[parameters, tuple]:
+    lvl = max(1, int(math.log2(N)) - 1)  # coarsest level 4 cells per side
+    base = dict(mu_liq=1e-3, mu_gas=1e-5, rho_liq=1000.0, rho_gas=1.0, sigma=0.07, Lx=1.0, Ly=1.0,
+                Lz=1.0, tFinal=1.0, Nx=N, Ny=N, Nz=N, stepMax=50, CFL=0.5, max_dt=1e-3,
+                out_period=1, tol=1e-10, nprocx=1, nprocy=1, nprocz=1, xper=False, yper=False,
+                zper=False, grav_z=9.81, solveNS=True, pressure_scheme="finite-difference",
+                pressureSolver="multigrid", mg_lvl=lvl, projection_method="RK4", iter_type="standard")
+    base.update(kw)
+    return parameters(**base), ("bubble3d", 0.125, 0.5, 0.5, 0.3)
 
 
 def mms_params(N: int = 128, rho_heavy: float = 1000.0, rho_light: float = 1.0, **kw) -> parameters:

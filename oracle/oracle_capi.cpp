@@ -60,12 +60,19 @@ int ora_ctx_create(const mist_params* p, void** out) {
     const Ext ce = c->m.cell_ext();
     c->VF.init(c->m.vof_ext());
     for (Arr3<double>* a : {&c->nx, &c->ny, &c->nz, &c->D, &c->u, &c->v, &c->w, &c->uf, &c->vf, &c->wf,
-                            &c->P, &c->divg, &c->RHS})
+                            &c->P, &c->divg, &c->RHS, &c->us, &c->vs, &c->ws, &c->Curve})
         a->init(ce);
     c->denx.init(c->m.face_ext(0));
     c->deny.init(c->m.face_ext(1));
     c->denz.init(c->m.face_ext(2));
+    c->viscx.init(c->m.face_ext(0));
+    c->viscy.init(c->m.face_ext(1));
+    c->viscz.init(c->m.face_ext(2));
+    c->sfx.init(c->m.face_ext(0));
+    c->sfy.init(c->m.face_ext(1));
+    c->sfz.init(c->m.face_ext(2));
     c->band.init(ce);
+    c->mask.init(ce);
     *out = c.release();
     return MIST_OK;
 }
@@ -77,7 +84,7 @@ const char* ora_last_error(void* h) { return h ? ((Ctx*)h)->err.c_str() : g_err.
 int ora_field_extents(void* h, int field, int64_t lo[3], int64_t n[3], int64_t* elem) {
     Ctx& c = *(Ctx*)h;
     const Ext* e = nullptr;
-    if (field == MIST_FIELD_BAND) { e = &c.band.e; *elem = 2; }
+    if (Arr3<int16_t>* b = field_i16(c, field)) { e = &b->e; *elem = 2; }
     else { Arr3<double>* a = field_f64(c, field); if (!a) return MIST_ERR_PARAM; e = &a->e; *elem = 8; }
     for (int d = 0; d < 3; ++d) { lo[d] = e->lo[d]; n[d] = e->n[d]; }
     return MIST_OK;
@@ -85,9 +92,9 @@ int ora_field_extents(void* h, int field, int64_t lo[3], int64_t n[3], int64_t* 
 
 int ora_set_field(void* h, int field, const void* src, size_t nbytes) {
     Ctx& c = *(Ctx*)h;
-    if (field == MIST_FIELD_BAND) {
-        if (nbytes != c.band.a.size() * 2) return MIST_ERR_PARAM;
-        std::memcpy(c.band.a.data(), src, nbytes);
+    if (Arr3<int16_t>* b = field_i16(c, field)) {
+        if (nbytes != b->a.size() * 2) return MIST_ERR_PARAM;
+        std::memcpy(b->a.data(), src, nbytes);
         return MIST_OK;
     }
     Arr3<double>* a = field_f64(c, field);
@@ -98,9 +105,9 @@ int ora_set_field(void* h, int field, const void* src, size_t nbytes) {
 
 int ora_get_field(void* h, int field, void* dst, size_t nbytes) {
     Ctx& c = *(Ctx*)h;
-    if (field == MIST_FIELD_BAND) {
-        if (nbytes != c.band.a.size() * 2) return MIST_ERR_PARAM;
-        std::memcpy(dst, c.band.a.data(), nbytes);
+    if (Arr3<int16_t>* b = field_i16(c, field)) {
+        if (nbytes != b->a.size() * 2) return MIST_ERR_PARAM;
+        std::memcpy(dst, b->a.data(), nbytes);
         return MIST_OK;
     }
     Arr3<double>* a = field_f64(c, field);
@@ -151,8 +158,12 @@ int ora_compute_dt(void* h, double* dt) { *dt = compute_dt(*(Ctx*)h); return MIS
 int ora_compute_band(void* h) { compute_band(*(Ctx*)h); return MIST_OK; }
 int ora_compute_normal(void* h) { compute_normal(*(Ctx*)h); return MIST_OK; }
 int ora_compute_plic(void* h) { compute_plic(*(Ctx*)h); return MIST_OK; }
-int ora_transport(void* h, double dt, double /*t*/) { return transport_vof(*(Ctx*)h, dt); }
-int ora_compute_props(void* h) { compute_dens(*(Ctx*)h); return MIST_OK; }
+// transport! (src/Transport.jl:2-268): with solveNS the momentum predictor too
+int ora_transport(void* h, double dt, double /*t*/) {
+    Ctx& c = *(Ctx*)h;
+    return c.p.solveNS ? transport_ns(c, dt) : transport_vof(c, dt);
+}
+int ora_compute_props(void* h) { compute_props(*(Ctx*)h); return MIST_OK; }
 int ora_pressure_rhs(void* h) { pressure_rhs(*(Ctx*)h); return MIST_OK; }
 int ora_corrector(void* h, double dt) { corrector(*(Ctx*)h, dt); return MIST_OK; }
 int ora_pressure_solve(void* h, double dt, double tol, int64_t precision, int64_t* cycles, double* resid) {
@@ -182,6 +193,94 @@ int ora_vof_step(void* h, double* t, double clamp_to, double* dt_used) {
     compute_band(c);
     *dt_used = dt;
     return st;
+}
+
+// ---- Navier-Stokes stages --------------------------------------------------------------
+int ora_set_wall_velocity(void* h, int side, const double vel[3]) {
+    Ctx& c = *(Ctx*)h;
+    if (side < 0 || side > 5) return MIST_ERR_PARAM;
+    for (int q = 0; q < 3; ++q) c.bc_wall[side][q] = vel[q];
+    return MIST_OK;
+}
+
+static bool velocity_triple(Ctx& c, int fu, int fv, int fw, Arr3<double>** a) {
+    const bool cell = fu == MIST_FIELD_U && fv == MIST_FIELD_V && fw == MIST_FIELD_W;
+    const bool pred = fu == MIST_FIELD_US && fv == MIST_FIELD_VS && fw == MIST_FIELD_WS;
+    if (!cell && !pred) return false;
+    a[0] = field_f64(c, fu); a[1] = field_f64(c, fv); a[2] = field_f64(c, fw);
+    return true;
+}
+
+int ora_apply_bc(void* h, int fu, int fv, int fw) {
+    Ctx& c = *(Ctx*)h;
+    Arr3<double>* a[3];
+    if (!velocity_triple(c, fu, fv, fw, a)) return MIST_ERR_PARAM;
+    apply_bc(c, *a[0], *a[1], *a[2]);
+    return MIST_OK;
+}
+
+int ora_interpolate_face(void* h, int fu, int fv, int fw) {
+    Ctx& c = *(Ctx*)h;
+    Arr3<double>* a[3];
+    if (!velocity_triple(c, fu, fv, fw, a)) return MIST_ERR_PARAM;
+    interpolate_face(c, *a[0], *a[1], *a[2]);
+    return MIST_OK;
+}
+
+int ora_interpolate_center(void* h) { interpolate_center(*(Ctx*)h); return MIST_OK; }
+int ora_divergence(void* h, double dt) { divergence(*(Ctx*)h, dt); return MIST_OK; }
+int ora_compute_curvature(void* h) { compute_curvature(*(Ctx*)h); return MIST_OK; }
+int ora_surface_force(void* h) {
+    Ctx& c = *(Ctx*)h;
+    mask_maker(c);
+    compute_sf(c);
+    return MIST_OK;
+}
+
+// run_solver initialisation with solveNS (src/NavierStokes_Parallel.jl:73-105): Neumann!(VF),
+// BC!(u,v,w), update_borders! of u, v, w and VF, interpolateFace!, compute_props!,
+// computeBand!, computeNormal!, computePLIC!, and the divergence diagnostic at the first dt.
+int ora_ns_init(void* h) {
+    Ctx& c = *(Ctx*)h;
+    neumann(c, c.VF);
+    apply_bc(c, c.u, c.v, c.w);
+    update_borders(c, c.u);
+    update_borders(c, c.v);
+    update_borders(c, c.w);
+    update_borders(c, c.VF);
+    interpolate_face(c, c.u, c.v, c.w);
+    compute_props(c);
+    compute_band(c);
+    compute_normal(c);
+    compute_plic(c);
+    divergence(c, compute_dt(c));
+    return MIST_OK;
+}
+
+int ora_ns_step(void* h, double* t, double clamp_to, double* dt_used, double tol, int64_t precision,
+                int64_t* cycles, double* resid) {
+    return ns_step(*(Ctx*)h, t, clamp_to, dt_used, tol, (int)precision, cycles, resid);
+}
+
+double ora_sl_flux_vol(void* h, int dir, int64_t i, int64_t j, int64_t k, double dt, double verts[24]) {
+    double v[8][3];
+    const double r = sl_flux_vol(*(Ctx*)h, dir, i, j, k, dt, v);
+    std::memcpy(verts, v, sizeof(v));
+    return r;
+}
+
+// the 17 tets of an FD-scheme band cell: its 5 back-traced tets, then the 12 correction tets
+int ora_correction_tets(void* h, int64_t i, int64_t j, int64_t k, double dt, double out[17 * 12]) {
+    const Ctx& c = *(const Ctx*)h;
+    double tets[17][4][3] = {}, verts[8][3], t5[5][4][3];
+    int64_t inds[17][4][3];
+    cell2verts(c, i, j, k, verts);
+    for (int n = 0; n < 8; ++n) project(c, verts[n], i, j, k, dt);
+    verts2tets5(verts, t5);
+    std::memcpy(tets, t5, sizeof(t5));
+    const bool ok = add_correction_tets(c, i, j, k, dt, tets, inds);
+    std::memcpy(out, tets, sizeof(tets));
+    return ok ? MIST_OK : MIST_ERR_OVERFLOW;
 }
 
 // Reductions over interior cells in FP64 (sum in long double for a stable checker).

@@ -92,7 +92,10 @@ struct Ctx {
     mist_params p{};
     Mesh m;
     Arr3<double> VF, nx, ny, nz, D, u, v, w, uf, vf, wf, P, divg, denx, deny, denz, RHS;
+    Arr3<double> us, vs, ws, viscx, viscy, viscz, Curve, sfx, sfy, sfz;
     Arr3<int16_t> band;
+    Arr3<int16_t> mask;          // bits 0..2: mask[i,j,k][1..3] of the reference
+    double bc_wall[6][3] = {};  // wall velocity of sides x-, x+, y-, y+, z-, z+
     std::string err;
     int status = 0;
     int64_t max_cut_level = 0;
@@ -124,6 +127,7 @@ void elvira_calc(const Ctx& c, int64_t i, int64_t j, int64_t k, double* onx, dou
 
 // ---- stages ----------------------------------------------------------------------
 Arr3<double>* field_f64(Ctx& c, int field);
+Arr3<int16_t>* field_i16(Ctx& c, int field);
 void neumann(Ctx& c, Arr3<double>& A);
 void neumann_i16(Ctx& c, Arr3<int16_t>& A);
 void update_borders(Ctx& c, Arr3<double>& A);
@@ -138,6 +142,22 @@ void compute_dens(Ctx& c);
 void pressure_rhs(Ctx& c);
 int pressure_solve(Ctx& c, double dt, double tol, int precision, int64_t* cycles, double* resid);
 void corrector(Ctx& c, double dt);
+void compute_props(Ctx& c);
+double sl_flux_vol(const Ctx& c, int dir, int64_t i, int64_t j, int64_t k, double dt, double verts[8][3]);
+bool add_correction_tets(const Ctx& c, int64_t i, int64_t j, int64_t k, double dt, double tets[17][4][3],
+                         int64_t inds[17][4][3]);
+void compute_curvature(Ctx& c);
+void mask_maker(Ctx& c);
+void compute_sf(Ctx& c);
+void apply_bc(Ctx& c, Arr3<double>& u, Arr3<double>& v, Arr3<double>& w);
+void interpolate_face(Ctx& c, const Arr3<double>& u, const Arr3<double>& v, const Arr3<double>& w);
+void interpolate_center(Ctx& c);
+void divergence(Ctx& c, double dt);
+int transport_ns(Ctx& c, double dt);
+int ns_step(Ctx& c, double* t, double clamp_to, double* dt_used, double tol, int precision, int64_t* cycles,
+            double* resid);
+CutScratch* new_cut_scratch();
+void delete_cut_scratch(CutScratch* s);
 double vf_circle(double xmin, double xmax, double ymin, double ymax, double rad, double xo, double yo);
 double vf_sphere(double xmin, double xmax, double ymin, double ymax, double zmin, double zmax,
                  double rad, double xo, double yo, double zo, bool bubble);

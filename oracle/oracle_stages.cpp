@@ -11,6 +11,12 @@ namespace ora {
 CutScratch* new_cut_scratch();
 void delete_cut_scratch(CutScratch* s);
 
+Arr3<int16_t>* field_i16(Ctx& c, int f) {
+    if (f == MIST_FIELD_BAND) return &c.band;
+    if (f == MIST_FIELD_MASK) return &c.mask;
+    return nullptr;
+}
+
 Arr3<double>* field_f64(Ctx& c, int f) {
     switch (f) {
         case MIST_FIELD_VOF: return &c.VF;
@@ -30,6 +36,16 @@ Arr3<double>* field_f64(Ctx& c, int f) {
         case MIST_FIELD_DENY: return &c.deny;
         case MIST_FIELD_DENZ: return &c.denz;
         case MIST_FIELD_RHS: return &c.RHS;
+        case MIST_FIELD_US: return &c.us;
+        case MIST_FIELD_VS: return &c.vs;
+        case MIST_FIELD_WS: return &c.ws;
+        case MIST_FIELD_VISCX: return &c.viscx;
+        case MIST_FIELD_VISCY: return &c.viscy;
+        case MIST_FIELD_VISCZ: return &c.viscz;
+        case MIST_FIELD_CURV: return &c.Curve;
+        case MIST_FIELD_SFX: return &c.sfx;
+        case MIST_FIELD_SFY: return &c.sfy;
+        case MIST_FIELD_SFZ: return &c.sfz;
         default: return nullptr;
     }
 }
