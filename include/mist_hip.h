@@ -148,8 +148,13 @@ int mist_compute_dt(mist_ctx* ctx, double* dt);         /* compute_dt  src/Tools
 int mist_compute_band(mist_ctx* ctx);                   /* computeBand!  src/VF.jl:16-74 */
 int mist_compute_normal(mist_ctx* ctx);                 /* computeNormal!/ELVIRA!  src/VF.jl:79, src/ELVIRA.jl:6-137 */
 int mist_compute_plic(mist_ctx* ctx);                   /* computePLIC!  src/VF.jl:94-103 */
-int mist_transport(mist_ctx* ctx, double dt, double t); /* VOF part of transport!  src/Transport.jl:2-268 */
-int mist_compute_props(mist_ctx* ctx);                  /* compute_dens!  src/Tools.jl:1212-1232 */
+/* transport!  src/Transport.jl:2-268. With solveNS = 0 the VOF part (new VF, normals, PLIC);
+ * with solveNS the momentum predictor too: us, vs, ws from the cut pre-images in the band
+ * (plus the 12 flux-correction tets per cell under the finite-difference scheme) and from
+ * convective fluxes elsewhere, then curvature, mask, surface force, viscous fluxes, gravity
+ * and the reflect boundary condition on us, vs, ws. */
+int mist_transport(mist_ctx* ctx, double dt, double t);
+int mist_compute_props(mist_ctx* ctx);                  /* compute_props!  src/Tools.jl:1186-1210 */
 int mist_pressure_rhs(mist_ctx* ctx);                   /* RHS = div(u*)  src/Pressure.jl:16-22 */
 /* Variable-density Poisson solve  A p = b,  A = dt div(rho_face^-1 grad) with wall faces
  * dropped (src/Pressure.jl:909-1033 scaling), b = RHS field, to ||b - A p||_inf <= tol.
@@ -169,6 +174,36 @@ int mist_vof_step(mist_ctx* ctx, double* t, double clamp_to, double* dt_used);
 /* Initialisation of run_solver for the VOF fields (src/NavierStokes_Parallel.jl:73-98):
  * Neumann!(VF), update_VF_borders!, computeBand!, computeNormal!, computePLIC!. */
 int mist_vof_init(mist_ctx* ctx);
+
+/* ---- Navier-Stokes stages (run_solver with solveNS) ------------------------------- */
+/* Sides of the boundary-condition descriptor. */
+enum { MIST_SIDE_XM = 0, MIST_SIDE_XP = 1, MIST_SIDE_YM = 2, MIST_SIDE_YP = 3, MIST_SIDE_ZM = 4, MIST_SIDE_ZP = 5 };
+/* Wall velocity of one side (default 0). Replaces the BC! closure of the examples
+ * (examples/example_viscous_bubble3D.jl:96-146, examples/example1.jl:94-136): on every
+ * non-periodic side the first ghost layer of each velocity component becomes
+ * -interior + 2 U_wall, axes in the order x, y, z. */
+int mist_set_wall_velocity(mist_ctx* ctx, int side, const double vel[3]);
+/* BC! on (MIST_FIELD_U, _V, _W) or (MIST_FIELD_US, _VS, _WS). */
+int mist_apply_bc(mist_ctx* ctx, int fu, int fv, int fw);
+/* interpolateFace!  src/Velocity.jl:23-37: uf, vf, wf from (u, v, w) or (us, vs, ws). */
+int mist_interpolate_face(mist_ctx* ctx, int fu, int fv, int fw);
+int mist_interpolate_center(mist_ctx* ctx);             /* interpolateCenter!  src/Velocity.jl:39-57 */
+int mist_divergence(mist_ctx* ctx, double dt);          /* divergence!  src/Velocity.jl:59-122 */
+int mist_compute_curvature(mist_ctx* ctx);              /* compute_curvature!  src/ELVIRA.jl:143-192 */
+/* mask_maker! + compute_sf!  src/Tools.jl:1582-1598, src/ELVIRA.jl:256-299 (the mask is
+ * never cleared, as in the reference). */
+int mist_surface_force(mist_ctx* ctx);
+/* Initialisation of run_solver with solveNS (src/NavierStokes_Parallel.jl:73-105):
+ * Neumann!(VF), BC!(u,v,w), update_borders! of u, v, w and VF, interpolateFace!(u,v,w),
+ * compute_props!, computeBand!, computeNormal!, computePLIC!, divergence! at the first dt. */
+int mist_ns_init(mist_ctx* ctx);
+/* One step of run_solver with solveNS (src/NavierStokes_Parallel.jl:125-168): dt, transport!,
+ * computeBand!, compute_props!, interpolateFace!(us,vs,ws), the pressure solve (as
+ * mist_pressure_solve with b = div(u*)), corrector!, interpolateCenter!, update_borders!, and
+ * divergence!. Needs pressure_scheme = finite-difference (the semi-Lagrangian scheme's
+ * nonlinear pressure equation is not built: MIST_ERR_PARAM). */
+int mist_ns_step(mist_ctx* ctx, double* t, double clamp_to, double* dt_used, double tol, int64_t precision,
+                 int64_t* cycles, double* final_resid);
 
 /* Batched PLIC primitives (first slice; host arrays of length m).
  * computeDist (src/VF.jl:109-156) for cells (i,j,k) with normals n and fractions vf. */
@@ -194,7 +229,9 @@ enum { MIST_PROF_ELVIRA = 0,     /* units: interface cells scored */
        MIST_PROF_MG_RBGS = 4,    /* units: cells of the finest level per red-black sweep */
        MIST_PROF_MG_RESID = 5,   /* units: cells */
        MIST_PROF_SL_GENERAL = 6, /* units: band cells left to the general cut walk */
-       MIST_PROF_COUNT = 7 };
+       MIST_PROF_NS_CUT = 7,     /* units: band cells of the momentum predictor */
+       MIST_PROF_NS_FIELDS = 8,  /* units: interior cells per stencil pass of the predictor */
+       MIST_PROF_COUNT = 9 };
 int mist_profile_enable(mist_ctx* ctx, int on);   /* on != 0 also resets the totals */
 int mist_profile_read(mist_ctx* ctx, int kernel, double* total_ms, int64_t* launches, int64_t* units);
 

@@ -238,10 +238,10 @@ int mist_ctx_create(const mist_params* pin, int device, mist_ctx** out) {
     for (int f = 0; f < MIST_FIELD_COUNT; ++f) {
         int64_t lo = imino, n[3] = {m.cnx, m.cny, m.cnz}, el = 8;
         if (f == MIST_FIELD_VOF) { lo = imino - 3; n[0] += 6; n[1] += 6; n[2] += 6; }
-        if (f == MIST_FIELD_DENX) n[0] += 1;
-        if (f == MIST_FIELD_DENY) n[1] += 1;
-        if (f == MIST_FIELD_DENZ) n[2] += 1;
-        if (f == MIST_FIELD_BAND) el = 2;
+        if (f == MIST_FIELD_DENX || f == MIST_FIELD_VISCX || f == MIST_FIELD_SFX) n[0] += 1;
+        if (f == MIST_FIELD_DENY || f == MIST_FIELD_VISCY || f == MIST_FIELD_SFY) n[1] += 1;
+        if (f == MIST_FIELD_DENZ || f == MIST_FIELD_VISCZ || f == MIST_FIELD_SFZ) n[2] += 1;
+        if (f == MIST_FIELD_BAND || f == MIST_FIELD_MASK) el = 2;
         for (int d = 0; d < 3; ++d) { ctx->flo[f][d] = lo; ctx->fn[f][d] = n[d]; }
         ctx->felem[f] = el;
         const size_t bytes = (size_t)(n[0] * n[1] * n[2] * el);
@@ -324,10 +324,16 @@ int mist_init_vof(mist_ctx* ctx, int shape, const double* args, int nargs) {
     return MIST_OK;
 }
 
+// face arrays (one extra face along their axis) and the int16 mask take no ghost fills
+static bool face_field(int f) {
+    return f == MIST_FIELD_DENX || f == MIST_FIELD_DENY || f == MIST_FIELD_DENZ || f == MIST_FIELD_VISCX ||
+           f == MIST_FIELD_VISCY || f == MIST_FIELD_VISCZ || f == MIST_FIELD_SFX || f == MIST_FIELD_SFY ||
+           f == MIST_FIELD_SFZ;
+}
+
 int mist_neumann(mist_ctx* ctx, int field) {
     CHECK_CTX();
-    if (field < 0 || field >= MIST_FIELD_COUNT || field == MIST_FIELD_DENX || field == MIST_FIELD_DENY ||
-        field == MIST_FIELD_DENZ)
+    if (field < 0 || field >= MIST_FIELD_COUNT || face_field(field) || field == MIST_FIELD_MASK)
         return fail(ctx, MIST_ERR_PARAM, "Neumann! applies to cell fields");
     RET_IF(mist::launch_neumann(ctx, field));
     if (field == MIST_FIELD_VOF) ctx->vf_version++;
@@ -336,8 +342,7 @@ int mist_neumann(mist_ctx* ctx, int field) {
 
 int mist_update_borders(mist_ctx* ctx, int field) {
     CHECK_CTX();
-    if (field < 0 || field >= MIST_FIELD_COUNT || field == MIST_FIELD_DENX || field == MIST_FIELD_DENY ||
-        field == MIST_FIELD_DENZ)
+    if (field < 0 || field >= MIST_FIELD_COUNT || face_field(field) || field == MIST_FIELD_MASK)
         return fail(ctx, MIST_ERR_PARAM, "update_borders! applies to cell fields");
     RET_IF(mist::launch_borders(ctx, field));
     if (field == MIST_FIELD_VOF && (ctx->p.xper || ctx->p.yper || ctx->p.zper)) ctx->vf_version++;
@@ -431,11 +436,34 @@ static int transport_body(mist_ctx* ctx, double dt) {
     return MIST_OK;
 }
 
+// transport! with the momentum predictor (src/Transport.jl:2-268)
+static int transport_ns_body(mist_ctx* ctx, double dt) {
+    RET_IF(mist_compute_normal(ctx));
+    if (!ctx->band_list_valid) RET_IF(mist_compute_band(ctx));
+    {
+        StageTimer tm(ctx, MIST_STAGE_TRANSPORT);
+        RET_IF(mist::launch_ns_advect(ctx, dt));
+    }
+    ctx->vf_version++;
+    RET_IF(mist_compute_normal(ctx));
+    {
+        StageTimer tm(ctx, MIST_STAGE_TRANSPORT);
+        RET_IF(mist::launch_ns_forces(ctx, dt));
+        RET_IF(mist::launch_reflect_bc(ctx, MIST_FIELD_US, MIST_FIELD_VS, MIST_FIELD_WS));
+    }
+    RET_IF(mist_compute_plic(ctx));
+    RET_IF(mist::launch_borders(ctx, MIST_FIELD_US));
+    RET_IF(mist::launch_borders(ctx, MIST_FIELD_VS));
+    RET_IF(mist::launch_borders(ctx, MIST_FIELD_WS));
+    ctx->band_list_valid = false;
+    return MIST_OK;
+}
+
 int mist_transport(mist_ctx* ctx, double dt, double t) {
     CHECK_CTX();
     (void)t;
     MIST_HIP_TRY(hipSetDevice(ctx->device));
-    RET_IF(transport_body(ctx, dt));
+    RET_IF(ctx->p.solveNS ? transport_ns_body(ctx, dt) : transport_body(ctx, dt));
     RET_IF(check_overflow(ctx));
     return MIST_OK;
 }
@@ -471,8 +499,132 @@ int mist_vof_step(mist_ctx* ctx, double* t, double clamp_to, double* dt_used) {
 
 int mist_compute_props(mist_ctx* ctx) {
     CHECK_CTX();
-    RET_IF(mist::launch_dens(ctx));
+    RET_IF(mist::launch_props(ctx));
     return MIST_OK;
+}
+
+// ---- Navier-Stokes stages ------------------------------------------------------------
+int mist_set_wall_velocity(mist_ctx* ctx, int side, const double vel[3]) {
+    CHECK_CTX();
+    if (side < 0 || side > 5 || !vel) return fail(ctx, MIST_ERR_PARAM, "side must be 0..5 and vel non-null");
+    for (int q = 0; q < 3; ++q) ctx->bc_wall[side][q] = vel[q];
+    return MIST_OK;
+}
+
+static bool velocity_triple(int fu, int fv, int fw) {
+    return (fu == MIST_FIELD_U && fv == MIST_FIELD_V && fw == MIST_FIELD_W) ||
+           (fu == MIST_FIELD_US && fv == MIST_FIELD_VS && fw == MIST_FIELD_WS);
+}
+
+int mist_apply_bc(mist_ctx* ctx, int fu, int fv, int fw) {
+    CHECK_CTX();
+    if (!velocity_triple(fu, fv, fw)) return fail(ctx, MIST_ERR_PARAM, "BC! takes (u, v, w) or (us, vs, ws)");
+    MIST_HIP_TRY(hipSetDevice(ctx->device));
+    RET_IF(mist::launch_reflect_bc(ctx, fu, fv, fw));
+    return MIST_OK;
+}
+
+int mist_interpolate_face(mist_ctx* ctx, int fu, int fv, int fw) {
+    CHECK_CTX();
+    if (!velocity_triple(fu, fv, fw))
+        return fail(ctx, MIST_ERR_PARAM, "interpolateFace! takes (u, v, w) or (us, vs, ws)");
+    MIST_HIP_TRY(hipSetDevice(ctx->device));
+    RET_IF(mist::launch_interp_face(ctx, fu, fv, fw));
+    return MIST_OK;
+}
+
+int mist_interpolate_center(mist_ctx* ctx) {
+    CHECK_CTX();
+    MIST_HIP_TRY(hipSetDevice(ctx->device));
+    RET_IF(mist::launch_interp_center(ctx));
+    return MIST_OK;
+}
+
+int mist_divergence(mist_ctx* ctx, double dt) {
+    CHECK_CTX();
+    if (!(dt > 0.0)) return fail(ctx, MIST_ERR_PARAM, "dt must be > 0");
+    MIST_HIP_TRY(hipSetDevice(ctx->device));
+    if (ctx->p.pressure_scheme == MIST_PS_SEMI_LAGRANGIAN && !ctx->band_list_valid) RET_IF(mist_compute_band(ctx));
+    RET_IF(mist::launch_divergence(ctx, dt));
+    return MIST_OK;
+}
+
+int mist_compute_curvature(mist_ctx* ctx) {
+    CHECK_CTX();
+    MIST_HIP_TRY(hipSetDevice(ctx->device));
+    RET_IF(mist::launch_curvature(ctx));
+    return MIST_OK;
+}
+
+int mist_surface_force(mist_ctx* ctx) {
+    CHECK_CTX();
+    MIST_HIP_TRY(hipSetDevice(ctx->device));
+    RET_IF(mist::launch_surface_force(ctx));
+    return MIST_OK;
+}
+
+int mist_ns_init(mist_ctx* ctx) {
+    CHECK_CTX();
+    MIST_HIP_TRY(hipSetDevice(ctx->device));
+    RET_IF(mist::launch_neumann(ctx, MIST_FIELD_VOF));
+    RET_IF(mist::launch_reflect_bc(ctx, MIST_FIELD_U, MIST_FIELD_V, MIST_FIELD_W));
+    RET_IF(mist::launch_borders(ctx, MIST_FIELD_U));
+    RET_IF(mist::launch_borders(ctx, MIST_FIELD_V));
+    RET_IF(mist::launch_borders(ctx, MIST_FIELD_W));
+    RET_IF(mist::launch_borders(ctx, MIST_FIELD_VOF));
+    ctx->vf_version++;
+    RET_IF(mist::launch_interp_face(ctx, MIST_FIELD_U, MIST_FIELD_V, MIST_FIELD_W));
+    RET_IF(mist::launch_props(ctx));
+    RET_IF(mist_compute_band(ctx));
+    RET_IF(mist_compute_normal(ctx));
+    RET_IF(mist_compute_plic(ctx));
+    double mx[3];
+    RET_IF(mist::launch_maxabs_uvw(ctx, mx));
+    RET_IF(mist::launch_divergence(ctx, dt_from_max(ctx, mx)));
+    return MIST_OK;
+}
+
+int mist_ns_step(mist_ctx* ctx, double* t, double clamp_to, double* dt_used, double tol, int64_t precision,
+                 int64_t* cycles, double* final_resid) {
+    CHECK_CTX();
+    if (!t) return fail(ctx, MIST_ERR_PARAM, "null t");
+    if (ctx->p.pressure_scheme != MIST_PS_FINITE_DIFFERENCE)
+        return fail(ctx, MIST_ERR_PARAM, "mist_ns_step: the semi-Lagrangian pressure equation is not built; "
+                                         "use pressure_scheme = finite-difference");
+    if (precision != MIST_MG_FP64 && precision != MIST_MG_MIXED) return fail(ctx, MIST_ERR_PARAM, "precision");
+    if (!(tol > 0.0)) return fail(ctx, MIST_ERR_PARAM, "tol must be > 0");
+    MIST_HIP_TRY(hipSetDevice(ctx->device));
+    double mx[3];
+    RET_IF(mist::launch_maxabs_uvw(ctx, mx));  // synchronises: the previous step is complete
+    RET_IF(check_overflow(ctx));
+    double dt = dt_from_max(ctx, mx);
+    if (clamp_to > 0.0 && *t + dt > clamp_to) dt = clamp_to - *t;
+    *t += dt;
+    if (dt_used) *dt_used = dt;
+    RET_IF(transport_ns_body(ctx, dt));
+    RET_IF(mist_compute_band(ctx));
+    RET_IF(mist::launch_props(ctx));
+    RET_IF(mist::launch_interp_face(ctx, MIST_FIELD_US, MIST_FIELD_VS, MIST_FIELD_WS));
+    RET_IF(mist::launch_pressure_rhs(ctx));
+    int64_t cyc = 0;
+    double res = 0.0;
+    int st;
+    {
+        StageTimer tm(ctx, MIST_STAGE_MG);
+        st = mist::pressure_solve(ctx, dt, tol, (int)precision, &cyc, &res);
+    }
+    if (cycles) *cycles = cyc;
+    if (final_resid) *final_resid = res;
+    if (st && st != MIST_ERR_NOCONV) { g_last_error = ctx->err; return st; }
+    RET_IF(mist::launch_corrector(ctx, dt));
+    RET_IF(mist::launch_interp_center(ctx));
+    RET_IF(mist::launch_borders(ctx, MIST_FIELD_U));
+    RET_IF(mist::launch_borders(ctx, MIST_FIELD_V));
+    RET_IF(mist::launch_borders(ctx, MIST_FIELD_W));
+    RET_IF(mist::launch_divergence(ctx, dt));
+    RET_IF(check_overflow(ctx));
+    if (st) g_last_error = ctx->err;
+    return st;
 }
 
 int mist_pressure_rhs(mist_ctx* ctx) {
@@ -508,7 +660,7 @@ int mist_corrector(mist_ctx* ctx, double dt) {
 int mist_reduce(mist_ctx* ctx, int kind, int field, double* out) {
     CHECK_CTX();
     if (!out || kind < 0 || kind > 3 || field < 0 || field >= MIST_FIELD_COUNT || field == MIST_FIELD_BAND ||
-        field == MIST_FIELD_DENX || field == MIST_FIELD_DENY || field == MIST_FIELD_DENZ)
+        field == MIST_FIELD_MASK || face_field(field))
         return fail(ctx, MIST_ERR_PARAM, "bad reduction");
     RET_IF(mist::launch_reduce(ctx, kind, field, out));
     return MIST_OK;

@@ -76,6 +76,7 @@ struct mist_ctx {
     double stage_ms[MIST_STAGE_COUNT] = {};
     bool timing = false;
     KernelProfile prof;
+    double bc_wall[6][3] = {};   // wall velocity of sides x-, x+, y-, y+, z-, z+ (reflect BC)
     std::string err;
 };
 
@@ -92,14 +93,24 @@ int launch_borders(mist_ctx* ctx, int field);
 int launch_band(mist_ctx* ctx);
 int launch_normals(mist_ctx* ctx);
 int launch_plic(mist_ctx* ctx);
+int launch_project_vertices(mist_ctx* ctx, double dt);
 int launch_transport(mist_ctx* ctx, double dt);
 int launch_init_vof(mist_ctx* ctx, int shape, const double* args);
-int launch_dens(mist_ctx* ctx);
+int launch_props(mist_ctx* ctx);
 int launch_pressure_rhs(mist_ctx* ctx);
 int launch_corrector(mist_ctx* ctx, double dt);
 int launch_reduce(mist_ctx* ctx, int kind, int field, double* out);
 int launch_plic_dist_batch(mist_ctx* ctx, const int64_t* ijk, const double* nrm, const double* vf, double* out, int64_t m);
 int launch_plic_vf_batch(mist_ctx* ctx, const int64_t* ijk, const double* nrm, const double* dist, double* out, int64_t m);
+// mist_ns.hip
+int launch_ns_advect(mist_ctx* ctx, double dt);
+int launch_ns_forces(mist_ctx* ctx, double dt);
+int launch_curvature(mist_ctx* ctx);
+int launch_surface_force(mist_ctx* ctx);
+int launch_reflect_bc(mist_ctx* ctx, int fu, int fv, int fw);
+int launch_interp_face(mist_ctx* ctx, int fu, int fv, int fw);
+int launch_interp_center(mist_ctx* ctx);
+int launch_divergence(mist_ctx* ctx, double dt);
 // mist_mg.hip
 int pressure_solve(mist_ctx* ctx, double dt, double tol, int precision, int64_t* cycles, double* resid);
 }  // namespace mist

@@ -7,43 +7,9 @@
 #include <vector>
 
 #include "mist_internal.h"
+#include "mist_kernels.h"
 
 namespace mist {
-
-constexpr double PI = 3.141592653589793;
-constexpr int CUT_MAXLVL = 16;
-
-__device__ __forceinline__ void cell_of(const DevMesh& m, int64_t q, int& i, int& j, int& k) {
-    const int a = (int)(q % m.cnx);
-    const int64_t r = q / m.cnx;
-    const int b = (int)(r % m.cny);
-    const int c = (int)(r / m.cny);
-    i = a + 1 - m.ng; j = b + 1 - m.ng; k = c + 1 - m.ng;
-}
-__device__ __forceinline__ void interior_of(const DevMesh& m, int q, int& i, int& j, int& k) {
-    i = q % m.Nx + 1;
-    const int r = q / m.Nx;
-    j = r % m.Ny + 1;
-    k = r / m.Ny + 1;
-}
-__device__ __forceinline__ int interior_id(const DevMesh& m, int i, int j, int k) {
-    return (i - 1) + m.Nx * ((j - 1) + m.Ny * (k - 1));
-}
-
-// Wave-aggregated append of `flag` items to list[*count].
-__device__ __forceinline__ void append(bool flag, int item, int* list, int* count) {
-    const unsigned long long mask = __ballot(flag);
-    if (mask == 0) return;
-    const int lane = __lane_id();
-    const int leader = __ffsll((long long)mask) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(count, __popcll(mask));
-    base = __shfl(base, leader);
-    if (flag) {
-        const unsigned long long lower = mask & ((1ull << lane) - 1ull);
-        list[base + __popcll(lower)] = item;
-    }
-}
 
 // ------------------------------------------------------------------------------------
 // defineVelocity! (src/Tools.jl:648-693), period T of cos(pi t / T) as a parameter
@@ -590,45 +556,6 @@ __global__ void k_plic(DevMesh m, DevParams p, const double* VF, const double* n
     }
 }
 
-// ------------------------------------------------------------------------------------
-// Back-trace of cell vertices (project!, src/Tools.jl:392-418). The back-traced position of
-// a vertex does not depend on which cell asks for it (the interpolation index search
-// converges to the same cell from any start), so each vertex of a band cell is traced once.
-// ------------------------------------------------------------------------------------
-__device__ void project_pt(const DevMesh& m, const DevParams& p, const double* uf, const double* vf,
-                           const double* wf, double pt[3], int i, int j, int k, double dt) {
-    if (p.zalesak) {
-        const double pi2 = 2.0 * PI;
-        const double x0 = pt[0], y0 = pt[1];
-        pt[0] = (0.5 + cos(pi2 * dt) * (x0 - 0.5)) + sin(pi2 * dt) * (y0 - 0.5);
-        pt[1] = (0.5 - sin(pi2 * dt) * (x0 - 0.5)) + cos(pi2 * dt) * (y0 - 0.5);
-        return;
-    }
-    double v1[3], v2[3], v3[3], v4[3], q[3];
-    if (p.projection == MIST_PROJ_RK4) {
-        vel_face(m, uf, vf, wf, pt, i, j, k, v1);
-        for (int n = 0; n < 3; ++n) q[n] = pt[n] - (0.5 * dt) * v1[n];
-        vel_face(m, uf, vf, wf, q, i, j, k, v2);
-        for (int n = 0; n < 3; ++n) q[n] = pt[n] - (0.5 * dt) * v2[n];
-        vel_face(m, uf, vf, wf, q, i, j, k, v3);
-        for (int n = 0; n < 3; ++n) q[n] = pt[n] - dt * v3[n];
-        vel_face(m, uf, vf, wf, q, i, j, k, v4);
-        for (int n = 0; n < 3; ++n) pt[n] = pt[n] + ((-dt) / 6.0) * (((v1[n] + 2.0 * v2[n]) + 2.0 * v3[n]) + v4[n]);
-    } else if (p.projection == MIST_PROJ_EULER) {
-        vel_face(m, uf, vf, wf, pt, i, j, k, v1);
-        for (int n = 0; n < 3; ++n) pt[n] = pt[n] + (-dt) * v1[n];
-    } else {
-        vel_face(m, uf, vf, wf, pt, i, j, k, v1);
-        for (int n = 0; n < 3; ++n) q[n] = pt[n] - (0.5 * dt) * v1[n];
-        vel_face(m, uf, vf, wf, q, i, j, k, v2);
-        for (int n = 0; n < 3; ++n) pt[n] = pt[n] + (-dt) * v2[n];
-    }
-}
-
-__device__ __forceinline__ int64_t vtx_id(const DevMesh& m, int I, int J, int K) {
-    return (int64_t)(I - 1) + (int64_t)(m.Nx + 1) * ((int64_t)(J - 1) + (int64_t)(m.Ny + 1) * (K - 1));
-}
-
 __global__ void k_project_vertices(DevMesh m, DevParams p, const int16_t* band, const double* uf, const double* vf,
                                    const double* wf, double dt, double* pv) {
     const int64_t n = (int64_t)(m.Nx + 1) * (m.Ny + 1) * (m.Nz + 1);
@@ -637,11 +564,12 @@ __global__ void k_project_vertices(DevMesh m, DevParams p, const int16_t* band, 
         const int64_t r = q / (m.Nx + 1);
         const int J = (int)(r % (m.Ny + 1)) + 1;
         const int K = (int)(r / (m.Ny + 1)) + 1;
+        // every vertex of a band cell, ghost cells included (a periodic image of a band cell
+        // puts an SL flux on the boundary face of its interior neighbour)
         bool need = false;
         for (int c = 0; c < 8; ++c) {
             const int i = I - 1 + (c & 1), j = J - 1 + ((c >> 1) & 1), k = K - 1 + (c >> 2);
-            if (i >= 1 && i <= m.Nx && j >= 1 && j <= m.Ny && k >= 1 && k <= m.Nz)
-                need |= abs((int)band[m.cell(i, j, k)]) <= p.nband;
+            need |= abs((int)band[m.cell(i, j, k)]) <= p.nband;
         }
         if (!need) continue;
         double pt[3] = {m.x[I], m.y[J], m.z[K]};
@@ -649,153 +577,6 @@ __global__ void k_project_vertices(DevMesh m, DevParams p, const int16_t* band, 
         pv[3 * q + 0] = pt[0];
         pv[3 * q + 1] = pt[1];
         pv[3 * q + 2] = pt[2];
-    }
-}
-
-// ------------------------------------------------------------------------------------
-// cutTet (src/VFgeom.jl:713-900) as an explicit depth-first walk. Each frame keeps what a
-// recursion level of the reference keeps in vert[:,:,lvl] and vert_ind[:,:,:,lvl], plus the
-// running sums of its children, so the sums nest exactly as the recursion adds them.
-// ------------------------------------------------------------------------------------
-struct CutFrame {
-    double v[8][3];
-    short vi[2][8][3];
-    double acc_vol, acc_liq;
-    unsigned char cs, child, nchild, flags;
-};
-
-__device__ __forceinline__ void cut_leaf(const DevMesh& m, const double* nx, const double* ny, const double* nz,
-                                         const double* D, const double tet[4][3], const int ind0[3], double& vol,
-                                         double& liq) {
-    const int64_t c = m.cell(ind0[0], ind0[1], ind0[2]);
-    const double n0 = nx[c], n1 = ny[c], n2 = nz[c], Dc = D[c];
-    double vert[8][3], d[4], dd[4];
-#pragma unroll
-    for (int n = 0; n < 4; ++n) {
-        vert[n][0] = tet[n][0]; vert[n][1] = tet[n][1]; vert[n][2] = tet[n][2];
-        d[n] = ((n0 * vert[n][0] + n1 * vert[n][1]) + n2 * vert[n][2]) - Dc;
-        dd[n] = d[n];
-    }
-    const int cs = d2case_snap(dd);  // the reference snaps a copy here
-    const int nv = CUT_NVERT[cs];
-    for (int n = 0; n < nv; ++n) {
-        const unsigned e = CUT_EDGE[cs][n];
-        const int v1 = e & 15, v2 = e >> 4;
-        const double mu = jmin(1.0, jmax(0.0, sdiv(-d[v1], d[v2] - d[v1])));
-        vert[4 + n][0] = (1.0 - mu) * vert[v1][0] + mu * vert[v2][0];
-        vert[4 + n][1] = (1.0 - mu) * vert[v1][1] + mu * vert[v2][1];
-        vert[4 + n][2] = (1.0 - mu) * vert[v1][2] + mu * vert[v2][2];
-    }
-    double V = 0.0, L = 0.0;
-    const int hi = CUT_NTETS[cs], lo = CUT_NNTET[cs];
-    for (int n = hi; n >= lo; --n) {
-        const unsigned q = CUT_VTET[cs][n - 1];
-        const double tv = tet_vol(vert[nib(q, 0)], vert[nib(q, 1)], vert[nib(q, 2)], vert[nib(q, 3)]);
-        V += tv; L += tv;
-    }
-    for (int n = 1; n <= lo - 1; ++n) {
-        const unsigned q = CUT_VTET[cs][n - 1];
-        V += tet_vol(vert[nib(q, 0)], vert[nib(q, 1)], vert[nib(q, 2)], vert[nib(q, 3)]);
-    }
-    vol = V;
-    liq = L;
-}
-
-// Returns false when the walk needs more than CUT_MAXLVL levels or a walk leaves the mesh.
-__device__ bool cut_tet(const DevMesh& m, const double* nx, const double* ny, const double* nz, const double* D,
-                        double tet[4][3], int ind[4][3], CutFrame* st, double& out_vol, double& out_liq,
-                        int& depth) {
-    int top = -1;
-    unsigned flags = 0;  // bit 0/1/2: x/y/z cuts done
-    bool have_node = true;
-    int ovf = 0;
-    depth = 0;
-    for (;;) {
-        if (have_node) {
-            int dir = -1, cut_ind = 0;
-            for (int a = 0; a < 3; ++a) {
-                if (flags & (1u << a)) continue;
-                const int mx = max(max(ind[0][a], ind[1][a]), max(ind[2][a], ind[3][a]));
-                const int mn = min(min(ind[0][a], ind[1][a]), min(ind[2][a], ind[3][a]));
-                if (mx > mn) { dir = a; cut_ind = mx; break; }
-                flags |= 1u << a;
-            }
-            if (dir < 0) {
-                double lv, ll;
-                cut_leaf(m, nx, ny, nz, D, tet, ind[0], lv, ll);
-                if (top < 0) { out_vol = lv; out_liq = ll; return ovf == 0; }
-                st[top].acc_vol += lv;
-                st[top].acc_liq += ll;
-            } else {
-                if (top + 1 >= CUT_MAXLVL) return false;
-                CutFrame& F = st[++top];
-                depth = max(depth, top + 1);
-                const double* cc = dir == 0 ? m.x : (dir == 1 ? m.y : m.z);
-                const double plane = cc[cut_ind];
-                double d[4], dd[4];
-                for (int n = 0; n < 4; ++n) { d[n] = tet[n][dir] - plane; dd[n] = d[n]; }
-                const int cs = d2case_snap(dd);
-                for (int n = 0; n < 4; ++n) {
-                    for (int q = 0; q < 3; ++q) {
-                        F.v[n][q] = tet[n][q];
-                        F.vi[0][n][q] = (short)ind[n][q];
-                        F.vi[1][n][q] = (short)ind[n][q];
-                    }
-                    const int lo1 = min(ind[n][dir], cut_ind - 1);
-                    F.vi[0][n][dir] = (short)lo1;
-                    F.vi[1][n][dir] = (short)max(lo1, cut_ind);
-                }
-                const int nv = CUT_NVERT[cs];
-                for (int n = 0; n < nv; ++n) {
-                    const unsigned e = CUT_EDGE[cs][n];
-                    const int v1 = e & 15, v2 = e >> 4;
-                    const double mu = jmin(1.0, jmax(0.0, sdiv(-d[v1], d[v2] - d[v1])));
-                    double pt[3];
-                    for (int q = 0; q < 3; ++q) {
-                        pt[q] = (1.0 - mu) * F.v[v1][q] + mu * F.v[v2][q];
-                        F.v[4 + n][q] = pt[q];
-                    }
-                    int I[3];
-                    pt2index(m, I, pt, F.vi[0][n][0], F.vi[0][n][1], F.vi[0][n][2], &ovf);
-                    for (int q = 0; q < 3; ++q) {
-                        int w = max(I[q], (int)min(F.vi[0][v1][q], F.vi[0][v2][q]));
-                        w = min(w, (int)max(F.vi[0][v1][q], F.vi[0][v2][q]));
-                        F.vi[0][4 + n][q] = (short)w;
-                        F.vi[1][4 + n][q] = (short)w;
-                    }
-                    F.vi[0][4 + n][dir] = (short)(cut_ind - 1);
-                    F.vi[1][4 + n][dir] = (short)cut_ind;
-                }
-                F.cs = (unsigned char)cs;
-                F.child = 0;
-                F.nchild = CUT_NTETS[cs];
-                F.flags = (unsigned char)flags;
-                F.acc_vol = 0.0;
-                F.acc_liq = 0.0;
-            }
-            have_node = false;
-        }
-        CutFrame& F = st[top];
-        if (F.child < F.nchild) {
-            const int n = F.child++;
-            const unsigned q = CUT_VTET[F.cs][n];
-            const int side = (CUT_SIDE2[F.cs] >> n) & 1;
-            for (int nn = 0; nn < 4; ++nn) {
-                const int v = nib(q, nn);
-                for (int a = 0; a < 3; ++a) {
-                    tet[nn][a] = F.v[v][a];
-                    ind[nn][a] = F.vi[side][v][a];
-                }
-            }
-            flags = F.flags;
-            have_node = true;
-        } else {
-            const double av = F.acc_vol, al = F.acc_liq;
-            --top;
-            if (top < 0) { out_vol = av; out_liq = al; return ovf == 0; }
-            st[top].acc_vol += av;
-            st[top].acc_liq += al;
-        }
     }
 }
 
@@ -827,9 +608,13 @@ __global__ __launch_bounds__(64) void k_sl_cut(DevMesh m, const double* pv, cons
                 tet[n][0] = pv[3 * vq + 0]; tet[n][1] = pv[3 * vq + 1]; tet[n][2] = pv[3 * vq + 2];
                 pt2index(m, ind[n], tet[n], i, j, k, &o);
             }
-            CutFrame st[CUT_MAXLVL];
+            CutFrame<2> st[CUT_MAXLVL];
             int depth = 0;
-            if (!cut_tet(m, nx, ny, nz, D, tet, ind, st, vol, liq, depth)) o = 1;
+            double r[2];
+            const LeafMomentum none{nullptr, nullptr, nullptr, 0.0, 0.0};
+            if (!cut_tet<2>(m, nx, ny, nz, D, none, tet, ind, st, r, depth)) o = 1;
+            vol = r[0];
+            liq = r[1];
             if (o) atomicOr(ovf, 1);
         }
         s_vol[lane] = vol;
@@ -1533,9 +1318,10 @@ __global__ void k_init_vof(DevMesh m, int shape, double a0, double a1, double a2
     }
 }
 
-// compute_dens! (src/Tools.jl:1212-1232)
-__global__ void k_dens(DevMesh m, double rl, double rg, const double* VF, double* denx, double* deny,
-                       double* denz) {
+// compute_props! (src/Tools.jl:1186-1210): face density and viscosity from the clamped face
+// average of VF
+__global__ void k_props(DevMesh m, double rl, double rg, double ml, double mg, const double* VF, double* denx,
+                        double* deny, double* denz, double* viscx, double* viscy, double* viscz) {
     const int64_t n = (int64_t)m.cnx * m.cny * m.cnz;
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
         int i, j, k;
@@ -1546,9 +1332,15 @@ __global__ void k_dens(DevMesh m, double rl, double rg, const double* VF, double
         const double vz = jmin(1.0, jmax(0.0, (vc + VF[m.vof(i, j, k - 1)]) / 2.0));
         // face arrays carry one extra face along their axis
         const int a = i - 1 + m.ng, b = j - 1 + m.ng, c = k - 1 + m.ng;
-        denx[a + (int64_t)(m.cnx + 1) * (b + (int64_t)m.cny * c)] = rl * vx + rg * (1.0 - vx);
-        deny[a + (int64_t)m.cnx * (b + (int64_t)(m.cny + 1) * c)] = rl * vy + rg * (1.0 - vy);
-        denz[a + (int64_t)m.cnx * (b + (int64_t)m.cny * c)] = rl * vz + rg * (1.0 - vz);
+        const int64_t fx = a + (int64_t)(m.cnx + 1) * (b + (int64_t)m.cny * c);
+        const int64_t fy = a + (int64_t)m.cnx * (b + (int64_t)(m.cny + 1) * c);
+        const int64_t fz = a + (int64_t)m.cnx * (b + (int64_t)m.cny * c);
+        denx[fx] = rl * vx + rg * (1.0 - vx);
+        deny[fy] = rl * vy + rg * (1.0 - vy);
+        denz[fz] = rl * vz + rg * (1.0 - vz);
+        viscx[fx] = vx * ml + (1.0 - vx) * mg;
+        viscy[fy] = vy * ml + (1.0 - vy) * mg;
+        viscz[fz] = vz * ml + (1.0 - vz) * mg;
     }
 }
 
@@ -1752,13 +1544,19 @@ int launch_plic(mist_ctx* ctx) {
     return launch_borders(ctx, MIST_FIELD_PLIC_D);
 }
 
-int launch_transport(mist_ctx* ctx, double dt) {
+int launch_project_vertices(mist_ctx* ctx, double dt) {
     const int64_t nv = (int64_t)(ctx->m.Nx + 1) * (ctx->m.Ny + 1) * (ctx->m.Nz + 1);
     hipEvent_t ev = prof_begin(ctx, MIST_PROF_PROJECT);
     LAUNCH(k_project_vertices, grid_for(nv, 256), 256, ctx->m, ctx->dp, (const int16_t*)ctx->f[MIST_FIELD_BAND],
            (const double*)ctx->f[MIST_FIELD_UF], (const double*)ctx->f[MIST_FIELD_VF],
            (const double*)ctx->f[MIST_FIELD_WF], dt, ctx->pv);
     prof_end(ctx, MIST_PROF_PROJECT, ev, nullptr, nv);
+    return MIST_OK;
+}
+
+int launch_transport(mist_ctx* ctx, double dt) {
+    if (int st = launch_project_vertices(ctx, dt)) return st;
+    hipEvent_t ev;
     const int64_t ncap = (int64_t)ctx->m.Nx * ctx->m.Ny * ctx->m.Nz;
     ev = prof_begin(ctx, MIST_PROF_SL_CUT);
     LAUNCH(k_fill_int, 1, 64, ctx->counters + 4, 5, 0);  // deferred cells, work queues, cut-list size
@@ -1819,10 +1617,12 @@ int launch_init_vof(mist_ctx* ctx, int shape, const double* args) {
     return MIST_OK;
 }
 
-int launch_dens(mist_ctx* ctx) {
+int launch_props(mist_ctx* ctx) {
     const int64_t n = (int64_t)ctx->m.cnx * ctx->m.cny * ctx->m.cnz;
-    LAUNCH(k_dens, grid_for(n, 256), 256, ctx->m, ctx->p.rho_liq, ctx->p.rho_gas, (const double*)ctx->f[MIST_FIELD_VOF],
-           (double*)ctx->f[MIST_FIELD_DENX], (double*)ctx->f[MIST_FIELD_DENY], (double*)ctx->f[MIST_FIELD_DENZ]);
+    LAUNCH(k_props, grid_for(n, 256), 256, ctx->m, ctx->p.rho_liq, ctx->p.rho_gas, ctx->p.mu_liq, ctx->p.mu_gas,
+           (const double*)ctx->f[MIST_FIELD_VOF], (double*)ctx->f[MIST_FIELD_DENX], (double*)ctx->f[MIST_FIELD_DENY],
+           (double*)ctx->f[MIST_FIELD_DENZ], (double*)ctx->f[MIST_FIELD_VISCX], (double*)ctx->f[MIST_FIELD_VISCY],
+           (double*)ctx->f[MIST_FIELD_VISCZ]);
     return MIST_OK;
 }
 

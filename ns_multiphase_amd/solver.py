@@ -13,7 +13,7 @@ import os
 
 import numpy as np
 
-from .params import FIELD, IC_SHAPES, REDUCE, MistParams, create_mesh, parameters
+from .params import FIELD, IC_SHAPES, REDUCE, SIDES, MistParams, create_mesh, parameters
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmist_hip.so")
@@ -70,6 +70,15 @@ def lib():
         "mist_stage_times": (i32, [vp, dp, i32]),
         "mist_profile_enable": (i32, [vp, i32]),
         "mist_profile_read": (i32, [vp, i32, dp, ip, ip]),
+        "mist_set_wall_velocity": (i32, [vp, i32, dp]),
+        "mist_apply_bc": (i32, [vp, i32, i32, i32]),
+        "mist_interpolate_face": (i32, [vp, i32, i32, i32]),
+        "mist_interpolate_center": (i32, [vp]),
+        "mist_divergence": (i32, [vp, d]),
+        "mist_compute_curvature": (i32, [vp]),
+        "mist_surface_force": (i32, [vp]),
+        "mist_ns_init": (i32, [vp]),
+        "mist_ns_step": (i32, [vp, dp, d, dp, d, i64, ip, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -226,7 +235,47 @@ class Context:
         self._check(lib().mist_plic_vf_batch(self.h, _ip(ijk), _dp(nrm), _dp(dist), _dp(out), len(dist)))
         return out
 
-    PROFILE_KERNELS = ("elvira", "sl_cut", "plic", "project", "mg_rbgs", "mg_resid", "sl_general")
+    # ---- Navier-Stokes stages (same names and arguments as the oracle's context) -----
+    def set_wall_velocity(self, side, vel):
+        v = np.ascontiguousarray(vel, np.float64).reshape(3)
+        self._check(lib().mist_set_wall_velocity(self.h, SIDES[side], _dp(v)))
+
+    def apply_bc(self, which="u"):
+        names = ("u", "v", "w") if which == "u" else ("us", "vs", "ws")
+        self._check(lib().mist_apply_bc(self.h, *(FIELD[n] for n in names)))
+
+    def interpolate_face(self, which="us"):
+        names = ("u", "v", "w") if which == "u" else ("us", "vs", "ws")
+        self._check(lib().mist_interpolate_face(self.h, *(FIELD[n] for n in names)))
+
+    def interpolate_center(self):
+        self._check(lib().mist_interpolate_center(self.h))
+
+    def divergence(self, dt):
+        self._check(lib().mist_divergence(self.h, dt))
+
+    def compute_curvature(self):
+        self._check(lib().mist_compute_curvature(self.h))
+
+    def surface_force(self):
+        self._check(lib().mist_surface_force(self.h))
+
+    def ns_init(self):
+        self._check(lib().mist_ns_init(self.h))
+
+    def ns_step(self, t, tol, precision="fp64", clamp_to=0.0, check=True):
+        tt = ctypes.c_double(t)
+        dt = ctypes.c_double()
+        cyc = ctypes.c_int64()
+        res = ctypes.c_double()
+        st = lib().mist_ns_step(self.h, ctypes.byref(tt), clamp_to, ctypes.byref(dt), tol,
+                                {"fp64": 0, "mixed": 1}[precision], ctypes.byref(cyc), ctypes.byref(res))
+        if check:
+            self._check(st)
+        return tt.value, dt.value, cyc.value, res.value
+
+    PROFILE_KERNELS = ("elvira", "sl_cut", "plic", "project", "mg_rbgs", "mg_resid", "sl_general",
+                       "ns_cut", "ns_fields")
 
     def profile(self, on=True):
         self._check(lib().mist_profile_enable(self.h, 1 if on else 0))
