@@ -5,6 +5,8 @@ Headline (`value`): SL-VOF cell-updates/s of the 3D deformation at 128^3 (sphere
 (0.35,0.35,0.35), period T=3, CFL 0.5, dt = 1/512): one step = compute_dt, defineVelocity!,
 transport! (ELVIRA normals, back-trace, tet cutting, VF update, PLIC refresh) and
 computeBand!; value = 128^3 x steps / wall time, inputs already on the device.
+`rising_bubble`: full Navier-Stokes steps of the 128^3 rising bubble (50 steps) with both
+multigrid precisions, and the oracle's step time on the host cores beside them.
 Beside it (`mg`): pressure multigrid solve time to max|b - A p| <= 1e-10 on the 128^3
 variable-density problem (density ratio 1000, exact p = cos cos cos), FP32 V-cycles in FP64
 defect correction vs FP64 V-cycles.
@@ -177,6 +179,87 @@ def run_mg(args, torch):
     return out
 
 
+def run_bubble(args, torch):
+    """Rising bubble, full run_solver steps with solveNS (transport with momentum predictor and
+    correction tets, band, props, pressure solve to 1e-10, corrector, interpolateCenter!,
+    divergence!), FP32-in-FP64 and FP64 multigrid."""
+    from ns_multiphase_amd import solver
+    from ns_multiphase_amd import workloads as W
+    param, ic = W.rising_bubble(args.bubble_N, stepMax=args.bubble_steps)
+    out = {}
+    for prec in ("mixed", "fp64"):
+        with solver.Context(param) as w:  # warm-up: first launches, hierarchy, graphs
+            w.init_vof(*ic)
+            w.ns_init()
+            t = 0.0
+            for _ in range(2):
+                t, _, _, _ = w.ns_step(t, param.tol, prec)
+        ctx = solver.Context(param)
+        ctx.init_vof(*ic)
+        ctx.ns_init()
+        torch.cuda.synchronize()
+        t = 0.0
+        cycles = 0
+        worst = 0.0
+        t0 = time.perf_counter()
+        for _ in range(args.bubble_steps):
+            t, dt, cyc, res = ctx.ns_step(t, param.tol, prec)
+            cycles += cyc
+            worst = max(worst, res)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        wmax = ctx.reduce("max", "w")
+        # per-kernel breakdown on a few more steps with event timing on (the multigrid then
+        # runs its iterations eagerly instead of as graphs)
+        ctx.profile(True)
+        nprof = 5
+        for _ in range(nprof):
+            t, dt, cyc, res = ctx.ns_step(t, param.tol, prec)
+        prof = ctx.profile_read()
+        ctx.close()
+        out[prec] = {"ms_per_step": 1e3 * elapsed / args.bubble_steps, "steps": args.bubble_steps,
+                     "mg_cycles_per_step": cycles / args.bubble_steps, "max_final_resid": worst,
+                     "max_w_at_end": wmax,
+                     "profiled_ms_per_step": {k: v[0] / nprof for k, v in prof.items() if v[1]}}
+        log(f"[bench] bubble {prec}: {out[prec]['ms_per_step']:.2f} ms/step, "
+            f"{out[prec]['mg_cycles_per_step']:.1f} MG cycles/step")
+    out["speedup_mixed_vs_fp64"] = out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"]
+    return param, ic, out
+
+
+def cpu_bubble(args, param, ic, nsamples=2):
+    """Oracle rising-bubble step time on the host cores, from device states of the run."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    from ns_multiphase_amd import solver
+    from ns_multiphase_amd.params import FIELD
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
+    threads = min(threads, 16)
+    pyoracle.set_threads(threads)
+    g = solver.Context(param)
+    g.init_vof(*ic)
+    g.ns_init()
+    o = pyoracle.OracleContext(param)
+    t = 0.0
+    at = [0] + [args.bubble_steps // 2] * (nsamples - 1)
+    done = 0
+    cpu_time = 0.0
+    for s in at:
+        while done < s:
+            t, _, _, _ = g.ns_step(t, param.tol, "fp64")
+            done += 1
+        for name in FIELD:
+            o.set(name, g.get(name))
+        t0 = time.perf_counter()
+        o.ns_step(t, param.tol, "fp64")
+        cpu_time += time.perf_counter() - t0
+    g.close()
+    log(f"  cpu bubble: {len(at)} oracle steps in {cpu_time:.1f} s")
+    return {"ms_per_step": 1e3 * cpu_time / len(at), "cores": threads, "kind": "port",
+            "sample": f"{len(at)} oracle steps of the {param.Nx}^3 rising bubble (C++ FP64 restatement, "
+                      f"FP64 multigrid to the same 1e-10 residual), from the device state at steps {at}"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -187,6 +270,9 @@ def main():
     ap.add_argument("--mg-reps", type=int, default=3)
     ap.add_argument("--no-mg", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--bubble-N", type=int, default=128)
+    ap.add_argument("--bubble-steps", type=int, default=50)
+    ap.add_argument("--no-bubble", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the hot path runs on a "
@@ -246,6 +332,18 @@ def main():
         mg["speedup_mixed_vs_fp64"] = mg["fp64"]["solve_ms"] / mg["mixed"]["solve_ms"]
         log(f"[bench] mg fp64 {mg['fp64']['solve_ms']:.2f} ms, mixed {mg['mixed']['solve_ms']:.2f} ms")
 
+    bubble = None
+    if not args.no_bubble:
+        log(f"[bench] rising bubble N={args.bubble_N}, {args.bubble_steps} steps")
+        bparam, bic, bubble = run_bubble(args, torch)
+        if not args.no_cpu_baseline:
+            bubble["cpu_baseline"] = cpu_bubble(args, bparam, bic)
+            bubble["speedup_vs_cpu"] = {prec: bubble["cpu_baseline"]["ms_per_step"] / bubble[prec]["ms_per_step"]
+                                        for prec in ("mixed", "fp64")}
+        bubble["workload"] = (f"rising bubble {args.bubble_N}^3, r=0.125 at (0.5,0.5,0.3), rho 1000/1, "
+                              f"mu 1e-3/1e-5, sigma 0.07, g=-9.81 z, no-slip walls, dt=1e-3, CFL 0.5, "
+                              f"finite-difference scheme with correction tets, MG tol 1e-10")
+
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (oracle)")
@@ -260,7 +358,7 @@ def main():
         "config": {"workload": f"3D deformation {args.N}^3, sphere r=0.15, T=3, CFL 0.5, dt=1/512, RK4, 5 tets",
                    "steps_timed_from_t0": steps, "volume_rel_change": (V1 - V0) / V0,
                    "mg_workload": f"variable-density MMS {args.mg_N}^3, rho 1000/1, tol 1e-10"},
-        "roofline": roof, "cpu_baseline": cpu, "mg": mg, "kernels": kernels,
+        "roofline": roof, "cpu_baseline": cpu, "mg": mg, "rising_bubble": bubble, "kernels": kernels,
         "rooflines": roofs,
     }
     print(json.dumps(line), flush=True)
