@@ -146,7 +146,8 @@ int mist_ctx_destroy(mist_ctx* ctx) {
     mist::mg_free(ctx->mg);
     for (int f = 0; f < MIST_FIELD_COUNT; ++f)
         if (ctx->f[f]) (void)hipFree(ctx->f[f]);
-    void* bufs[] = {ctx->geom, ctx->trig, ctx->pv, ctx->band_list, ctx->iface_list, ctx->defer_list, ctx->cut_list, ctx->counters, ctx->red};
+    void* bufs[] = {ctx->geom, ctx->trig, ctx->pv, ctx->band_list, ctx->iface_list, ctx->defer_list, ctx->cut_list,
+                    ctx->counters, ctx->red, ctx->ns_sums};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (ctx->h_counters) (void)hipHostFree(ctx->h_counters);
@@ -258,6 +259,8 @@ int mist_ctx_create(const mist_params* pin, int device, mist_ctx** out) {
         hipMalloc(&ctx->cut_list, (size_t)ncell * 24) != hipSuccess ||
         hipMalloc(&ctx->counters, 16 * sizeof(int)) != hipSuccess ||
         hipMalloc(&ctx->red, (size_t)(4 * ctx->red_blocks + 16) * sizeof(double)) != hipSuccess)
+        return bail(MIST_ERR_OOM, "work allocation");
+    if (p.solveNS && hipMalloc(&ctx->ns_sums, (size_t)ncell * 6 * sizeof(double)) != hipSuccess)
         return bail(MIST_ERR_OOM, "work allocation");
     if (hipMemset(ctx->counters, 0, 16 * sizeof(int)) != hipSuccess) return bail(MIST_ERR_HIP, "clear");
     if (hipHostMalloc(&ctx->h_counters, 16 * sizeof(int), hipHostMallocDefault) != hipSuccess ||

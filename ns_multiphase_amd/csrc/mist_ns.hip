@@ -157,16 +157,18 @@ __device__ inline void ns_tet(const DevMesh& m, const double* pv, const double* 
     }
 }
 
-// Semi-Lagrangian band cells of transport! with the momentum sums (src/Transport.jl:107-150):
-// one lane per (cell, tet), T tets per cell (5, or 17 with the correction tets), 64/T cells
-// per block. The first lane of a cell adds its tets' sums in tet order and writes VF (in
-// place: the cuts read only normals and D) and us, vs, ws = (rho u vol) / (rho vol).
-template <int T>
-__global__ __launch_bounds__(64) void k_ns_cut(DevMesh m, const double* pv, const int* list, const int* count,
-                                               const double* nx, const double* ny, const double* nz,
-                                               const double* D, LeafMomentum lm, const double* uf,
-                                               const double* vf, const double* wf, double dt, double* VF,
-                                               double* us, double* vs, double* ws, int* ovf) {
+// Tets FIRST .. FIRST+T-1 of the cells of `list` through the general cut walk with the six
+// momentum sums, one lane per (cell, tet), 64/T cells per block. Without FINAL the cell's
+// sums go to sums[6 q ..]; with FINAL they are added to the sums already there (the tree
+// pass's five tets) and the cell's VF (in place: the cuts read only normals and D) and
+// us, vs, ws = (rho u vol) / (rho vol) are written (src/Transport.jl:107-150). The tetsign of
+// the reference multiplies every tet of a cell alike and cancels in these ratios.
+template <int T, int FIRST, bool FINAL>
+__global__ __launch_bounds__(64) void k_ns_walk(DevMesh m, const double* pv, const int* list, const int* count,
+                                                const double* nx, const double* ny, const double* nz,
+                                                const double* D, LeafMomentum lm, const double* uf,
+                                                const double* vf, const double* wf, double dt, double* sums,
+                                                double* VF, double* us, double* vs, double* ws, int* ovf) {
     constexpr int CPB = 64 / T;
     __shared__ double s_acc[6][64];
     const int lane = threadIdx.x;
@@ -176,11 +178,12 @@ __global__ __launch_bounds__(64) void k_ns_cut(DevMesh m, const double* pv, cons
         const int cidx = base + cl;
         const bool active = cl < CPB && cidx < nb;
         double r[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        int i = 1, j = 1, k = 1;
+        int q = 0, i = 1, j = 1, k = 1;
         if (active) {
-            interior_of(m, list[cidx], i, j, k);
+            q = list[cidx];
+            interior_of(m, q, i, j, k);
             double tet[4][3];
-            ns_tet(m, pv, uf, vf, wf, dt, i, j, k, t, tet);
+            ns_tet(m, pv, uf, vf, wf, dt, i, j, k, FIRST + t, tet);
             int ind[4][3];
             int o = 0;
             for (int n = 0; n < 4; ++n) pt2index(m, ind[n], tet[n], i, j, k, &o);
@@ -190,21 +193,46 @@ __global__ __launch_bounds__(64) void k_ns_cut(DevMesh m, const double* pv, cons
             if (o) atomicOr(ovf, 1);
         }
 #pragma unroll
-        for (int q = 0; q < 6; ++q) s_acc[q][lane] = r[q];
+        for (int a = 0; a < 6; ++a) s_acc[a][lane] = r[a];
         __syncthreads();
         if (active && t == 0) {
-            const double ts = ((i + j + k) % 2) == 0 ? 1.0 : -1.0;
-            double S[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            double S[6];
+            double* o = sums + 6 * (int64_t)q;
+#pragma unroll
+            for (int a = 0; a < 6; ++a) S[a] = FINAL ? o[a] : 0.0;
             for (int n = 0; n < T; ++n)
 #pragma unroll
-                for (int q = 0; q < 6; ++q) S[q] += ts * s_acc[q][lane + n];
-            VF[m.vof(i, j, k)] = S[1] / S[0];
-            const int64_t c = m.cell(i, j, k);
-            us[c] = S[2] / S[5];
-            vs[c] = S[3] / S[5];
-            ws[c] = S[4] / S[5];
+                for (int a = 0; a < 6; ++a) S[a] += s_acc[a][lane + n];
+            if (FINAL) {
+                VF[m.vof(i, j, k)] = S[1] / S[0];
+                const int64_t c = m.cell(i, j, k);
+                us[c] = S[2] / S[5];
+                vs[c] = S[3] / S[5];
+                ws[c] = S[4] / S[5];
+            } else {
+#pragma unroll
+                for (int a = 0; a < 6; ++a) o[a] = S[a];
+            }
         }
         __syncthreads();
+    }
+}
+
+// Band cells of the semi-Lagrangian scheme (no correction tets): VF and us, vs, ws from the
+// five tets' sums.
+__global__ void k_ns_finalize(DevMesh m, const int* list, const int* count, const double* sums, double* VF,
+                              double* us, double* vs, double* ws) {
+    const int nb = *count;
+    for (int cidx = blockIdx.x * blockDim.x + threadIdx.x; cidx < nb; cidx += gridDim.x * blockDim.x) {
+        const int q = list[cidx];
+        int i, j, k;
+        interior_of(m, q, i, j, k);
+        const double* S = sums + 6 * (int64_t)q;
+        VF[m.vof(i, j, k)] = S[1] / S[0];
+        const int64_t c = m.cell(i, j, k);
+        us[c] = S[2] / S[5];
+        vs[c] = S[3] / S[5];
+        ws[c] = S[4] / S[5];
     }
 }
 
@@ -557,19 +585,29 @@ int launch_ns_advect(mist_ctx* ctx, double dt) {
     const bool fd = ctx->p.pressure_scheme == MIST_PS_FINITE_DIFFERENCE;
     const LeafMomentum lm{FLD(MIST_FIELD_U), FLD(MIST_FIELD_V), FLD(MIST_FIELD_W), ctx->p.rho_liq, ctx->p.rho_gas};
     const int64_t ncap = (int64_t)ctx->m.Nx * ctx->m.Ny * ctx->m.Nz;
+    double* sums = ctx->ns_sums;
+    const double* nx = (const double*)FLD(MIST_FIELD_NX);
+    const double* ny = (const double*)FLD(MIST_FIELD_NY);
+    const double* nz = (const double*)FLD(MIST_FIELD_NZ);
+    const double* D = (const double*)FLD(MIST_FIELD_PLIC_D);
+    const double* uf = (const double*)FLD(MIST_FIELD_UF);
+    const double* vf = (const double*)FLD(MIST_FIELD_VF);
+    const double* wf = (const double*)FLD(MIST_FIELD_WF);
     hipEvent_t ev = prof_begin(ctx, MIST_PROF_NS_CUT);
+    // the five back-traced tets: cut trees per work-group, then the general walk for the rest
+    if ((st = launch_tree_momentum(ctx, sums))) return st;
+    NSL((k_ns_walk<5, 0, false>), 32, 64, ctx->m, (const double*)ctx->pv, (const int*)ctx->defer_list,
+        (const int*)(ctx->counters + 6), nx, ny, nz, D, lm, uf, vf, wf, dt, sums, FLD(MIST_FIELD_VOF),
+        FLD(MIST_FIELD_US), FLD(MIST_FIELD_VS), FLD(MIST_FIELD_WS), ctx->counters + 2);
+    // the twelve correction tets (finite-difference scheme), then VF and us, vs, ws
     if (fd)
-        NSL(k_ns_cut<17>, ns_grid(ncap, 3, 8192), 64, ctx->m, (const double*)ctx->pv, (const int*)ctx->band_list,
-            (const int*)ctx->counters, (const double*)FLD(MIST_FIELD_NX), (const double*)FLD(MIST_FIELD_NY),
-            (const double*)FLD(MIST_FIELD_NZ), (const double*)FLD(MIST_FIELD_PLIC_D), lm,
-            (const double*)FLD(MIST_FIELD_UF), (const double*)FLD(MIST_FIELD_VF), (const double*)FLD(MIST_FIELD_WF),
-            dt, FLD(MIST_FIELD_VOF), FLD(MIST_FIELD_US), FLD(MIST_FIELD_VS), FLD(MIST_FIELD_WS), ctx->counters + 2);
+        NSL((k_ns_walk<12, 5, true>), ns_grid(ncap, 5, 8192), 64, ctx->m, (const double*)ctx->pv,
+            (const int*)ctx->band_list, (const int*)ctx->counters, nx, ny, nz, D, lm, uf, vf, wf, dt, sums,
+            FLD(MIST_FIELD_VOF), FLD(MIST_FIELD_US), FLD(MIST_FIELD_VS), FLD(MIST_FIELD_WS), ctx->counters + 2);
     else
-        NSL(k_ns_cut<5>, ns_grid(ncap, 12, 8192), 64, ctx->m, (const double*)ctx->pv, (const int*)ctx->band_list,
-            (const int*)ctx->counters, (const double*)FLD(MIST_FIELD_NX), (const double*)FLD(MIST_FIELD_NY),
-            (const double*)FLD(MIST_FIELD_NZ), (const double*)FLD(MIST_FIELD_PLIC_D), lm,
-            (const double*)FLD(MIST_FIELD_UF), (const double*)FLD(MIST_FIELD_VF), (const double*)FLD(MIST_FIELD_WF),
-            dt, FLD(MIST_FIELD_VOF), FLD(MIST_FIELD_US), FLD(MIST_FIELD_VS), FLD(MIST_FIELD_WS), ctx->counters + 2);
+        NSL(k_ns_finalize, ns_grid(ncap, 256, 4096), 256, ctx->m, (const int*)ctx->band_list,
+            (const int*)ctx->counters, (const double*)sums, FLD(MIST_FIELD_VOF), FLD(MIST_FIELD_US),
+            FLD(MIST_FIELD_VS), FLD(MIST_FIELD_WS));
     prof_end(ctx, MIST_PROF_NS_CUT, ev, ctx->counters + 0, 0);
     ev = prof_begin(ctx, MIST_PROF_NS_FIELDS);
     NSL(k_ns_convect, ns_grid(ncap, 256), 256, ctx->m, (int)ctx->p.nband, fd ? 0 : 1,

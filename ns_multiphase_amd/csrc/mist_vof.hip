@@ -919,10 +919,12 @@ static_assert(sizeof(CutCell) == 24, "the context allocates 24 bytes per cut-lis
 // the same additions (or vLiq none) and the new VF is exactly 1 (or 0): it is written here. The
 // other cells go to the cut list; those whose corners span more than 4 cells along an axis to
 // the general walk.
+// With the momentum sums (`mom`) every cell is cut: the leaves' cell velocities weight the
+// momentum even where every leaf is on one side of its plane.
 __global__ __launch_bounds__(256) void k_sl_classify(DevMesh m, const double* pv, const int* list, const int* count,
                                                      const double* nx, const double* ny, const double* nz,
                                                      const double* D, double* VF, int* ovf, CutCell* cut,
-                                                     int* ncut, int* general, int* ngeneral) {
+                                                     int* ncut, int* general, int* ngeneral, int mom) {
     const int nb = *count;
     const int nloop = (nb + 63) & ~63;  // whole waves in the loop (the appends use ballots)
     for (int cidx = blockIdx.x * blockDim.x + threadIdx.x; cidx < nloop; cidx += gridDim.x * blockDim.x) {
@@ -960,7 +962,7 @@ __global__ __launch_bounds__(256) void k_sl_classify(DevMesh m, const double* pv
                 for (int c = 0; c < 8; ++c)
 #pragma unroll
                     for (int a = 0; a < 3; ++a) offs |= (unsigned long long)(I[c][a] - lo[a]) << (6 * c + 2 * a);
-                bool liq = true, gas = true;
+                bool liq = !mom, gas = !mom;
                 for (int cz = lo[2]; cz <= hi[2] && (liq || gas); ++cz)
                     for (int cy = lo[1]; cy <= hi[1] && (liq || gas); ++cy)
                         for (int cx = lo[0]; cx <= hi[0] && (liq || gas); ++cx) {
@@ -981,13 +983,19 @@ __global__ __launch_bounds__(256) void k_sl_classify(DevMesh m, const double* pv
     }
 }
 
-template <class C>
+// MOM: the six sums of the momentum predictor (vol, vLiq, rho u vol, rho v vol, rho w vol,
+// rho vol with the leaf cell's centre velocity, src/VFgeom.jl:791-827) go to sums[6 q .. 6 q + 5]
+// instead of VF.
+template <class C, bool MOM>
 __global__ __launch_bounds__(C::NT) void k_sl_tree(DevMesh m, const double* pv, const CutCell* cut, const int* idx,
                                                 const int* count, const double* nx, const double* ny,
                                                 const double* nz, const double* D, double* VF, int* defer,
-                                                int* ndefer, int defer_cell, int* queue) {
+                                                int* ndefer, int defer_cell, int* queue, LeafMomentum lm,
+                                                double* sums) {
     constexpr int NT = C::NT;
     __shared__ TreeShared<C> S;
+    __shared__ double s_vel[MOM ? 64 : 1][3];
+    __shared__ double s_red[MOM ? 4 : 1][NT / 64];
     const int tid = threadIdx.x;
     const int nb = *count;
     // the next cell is claimed while the current one is cut (the atomic's latency is hidden)
@@ -1035,6 +1043,11 @@ __global__ __launch_bounds__(C::NT) void k_sl_tree(DevMesh m, const double* pv, 
             S.nb[l][3] = Dc;
             S.cls[l] = (signed char)box_class(n0, n1, n2, Dc, m.x[lo[0] + ox], m.x[lo[0] + ox + 1], m.y[lo[1] + oy],
                                               m.y[lo[1] + oy + 1], m.z[lo[2] + oz], m.z[lo[2] + oz + 1]);
+            if constexpr (MOM) {
+                s_vel[l][0] = lm.u[c];
+                s_vel[l][1] = lm.v[c];
+                s_vel[l][2] = lm.w[c];
+            }
         }
         // ---- the five tets (src/VFgeom.jl:235-256) ---------------------------------------
         if (tid < 5) {
@@ -1105,6 +1118,7 @@ __global__ __launch_bounds__(C::NT) void k_sl_tree(DevMesh m, const double* pv, 
             __syncthreads();
             // ---- leaves: last z cut and the PLIC cut (src/VFgeom.jl:757-830) ------------------
             double Vs = 0.0, Ls = 0.0;
+            double Ms[4] = {0.0, 0.0, 0.0, 0.0};  // rho u vol, rho v vol, rho w vol, rho vol
             for (int L = tid; L < nleaf; L += NT) {
                 const int e = S.lnode[L];
                 const int c = L - S.loff[e];
@@ -1163,10 +1177,31 @@ __global__ __launch_bounds__(C::NT) void k_sl_tree(DevMesh m, const double* pv, 
                 const unsigned ox = ob & 3u, oy = (ob >> 2) & 3u, oz = (ob >> 4) & 3u;
                 const int l = (int)ox + bx * ((int)oy + by * (int)oz);
                 const int lcls = S.cls[l];
+                double uc = 0.0, vc = 0.0, wc = 0.0;
+                if constexpr (MOM) {
+                    uc = s_vel[l][0];
+                    vc = s_vel[l][1];
+                    wc = s_vel[l][2];
+                }
+                auto on_liq = [&](double tv) {
+                    Vs += tv;
+                    Ls += tv;
+                    if constexpr (MOM) {
+                        Ms[0] += (tv * uc) * lm.rho_liq; Ms[1] += (tv * vc) * lm.rho_liq;
+                        Ms[2] += (tv * wc) * lm.rho_liq; Ms[3] += tv * lm.rho_liq;
+                    }
+                };
+                auto on_gas = [&](double tv) {
+                    Vs += tv;
+                    if constexpr (MOM) {
+                        Ms[0] += (tv * uc) * lm.rho_gas; Ms[1] += (tv * vc) * lm.rho_gas;
+                        Ms[2] += (tv * wc) * lm.rho_gas; Ms[3] += tv * lm.rho_gas;
+                    }
+                };
                 if (lcls != 0) {  // every vertex strictly on one side: the d2case case is 1 or 16
                     const double tv = tet_vol3(Q[0], Q[1], Q[2], Q[3]);
-                    Vs += tv;
-                    if (lcls > 0) Ls += tv;
+                    if (lcls > 0) on_liq(tv);
+                    else on_gas(tv);
                     continue;
                 }
                 const double n0 = S.nb[l][0], n1 = S.nb[l][1], n2 = S.nb[l][2], Dc = S.nb[l][3];
@@ -1179,10 +1214,10 @@ __global__ __launch_bounds__(C::NT) void k_sl_tree(DevMesh m, const double* pv, 
                 const int cs = d2case_snap(dd);
                 if (cs == 0 || cs == 15) {  // one sub-tet, the leaf itself (vertex order 1,2,3,4)
                     const double tv = tet_vol3(Q[0], Q[1], Q[2], Q[3]);
-                    Vs += tv;
-                    if (cs == 0) Ls += tv;
+                    if (cs == 0) on_liq(tv);
+                    else on_gas(tv);
                 } else {
-                    cut_dispatch<true>(cs, Q, d, [&](double tv) { Vs += tv; Ls += tv; }, [&](double tv) { Vs += tv; });
+                    cut_dispatch<true>(cs, Q, d, on_liq, on_gas);
                 }
             }
             // ---- block sums in a fixed order ----------------------------------------------------
@@ -1190,23 +1225,46 @@ __global__ __launch_bounds__(C::NT) void k_sl_tree(DevMesh m, const double* pv, 
             for (int off = 32; off >= 1; off >>= 1) {
                 Vs += __shfl_xor(Vs, off);
                 Ls += __shfl_xor(Ls, off);
+                if constexpr (MOM) {
+#pragma unroll
+                    for (int a = 0; a < 4; ++a) Ms[a] += __shfl_xor(Ms[a], off);
+                }
             }
             if constexpr (NT > 64) {
                 if ((tid & 63) == 0) {
                     S.red[0][tid >> 6] = Vs;
                     S.red[1][tid >> 6] = Ls;
+                    if constexpr (MOM) {
+#pragma unroll
+                        for (int a = 0; a < 4; ++a) s_red[a][tid >> 6] = Ms[a];
+                    }
                 }
                 __syncthreads();
                 if (tid == 0) {
                     Vs = S.red[0][0];
                     Ls = S.red[1][0];
+                    if constexpr (MOM) {
+#pragma unroll
+                        for (int a = 0; a < 4; ++a) Ms[a] = s_red[a][0];
+                    }
                     for (int w = 1; w < NT / 64; ++w) {
                         Vs += S.red[0][w];
                         Ls += S.red[1][w];
+                        if constexpr (MOM) {
+#pragma unroll
+                            for (int a = 0; a < 4; ++a) Ms[a] += s_red[a][w];
+                        }
                     }
                 }
             }
-            if (tid == 0) VF[m.vof(i, j, k)] = Ls / Vs;
+            if (tid == 0) {
+                if constexpr (MOM) {
+                    double* o = sums + 6 * (int64_t)q;
+                    o[0] = Vs; o[1] = Ls; o[2] = Ms[0]; o[3] = Ms[1]; o[4] = Ms[2]; o[5] = Ms[3];
+                } else {
+                    VF[m.vof(i, j, k)] = Ls / Vs;
+                }
+            }
         }
         if (tid == 0) S.next = claim;
         __syncthreads();
@@ -1562,7 +1620,7 @@ int launch_transport(mist_ctx* ctx, double dt) {
     LAUNCH(k_fill_int, 1, 64, ctx->counters + 4, 5, 0);  // deferred cells, work queues, cut-list size
     if (!ctx->tree_blocks) {
         int per = 0, dev = 0, cus = 0;
-        MIST_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_sl_tree<TreeMain>, TreeMain::NT, 0));
+        MIST_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_sl_tree<TreeMain, false>, TreeMain::NT, 0));
         MIST_HIP_TRY(hipGetDevice(&dev));
         MIST_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         ctx->tree_cus = std::max(1, cus);
@@ -1577,23 +1635,58 @@ int launch_transport(mist_ctx* ctx, double dt) {
     // bulk cells, index boxes; cells to cut go to the cut list
     LAUNCH(k_sl_classify, grid_for(ncap, 256, 8192), 256, ctx->m, (const double*)ctx->pv, (const int*)ctx->band_list,
            (const int*)(ctx->counters + 0), nx, ny, nz, D, VF, ctx->counters + 2, cut, ctx->counters + 8,
-           ctx->defer_list, ctx->counters + 6);
-    LAUNCH(k_sl_tree<TreeMain>, ctx->tree_blocks, TreeMain::NT, ctx->m, (const double*)ctx->pv, (const CutCell*)cut,
-           (const int*)nullptr, (const int*)(ctx->counters + 8), nx, ny, nz, D, VF, ctx->iface_list,
-           ctx->counters + 4, 0, ctx->counters + 5);
+           ctx->defer_list, ctx->counters + 6, 0);
+    const LeafMomentum none{nullptr, nullptr, nullptr, 0.0, 0.0};
+    LAUNCH((k_sl_tree<TreeMain, false>), ctx->tree_blocks, TreeMain::NT, ctx->m, (const double*)ctx->pv,
+           (const CutCell*)cut, (const int*)nullptr, (const int*)(ctx->counters + 8), nx, ny, nz, D, VF,
+           ctx->iface_list, ctx->counters + 4, 0, ctx->counters + 5, none, (double*)nullptr);
     prof_end(ctx, MIST_PROF_SL_CUT, ev, ctx->counters + 0, 0);
     // cells whose cut tree outgrows the main pass: the same walk with larger capacities; then
     // the general walk for those and for cells whose corners span more than four cells
     ev = prof_begin(ctx, MIST_PROF_SL_GENERAL);
-    LAUNCH(k_sl_tree<TreeWide>, 32, TreeWide::NT, ctx->m, (const double*)ctx->pv, (const CutCell*)cut,
+    LAUNCH((k_sl_tree<TreeWide, false>), 32, TreeWide::NT, ctx->m, (const double*)ctx->pv, (const CutCell*)cut,
            (const int*)ctx->iface_list, (const int*)(ctx->counters + 4), nx, ny, nz, D, VF, ctx->defer_list,
-           ctx->counters + 6, 1, ctx->counters + 7);
+           ctx->counters + 6, 1, ctx->counters + 7, none, (double*)nullptr);
     LAUNCH(k_sl_cut, 32, 64, ctx->m, (const double*)ctx->pv, (const int*)ctx->defer_list,
            (const int*)(ctx->counters + 6), nx, ny, nz, D, VF, ctx->counters + 2);
     prof_end(ctx, MIST_PROF_SL_GENERAL, ev, ctx->counters + 6, 0);
     int st;
     if ((st = launch_neumann(ctx, MIST_FIELD_VOF))) return st;
     return launch_borders(ctx, MIST_FIELD_VOF);
+}
+
+// The five back-traced tets of every band cell with the six momentum sums into `sums`
+// (indexed by interior cell id): the classify / main tree / wide tree pipeline of
+// launch_transport without the bulk shortcut; cells left for the general walk are listed in
+// defer_list (count counters[6]) for the caller.
+int launch_tree_momentum(mist_ctx* ctx, double* sums) {
+    const int64_t ncap = (int64_t)ctx->m.Nx * ctx->m.Ny * ctx->m.Nz;
+    LAUNCH(k_fill_int, 1, 64, ctx->counters + 4, 5, 0);
+    if (!ctx->tree_blocks_mom) {
+        int per = 0, dev = 0, cus = 0;
+        MIST_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_sl_tree<TreeMain, true>, TreeMain::NT, 0));
+        MIST_HIP_TRY(hipGetDevice(&dev));
+        MIST_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        ctx->tree_blocks_mom = std::max(1, per) * std::max(1, cus);
+    }
+    const double* nx = (const double*)ctx->f[MIST_FIELD_NX];
+    const double* ny = (const double*)ctx->f[MIST_FIELD_NY];
+    const double* nz = (const double*)ctx->f[MIST_FIELD_NZ];
+    const double* D = (const double*)ctx->f[MIST_FIELD_PLIC_D];
+    double* VF = (double*)ctx->f[MIST_FIELD_VOF];
+    CutCell* cut = (CutCell*)ctx->cut_list;
+    const LeafMomentum lm{(const double*)ctx->f[MIST_FIELD_U], (const double*)ctx->f[MIST_FIELD_V],
+                          (const double*)ctx->f[MIST_FIELD_W], ctx->p.rho_liq, ctx->p.rho_gas};
+    LAUNCH(k_sl_classify, grid_for(ncap, 256, 8192), 256, ctx->m, (const double*)ctx->pv, (const int*)ctx->band_list,
+           (const int*)(ctx->counters + 0), nx, ny, nz, D, VF, ctx->counters + 2, cut, ctx->counters + 8,
+           ctx->defer_list, ctx->counters + 6, 1);
+    LAUNCH((k_sl_tree<TreeMain, true>), ctx->tree_blocks_mom, TreeMain::NT, ctx->m, (const double*)ctx->pv,
+           (const CutCell*)cut, (const int*)nullptr, (const int*)(ctx->counters + 8), nx, ny, nz, D, VF,
+           ctx->iface_list, ctx->counters + 4, 0, ctx->counters + 5, lm, sums);
+    LAUNCH((k_sl_tree<TreeWide, true>), 32, TreeWide::NT, ctx->m, (const double*)ctx->pv, (const CutCell*)cut,
+           (const int*)ctx->iface_list, (const int*)(ctx->counters + 4), nx, ny, nz, D, VF, ctx->defer_list,
+           ctx->counters + 6, 1, ctx->counters + 7, lm, sums);
+    return MIST_OK;
 }
 
 int launch_init_vof(mist_ctx* ctx, int shape, const double* args) {
