@@ -107,7 +107,9 @@ enum {
     MIST_FIELD_CURV = 24,   /* Curve: interface curvature (height function) */
     MIST_FIELD_SFX = 25, MIST_FIELD_SFY = 26, MIST_FIELD_SFZ = 27, /* face surface force */
     MIST_FIELD_MASK = 28,   /* mask: int16, bit d set = mask[i,j,k][d+1] of the reference */
-    MIST_FIELD_COUNT = 29
+    /* adjoints (mist_transport_vjp): of VF (VF extents) and of the face velocities */
+    MIST_FIELD_VOF_BAR = 29, MIST_FIELD_UF_BAR = 30, MIST_FIELD_VFACE_BAR = 31, MIST_FIELD_WF_BAR = 32,
+    MIST_FIELD_COUNT = 33
 };
 
 enum { MIST_REDUCE_SUM = 0, MIST_REDUCE_MAX = 1, MIST_REDUCE_MIN = 2, MIST_REDUCE_MAXABS = 3 };
@@ -204,6 +206,22 @@ int mist_ns_init(mist_ctx* ctx);
  * nonlinear pressure equation is not built: MIST_ERR_PARAM). */
 int mist_ns_step(mist_ctx* ctx, double* t, double clamp_to, double* dt_used, double tol, int64_t precision,
                  int64_t* cycles, double* final_resid);
+
+/* ---- adjoint of the SL-VOF step (no reference counterpart) ------------------------- */
+/* Reverse mode of one transport! step of the VOF part (solveNS = 0, 5 tets) followed by
+ * computeBand!, with the band held fixed. On entry the context holds the step's input
+ * state (VF with its ghosts, as the forward step found it; normals, PLIC constants and band
+ * are recomputed for it when stale) and MIST_FIELD_VOF_BAR holds dL/dVF of the step's
+ * output. On return MIST_FIELD_VOF_BAR holds dL/dVF of the input, and dL/d(uf, vf, wf) of
+ * this step has been added to MIST_FIELD_UF_BAR, _VFACE_BAR, _WF_BAR (zero them before the
+ * first step of a sweep). Branches (cut cases, clamps, ELVIRA's choice, cell searches) are
+ * those of the forward evaluation. The first call allocates the work arrays. */
+int mist_transport_vjp(mist_ctx* ctx, double dt);
+/* Step checkpoints: copy the whole VF array (with its ghosts) to device slot `slot`
+ * (0 <= slot < 4096, allocated on first use) and back; a restore also recomputes the band,
+ * normals and PLIC constants of the restored VF, the state a step starts from. */
+int mist_checkpoint_save(mist_ctx* ctx, int slot);
+int mist_checkpoint_restore(mist_ctx* ctx, int slot);
 
 /* Batched PLIC primitives (first slice; host arrays of length m).
  * computeDist (src/VF.jl:109-156) for cells (i,j,k) with normals n and fractions vf. */

@@ -1,0 +1,466 @@
+// mist_adjoint.hip -- reverse mode of one SL-VOF transport step (the VOF part of transport!,
+// src/Transport.jl:2-268, with computeBand! held fixed): given the adjoint of the new liquid
+// fractions, the adjoint of the old ones and of the face velocities. No reference
+// counterpart: the reference has no derivative of its transport.
+//
+// The forward step is VF' = T(VF, uf): normals n(VF) by ELVIRA, plane constants D(n, VF) by
+// computeDist, back-traced corners x(uf) by RK4, and in each band cell VF' = vLiq / vol of
+// the cut pre-image (elsewhere VF' = VF); then the ghost fills of VF'. The reverse pass runs
+// these in the opposite order on the state of the step (its VF, normals, D, band and the
+// ELVIRA candidate each interface cell kept):
+//   1. ghost fills of VF' reversed (periodic copies, then the one-layer Neumann copies);
+//   2. outside the band the adjoint passes through; in each band cell the five tets are cut
+//      again, then walked in reverse with seeds dVF'/dvLiq = 1/vol, dVF'/dvol = -vLiq/vol^2:
+//      leaf cuts give plane adjoints (n, D) of their cells, mesh-plane cuts pass vertex
+//      adjoints up the tree to the cell's back-traced corners;
+//   3. the corners' adjoints through the RK4 back-trace to uf, vf, wf;
+//   4. the plane-constant adjoints through computeDist to n and VF (after D's periodic copies
+//      are reversed), then the normal adjoints through the chosen ELVIRA candidate to VF.
+// Adjoints shared by several threads (corners, planes, faces, stencil neighbours) are added
+// with atomics, so their rounding depends on the order of arrival.
+#include <algorithm>
+#include <cmath>
+
+#include "mist_adjoint.h"
+#include "mist_internal.h"
+#include "mist_kernels.h"
+
+namespace mist {
+
+// ---- reverse of the ghost fills ------------------------------------------------------
+struct AdjExt {
+    int lo[3];
+    int n[3];
+    __device__ __forceinline__ int64_t at(int i, int j, int k) const {
+        return (int64_t)(i - lo[0]) + (int64_t)n[0] * ((int64_t)(j - lo[1]) + (int64_t)n[1] * (k - lo[2]));
+    }
+};
+
+// reverse of k_neumann_axis: ghost[mn-1] = A[mn], ghost[mx+1] = A[mx]
+__global__ void k_adj_neumann_axis(double* g, AdjExt e, int d, int mn, int mx) {
+    const int d1 = (d + 1) % 3, d2 = (d + 2) % 3;
+    const int64_t tot = (int64_t)e.n[d1] * e.n[d2];
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
+        int id[3];
+        id[d1] = e.lo[d1] + (int)(q % e.n[d1]);
+        id[d2] = e.lo[d2] + (int)(q / e.n[d1]);
+        id[d] = mx + 1;
+        const int64_t ghi = e.at(id[0], id[1], id[2]);
+        id[d] = mn - 1;
+        const int64_t glo = e.at(id[0], id[1], id[2]);
+        const double a = g[ghi], b = g[glo];
+        g[ghi] = 0.0;
+        g[glo] = 0.0;
+        id[d] = mx;
+        g[e.at(id[0], id[1], id[2])] += a;
+        id[d] = mn;
+        g[e.at(id[0], id[1], id[2])] += b;
+    }
+}
+
+// reverse of k_borders_axis (the statements of each column undone in reverse order)
+__global__ void k_adj_borders_axis(double* g, AdjExt e, int d, int mn, int mx, int mno, int mxo, int ng) {
+    const int d1 = (d + 1) % 3, d2 = (d + 2) % 3;
+    const int64_t tot = (int64_t)e.n[d1] * e.n[d2];
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
+        int id[3], is[3];
+        id[d1] = is[d1] = e.lo[d1] + (int)(q % e.n[d1]);
+        id[d2] = is[d2] = e.lo[d2] + (int)(q / e.n[d1]);
+        const int len2 = mn - mno;
+        for (int s = len2 - 1; s >= 0; --s) {
+            id[d] = mno + s; is[d] = mx - ng + 1 + s;
+            const int64_t a = e.at(id[0], id[1], id[2]), b = e.at(is[0], is[1], is[2]);
+            const double v = g[a];
+            g[a] = 0.0;
+            g[b] += v;
+        }
+        const int len1 = mxo - mx;
+        for (int s = 0; s < len1; ++s) {
+            id[d] = mx + 1 + s; is[d] = mn + s;
+            const int64_t a = e.at(id[0], id[1], id[2]), b = e.at(is[0], is[1], is[2]);
+            const double v = g[a];
+            g[a] = 0.0;
+            g[b] += v;
+        }
+    }
+}
+
+__global__ void k_adj_fill(double* a, int64_t n, double v) {
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) a[q] = v;
+}
+
+// interior cells outside the band: VF' = VF
+__global__ void k_adj_identity(DevMesh m, int nband, const int16_t* band, const double* lam, double* lam_new) {
+    const int n = m.Nx * m.Ny * m.Nz;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        int i, j, k;
+        interior_of(m, q, i, j, k);
+        if (abs((int)band[m.cell(i, j, k)]) <= nband) continue;
+        const int64_t v = m.vof(i, j, k);
+        lam_new[v] += lam[v];
+    }
+}
+
+// ---- reverse cut walk -----------------------------------------------------------------
+struct AdjFrame {
+    double v[8][3];
+    double g[8][3];
+    double d[4];
+    short vi[2][8][3];
+    unsigned char cs, child, nchild, flags, dir;
+};
+
+// cutTet of one tet in reverse: the walk of cut_tet, each frame keeping the adjoints of its 8
+// vertices; a leaf adds its vertex adjoints to its parent's slots and its plane adjoints to
+// the leaf cell (gpl: 4 arrays nx, ny, nz, D); a finished frame passes its cut vertices'
+// adjoints to their edge ends, then its 4 vertices' adjoints to the parent. groot receives
+// the adjoints of the tet's vertices. Returns false when the walk overflows.
+__device__ bool cut_tet_adj(const DevMesh& m, const double* nx, const double* ny, const double* nz, const double* D,
+                            double tet[4][3], int ind[4][3], AdjFrame* st, double gL, double gV, double groot[4][3],
+                            double* const gpl[4]) {
+    int top = -1;
+    unsigned flags = 0;
+    bool have_node = true;
+    int ovf = 0;
+    for (;;) {
+        if (have_node) {
+            int dir = -1, cut_ind = 0;
+            for (int a = 0; a < 3; ++a) {
+                if (flags & (1u << a)) continue;
+                const int mx = max(max(ind[0][a], ind[1][a]), max(ind[2][a], ind[3][a]));
+                const int mn = min(min(ind[0][a], ind[1][a]), min(ind[2][a], ind[3][a]));
+                if (mx > mn) { dir = a; cut_ind = mx; break; }
+                flags |= 1u << a;
+            }
+            if (dir < 0) {
+                const int64_t c = m.cell(ind[0][0], ind[0][1], ind[0][2]);
+                double gx[4][3] = {}, gn[3] = {0.0, 0.0, 0.0}, gD = 0.0;
+                leaf_adj(tet, nx[c], ny[c], nz[c], D[c], gL, gV, gx, gn, gD);
+                if (gn[0] != 0.0) atomicAdd(gpl[0] + c, gn[0]);
+                if (gn[1] != 0.0) atomicAdd(gpl[1] + c, gn[1]);
+                if (gn[2] != 0.0) atomicAdd(gpl[2] + c, gn[2]);
+                if (gD != 0.0) atomicAdd(gpl[3] + c, gD);
+                if (top < 0) {
+                    for (int v = 0; v < 4; ++v)
+                        for (int q = 0; q < 3; ++q) groot[v][q] = gx[v][q];
+                    return ovf == 0;
+                }
+                AdjFrame& P = st[top];
+                const unsigned qv = CUT_VTET[P.cs][P.child - 1];
+                for (int nn = 0; nn < 4; ++nn)
+                    for (int q = 0; q < 3; ++q) P.g[nib(qv, nn)][q] += gx[nn][q];
+            } else {
+                if (top + 1 >= CUT_MAXLVL) return false;
+                AdjFrame& F = st[++top];
+                const double* cc = dir == 0 ? m.x : (dir == 1 ? m.y : m.z);
+                const double plane = cc[cut_ind];
+                double dd[4];
+                for (int n = 0; n < 4; ++n) {
+                    F.d[n] = tet[n][dir] - plane;
+                    dd[n] = F.d[n];
+                }
+                const int cs = d2case_snap(dd);
+                for (int n = 0; n < 4; ++n) {
+                    for (int q = 0; q < 3; ++q) {
+                        F.v[n][q] = tet[n][q];
+                        F.vi[0][n][q] = (short)ind[n][q];
+                        F.vi[1][n][q] = (short)ind[n][q];
+                    }
+                    const int lo1 = min(ind[n][dir], cut_ind - 1);
+                    F.vi[0][n][dir] = (short)lo1;
+                    F.vi[1][n][dir] = (short)max(lo1, cut_ind);
+                }
+                const int nv = CUT_NVERT[cs];
+                for (int n = 0; n < nv; ++n) {
+                    const unsigned e = CUT_EDGE[cs][n];
+                    const int v1 = e & 15, v2 = e >> 4;
+                    const double mu = jmin(1.0, jmax(0.0, sdiv(-F.d[v1], F.d[v2] - F.d[v1])));
+                    double pt[3];
+                    for (int q = 0; q < 3; ++q) {
+                        pt[q] = (1.0 - mu) * F.v[v1][q] + mu * F.v[v2][q];
+                        F.v[4 + n][q] = pt[q];
+                    }
+                    int I[3];
+                    pt2index(m, I, pt, F.vi[0][n][0], F.vi[0][n][1], F.vi[0][n][2], &ovf);
+                    for (int q = 0; q < 3; ++q) {
+                        int w = max(I[q], (int)min(F.vi[0][v1][q], F.vi[0][v2][q]));
+                        w = min(w, (int)max(F.vi[0][v1][q], F.vi[0][v2][q]));
+                        F.vi[0][4 + n][q] = (short)w;
+                        F.vi[1][4 + n][q] = (short)w;
+                    }
+                    F.vi[0][4 + n][dir] = (short)(cut_ind - 1);
+                    F.vi[1][4 + n][dir] = (short)cut_ind;
+                }
+                for (int n = 0; n < 8; ++n)
+                    for (int q = 0; q < 3; ++q) F.g[n][q] = 0.0;
+                F.cs = (unsigned char)cs;
+                F.child = 0;
+                F.nchild = CUT_NTETS[cs];
+                F.flags = (unsigned char)flags;
+                F.dir = (unsigned char)dir;
+            }
+            have_node = false;
+        }
+        AdjFrame& F = st[top];
+        if (F.child < F.nchild) {
+            const int n = F.child++;
+            const unsigned q = CUT_VTET[F.cs][n];
+            const int side = (CUT_SIDE2[F.cs] >> n) & 1;
+            for (int nn = 0; nn < 4; ++nn) {
+                const int v = nib(q, nn);
+                for (int a = 0; a < 3; ++a) {
+                    tet[nn][a] = F.v[v][a];
+                    ind[nn][a] = F.vi[side][v][a];
+                }
+            }
+            flags = F.flags;
+            have_node = true;
+        } else {
+            // the mesh-plane cut of this node in reverse: d = x[dir] - plane
+            const int nv = CUT_NVERT[F.cs];
+            for (int e = nv - 1; e >= 0; --e) {
+                const unsigned ed = CUT_EDGE[F.cs][e];
+                const int a = ed & 15, b = ed >> 4;
+                double gda, gdb;
+                lerp_adj(F.v[a], F.v[b], F.d[a], F.d[b], F.g[4 + e], F.g[a], F.g[b], gda, gdb);
+                F.g[a][F.dir] += gda;
+                F.g[b][F.dir] += gdb;
+            }
+            double g4[4][3];
+            for (int v = 0; v < 4; ++v)
+                for (int q = 0; q < 3; ++q) g4[v][q] = F.g[v][q];
+            --top;
+            if (top < 0) {
+                for (int v = 0; v < 4; ++v)
+                    for (int q = 0; q < 3; ++q) groot[v][q] = g4[v][q];
+                return ovf == 0;
+            }
+            AdjFrame& P = st[top];
+            const unsigned qv = CUT_VTET[P.cs][P.child - 1];
+            for (int nn = 0; nn < 4; ++nn)
+                for (int q = 0; q < 3; ++q) P.g[nib(qv, nn)][q] += g4[nn][q];
+        }
+    }
+}
+
+// Band cells: one lane per (cell, tet), 12 cells per block. The tets are cut forward for the
+// cell's vLiq and vol, then in reverse with the seeds of VF' = vLiq / vol.
+__global__ __launch_bounds__(64) void k_adj_cut(DevMesh m, const double* pv, const int* list, const int* count,
+                                                const double* nx, const double* ny, const double* nz,
+                                                const double* D, const double* lam, double* gpv, double* gnx,
+                                                double* gny, double* gnz, double* gD, int* ovf) {
+    __shared__ double s_vol[64], s_liq[64];
+    const int lane = threadIdx.x;
+    const int cl = lane / 5, t = lane - 5 * cl;
+    const int nb = *count;
+    double* const gpl[4] = {gnx, gny, gnz, gD};
+    const LeafMomentum none{nullptr, nullptr, nullptr, 0.0, 0.0};
+    for (int base = blockIdx.x * 12; base < nb; base += gridDim.x * 12) {
+        const int cidx = base + cl;
+        const bool active = lane < 60 && cidx < nb;
+        int i = 1, j = 1, k = 1;
+        double tet[4][3];
+        int ind[4][3];
+        int64_t vq[4] = {0, 0, 0, 0};
+        double vol = 0.0, liq = 0.0;
+        int o = 0;
+        if (active) {
+            interior_of(m, list[cidx], i, j, k);
+            const bool even = ((i + j + k) % 2) == 0;
+            const unsigned tv = TET5[t];
+            for (int n = 0; n < 4; ++n) {
+                const int c = vert_corner(nib(tv, n), even);
+                vq[n] = vtx_id(m, i + (c & 1), j + ((c >> 1) & 1), k + (c >> 2));
+                tet[n][0] = pv[3 * vq[n] + 0]; tet[n][1] = pv[3 * vq[n] + 1]; tet[n][2] = pv[3 * vq[n] + 2];
+            }
+            double t2[4][3];
+            int i2[4][3];
+            for (int n = 0; n < 4; ++n) {
+                pt2index(m, ind[n], tet[n], i, j, k, &o);
+                for (int q = 0; q < 3; ++q) { t2[n][q] = tet[n][q]; i2[n][q] = ind[n][q]; }
+            }
+            CutFrame<2> st[CUT_MAXLVL];
+            int depth = 0;
+            double r[2];
+            if (!cut_tet<2>(m, nx, ny, nz, D, none, t2, i2, st, r, depth)) o = 1;
+            vol = r[0];
+            liq = r[1];
+        }
+        s_vol[lane] = vol;
+        s_liq[lane] = liq;
+        __syncthreads();
+        if (active) {
+            const int f = lane - t;  // first lane of the cell
+            double V = 0.0, L = 0.0;
+            for (int q = 0; q < 5; ++q) { V += s_vol[f + q]; L += s_liq[f + q]; }
+            const double lv = lam[m.vof(i, j, k)];
+            if (lv != 0.0) {
+                // VF' = L / V (the tetsign of the cell multiplies L and V alike)
+                const double gL = lv / V, gV = -lv * L / (V * V);
+                AdjFrame st[CUT_MAXLVL];
+                double groot[4][3];
+                if (!cut_tet_adj(m, nx, ny, nz, D, tet, ind, st, gL, gV, groot, gpl)) o = 1;
+                for (int n = 0; n < 4; ++n)
+                    for (int q = 0; q < 3; ++q)
+                        if (groot[n][q] != 0.0) atomicAdd(gpv + 3 * vq[n] + q, groot[n][q]);
+            }
+            if (o) atomicOr(ovf, 1);
+        }
+        __syncthreads();
+    }
+}
+
+// Corners of band cells: the adjoint of the back-traced position through project!
+__global__ void k_adj_backtrace(DevMesh m, DevParams p, const int16_t* band, const double* uf, const double* vf,
+                                const double* wf, double dt, const double* gpv, double* guf, double* gvf,
+                                double* gwf) {
+    const int64_t n = (int64_t)(m.Nx + 1) * (m.Ny + 1) * (m.Nz + 1);
+    const double* const F[3] = {uf, vf, wf};
+    double* const G[3] = {guf, gvf, gwf};
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        const double g[3] = {gpv[3 * q + 0], gpv[3 * q + 1], gpv[3 * q + 2]};
+        if (g[0] == 0.0 && g[1] == 0.0 && g[2] == 0.0) continue;
+        const int I = (int)(q % (m.Nx + 1)) + 1;
+        const int64_t r = q / (m.Nx + 1);
+        const int J = (int)(r % (m.Ny + 1)) + 1;
+        const int K = (int)(r / (m.Ny + 1)) + 1;
+        const double x[3] = {m.x[I], m.y[J], m.z[K]};
+        project_adj(m, p, F, x, min(I, m.Nx), min(J, m.Ny), min(K, m.Nz), dt, g,
+                    [&](int c, int64_t idx, double w) {
+                        if (w != 0.0) atomicAdd(G[c] + idx, w);
+                    });
+    }
+}
+
+// computeDist of every cell (computePLIC!, src/VF.jl:94-103) in reverse
+__global__ void k_adj_plic(DevMesh m, DevParams p, const double* VF, const double* nx, const double* ny,
+                           const double* nz, const double* gD, double* gnx, double* gny, double* gnz,
+                           double* lam_new) {
+    const int64_t n = (int64_t)m.cnx * m.cny * m.cnz;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        const double g = gD[q];
+        if (g == 0.0) continue;
+        int i, j, k;
+        cell_of(m, q, i, j, k);
+        double gn[3] = {0.0, 0.0, 0.0}, gv = 0.0;
+        compute_dist_adj(m, p, i, j, k, nx[q], ny[q], nz[q], VF[m.vof(i, j, k)], g, gn, gv);
+        gnx[q] += gn[0];
+        gny[q] += gn[1];
+        gnz[q] += gn[2];
+        if (gv != 0.0) lam_new[m.vof(i, j, k)] += gv;
+    }
+}
+
+// ELVIRA normals of the interface cells in reverse, through the candidate the forward kept
+__global__ void k_adj_elvira(DevMesh m, DevParams p, const double* VF, const int* best, const double* gnx,
+                             const double* gny, const double* gnz, double* lam_new) {
+    const int n = m.Nx * m.Ny * m.Nz;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        int i, j, k;
+        interior_of(m, q, i, j, k);
+        const int64_t c = m.cell(i, j, k);
+        const double g[3] = {gnx[c], gny[c], gnz[c]};
+        if (g[0] == 0.0 && g[1] == 0.0 && g[2] == 0.0) continue;
+        const double vf = VF[m.vof(i, j, k)];
+        if (!(vf >= p.VFlo && vf <= p.VFhi)) continue;  // axis normals: constant
+        const int cand = best[q];
+        if (cand < 0) continue;
+        elvira_cand_adj(
+            m, i, j, k, cand, [&](int di, int dj, int dk) { return VF[m.vof(i + di, j + dj, k + dk)]; }, g,
+            [&](int di, int dj, int dk, double w) {
+                if (w != 0.0) atomicAdd(lam_new + m.vof(i + di, j + dj, k + dk), w);
+            });
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// launcher
+// ------------------------------------------------------------------------------------
+static inline int adj_grid(int64_t n, int block, int cap = 65536) {
+    int64_t g = (n + block - 1) / block;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(g, cap));
+}
+
+#define ADJL(kernel, grid, block, ...)                                                   \
+    do {                                                                                 \
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, ctx->stream, __VA_ARGS__); \
+        MIST_HIP_TRY(hipGetLastError());                                                 \
+    } while (0)
+
+static AdjExt adj_ext(mist_ctx* ctx, int field) {
+    AdjExt e;
+    for (int d = 0; d < 3; ++d) { e.lo[d] = (int)ctx->flo[field][d]; e.n[d] = (int)ctx->fn[field][d]; }
+    return e;
+}
+
+// reverse of launch_borders on a double array with the extents of `field`
+static int adj_borders(mist_ctx* ctx, double* g, int field) {
+    const AdjExt e = adj_ext(ctx, field);
+    const int64_t per[3] = {ctx->p.xper, ctx->p.yper, ctx->p.zper};
+    const int mx[3] = {ctx->m.Nx, ctx->m.Ny, ctx->m.Nz};
+    const int ng = ctx->m.ng;
+    for (int d = 2; d >= 0; --d) {
+        if (!per[d]) continue;
+        const int64_t tot = (int64_t)e.n[(d + 1) % 3] * e.n[(d + 2) % 3];
+        ADJL(k_adj_borders_axis, adj_grid(tot, 256), 256, g, e, d, 1, mx[d], 1 - ng, mx[d] + ng, ng);
+    }
+    return MIST_OK;
+}
+
+static int adj_neumann(mist_ctx* ctx, double* g, int field) {
+    const AdjExt e = adj_ext(ctx, field);
+    const int mx[3] = {ctx->m.Nx, ctx->m.Ny, ctx->m.Nz};
+    for (int d = 2; d >= 0; --d) {
+        const int64_t tot = (int64_t)e.n[(d + 1) % 3] * e.n[(d + 2) % 3];
+        ADJL(k_adj_neumann_axis, adj_grid(tot, 256), 256, g, e, d, 1, mx[d]);
+    }
+    return MIST_OK;
+}
+
+int launch_transport_vjp(mist_ctx* ctx, double dt) {
+    int st;
+    const mist::DevMesh& m = ctx->m;
+    const int64_t ncell_ext = (int64_t)m.cnx * m.cny * m.cnz;
+    const int64_t nvof = (int64_t)m.vnx * m.vny * m.vnz;
+    const int64_t nv = (int64_t)(m.Nx + 1) * (m.Ny + 1) * (m.Nz + 1);
+    const int64_t ncap = (int64_t)m.Nx * m.Ny * m.Nz;
+    double* lam = (double*)ctx->f[MIST_FIELD_VOF_BAR];
+    double* lam_new = ctx->adj_lam;
+    double* gpl[4] = {ctx->adj_plane, ctx->adj_plane + ncell_ext, ctx->adj_plane + 2 * ncell_ext,
+                      ctx->adj_plane + 3 * ncell_ext};
+    // pre-images of this state's band cells under the current face velocities
+    if ((st = launch_project_vertices(ctx, dt))) return st;
+    // 1. ghost fills of VF' in reverse: periodic copies, then Neumann
+    if ((st = adj_borders(ctx, lam, MIST_FIELD_VOF))) return st;
+    if ((st = adj_neumann(ctx, lam, MIST_FIELD_VOF))) return st;
+    ADJL(k_adj_fill, adj_grid(nvof, 256), 256, lam_new, nvof, 0.0);
+    ADJL(k_adj_fill, adj_grid(4 * ncell_ext, 256), 256, ctx->adj_plane, 4 * ncell_ext, 0.0);
+    ADJL(k_adj_fill, adj_grid(3 * nv, 256), 256, ctx->adj_pv, 3 * nv, 0.0);
+    // 2. identity outside the band; the band cells' cut in reverse
+    ADJL(k_adj_identity, adj_grid(ncap, 256), 256, m, (int)ctx->p.nband, (const int16_t*)ctx->f[MIST_FIELD_BAND],
+         (const double*)lam, lam_new);
+    const double* nx = (const double*)ctx->f[MIST_FIELD_NX];
+    const double* ny = (const double*)ctx->f[MIST_FIELD_NY];
+    const double* nz = (const double*)ctx->f[MIST_FIELD_NZ];
+    const double* D = (const double*)ctx->f[MIST_FIELD_PLIC_D];
+    ADJL(k_adj_cut, adj_grid(ncap, 12, 8192), 64, m, (const double*)ctx->pv, (const int*)ctx->band_list,
+         (const int*)ctx->counters, nx, ny, nz, D, (const double*)lam, ctx->adj_pv, gpl[0], gpl[1], gpl[2], gpl[3],
+         ctx->counters + 2);
+    // 3. the back-trace in reverse, into the face-velocity adjoints
+    ADJL(k_adj_backtrace, adj_grid(nv, 128), 128, m, ctx->dp, (const int16_t*)ctx->f[MIST_FIELD_BAND],
+         (const double*)ctx->f[MIST_FIELD_UF], (const double*)ctx->f[MIST_FIELD_VF],
+         (const double*)ctx->f[MIST_FIELD_WF], dt, (const double*)ctx->adj_pv, (double*)ctx->f[MIST_FIELD_UF_BAR],
+         (double*)ctx->f[MIST_FIELD_VFACE_BAR], (double*)ctx->f[MIST_FIELD_WF_BAR]);
+    // 4. computePLIC! (after D's periodic copies) and ELVIRA (after the normals' copies)
+    if ((st = adj_borders(ctx, gpl[3], MIST_FIELD_PLIC_D))) return st;
+    ADJL(k_adj_plic, adj_grid(ncell_ext, 256), 256, m, ctx->dp, (const double*)ctx->f[MIST_FIELD_VOF], nx, ny, nz,
+         (const double*)gpl[3], gpl[0], gpl[1], gpl[2], lam_new);
+    for (int a = 0; a < 3; ++a)
+        if ((st = adj_borders(ctx, gpl[a], MIST_FIELD_NX + a))) return st;
+    ADJL(k_adj_elvira, adj_grid(ncap, 256), 256, m, ctx->dp, (const double*)ctx->f[MIST_FIELD_VOF],
+         (const int*)ctx->elv_best, (const double*)gpl[0], (const double*)gpl[1], (const double*)gpl[2], lam_new);
+    MIST_HIP_TRY(hipMemcpyAsync(lam, lam_new, nvof * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    return MIST_OK;
+}
+
+}  // namespace mist

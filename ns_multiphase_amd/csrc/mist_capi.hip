@@ -147,8 +147,10 @@ int mist_ctx_destroy(mist_ctx* ctx) {
     for (int f = 0; f < MIST_FIELD_COUNT; ++f)
         if (ctx->f[f]) (void)hipFree(ctx->f[f]);
     void* bufs[] = {ctx->geom, ctx->trig, ctx->pv, ctx->band_list, ctx->iface_list, ctx->defer_list, ctx->cut_list,
-                    ctx->counters, ctx->red, ctx->ns_sums};
+                    ctx->counters, ctx->red, ctx->ns_sums, ctx->elv_best, ctx->adj_lam, ctx->adj_plane, ctx->adj_pv};
     for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    for (double* b : ctx->ckpt)
         if (b) (void)hipFree(b);
     if (ctx->h_counters) (void)hipHostFree(ctx->h_counters);
     if (ctx->h_red) (void)hipHostFree(ctx->h_red);
@@ -238,7 +240,7 @@ int mist_ctx_create(const mist_params* pin, int device, mist_ctx** out) {
     // fields
     for (int f = 0; f < MIST_FIELD_COUNT; ++f) {
         int64_t lo = imino, n[3] = {m.cnx, m.cny, m.cnz}, el = 8;
-        if (f == MIST_FIELD_VOF) { lo = imino - 3; n[0] += 6; n[1] += 6; n[2] += 6; }
+        if (f == MIST_FIELD_VOF || f == MIST_FIELD_VOF_BAR) { lo = imino - 3; n[0] += 6; n[1] += 6; n[2] += 6; }
         if (f == MIST_FIELD_DENX || f == MIST_FIELD_VISCX || f == MIST_FIELD_SFX) n[0] += 1;
         if (f == MIST_FIELD_DENY || f == MIST_FIELD_VISCY || f == MIST_FIELD_SFY) n[1] += 1;
         if (f == MIST_FIELD_DENZ || f == MIST_FIELD_VISCZ || f == MIST_FIELD_SFZ) n[2] += 1;
@@ -257,6 +259,7 @@ int mist_ctx_create(const mist_params* pin, int device, mist_ctx** out) {
         hipMalloc(&ctx->iface_list, (size_t)ncell * sizeof(int)) != hipSuccess ||
         hipMalloc(&ctx->defer_list, (size_t)ncell * sizeof(int)) != hipSuccess ||
         hipMalloc(&ctx->cut_list, (size_t)ncell * 24) != hipSuccess ||
+        hipMalloc(&ctx->elv_best, (size_t)ncell * sizeof(int)) != hipSuccess ||
         hipMalloc(&ctx->counters, 16 * sizeof(int)) != hipSuccess ||
         hipMalloc(&ctx->red, (size_t)(4 * ctx->red_blocks + 16) * sizeof(double)) != hipSuccess)
         return bail(MIST_ERR_OOM, "work allocation");
@@ -718,6 +721,58 @@ int mist_plic_vf_batch(mist_ctx* ctx, const int64_t* ijk, const double* nrm, con
                        int64_t m) {
     return batch(ctx, ijk, nrm, dist, vf_out, m, false);
 }
+
+// ---- adjoint ---------------------------------------------------------------------------
+int mist_transport_vjp(mist_ctx* ctx, double dt) {
+    CHECK_CTX();
+    if (ctx->p.solveNS) return fail(ctx, MIST_ERR_PARAM, "mist_transport_vjp: the adjoint covers the VOF transport (solveNS = 0)");
+    if (!(dt > 0.0)) return fail(ctx, MIST_ERR_PARAM, "dt must be > 0");
+    MIST_HIP_TRY(hipSetDevice(ctx->device));
+    const mist::DevMesh& m = ctx->m;
+    if (!ctx->adj_lam) {
+        const size_t nvof = (size_t)m.vnx * m.vny * m.vnz;
+        const size_t ncell_ext = (size_t)m.cnx * m.cny * m.cnz;
+        const size_t nv = (size_t)(m.Nx + 1) * (m.Ny + 1) * (m.Nz + 1);
+        if (hipMalloc(&ctx->adj_lam, nvof * sizeof(double)) != hipSuccess ||
+            hipMalloc(&ctx->adj_plane, 4 * ncell_ext * sizeof(double)) != hipSuccess ||
+            hipMalloc(&ctx->adj_pv, 3 * nv * sizeof(double)) != hipSuccess)
+            return fail(ctx, MIST_ERR_OOM, "adjoint work allocation");
+    }
+    // the forward state of the step: normals (and the ELVIRA choice), PLIC constants, band
+    ctx->normals_version = 0;  // recompute: the kept ELVIRA candidates must be this VF's
+    RET_IF(mist_compute_normal(ctx));
+    RET_IF(mist_compute_plic(ctx));
+    if (!ctx->band_list_valid) RET_IF(mist_compute_band(ctx));
+    RET_IF(mist::launch_transport_vjp(ctx, dt));
+    RET_IF(check_overflow(ctx));
+    return MIST_OK;
+}
+
+static int ckpt_copy(mist_ctx* ctx, int slot, bool save) {
+    CHECK_CTX();
+    if (slot < 0 || slot >= 4096) return fail(ctx, MIST_ERR_PARAM, "checkpoint slot must be 0..4095");
+    MIST_HIP_TRY(hipSetDevice(ctx->device));
+    const size_t bytes = field_bytes(ctx, MIST_FIELD_VOF);
+    if ((size_t)slot >= ctx->ckpt.size()) ctx->ckpt.resize((size_t)slot + 1, nullptr);
+    if (!ctx->ckpt[slot]) {
+        if (!save) return fail(ctx, MIST_ERR_STATE, "checkpoint slot " + std::to_string(slot) + " is empty");
+        if (hipMalloc(&ctx->ckpt[slot], bytes) != hipSuccess) return fail(ctx, MIST_ERR_OOM, "checkpoint allocation");
+    }
+    if (save) {
+        MIST_HIP_TRY(hipMemcpyAsync(ctx->ckpt[slot], ctx->f[MIST_FIELD_VOF], bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    } else {
+        MIST_HIP_TRY(hipMemcpyAsync(ctx->f[MIST_FIELD_VOF], ctx->ckpt[slot], bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        ctx->vf_version++;
+        // the rest of the state a step starts from: band, normals and PLIC constants of VF
+        RET_IF(mist_compute_band(ctx));
+        RET_IF(mist_compute_normal(ctx));
+        RET_IF(mist_compute_plic(ctx));
+    }
+    return MIST_OK;
+}
+
+int mist_checkpoint_save(mist_ctx* ctx, int slot) { return ckpt_copy(ctx, slot, true); }
+int mist_checkpoint_restore(mist_ctx* ctx, int slot) { return ckpt_copy(ctx, slot, false); }
 
 int mist_profile_enable(mist_ctx* ctx, int on) {
     CHECK_CTX();

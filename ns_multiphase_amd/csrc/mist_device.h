@@ -52,20 +52,20 @@ struct DevParams {
 };
 
 // Julia min/max for non-NaN arguments with signed zeros ordered -0.0 < 0.0.
-__device__ __forceinline__ double jmax(double a, double b) {
+__host__ __device__ __forceinline__ double jmax(double a, double b) {
     if (a != a) return a;
     if (b != b) return b;
-    if (a == b) return signbit(a) ? b : a;
+    if (a == b) return __builtin_signbit(a) ? b : a;
     return a > b ? a : b;
 }
-__device__ __forceinline__ double jmin(double a, double b) {
+__host__ __device__ __forceinline__ double jmin(double a, double b) {
     if (a != a) return a;
     if (b != b) return b;
-    if (a == b) return signbit(a) ? a : b;
+    if (a == b) return __builtin_signbit(a) ? a : b;
     return a < b ? a : b;
 }
 // a /^ b  (src/Tools.jl:375-377)
-__device__ __forceinline__ double sdiv(double a, double b) {
+__host__ __device__ __forceinline__ double sdiv(double a, double b) {
     return fabs(b) <= EPS ? __builtin_huge_val() : a / b;
 }
 
@@ -126,10 +126,10 @@ __constant__ const unsigned short TET5[5] = {
     (0 | (4 << 4) | (5 << 8) | (6 << 12)), (3 | (6 << 4) | (5 << 8) | (7 << 12)),
     (0 | (3 << 4) | (6 << 8) | (5 << 12))};
 
-__device__ __forceinline__ int nib(unsigned v, int q) { return (v >> (4 * q)) & 15; }
+__host__ __device__ __forceinline__ int nib(unsigned v, int q) { return (v >> (4 * q)) & 15; }
 
 // tet_vol (src/VFgeom.jl:669-681)
-__device__ __forceinline__ double tet_vol(const double* p0, const double* p1, const double* p2, const double* p3) {
+__host__ __device__ __forceinline__ double tet_vol(const double* p0, const double* p1, const double* p2, const double* p3) {
     const double a1 = p0[0] - p3[0], a2 = p0[1] - p3[1], a3 = p0[2] - p3[2];
     const double b1 = p1[0] - p3[0], b2 = p1[1] - p3[1], b3 = p1[2] - p3[2];
     const double c1 = p2[0] - p3[0], c2 = p2[1] - p3[1], c3 = p2[2] - p3[2];
@@ -137,7 +137,7 @@ __device__ __forceinline__ double tet_vol(const double* p0, const double* p1, co
 }
 
 // d2case (src/VFgeom.jl:693-708) on a copy held in registers; d is snapped in place.
-__device__ __forceinline__ int d2case_snap(double d[4]) {
+__host__ __device__ __forceinline__ int d2case_snap(double d[4]) {
     int npos = 0, nneg = 0;
 #pragma unroll
     for (int n = 0; n < 4; ++n) { npos += d[n] > 0.0; nneg += d[n] < 0.0; }
@@ -149,7 +149,7 @@ __device__ __forceinline__ int d2case_snap(double d[4]) {
 }
 
 // computeAlpha (src/VF.jl:164-214), Scardovelli & Zaleski (2000).
-__device__ inline double compute_alpha(double a, double b, double c, double VF) {
+__host__ __device__ inline double compute_alpha(double a, double b, double c, double VF) {
     // sort ascending
     double m1 = a, m2 = b, m3 = c, t;
     if (m1 > m2) { t = m1; m1 = m2; m2 = t; }
@@ -194,7 +194,7 @@ __device__ inline double compute_alpha(double a, double b, double c, double VF) 
 }
 
 // computeDist (src/VF.jl:109-156)
-__device__ inline double compute_dist(const DevMesh& m, const DevParams& p, int i, int j, int k, double nx,
+__host__ __device__ inline double compute_dist(const DevMesh& m, const DevParams& p, int i, int j, int k, double nx,
                                       double ny, double nz, double VF) {
     if (VF < p.VFlo) return -((m.Lx + m.Ly) + m.Lz);
     if (VF > p.VFhi) return +((m.Lx + m.Ly) + m.Lz);
@@ -218,7 +218,7 @@ __device__ inline double compute_dist(const DevMesh& m, const DevParams& p, int 
 // Corner c (bits: x = c&1, y = c>>1&1, z = c>>2) of cell (i,j,k) in the reference vertex
 // numbering of cell2verts! (src/VFgeom.jl:190-216): even cells number corners in lexical
 // order, odd cells mirror x. Returns the lexical corner index of reference vertex v.
-__device__ __forceinline__ int vert_corner(int v, bool even) { return even ? v : (v ^ 1); }
+__host__ __device__ __forceinline__ int vert_corner(int v, bool even) { return even ? v : (v ^ 1); }
 
 
 // ---- register-resident tet cutting -------------------------------------------------
@@ -252,10 +252,10 @@ struct V3 {
     double x, y, z;
 };
 
-__device__ __forceinline__ V3 lerp3(const V3& a, const V3& b, double mu) {
+__host__ __device__ __forceinline__ V3 lerp3(const V3& a, const V3& b, double mu) {
     return V3{(1.0 - mu) * a.x + mu * b.x, (1.0 - mu) * a.y + mu * b.y, (1.0 - mu) * a.z + mu * b.z};
 }
-__device__ __forceinline__ double tet_vol3(const V3& p0, const V3& p1, const V3& p2, const V3& p3) {
+__host__ __device__ __forceinline__ double tet_vol3(const V3& p0, const V3& p1, const V3& p2, const V3& p3) {
     const double a1 = p0.x - p3.x, a2 = p0.y - p3.y, a3 = p0.z - p3.z;
     const double b1 = p1.x - p3.x, b2 = p1.y - p3.y, b3 = p1.z - p3.z;
     const double c1 = p2.x - p3.x, c2 = p2.y - p3.y, c3 = p2.z - p3.z;
@@ -385,7 +385,7 @@ __device__ __forceinline__ double plic2vf_full(double x0, double x1, double y0, 
 
 // pt2index! (src/Tools.jl:424-434). The walk is bounded by the coordinate arrays; a walk
 // that would leave them sets *ovf and stops at the last valid cell.
-__device__ __forceinline__ void pt2index(const DevMesh& m, int I[3], const double pt[3], int i, int j, int k,
+__host__ __device__ __forceinline__ void pt2index(const DevMesh& m, int I[3], const double pt[3], int i, int j, int k,
                                          int* ovf) {
     const int lo = 1 - m.ng;
     I[0] = i; I[1] = j; I[2] = k;
@@ -399,13 +399,13 @@ __device__ __forceinline__ void pt2index(const DevMesh& m, int I[3], const doubl
 
 // Index search of get_velocity_{u,v,w}face (src/Tools.jl:520-540): i with c[i] <= p < c[i+1]
 // clamped to [lo, hi-1]. The result does not depend on the starting index.
-__device__ __forceinline__ int find_cell(double p, int i, int lo, int hi, const double* c) {
+__host__ __device__ __forceinline__ int find_cell(double p, int i, int lo, int hi, const double* c) {
     while (p - c[i] < 0.0 && i > lo) i = i - 1;
     while (p - c[i + 1] >= 0.0 && i + 1 < hi) i = i + 1;
     return i;
 }
 
-__device__ __forceinline__ double trilin(const double* A, const DevMesh& m, int i, int j, int k, double wx1,
+__host__ __device__ __forceinline__ double trilin(const double* A, const DevMesh& m, int i, int j, int k, double wx1,
                                          double wx2, double wy1, double wy2, double wz1, double wz2) {
     const int64_t sx = 1, sy = m.cnx, sz = (int64_t)m.cnx * m.cny;
     const int64_t b = m.cell(i, j, k);
@@ -416,7 +416,7 @@ __device__ __forceinline__ double trilin(const double* A, const DevMesh& m, int 
 }
 
 // get_velocity_{u,v,w}face (src/Tools.jl:516-641), starting index (i,j,k).
-__device__ inline void vel_face(const DevMesh& m, const double* uf, const double* vf, const double* wf,
+__host__ __device__ inline void vel_face(const DevMesh& m, const double* uf, const double* vf, const double* wf,
                                 const double pt[3], int i0, int j0, int k0, double v[3]) {
     const int lo = 1 - m.ng;
     int i, j, k;

@@ -79,6 +79,11 @@ struct mist_ctx {
     bool timing = false;
     KernelProfile prof;
     double bc_wall[6][3] = {};   // wall velocity of sides x-, x+, y-, y+, z-, z+ (reflect BC)
+    int* elv_best = nullptr;     // ELVIRA candidate kept per interior cell (-1: none)
+    double* adj_lam = nullptr;   // adjoint work: dL/dVF of the step input (VF extents)
+    double* adj_plane = nullptr; // adjoint work: planes nx, ny, nz, D (4 cell-extent arrays)
+    double* adj_pv = nullptr;    // adjoint work: back-traced corners (3 per grid vertex)
+    std::vector<double*> ckpt;   // VF checkpoints (device)
     std::string err;
 };
 
@@ -114,6 +119,8 @@ int launch_reflect_bc(mist_ctx* ctx, int fu, int fv, int fw);
 int launch_interp_face(mist_ctx* ctx, int fu, int fv, int fw);
 int launch_interp_center(mist_ctx* ctx);
 int launch_divergence(mist_ctx* ctx, double dt);
+// mist_adjoint.hip
+int launch_transport_vjp(mist_ctx* ctx, double dt);
 // mist_mg.hip
 int pressure_solve(mist_ctx* ctx, double dt, double tol, int precision, int64_t* cycles, double* resid);
 }  // namespace mist

@@ -48,7 +48,7 @@ __device__ __forceinline__ void append(bool flag, int item, int* list, int* coun
 // a vertex does not depend on which cell asks for it (the interpolation index search
 // converges to the same cell from any start), so each vertex of a band cell is traced once.
 // ------------------------------------------------------------------------------------
-__device__ inline void project_pt(const DevMesh& m, const DevParams& p, const double* uf, const double* vf,
+__host__ __device__ inline void project_pt(const DevMesh& m, const DevParams& p, const double* uf, const double* vf,
                            const double* wf, double pt[3], int i, int j, int k, double dt) {
     if (p.zalesak) {
         const double pi2 = 2.0 * PI;

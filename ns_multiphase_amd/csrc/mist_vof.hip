@@ -336,7 +336,7 @@ constexpr int ELV_PAIRS = 27 * 9;  // candidate-neighbour pairs of one z-plane o
 
 __global__ __launch_bounds__(ELV_THREADS, 4) void k_elvira(DevMesh m, DevParams p, const double* VF, const int* list,
                                                         const int* count, int fam_mask, double* nxo, double* nyo,
-                                                        double* nzo) {
+                                                        double* nzo, int* best) {
     __shared__ double sVF[27];
     __shared__ double sC[27][4];
     __shared__ int sOk[27];
@@ -536,9 +536,12 @@ __global__ __launch_bounds__(ELV_THREADS, 4) void k_elvira(DevMesh m, DevParams 
                 const int ol = __shfl_xor(bl, off, 32);
                 if (ol < 64 && (bl == 64 || oe < be || (oe == be && ol < bl))) { be = oe; bl = ol; }
             }
-            if (lane == 0 && bl < 64) {
-                const int64_t c = m.cell(i, j, k);
-                nxo[c] = sC[bl][0]; nyo[c] = sC[bl][1]; nzo[c] = sC[bl][2];
+            if (lane == 0) {
+                if (bl < 64) {
+                    const int64_t c = m.cell(i, j, k);
+                    nxo[c] = sC[bl][0]; nyo[c] = sC[bl][1]; nzo[c] = sC[bl][2];
+                }
+                best[list[cidx]] = bl < 64 ? bl : -1;
             }
         }
         __syncthreads();
@@ -1513,7 +1516,8 @@ int launch_maxabs_uvw(mist_ctx* ctx, double out[3]) {
 int launch_reduce(mist_ctx* ctx, int kind, int field, double* out) {
     const int64_t n = (int64_t)ctx->m.Nx * ctx->m.Ny * ctx->m.Nz;
     const int g = std::min(ctx->red_blocks, grid_for(n, RED_THREADS));
-    LAUNCH(k_reduce_interior, g, RED_THREADS, ctx->m, (const double*)ctx->f[field], field == MIST_FIELD_VOF ? 1 : 0,
+    LAUNCH(k_reduce_interior, g, RED_THREADS, ctx->m, (const double*)ctx->f[field],
+           (field == MIST_FIELD_VOF || field == MIST_FIELD_VOF_BAR) ? 1 : 0,
            kind, ctx->red);
     LAUNCH(k_reduce_partials, 1, RED_THREADS, ctx->red, g, 1, kind == MIST_REDUCE_MAXABS ? (int)MIST_REDUCE_MAX : kind,
            ctx->red + 3 * ctx->red_blocks);
@@ -1584,7 +1588,7 @@ int launch_normals(mist_ctx* ctx) {
     const int64_t ncap = (int64_t)ctx->m.Nx * ctx->m.Ny * ctx->m.Nz;
     hipEvent_t ev = prof_begin(ctx, MIST_PROF_ELVIRA);
     LAUNCH(k_elvira, grid_for(ncap, 1, 16384), ELV_THREADS, ctx->m, ctx->dp, (const double*)ctx->f[MIST_FIELD_VOF],
-           (const int*)ctx->iface_list, (const int*)(ctx->counters + 1), fam, nx, ny, nz);
+           (const int*)ctx->iface_list, (const int*)(ctx->counters + 1), fam, nx, ny, nz, ctx->elv_best);
     prof_end(ctx, MIST_PROF_ELVIRA, ev, ctx->counters + 1, 0);
     int st;
     if ((st = launch_borders(ctx, MIST_FIELD_NX))) return st;

@@ -30,6 +30,7 @@ FIELD = {
     "uf": 9, "vf": 10, "wf": 11, "P": 12, "divg": 13, "denx": 14, "deny": 15, "denz": 16,
     "RHS": 17, "us": 18, "vs": 19, "ws": 20, "viscx": 21, "viscy": 22, "viscz": 23,
     "Curve": 24, "sfx": 25, "sfy": 26, "sfz": 27, "mask": 28,
+    "VF_bar": 29, "uf_bar": 30, "vf_bar": 31, "wf_bar": 32,
 }
 # wall sides of the boundary-condition descriptor: x-, x+, y-, y+, z-, z+
 SIDES = {"xm": 0, "xp": 1, "ym": 2, "yp": 3, "zm": 4, "zp": 5}

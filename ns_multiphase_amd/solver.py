@@ -79,6 +79,9 @@ def lib():
         "mist_surface_force": (i32, [vp]),
         "mist_ns_init": (i32, [vp]),
         "mist_ns_step": (i32, [vp, dp, d, dp, d, i64, ip, dp]),
+        "mist_transport_vjp": (i32, [vp, d]),
+        "mist_checkpoint_save": (i32, [vp, i32]),
+        "mist_checkpoint_restore": (i32, [vp, i32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -273,6 +276,16 @@ class Context:
         if check:
             self._check(st)
         return tt.value, dt.value, cyc.value, res.value
+
+    # ---- adjoint of the SL-VOF step (no reference counterpart) -------------------------
+    def transport_vjp(self, dt):
+        self._check(lib().mist_transport_vjp(self.h, dt))
+
+    def checkpoint_save(self, slot):
+        self._check(lib().mist_checkpoint_save(self.h, slot))
+
+    def checkpoint_restore(self, slot):
+        self._check(lib().mist_checkpoint_restore(self.h, slot))
 
     PROFILE_KERNELS = ("elvira", "sl_cut", "plic", "project", "mg_rbgs", "mg_resid", "sl_general",
                        "ns_cut", "ns_fields")

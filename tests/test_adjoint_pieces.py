@@ -1,0 +1,166 @@
+"""CPU checks of the adjoint pieces of the SL-VOF step (ns_multiphase_amd/csrc/mist_adjoint.h):
+each reverse-mode function against central differences of the forward function it reverses
+(the library's own forward code, compiled for the host through tests/adjoint_harness.hip).
+Inputs are drawn away from branch switches, where the map is smooth."""
+import ctypes
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(BUILD, "libadjh.so")
+SRC = [os.path.join(HERE, "adjoint_harness.hip")] + [
+    os.path.join(HERE, "..", "ns_multiphase_amd", "csrc", f) for f in ("mist_adjoint.h", "mist_device.h", "mist_kernels.h")]
+
+dp = ctypes.POINTER(ctypes.c_double)
+
+
+@pytest.fixture(scope="module")
+def H():
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    os.makedirs(BUILD, exist_ok=True)
+    if not os.path.exists(LIB) or os.path.getmtime(LIB) < max(os.path.getmtime(s) for s in SRC):
+        subprocess.run([hipcc, "--offload-arch=gfx950", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+                        SRC[0], "-o", LIB + ".tmp"], check=True, capture_output=True)
+        os.replace(LIB + ".tmp", LIB)
+    L = ctypes.CDLL(LIB)
+    L.h_alpha.restype = ctypes.c_double
+    L.h_dist.restype = ctypes.c_double
+    for f in ("h_alpha", "h_alpha_adj"):
+        getattr(L, f).argtypes = [ctypes.c_double] * 4 + ([dp] if f.endswith("adj") else [])
+    L.h_leaf.argtypes = [dp, dp, ctypes.c_double, dp]
+    L.h_leaf_adj.argtypes = [dp, dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp, dp, dp]
+    L.h_dist.argtypes = [ctypes.c_int] * 3 + [dp, ctypes.c_double]
+    L.h_dist_adj.argtypes = [ctypes.c_int] * 3 + [dp, ctypes.c_double, dp]
+    L.h_mesh.argtypes = [ctypes.POINTER(ctypes.c_int64), dp, ctypes.c_int, ctypes.c_double]
+    L.h_project.argtypes = [dp, dp, dp, dp] + [ctypes.c_int] * 3 + [ctypes.c_double, dp]
+    L.h_project_adj.argtypes = [dp, dp, dp, dp] + [ctypes.c_int] * 3 + [ctypes.c_double, dp, dp, dp, dp]
+    L.h_elvira_cand.argtypes = [dp] + [ctypes.c_int] * 4 + [dp]
+    L.h_elvira_cand_adj.argtypes = [dp] + [ctypes.c_int] * 4 + [dp, dp]
+    return L
+
+
+def P(a):
+    return np.ascontiguousarray(a, np.float64).ctypes.data_as(dp)
+
+
+def fd(f, x, h):
+    g = np.zeros_like(x)
+    for q in range(x.size):
+        e = np.zeros_like(x)
+        e.flat[q] = h
+        g.flat[q] = (f(x + e) - f(x - e)) / (2 * h)
+    return g
+
+
+def test_leaf_cut_adjoint(H):
+    rng = np.random.default_rng(3)
+    checked = 0
+    for trial in range(60):
+        x = rng.random((4, 3))
+        n = rng.standard_normal(3)
+        n /= np.linalg.norm(n)
+        # plane through a point of the tet, so that it is cut
+        D = float(n @ (x.mean(0) + 0.1 * rng.standard_normal(3)))
+        d = x @ n - D
+        if np.abs(d).min() < 1e-3:
+            continue  # near a case switch
+        for gL, gV in ((1.0, 0.0), (0.0, 1.0), (0.7, -0.3)):
+            def F(z):
+                out = np.zeros(2)
+                H.h_leaf(P(z[:12]), P(z[12:15]), float(z[15]), P(out))
+                return gL * out[0] + gV * out[1]
+            z = np.concatenate([x.ravel(), n, [D]])
+            num = fd(F, z, 1e-6)
+            gx, gn, gD = np.zeros(12), np.zeros(3), np.zeros(1)
+            H.h_leaf_adj(P(x), P(n), D, gL, gV, P(gx), P(gn), P(gD))
+            ana = np.concatenate([gx, gn, gD])
+            assert np.allclose(ana, num, rtol=1e-6, atol=1e-8), (trial, ana, num)
+        checked += 1
+    assert checked >= 30
+
+
+@pytest.mark.parametrize("VF", [0.003, 0.05, 0.2, 0.45])
+def test_compute_alpha_adjoint_all_branches(H, VF):
+    rng = np.random.default_rng(int(VF * 1000))
+    for _ in range(40):
+        m = rng.random(3) + 0.05
+        m /= m.sum()  # |n . h| normalised like computeDist
+        z = np.concatenate([m, [VF]])
+        num = fd(lambda t: H.h_alpha(*map(float, t)), z, 1e-7)
+        out = np.zeros(4)
+        H.h_alpha_adj(*map(float, z), P(out))
+        assert np.allclose(out, num, rtol=1e-5, atol=1e-7), (z, out, num)
+
+
+def _mesh(H, N=(8, 9, 10), L=(1.0, 1.2, 0.9)):
+    H.h_mesh((ctypes.c_int64 * 3)(*N), P(np.array(L)), 3, 1e-10)
+    return N, L
+
+
+def test_compute_dist_adjoint(H):
+    _mesh(H)
+    rng = np.random.default_rng(5)
+    for _ in range(80):
+        n = rng.standard_normal(3)
+        n /= np.linalg.norm(n)
+        if np.abs(n).min() < 0.05:
+            continue
+        VF = float(rng.uniform(0.02, 0.98))
+        i, j, k = (int(v) for v in rng.integers(1, 7, 3))
+        z = np.concatenate([n, [VF]])
+        num = fd(lambda t: H.h_dist(i, j, k, P(t[:3]), float(t[3])), z, 1e-7)
+        out = np.zeros(4)
+        H.h_dist_adj(i, j, k, P(n), VF, P(out))
+        assert np.allclose(out, num, rtol=1e-5, atol=1e-7), (n, VF, out, num)
+
+
+def test_backtrace_adjoint(H):
+    N, L = _mesh(H)
+    ng = 3
+    shape = (N[2] + 2 * ng, N[1] + 2 * ng, N[0] + 2 * ng)  # k, j, i (i fastest in memory)
+    rng = np.random.default_rng(8)
+    uf, vf, wf = (0.5 * rng.standard_normal(shape) for _ in range(3))
+    dt = 0.05
+    for trial in range(12):
+        I, J, K = (int(v) for v in rng.integers(2, 8, 3))
+        x = np.array([I * L[0] / N[0], J * L[1] / N[1], K * L[2] / N[2]]) + 0.013 * rng.standard_normal(3)
+        g = rng.standard_normal(3)
+        gu, gv, gw = (np.zeros(shape) for _ in range(3))
+        H.h_project_adj(P(uf), P(vf), P(wf), P(x), I, J, K, dt, P(g), P(gu), P(gv), P(gw))
+        du, dv, dw = (rng.standard_normal(shape) for _ in range(3))
+
+        def F(h):
+            out = np.zeros(3)
+            H.h_project(P(uf + h * du), P(vf + h * dv), P(wf + h * dw), P(x), I, J, K, dt, P(out))
+            return float(g @ out)
+        num = (F(1e-6) - F(-1e-6)) / 2e-6
+        ana = float((gu * du).sum() + (gv * dv).sum() + (gw * dw).sum())
+        assert ana == pytest.approx(num, rel=1e-6, abs=1e-9), trial
+
+
+@pytest.mark.parametrize("cand", [0, 4, 8, 9, 13, 17, 18, 22, 26])
+def test_elvira_candidate_adjoint(H, cand):
+    N, L = _mesh(H)
+    ng = 3
+    shape = (N[2] + 2 * ng + 6, N[1] + 2 * ng + 6, N[0] + 2 * ng + 6)
+    rng = np.random.default_rng(cand)
+    VF = rng.random(shape)
+    i, j, k = 4, 5, 5
+    gn = rng.standard_normal(3)
+    gVF = np.zeros(shape)
+    H.h_elvira_cand_adj(P(VF), i, j, k, cand, P(gn), P(gVF))
+    d = rng.standard_normal(shape)
+
+    def F(h):
+        out = np.zeros(3)
+        H.h_elvira_cand(P(VF + h * d), i, j, k, cand, P(out))
+        return float(gn @ out)
+    num = (F(1e-7) - F(-1e-7)) / 2e-7
+    assert float((gVF * d).sum()) == pytest.approx(num, rel=1e-6, abs=1e-9)

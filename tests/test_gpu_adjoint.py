@@ -1,0 +1,126 @@
+"""The adjoint of the SL-VOF transport on the GPU (mist_transport_vjp, checkpointed sweeps):
+directional derivatives of one step against central differences of the library's forward
+step, and the gradient of the adjoint workload's loss against central differences of the
+loss computed by the library and by the oracle (the CPU restatement of the reference's
+transport; no Julia / ForwardDiff here, so central differences of the reference algorithm
+stand in for a forward-mode derivative of it).
+
+The map is piecewise smooth: cut cases, clamps, the ELVIRA argmin, the band and the cell
+searches switch along a path. Differences with h = 1e-6 straddle such a switch in a few
+directions; the bars count the directions that agree to 1e-6 relative and bound the rest.
+"""
+import numpy as np
+import pytest
+
+from ns_multiphase_amd import adjoint as A
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mist():
+    from ns_multiphase_amd import solver
+    solver.lib()
+    return solver
+
+
+def _setup(mist, N, nsteps, every, seed=1):
+    p = A.adjoint_params(N)
+    ctx = mist.Context(p)
+    # off the grid's symmetry: a circle centred on it has diagonal cells where two ELVIRA
+    # candidates give the same normal with different derivatives (a kink of the map)
+    ctx.init_vof("circle", 0.2, 0.47, 0.52)
+    vf0 = ctx.get("VF")
+    basis = A.mode_faces(ctx.mesh)
+    amps = np.random.default_rng(seed).normal(0.0, 0.1, 16)
+    dt = A.stable_dt(ctx.mesh, basis, np.random.default_rng(2).normal(0.0, 0.1, 16))
+    dt = min(dt, A.stable_dt(ctx.mesh, basis, amps))
+    run = A.SLVOFRun(ctx, vf0, basis, dt, nsteps, every)
+    return p, ctx, run, amps
+
+
+def _interior(mesh, a):
+    o = mesh.nghost + 3
+    return a[o:o + mesh.Nx, o:o + mesh.Ny, o:o + mesh.Nz]
+
+
+def test_one_step_vjp_directional(mist):
+    """<lam, dVF'> for a velocity direction and for a VF direction, one step."""
+    p, ctx, run, amps = _setup(mist, 64, 1, 1)
+    m = ctx.mesh
+    rng = np.random.default_rng(4)
+    VF1 = run.forward(amps)
+    lam = np.zeros_like(VF1)
+    _interior(m, lam)[...] = rng.standard_normal((m.Nx, m.Ny, m.Nz))
+    g_amp = run.backward(lam)
+    lam0 = A.fold_vof_ghosts(m, p, ctx.get("VF_bar"))  # the initial ghost fill, reversed
+    # velocity direction
+    da = rng.standard_normal(16)
+    h = 1e-6
+    fp = float((lam * run.forward(amps + h * da, False)).sum())
+    fm = float((lam * run.forward(amps - h * da, False)).sum())
+    assert float(g_amp @ da) == pytest.approx((fp - fm) / (2 * h), rel=1e-6)
+    # VF direction: interface cells away from the walls (their ghosts are not filled here)
+    vf0 = run.vf0
+    d = np.zeros_like(vf0)
+    mid = _interior(m, vf0)
+    sel = np.zeros(mid.shape, bool)
+    sel[4:-4, 4:-4, :] = (mid[4:-4, 4:-4, :] > 0.02) & (mid[4:-4, 4:-4, :] < 0.98)
+    _interior(m, d)[sel] = rng.standard_normal(int(sel.sum()))
+    saved = run.vf0
+    run.vf0 = saved + h * d
+    fp = float((lam * run.forward(amps, False)).sum())
+    run.vf0 = saved - h * d
+    fm = float((lam * run.forward(amps, False)).sum())
+    run.vf0 = saved
+    assert float((lam0 * d).sum()) == pytest.approx((fp - fm) / (2 * h), rel=1e-5)
+
+
+def test_gradient_matches_central_differences(mist):
+    """Loss gradient over 24 steps with checkpoints every 8 (64 x 64) against central
+    differences of the library's forward loss, per amplitude."""
+    p, ctx, run, amps = _setup(mist, 64, 24, 8)
+    tgt_amps = np.random.default_rng(2).normal(0.0, 0.1, 16)
+    target = run.forward(tgt_amps, False)
+    L, g = run.loss_and_grad(amps, target)
+    h = 1e-6
+    fd = np.array([(run.loss(amps + h * e, target) - run.loss(amps - h * e, target)) / (2 * h)
+                   for e in np.eye(16)])
+    rel = np.abs(g - fd) / np.maximum(np.abs(fd), 1e-3 * np.abs(fd).max())
+    within = int((rel <= 1e-6).sum())
+    print(f"adjoint vs FD: {within}/16 within 1e-6, max rel {rel.max():.2e}")
+    assert within >= 12
+    assert np.linalg.norm(g - fd) <= 1e-3 * np.linalg.norm(fd)
+
+
+def test_gradient_matches_oracle_differences(mist, oracle):
+    """The same gradient against central differences of the oracle's loss (the reference's
+    algorithm on the CPU) at 64 x 64, 16 steps."""
+    p, ctx, run, amps = _setup(mist, 64, 16, 8)
+    tgt_amps = np.random.default_rng(2).normal(0.0, 0.1, 16)
+    target = run.forward(tgt_amps, False)
+    L, g = run.loss_and_grad(amps, target)
+    o = oracle.OracleContext(p)
+    m = ctx.mesh
+    oo = m.nghost + 3
+    sl = (slice(oo, oo + m.Nx), slice(oo, oo + m.Ny), slice(oo, oo + m.Nz))
+
+    def oracle_loss(a):
+        uf, vf = A.faces_of(run.basis, a)
+        o.set("uf", uf)
+        o.set("vf", vf)
+        o.set("wf", np.zeros_like(uf))
+        o.set("VF", run.vf0)
+        o.vof_init()
+        for _ in range(run.nsteps):
+            o.transport(run.dt)
+            o.compute_band()
+        return 0.5 * float(((o.get("VF")[sl] - target[sl]) ** 2).sum())
+
+    assert oracle_loss(amps) == pytest.approx(L, rel=1e-10)
+    h = 1e-6
+    fd = np.array([(oracle_loss(amps + h * e) - oracle_loss(amps - h * e)) / (2 * h) for e in np.eye(16)])
+    rel = np.abs(g - fd) / np.maximum(np.abs(fd), 1e-3 * np.abs(fd).max())
+    print(f"adjoint vs oracle FD: {int((rel <= 1e-6).sum())}/16 within 1e-6, max rel {rel.max():.2e}")
+    assert int((rel <= 1e-6).sum()) >= 12
+    assert np.linalg.norm(g - fd) <= 1e-3 * np.linalg.norm(fd)

@@ -398,3 +398,35 @@ def test_rising_bubble_steps(oracle):
     assert z[-1] - z[0] > 1e-5
     w = c.get("w")[g:-g, g:-g, g:-g]
     assert 0.0 < w.max() < 1.0
+
+
+# ---- the adjoint workload's velocity field ------------------------------------------------
+def test_streamfunction_faces_divergence_free(oracle):
+    """Corner-difference faces of the 16-mode stream function: zero discrete divergence and
+    zero wall flux, for any amplitudes."""
+    from ns_multiphase_amd import adjoint as A
+    from ns_multiphase_amd.params import create_mesh
+    p = A.adjoint_params(32)
+    m = create_mesh(p)
+    B = A.mode_faces(m)
+    uf, vf = A.faces_of(B, np.random.default_rng(1).normal(0, 0.1, 16))
+    g = m.nghost
+    div = (uf[1:, :-1] - uf[:-1, :-1]) / m.dx + (vf[:-1, 1:] - vf[:-1, :-1]) / m.dy
+    assert np.abs(div).max() <= 1e-12
+    assert np.abs(uf[g]).max() == 0.0 and np.abs(uf[g + m.Nx]).max() <= 1e-13  # sin(k pi) rounds
+    assert np.abs(vf[:, g]).max() == 0.0 and np.abs(vf[:, g + m.Ny]).max() <= 1e-13
+
+
+def test_fold_vof_ghosts_is_the_adjoint_of_the_fill(oracle):
+    """<fold(lam), x> = <lam, fill(x)> for the Neumann + periodic fill of VF (the oracle's)."""
+    from ns_multiphase_amd import adjoint as A
+    p = A.adjoint_params(12)
+    c = oracle.OracleContext(p)
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(c.get("VF").shape)
+    c.set("VF", x)
+    c.neumann("VF")
+    c.update_borders("VF")
+    fx = c.get("VF")
+    lam = rng.standard_normal(x.shape)
+    assert float((A.fold_vof_ghosts(c.mesh, p, lam) * x).sum()) == pytest.approx(float((lam * fx).sum()), rel=1e-12)
