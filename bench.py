@@ -7,6 +7,9 @@ transport! (ELVIRA normals, back-trace, tet cutting, VF update, PLIC refresh) an
 computeBand!; value = 128^3 x steps / wall time, inputs already on the device.
 `rising_bubble`: full Navier-Stokes steps of the 128^3 rising bubble (50 steps) with both
 multigrid precisions, and the oracle's step time on the host cores beside them.
+`adjoint`: the loss gradient through 64 checkpointed SL-VOF steps at 256^2 (16 stream-function
+amplitudes) against the forward's cost and central differences. `large`: the 256^3 pressure
+MMS in both precisions and 20 deformation steps at 256^3.
 Beside it (`mg`): pressure multigrid solve time to max|b - A p| <= 1e-10 on the 128^3
 variable-density problem (density ratio 1000, exact p = cos cos cos), FP32 V-cycles in FP64
 defect correction vs FP64 V-cycles.
@@ -260,6 +263,99 @@ def cpu_bubble(args, param, ic, nsamples=2):
                       f"FP64 multigrid to the same 1e-10 residual), from the device state at steps {at}"}
 
 
+def run_adjoint(args, torch):
+    """Gradient of the SL-VOF loss with respect to 16 stream-function amplitudes: 256^2, 64
+    steps, checkpoints every 8; timed against the plain forward loss, and checked per
+    amplitude against central differences (h = 1e-6) of the forward loss."""
+    import numpy as np
+    from ns_multiphase_amd import adjoint as A
+    from ns_multiphase_amd import solver
+    N, nsteps, every = args.adj_N, 64, 8
+    p = A.adjoint_params(N)
+    ctx = solver.Context(p)
+    ctx.init_vof("circle", 0.2, 0.5, 0.5)
+    vf0 = ctx.get("VF")
+    basis = A.mode_faces(ctx.mesh)
+    a1 = np.random.default_rng(1).normal(0.0, 0.1, 16)
+    a2 = np.random.default_rng(2).normal(0.0, 0.1, 16)
+    dt = min(A.stable_dt(ctx.mesh, basis, a1), A.stable_dt(ctx.mesh, basis, a2))
+    run = A.SLVOFRun(ctx, vf0, basis, dt, nsteps, every)
+    target = run.forward(a2, checkpoints=False)
+    run.loss_and_grad(a1, target)  # warm-up
+    reps = 3
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        L = run.loss(a1, target)
+    torch.cuda.synchronize()
+    t_fwd = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        L2, g = run.loss_and_grad(a1, target)
+    torch.cuda.synchronize()
+    t_grad = (time.perf_counter() - t0) / reps
+    h = 1e-6
+    fd = np.array([(run.loss(a1 + h * e, target) - run.loss(a1 - h * e, target)) / (2 * h) for e in np.eye(16)])
+    rel = np.abs(g - fd) / np.maximum(np.abs(fd), 1e-3 * np.abs(fd).max())
+    ctx.close()
+    out = {"workload": f"{N}^2 x 1, circle r=0.2 at (0.5,0.5), 16-mode stream function (seed 1 control, seed 2 "
+                       f"target), {nsteps} steps, dt={dt:.3e}, checkpoints every {every}",
+           "loss": L, "forward_ms": 1e3 * t_fwd, "loss_and_gradient_ms": 1e3 * t_grad,
+           "gradient_cost_vs_forward": t_grad / t_fwd,
+           "fd_check": {"h": h, "within_1e-6": int((rel <= 1e-6).sum()), "directions": 16,
+                        "max_rel": float(rel.max()), "median_rel": float(np.median(rel))}}
+    log(f"[bench] adjoint: forward {out['forward_ms']:.1f} ms, loss+gradient {out['loss_and_gradient_ms']:.1f} ms "
+        f"({out['gradient_cost_vs_forward']:.2f}x), FD {out['fd_check']['within_1e-6']}/16 within 1e-6")
+    return out
+
+
+def run_large(args, torch):
+    """256^3: the variable-density pressure MMS (mixed vs FP64 to 1e-10, L-inf error against
+    the exact pressure), then 20 SL-VOF steps of the 3D deformation (dt = 1/1024)."""
+    import numpy as np
+    from ns_multiphase_amd import solver
+    from ns_multiphase_amd import workloads as W
+    N = args.large_N
+    out = {"workload": f"{N}^3 MMS (rho 1000/1, tol 1e-10) + 20 steps of 3D deformation {N}^3"}
+    dt = 1e-3
+    p = W.mms_params(N)
+    for prec in ("fp64", "mixed"):
+        ctx = solver.Context(p)
+        W.mms_setup(ctx, dt)
+        P0 = ctx.get("P")
+        ctx.pressure_solve(dt, 1e-10, prec)  # warm: hierarchy and graphs
+        ctx.set("P", P0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        cyc, res = ctx.pressure_solve(dt, 1e-10, prec)
+        torch.cuda.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0)
+        Pn = W.interior(ctx.mesh, ctx.get("P"))
+        err = float(np.abs(Pn - Pn.mean() - W.mms_exact(ctx.mesh)).max())
+        out[f"mg_{prec}"] = {"solve_ms": ms, "cycles": cyc, "final_resid": res, "linf_err_vs_exact": err}
+        ctx.close()
+        log(f"[bench] {N}^3 MMS {prec}: {ms:.1f} ms, {cyc} cycles, L-inf {err:.2e}")
+    out["mg_speedup_mixed_vs_fp64"] = out["mg_fp64"]["solve_ms"] / out["mg_mixed"]["solve_ms"]
+    param, ic = W.deformation3d(N)
+    ctx = solver.Context(param)
+    ctx.init_vof(*ic)
+    ctx.vof_init()
+    t = 0.0
+    for _ in range(2):
+        t, _ = ctx.vof_step(t, param.tFinal)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(20):
+        t, _ = ctx.vof_step(t, param.tFinal)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    out["deformation_20_steps"] = {"ms_per_step": 1e3 * el / 20, "cell_updates_per_s": N ** 3 * 20 / el,
+                                   "dt": param.max_dt}
+    ctx.close()
+    log(f"[bench] {N}^3 deformation: {1e3 * el / 20:.2f} ms/step")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -273,6 +369,10 @@ def main():
     ap.add_argument("--bubble-N", type=int, default=128)
     ap.add_argument("--bubble-steps", type=int, default=50)
     ap.add_argument("--no-bubble", action="store_true")
+    ap.add_argument("--adj-N", type=int, default=256)
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--large-N", type=int, default=256)
+    ap.add_argument("--no-large", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the hot path runs on a "
@@ -344,6 +444,15 @@ def main():
                               f"mu 1e-3/1e-5, sigma 0.07, g=-9.81 z, no-slip walls, dt=1e-3, CFL 0.5, "
                               f"finite-difference scheme with correction tets, MG tol 1e-10")
 
+    adjoint = None
+    if not args.no_adjoint:
+        log(f"[bench] adjoint {args.adj_N}^2")
+        adjoint = run_adjoint(args, torch)
+    large = None
+    if not args.no_large:
+        log(f"[bench] large {args.large_N}^3")
+        large = run_large(args, torch)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (oracle)")
@@ -358,7 +467,8 @@ def main():
         "config": {"workload": f"3D deformation {args.N}^3, sphere r=0.15, T=3, CFL 0.5, dt=1/512, RK4, 5 tets",
                    "steps_timed_from_t0": steps, "volume_rel_change": (V1 - V0) / V0,
                    "mg_workload": f"variable-density MMS {args.mg_N}^3, rho 1000/1, tol 1e-10"},
-        "roofline": roof, "cpu_baseline": cpu, "mg": mg, "rising_bubble": bubble, "kernels": kernels,
+        "roofline": roof, "cpu_baseline": cpu, "mg": mg, "rising_bubble": bubble, "adjoint": adjoint,
+        "large": large, "kernels": kernels,
         "rooflines": roofs,
     }
     print(json.dumps(line), flush=True)

@@ -116,10 +116,10 @@ struct AdjFrame {
 // adjoints to their edge ends, then its 4 vertices' adjoints to the parent. groot receives
 // the adjoints of the tet's vertices. Returns false when the walk overflows.
 __device__ bool cut_tet_adj(const DevMesh& m, const double* nx, const double* ny, const double* nz, const double* D,
-                            double tet[4][3], int ind[4][3], AdjFrame* st, double gL, double gV, double groot[4][3],
-                            double* const gpl[4]) {
+                            double tet[4][3], int ind[4][3], unsigned flags0, AdjFrame* st, double gL, double gV,
+                            double groot[4][3], double* const gpl[4]) {
     int top = -1;
-    unsigned flags = 0;
+    unsigned flags = flags0;
     bool have_node = true;
     int ovf = 0;
     for (;;) {
@@ -243,70 +243,182 @@ __device__ bool cut_tet_adj(const DevMesh& m, const double* nx, const double* ny
     }
 }
 
-// Band cells: one lane per (cell, tet), 12 cells per block. The tets are cut forward for the
-// cell's vLiq and vol, then in reverse with the seeds of VF' = vLiq / vol.
-__global__ __launch_bounds__(64) void k_adj_cut(DevMesh m, const double* pv, const int* list, const int* count,
-                                                const double* nx, const double* ny, const double* nz,
-                                                const double* D, const double* lam, double* gpv, double* gnx,
-                                                double* gny, double* gnz, double* gD, int* ovf) {
+// Cells left to the general walk by the tree pass: their five tets' vol and vLiq into sums.
+__global__ __launch_bounds__(64) void k_adj_gen_sums(DevMesh m, const double* pv, const int* list, const int* count,
+                                                     const double* nx, const double* ny, const double* nz,
+                                                     const double* D, double* sums, int* ovf) {
     __shared__ double s_vol[64], s_liq[64];
     const int lane = threadIdx.x;
     const int cl = lane / 5, t = lane - 5 * cl;
     const int nb = *count;
-    double* const gpl[4] = {gnx, gny, gnz, gD};
     const LeafMomentum none{nullptr, nullptr, nullptr, 0.0, 0.0};
     for (int base = blockIdx.x * 12; base < nb; base += gridDim.x * 12) {
         const int cidx = base + cl;
         const bool active = lane < 60 && cidx < nb;
-        int i = 1, j = 1, k = 1;
-        double tet[4][3];
-        int ind[4][3];
-        int64_t vq[4] = {0, 0, 0, 0};
-        double vol = 0.0, liq = 0.0;
-        int o = 0;
+        double r[2] = {0.0, 0.0};
+        int q = 0;
         if (active) {
-            interior_of(m, list[cidx], i, j, k);
+            q = list[cidx];
+            int i, j, k;
+            interior_of(m, q, i, j, k);
             const bool even = ((i + j + k) % 2) == 0;
             const unsigned tv = TET5[t];
+            double tet[4][3];
+            int ind[4][3], o = 0;
             for (int n = 0; n < 4; ++n) {
                 const int c = vert_corner(nib(tv, n), even);
-                vq[n] = vtx_id(m, i + (c & 1), j + ((c >> 1) & 1), k + (c >> 2));
-                tet[n][0] = pv[3 * vq[n] + 0]; tet[n][1] = pv[3 * vq[n] + 1]; tet[n][2] = pv[3 * vq[n] + 2];
-            }
-            double t2[4][3];
-            int i2[4][3];
-            for (int n = 0; n < 4; ++n) {
+                const int64_t vq = vtx_id(m, i + (c & 1), j + ((c >> 1) & 1), k + (c >> 2));
+                tet[n][0] = pv[3 * vq + 0]; tet[n][1] = pv[3 * vq + 1]; tet[n][2] = pv[3 * vq + 2];
                 pt2index(m, ind[n], tet[n], i, j, k, &o);
-                for (int q = 0; q < 3; ++q) { t2[n][q] = tet[n][q]; i2[n][q] = ind[n][q]; }
             }
             CutFrame<2> st[CUT_MAXLVL];
             int depth = 0;
-            double r[2];
-            if (!cut_tet<2>(m, nx, ny, nz, D, none, t2, i2, st, r, depth)) o = 1;
-            vol = r[0];
-            liq = r[1];
-        }
-        s_vol[lane] = vol;
-        s_liq[lane] = liq;
-        __syncthreads();
-        if (active) {
-            const int f = lane - t;  // first lane of the cell
-            double V = 0.0, L = 0.0;
-            for (int q = 0; q < 5; ++q) { V += s_vol[f + q]; L += s_liq[f + q]; }
-            const double lv = lam[m.vof(i, j, k)];
-            if (lv != 0.0) {
-                // VF' = L / V (the tetsign of the cell multiplies L and V alike)
-                const double gL = lv / V, gV = -lv * L / (V * V);
-                AdjFrame st[CUT_MAXLVL];
-                double groot[4][3];
-                if (!cut_tet_adj(m, nx, ny, nz, D, tet, ind, st, gL, gV, groot, gpl)) o = 1;
-                for (int n = 0; n < 4; ++n)
-                    for (int q = 0; q < 3; ++q)
-                        if (groot[n][q] != 0.0) atomicAdd(gpv + 3 * vq[n] + q, groot[n][q]);
-            }
+            if (!cut_tet<2>(m, nx, ny, nz, D, none, tet, ind, st, r, depth)) o = 1;
             if (o) atomicOr(ovf, 1);
         }
+        s_vol[lane] = r[0];
+        s_liq[lane] = r[1];
         __syncthreads();
+        if (active && t == 0) {
+            double V = 0.0, L = 0.0;
+            for (int n = 0; n < 5; ++n) { V += s_vol[lane + n]; L += s_liq[lane + n]; }
+            sums[6 * (int64_t)q + 0] = V;
+            sums[6 * (int64_t)q + 1] = L;
+        }
+        __syncthreads();
+    }
+}
+
+// Band cells in reverse: one lane per (cell, tet, child of the tet's first cut), 30 lanes per
+// cell, two cells per block. Each lane replays the first cut of its tet, walks its child's
+// subtree in reverse with the seeds of VF' = vLiq / vol (vol and vLiq of the cell from the
+// forward sums), takes the child's vertex adjoints back through the first cut to the tet's
+// vertices and adds them to the cell's back-traced corners. A tet without cuts is a leaf,
+// reversed by its first lane alone.
+__global__ __launch_bounds__(64) void k_adj_cut(DevMesh m, const double* pv, const int* list, const int* count,
+                                                const double* nx, const double* ny, const double* nz,
+                                                const double* D, const double* lam, const double* sums,
+                                                double* gpv, double* gnx, double* gny, double* gnz, double* gD,
+                                                int* ovf) {
+    const int lane = threadIdx.x;
+    const int cl = lane / 30, r = lane - 30 * cl, t = r / 6, ch = r - 6 * t;
+    const int nb = *count;
+    double* const gpl[4] = {gnx, gny, gnz, gD};
+    for (int base = blockIdx.x * 2; base < nb; base += gridDim.x * 2) {
+        const int cidx = base + cl;
+        if (cl >= 2 || cidx >= nb) continue;
+        const int q = list[cidx];
+        int i, j, k;
+        interior_of(m, q, i, j, k);
+        const double lv = lam[m.vof(i, j, k)];
+        if (lv == 0.0) continue;
+        const double V = sums[6 * (int64_t)q + 0], L = sums[6 * (int64_t)q + 1];
+        const double gL = lv / V, gV = -lv * L / (V * V);  // VF' = L / V
+        const bool even = ((i + j + k) % 2) == 0;
+        const unsigned tv = TET5[t];
+        double tet[4][3];
+        int ind[4][3], o = 0;
+        int64_t vq[4];
+        for (int n = 0; n < 4; ++n) {
+            const int c = vert_corner(nib(tv, n), even);
+            vq[n] = vtx_id(m, i + (c & 1), j + ((c >> 1) & 1), k + (c >> 2));
+            tet[n][0] = pv[3 * vq[n] + 0]; tet[n][1] = pv[3 * vq[n] + 1]; tet[n][2] = pv[3 * vq[n] + 2];
+            pt2index(m, ind[n], tet[n], i, j, k, &o);
+        }
+        // the first node of the walk: which plane cuts the tet (src/VFgeom.jl:738-756)
+        unsigned flags = 0;
+        int dir = -1, cut_ind = 0;
+        for (int a = 0; a < 3; ++a) {
+            const int mx = max(max(ind[0][a], ind[1][a]), max(ind[2][a], ind[3][a]));
+            const int mn = min(min(ind[0][a], ind[1][a]), min(ind[2][a], ind[3][a]));
+            if (mx > mn) { dir = a; cut_ind = mx; break; }
+            flags |= 1u << a;
+        }
+        double g4[4][3] = {};
+        if (dir < 0) {
+            if (ch != 0) continue;
+            const int64_t c = m.cell(ind[0][0], ind[0][1], ind[0][2]);
+            double gn[3] = {0.0, 0.0, 0.0}, gd = 0.0;
+            leaf_adj(tet, nx[c], ny[c], nz[c], D[c], gL, gV, g4, gn, gd);
+            atomicAdd(gpl[0] + c, gn[0]);
+            atomicAdd(gpl[1] + c, gn[1]);
+            atomicAdd(gpl[2] + c, gn[2]);
+            atomicAdd(gpl[3] + c, gd);
+        } else {
+            const double* cc = dir == 0 ? m.x : (dir == 1 ? m.y : m.z);
+            const double plane = cc[cut_ind];
+            double v[8][3], d[4], dd[4];
+            short vi[2][8][3];
+            for (int n = 0; n < 4; ++n) {
+                d[n] = tet[n][dir] - plane;
+                dd[n] = d[n];
+            }
+            const int cs = d2case_snap(dd);
+            if (ch >= CUT_NTETS[cs]) continue;
+            for (int n = 0; n < 4; ++n) {
+                for (int a = 0; a < 3; ++a) {
+                    v[n][a] = tet[n][a];
+                    vi[0][n][a] = (short)ind[n][a];
+                    vi[1][n][a] = (short)ind[n][a];
+                }
+                const int lo1 = min(ind[n][dir], cut_ind - 1);
+                vi[0][n][dir] = (short)lo1;
+                vi[1][n][dir] = (short)max(lo1, cut_ind);
+            }
+            const int nv = CUT_NVERT[cs];
+            for (int n = 0; n < nv; ++n) {
+                const unsigned e = CUT_EDGE[cs][n];
+                const int v1 = e & 15, v2 = e >> 4;
+                const double mu = jmin(1.0, jmax(0.0, sdiv(-d[v1], d[v2] - d[v1])));
+                double pt[3];
+                for (int a = 0; a < 3; ++a) {
+                    pt[a] = (1.0 - mu) * v[v1][a] + mu * v[v2][a];
+                    v[4 + n][a] = pt[a];
+                }
+                int I[3];
+                pt2index(m, I, pt, vi[0][n][0], vi[0][n][1], vi[0][n][2], &o);
+                for (int a = 0; a < 3; ++a) {
+                    int w = max(I[a], (int)min(vi[0][v1][a], vi[0][v2][a]));
+                    w = min(w, (int)max(vi[0][v1][a], vi[0][v2][a]));
+                    vi[0][4 + n][a] = (short)w;
+                    vi[1][4 + n][a] = (short)w;
+                }
+                vi[0][4 + n][dir] = (short)(cut_ind - 1);
+                vi[1][4 + n][dir] = (short)cut_ind;
+            }
+            // this lane's child: its subtree in reverse
+            const unsigned qv = CUT_VTET[cs][ch];
+            const int side = (CUT_SIDE2[cs] >> ch) & 1;
+            double ct[4][3];
+            int ci[4][3];
+            for (int nn = 0; nn < 4; ++nn) {
+                const int vv = nib(qv, nn);
+                for (int a = 0; a < 3; ++a) {
+                    ct[nn][a] = v[vv][a];
+                    ci[nn][a] = vi[side][vv][a];
+                }
+            }
+            AdjFrame st[CUT_MAXLVL];
+            double gc[4][3];
+            if (!cut_tet_adj(m, nx, ny, nz, D, ct, ci, flags, st, gL, gV, gc, gpl)) o = 1;
+            double g8[8][3] = {};
+            for (int nn = 0; nn < 4; ++nn)
+                for (int a = 0; a < 3; ++a) g8[nib(qv, nn)][a] += gc[nn][a];
+            for (int e = nv - 1; e >= 0; --e) {
+                const unsigned ed = CUT_EDGE[cs][e];
+                const int a = ed & 15, b = ed >> 4;
+                double gda, gdb;
+                lerp_adj(v[a], v[b], d[a], d[b], g8[4 + e], g8[a], g8[b], gda, gdb);
+                g8[a][dir] += gda;
+                g8[b][dir] += gdb;
+            }
+            for (int n = 0; n < 4; ++n)
+                for (int a = 0; a < 3; ++a) g4[n][a] = g8[n][a];
+        }
+        for (int n = 0; n < 4; ++n)
+            for (int a = 0; a < 3; ++a)
+                if (g4[n][a] != 0.0) atomicAdd(gpv + 3 * vq[n] + a, g4[n][a]);
+        if (o) atomicOr(ovf, 1);
     }
 }
 
@@ -443,9 +555,13 @@ int launch_transport_vjp(mist_ctx* ctx, double dt) {
     const double* ny = (const double*)ctx->f[MIST_FIELD_NY];
     const double* nz = (const double*)ctx->f[MIST_FIELD_NZ];
     const double* D = (const double*)ctx->f[MIST_FIELD_PLIC_D];
-    ADJL(k_adj_cut, adj_grid(ncap, 12, 8192), 64, m, (const double*)ctx->pv, (const int*)ctx->band_list,
-         (const int*)ctx->counters, nx, ny, nz, D, (const double*)lam, ctx->adj_pv, gpl[0], gpl[1], gpl[2], gpl[3],
-         ctx->counters + 2);
+    // the cells' vol and vLiq: the tree pass, the general walk for the cells it leaves
+    if ((st = launch_tree_momentum(ctx, ctx->adj_sums))) return st;
+    ADJL(k_adj_gen_sums, 32, 64, m, (const double*)ctx->pv, (const int*)ctx->defer_list,
+         (const int*)(ctx->counters + 6), nx, ny, nz, D, ctx->adj_sums, ctx->counters + 2);
+    ADJL(k_adj_cut, adj_grid(ncap, 2, 16384), 64, m, (const double*)ctx->pv, (const int*)ctx->band_list,
+         (const int*)ctx->counters, nx, ny, nz, D, (const double*)lam, (const double*)ctx->adj_sums, ctx->adj_pv,
+         gpl[0], gpl[1], gpl[2], gpl[3], ctx->counters + 2);
     // 3. the back-trace in reverse, into the face-velocity adjoints
     ADJL(k_adj_backtrace, adj_grid(nv, 128), 128, m, ctx->dp, (const int16_t*)ctx->f[MIST_FIELD_BAND],
          (const double*)ctx->f[MIST_FIELD_UF], (const double*)ctx->f[MIST_FIELD_VF],

@@ -147,7 +147,8 @@ int mist_ctx_destroy(mist_ctx* ctx) {
     for (int f = 0; f < MIST_FIELD_COUNT; ++f)
         if (ctx->f[f]) (void)hipFree(ctx->f[f]);
     void* bufs[] = {ctx->geom, ctx->trig, ctx->pv, ctx->band_list, ctx->iface_list, ctx->defer_list, ctx->cut_list,
-                    ctx->counters, ctx->red, ctx->ns_sums, ctx->elv_best, ctx->adj_lam, ctx->adj_plane, ctx->adj_pv};
+                    ctx->counters, ctx->red, ctx->ns_sums, ctx->elv_best, ctx->adj_lam, ctx->adj_plane, ctx->adj_pv,
+                    ctx->adj_sums};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (double* b : ctx->ckpt)
@@ -292,6 +293,8 @@ int mist_field_device_ptr(mist_ctx* ctx, int field, void** dptr) {
     // the caller may write through the pointer: cached derived state is no longer trusted
     if (field == MIST_FIELD_VOF) ctx->vf_version++;
     if (field == MIST_FIELD_NX || field == MIST_FIELD_NY || field == MIST_FIELD_NZ) ctx->normals_version = 0;
+    if (field == MIST_FIELD_NX || field == MIST_FIELD_NY || field == MIST_FIELD_NZ || field == MIST_FIELD_PLIC_D)
+        ctx->plic_version = 0;
     if (field == MIST_FIELD_BAND) ctx->band_list_valid = false;
     return MIST_OK;
 }
@@ -306,6 +309,8 @@ int mist_upload(mist_ctx* ctx, int field, const void* host, size_t nbytes) {
     MIST_HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (field == MIST_FIELD_VOF) ctx->vf_version++;
     if (field == MIST_FIELD_NX || field == MIST_FIELD_NY || field == MIST_FIELD_NZ) ctx->normals_version = 0;
+    if (field == MIST_FIELD_NX || field == MIST_FIELD_NY || field == MIST_FIELD_NZ || field == MIST_FIELD_PLIC_D)
+        ctx->plic_version = 0;
     if (field == MIST_FIELD_BAND) ctx->band_list_valid = false;
     return MIST_OK;
 }
@@ -415,6 +420,8 @@ int mist_compute_plic(mist_ctx* ctx) {
     CHECK_CTX();
     StageTimer tm(ctx, MIST_STAGE_PLIC);
     RET_IF(mist::launch_plic(ctx));
+    // D is this VF's when the normals are (they are recomputed whenever VF changes)
+    ctx->plic_version = ctx->normals_version == ctx->vf_version ? ctx->vf_version : 0;
     return MIST_OK;
 }
 
@@ -733,15 +740,17 @@ int mist_transport_vjp(mist_ctx* ctx, double dt) {
         const size_t nvof = (size_t)m.vnx * m.vny * m.vnz;
         const size_t ncell_ext = (size_t)m.cnx * m.cny * m.cnz;
         const size_t nv = (size_t)(m.Nx + 1) * (m.Ny + 1) * (m.Nz + 1);
+        const size_t ncell = (size_t)m.Nx * m.Ny * m.Nz;
         if (hipMalloc(&ctx->adj_lam, nvof * sizeof(double)) != hipSuccess ||
             hipMalloc(&ctx->adj_plane, 4 * ncell_ext * sizeof(double)) != hipSuccess ||
-            hipMalloc(&ctx->adj_pv, 3 * nv * sizeof(double)) != hipSuccess)
+            hipMalloc(&ctx->adj_pv, 3 * nv * sizeof(double)) != hipSuccess ||
+            hipMalloc(&ctx->adj_sums, 6 * ncell * sizeof(double)) != hipSuccess)
             return fail(ctx, MIST_ERR_OOM, "adjoint work allocation");
     }
-    // the forward state of the step: normals (and the ELVIRA choice), PLIC constants, band
-    ctx->normals_version = 0;  // recompute: the kept ELVIRA candidates must be this VF's
+    // the forward state of the step: normals (and the ELVIRA choice kept with them), PLIC
+    // constants and band of this VF, recomputed when stale
     RET_IF(mist_compute_normal(ctx));
-    RET_IF(mist_compute_plic(ctx));
+    if (ctx->plic_version != ctx->vf_version) RET_IF(mist_compute_plic(ctx));
     if (!ctx->band_list_valid) RET_IF(mist_compute_band(ctx));
     RET_IF(mist::launch_transport_vjp(ctx, dt));
     RET_IF(check_overflow(ctx));

@@ -83,6 +83,8 @@ struct mist_ctx {
     double* adj_lam = nullptr;   // adjoint work: dL/dVF of the step input (VF extents)
     double* adj_plane = nullptr; // adjoint work: planes nx, ny, nz, D (4 cell-extent arrays)
     double* adj_pv = nullptr;    // adjoint work: back-traced corners (3 per grid vertex)
+    double* adj_sums = nullptr;  // adjoint work: 6 cut sums per interior cell (vol, vLiq used)
+    uint64_t plic_version = 0;   // VF version the PLIC constants were computed for
     std::vector<double*> ckpt;   // VF checkpoints (device)
     std::string err;
 };
