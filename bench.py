@@ -79,8 +79,9 @@ def counted_flops(N):
 # HBM bytes per unit (interface cell for k_elvira, band cell for the cut pass) from
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per dispatch on 20 steps of this workload, FETCH_SIZE
 # scaled by 2.0 and WRITE_SIZE by 1.0 (what the counters read for a plain 256 MiB device copy);
-# see profiles/pmc_traffic_deformation3d_128.md.
-TRAFFIC_PER_UNIT = {"k_elvira": 669.0, "k_sl_tree": 1006.0}
+# see profiles/pmc_traffic_deformation3d_128.md. k_elvira's figure is almost all WRITE_SIZE: its
+# spilled registers (scratch stores), not field traffic.
+TRAFFIC_PER_UNIT = {"k_elvira": 32992.0, "k_sl_tree": 859.0}
 
 
 def run_vof(args, torch):
@@ -373,7 +374,20 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--large-N", type=int, default=256)
     ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 256 MiB on the device 3 times (the plain copy that scales the PMC "
+                         "FETCH_SIZE / WRITE_SIZE counters to bytes) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = 64 * 1024 * 1024
+        a = torch.ones(n, device="cuda", dtype=torch.float32)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_per_copy": 4 * n, "copies": 3}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the hot path runs on a "
               "single MI355X; multi-GPU sharding is a design only)", file=sys.stderr)
