@@ -24,6 +24,7 @@
 #include "mist_adjoint.h"
 #include "mist_internal.h"
 #include "mist_kernels.h"
+#include "mist_tree.h"
 
 namespace mist {
 
@@ -422,6 +423,498 @@ __global__ __launch_bounds__(64) void k_adj_cut(DevMesh m, const double* pv, con
     }
 }
 
+// ---- reverse of the cut tree ----------------------------------------------------------
+// Gradient of tet_vol3 (the volume -det(p0-p3, p1-p3, p2-p3) / 6) with weight w, into g0..g3.
+__device__ __forceinline__ void tet_vol3_grad(const V3& p0, const V3& p1, const V3& p2, const V3& p3, double w,
+                                              V3& g0, V3& g1, V3& g2, V3& g3) {
+    const double a1 = p0.x - p3.x, a2 = p0.y - p3.y, a3 = p0.z - p3.z;
+    const double b1 = p1.x - p3.x, b2 = p1.y - p3.y, b3 = p1.z - p3.z;
+    const double c1 = p2.x - p3.x, c2 = p2.y - p3.y, c3 = p2.z - p3.z;
+    const double s = -w / 6.0;
+    const V3 ga{s * (b2 * c3 - c2 * b3), s * (c1 * b3 - b1 * c3), s * (b1 * c2 - c1 * b2)};
+    const V3 gb{s * (c2 * a3 - a2 * c3), s * (a1 * c3 - c1 * a3), s * (c1 * a2 - a1 * c2)};
+    const V3 gc{s * (a2 * b3 - b2 * a3), s * (b1 * a3 - a1 * b3), s * (a1 * b2 - b1 * a2)};
+    g0.x += ga.x; g0.y += ga.y; g0.z += ga.z;
+    g1.x += gb.x; g1.y += gb.y; g1.z += gb.z;
+    g2.x += gc.x; g2.y += gc.y; g2.z += gc.z;
+    g3.x -= (ga.x + gb.x) + gc.x; g3.y -= (ga.y + gb.y) + gc.y; g3.z -= (ga.z + gb.z) + gc.z;
+}
+
+// lerp_adj on V3: x = lerp3(a, b, mu(da, db)); adds gx's share to ga, gb, returns gda, gdb.
+__device__ __forceinline__ void lerp3_adj(const V3& a, const V3& b, double da, double db, const V3& gx, V3& ga, V3& gb,
+                                          double& gda, double& gdb) {
+    const double A[3] = {a.x, a.y, a.z}, B[3] = {b.x, b.y, b.z}, G[3] = {gx.x, gx.y, gx.z};
+    double sa[3] = {0.0, 0.0, 0.0}, sb[3] = {0.0, 0.0, 0.0};
+    lerp_adj(A, B, da, db, G, sa, sb, gda, gdb);
+    ga.x += sa[0]; ga.y += sa[1]; ga.z += sa[2];
+    gb.x += sb[0]; gb.y += sb[1]; gb.z += sb[2];
+}
+
+// Reverse of cut_case<CS, true> (the PLIC cut of a leaf): the liquid sub-tets carry gL + gV,
+// the gas ones gV; the cut vertices pass their adjoints to their edge ends and to the vertex
+// distances dm. Adds to gP and gd.
+template <int CS>
+__device__ __forceinline__ void cut_case_adj(const V3 (&P)[4], const double (&dm)[4], double gL, double gV,
+                                             V3 (&gP)[4], double (&gd)[4]) {
+    constexpr int NV = CutTab::nvert[CS], NT = CutTab::ntets[CS], NN = CutTab::nntet[CS];
+    V3 Q[8];
+    Q[0] = P[0]; Q[1] = P[1]; Q[2] = P[2]; Q[3] = P[3];
+    sfor<0, NV>([&](auto I) {
+        constexpr int n = decltype(I)::value;
+        constexpr int v1 = CutTab::edge[CS][n] & 15, v2 = CutTab::edge[CS][n] >> 4;
+        const double mu = jmin(1.0, jmax(0.0, sdiv(-dm[v1], dm[v2] - dm[v1])));
+        Q[4 + n] = lerp3(Q[v1], Q[v2], mu);
+    });
+    V3 g[8];
+    sfor<0, 8>([&](auto I) { g[decltype(I)::value] = V3{0.0, 0.0, 0.0}; });
+    sfor<0, NT - NN + 1>([&](auto I) {
+        constexpr int n = NT - decltype(I)::value;
+        constexpr int a = vtet_of(CS, n - 1, 0), b = vtet_of(CS, n - 1, 1), c = vtet_of(CS, n - 1, 2),
+                      d = vtet_of(CS, n - 1, 3);
+        tet_vol3_grad(Q[a], Q[b], Q[c], Q[d], gL + gV, g[a], g[b], g[c], g[d]);
+    });
+    sfor<1, NN>([&](auto I) {
+        constexpr int n = decltype(I)::value;
+        constexpr int a = vtet_of(CS, n - 1, 0), b = vtet_of(CS, n - 1, 1), c = vtet_of(CS, n - 1, 2),
+                      d = vtet_of(CS, n - 1, 3);
+        tet_vol3_grad(Q[a], Q[b], Q[c], Q[d], gV, g[a], g[b], g[c], g[d]);
+    });
+    sfor<0, NV>([&](auto I) {
+        constexpr int n = NV - 1 - decltype(I)::value;
+        constexpr int v1 = CutTab::edge[CS][n] & 15, v2 = CutTab::edge[CS][n] >> 4;
+        double ga, gb;
+        lerp3_adj(Q[v1], Q[v2], dm[v1], dm[v2], g[4 + n], g[v1], g[v2], ga, gb);
+        gd[v1] += ga;
+        gd[v2] += gb;
+    });
+    sfor<0, 4>([&](auto I) {
+        constexpr int v = decltype(I)::value;
+        gP[v].x += g[v].x; gP[v].y += g[v].y; gP[v].z += g[v].z;
+    });
+}
+
+__device__ __forceinline__ void cut_dispatch_adj(int cs, const V3 (&P)[4], const double (&dm)[4], double gL,
+                                                 double gV, V3 (&gP)[4], double (&gd)[4]) {
+    switch (cs) {
+#define MIST_CASE(k) \
+    case k: cut_case_adj<k>(P, dm, gL, gV, gP, gd); break;
+        MIST_CASE(0) MIST_CASE(1) MIST_CASE(2) MIST_CASE(3) MIST_CASE(4) MIST_CASE(5) MIST_CASE(6) MIST_CASE(7)
+        MIST_CASE(8) MIST_CASE(9) MIST_CASE(10) MIST_CASE(11) MIST_CASE(12) MIST_CASE(13) MIST_CASE(14) MIST_CASE(15)
+#undef MIST_CASE
+        default: break;
+    }
+}
+
+// A leaf of the tree of a cell: its four vertices, the cell it lies in (offset l in the index
+// box) and how it was made from its node (the node's last z cut, src/VFgeom.jl:853-862, or the
+// node itself). The same arithmetic as the leaf loop of k_sl_tree.
+struct TreeLeaf {
+    V3 Q[4];
+    TreeNode N;
+    int l;
+    bool zcut;
+    double P;       // the z-plane of the cut
+    int cs;         // its case
+    unsigned qv;    // the leaf's vertices among the node's 4 + cut vertices
+};
+
+template <class C>
+__device__ __forceinline__ void tree_leaf(const TreeShared<C>& S, int cur, int L, const int (&span)[3], int bx, int by,
+                                          TreeLeaf& T) {
+    const int e = S.lnode[L];
+    const int c = L - S.loff[e];
+    const TreeNode N = S.node[cur][e];
+    T.N = N;
+    const int o0 = boff(N.bits, 0, 2), o1 = boff(N.bits, 1, 2), o2 = boff(N.bits, 2, 2), o3 = boff(N.bits, 3, 2);
+    const int mx = max(max(o0, o1), max(o2, o3));
+    unsigned ob;
+    T.zcut = mx > min(min(o0, o1), min(o2, o3));
+    if (T.zcut) {
+        const double P = S.plane[2][mx];
+        double dd[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) dd[v] = S.pz[tn_id(N, v)] - P;
+        const int cs = d2case_snap(dd);
+        const unsigned qv = CUT_VTET[cs][c];
+        const bool s2 = (CUT_SIDE2[cs] >> c) & 1;
+        T.P = P;
+        T.cs = cs;
+        T.qv = qv;
+        int e1v = 0, e2v = 0;
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            const int lv = nib(qv, kk);
+            const bool cutv = lv >= 4;
+            const unsigned eg = CUT_EDGE[cs][cutv ? lv - 4 : 0];
+            const int v1 = cutv ? (int)(eg & 15) : lv, v2 = cutv ? (int)(eg >> 4) : lv;
+            const V3 A = pool_v(S, tn_id(N, v1)), B = pool_v(S, tn_id(N, v2));
+            const double d1 = A.z - P, d2 = B.z - P;
+            const double mu = cutv ? jmin(1.0, jmax(0.0, sdiv(-d1, d2 - d1))) : 0.0;
+            T.Q[kk] = lerp3(A, B, mu);
+            if (kk == 0) { e1v = v1; e2v = v2; }
+        }
+        const int lv0 = nib(qv, 0);
+        if (lv0 < 4) {
+            ob = (N.bits >> (6 * lv0)) & 63u;
+            const unsigned oa = ob >> 4;
+            ob = (ob & 15u) | ((s2 ? (unsigned)mx : min(oa, (unsigned)(mx - 1))) << 4);
+        } else {
+            ob = (unsigned)(s2 ? mx : mx - 1) << 4;
+#pragma unroll
+            for (int ax = 0; ax < 2; ++ax) {
+                const int ea = boff(N.bits, e1v, ax), eb = boff(N.bits, e2v, ax);
+                const unsigned oo = ea == eb ? (unsigned)ea
+                                             : walk_clamp(ax == 0 ? T.Q[0].x : T.Q[0].y, boff(N.bits, lv0 - 4, ax),
+                                                          ea, eb, S.plane[ax], span[ax]);
+                ob |= oo << (2 * ax);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) T.Q[v] = pool_v(S, tn_id(N, v));
+        ob = N.bits & 63u;
+        T.P = 0.0;
+        T.cs = 0;
+        T.qv = 0;
+    }
+    const unsigned ox = ob & 3u, oy = (ob >> 2) & 3u, oz = (ob >> 4) & 3u;
+    T.l = (int)ox + bx * ((int)oy + by * (int)oz);
+}
+
+template <class C>
+struct TreeAdjShared {
+    unsigned par[C::PV];   // how each pool vertex was made (par_pack)
+    double g[3][C::PV];    // adjoints of the pool vertices
+    double gnb[64][4];     // adjoints of the box cells' nx, ny, nz, D
+    int lvl[12];           // pool size at the start of each mesh-plane level, then at the end
+    double VL[2];          // the cell's vol and vLiq
+};
+
+template <class C>
+__device__ __forceinline__ void pool_add(TreeAdjShared<C>& A, unsigned id, const V3& g) {
+    if (g.x != 0.0) atomicAdd(&A.g[0][id], g.x);
+    if (g.y != 0.0) atomicAdd(&A.g[1][id], g.y);
+    if (g.z != 0.0) atomicAdd(&A.g[2][id], g.z);
+}
+
+// Band cells in reverse, one work-group per cell as in k_sl_tree: the tree of the cell is built
+// again (recording how each pool vertex was made), the leaves give vol and vLiq in the forward
+// order, then the leaves are reversed with the seeds of VF' = vLiq / vol (their PLIC cuts give
+// the box cells' plane adjoints, their last z cuts pass vertex adjoints to the node's
+// vertices), the mesh-plane levels are reversed from the last one down, and the adjoints of
+// the 8 back-traced corners and of the planes are added to gpv and gpl. Cells with a zero
+// adjoint are skipped; cells that outgrow the pools go to `defer` like the forward pass.
+template <class C>
+__global__ __launch_bounds__(C::NT) void k_adj_tree(DevMesh m, const double* pv, const CutCell* cut, const int* idx,
+                                                  const int* count, const double* nx, const double* ny,
+                                                  const double* nz, const double* D, const double* lam,
+                                                  double* gpv, double* gnx, double* gny, double* gnz, double* gD,
+                                                  int* defer, int* ndefer, int defer_cell, int* queue) {
+    constexpr int NT = C::NT;
+    __shared__ TreeShared<C> S;
+    __shared__ TreeAdjShared<C> A;
+    const int tid = threadIdx.x;
+    const int nb = *count;
+    double* const gpl[4] = {gnx, gny, gnz, gD};
+    if (tid == 0) S.next = atomicAdd(queue, 1);
+    __syncthreads();
+    int cidx = S.next;
+    while (cidx < nb) {
+        __syncthreads();
+        int claim = 0;
+        if (tid == 0) claim = atomicAdd(queue, 1);
+        const int entry = idx ? idx[cidx] : cidx;
+        const CutCell cc = cut[entry];
+        const int q = cc.q;
+        int i, j, k;
+        interior_of(m, q, i, j, k);
+        const double lv = lam[m.vof(i, j, k)];
+        if (lv != 0.0) {
+            const bool even = ((i + j + k) % 2) == 0;
+            const int lo[3] = {cc.lo0, cc.lo1, cc.lo2};
+            int span[3] = {0, 0, 0};
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+#pragma unroll
+                for (int a = 0; a < 3; ++a) span[a] = max(span[a], (int)((cc.offs >> (6 * c + 2 * a)) & 3ull));
+            const int bx = span[0] + 1, by = span[1] + 1, bz = span[2] + 1;
+            const int ncell = bx * by * bz;
+            if (tid < 8) {
+                const int64_t vq = vtx_id(m, i + (tid & 1), j + ((tid >> 1) & 1), k + (tid >> 2));
+                S.px[tid] = pv[3 * vq + 0];
+                S.py[tid] = pv[3 * vq + 1];
+                S.pz[tid] = pv[3 * vq + 2];
+            }
+            if (tid < 15) {
+                const int a = tid / 5, off = tid - 5 * a;
+                const int la = a == 0 ? lo[0] : (a == 1 ? lo[1] : lo[2]);
+                const int sa = a == 0 ? span[0] : (a == 1 ? span[1] : span[2]);
+                if (off <= sa + 1) S.plane[a][off] = (a == 0 ? m.x : (a == 1 ? m.y : m.z))[la + off];
+            }
+            for (int l = tid; l < ncell; l += NT) {
+                const int ox = l % bx, oy = (l / bx) % by, oz = l / (bx * by);
+                const int64_t c = m.cell(lo[0] + ox, lo[1] + oy, lo[2] + oz);
+                const double n0 = nx[c], n1 = ny[c], n2 = nz[c], Dc = D[c];
+                S.nb[l][0] = n0;
+                S.nb[l][1] = n1;
+                S.nb[l][2] = n2;
+                S.nb[l][3] = Dc;
+                S.cls[l] = (signed char)box_class(n0, n1, n2, Dc, m.x[lo[0] + ox], m.x[lo[0] + ox + 1],
+                                                  m.y[lo[1] + oy], m.y[lo[1] + oy + 1], m.z[lo[2] + oz],
+                                                  m.z[lo[2] + oz + 1]);
+                A.gnb[l][0] = 0.0;
+                A.gnb[l][1] = 0.0;
+                A.gnb[l][2] = 0.0;
+                A.gnb[l][3] = 0.0;
+            }
+            if (tid < 5) {
+                const unsigned tv = TET5[tid];
+                unsigned ids[4], bits = 0;
+#pragma unroll
+                for (int n = 0; n < 4; ++n) {
+                    const int c = vert_corner(nib(tv, n), even);
+                    ids[n] = (unsigned)c;
+                    bits |= (unsigned)((cc.offs >> (6 * c)) & 63ull) << (6 * n);
+                }
+                S.node[0][tid] = TreeNode{ids[0] | (ids[1] << 16), ids[2] | (ids[3] << 16), bits};
+            }
+            __syncthreads();
+            // ---- the tree again, recording each pool vertex's edge and plane ---------------
+            int n = 5, npool = 8, cur = 0, nl = 0;
+            bool ok = true;
+            for (int axis = 0; axis < 3 && ok; ++axis) {
+                const int minspan = axis == 2 ? 2 : 1;
+                for (int rep = 0; rep < 3 && ok; ++rep) {
+                    int any = 0;
+                    for (int e = tid; e < n; e += NT) {
+                        const unsigned bits = S.node[cur][e].bits;
+                        const int o0 = boff(bits, 0, axis), o1 = boff(bits, 1, axis), o2 = boff(bits, 2, axis),
+                                  o3 = boff(bits, 3, axis);
+                        any |= max(max(o0, o1), max(o2, o3)) - min(min(o0, o1), min(o2, o3)) >= minspan;
+                    }
+                    if (!__syncthreads_or(any)) break;
+                    if (tid == 0) A.lvl[nl] = npool;
+                    ++nl;
+                    ok = tree_level<C>(S, axis, minspan, cur, n, npool, Span3{{span[0], span[1], span[2]}}, A.par);
+                    cur ^= 1;
+                }
+            }
+            if (tid == 0) A.lvl[nl] = npool;
+            int nleaf = 0;
+            for (int b = 0; b < n && ok; b += NT) {
+                const int e = b + tid;
+                int cnt = 0;
+                if (e < n) {
+                    const TreeNode N = S.node[cur][e];
+                    const int o0 = boff(N.bits, 0, 2), o1 = boff(N.bits, 1, 2), o2 = boff(N.bits, 2, 2),
+                              o3 = boff(N.bits, 3, 2);
+                    const int mx = max(max(o0, o1), max(o2, o3));
+                    if (mx > min(min(o0, o1), min(o2, o3))) {
+                        const double P = S.plane[2][mx];
+                        double dd[4];
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) dd[v] = S.pz[tn_id(N, v)] - P;
+                        cnt = CUT_NTETS[d2case_snap(dd)];
+                    } else {
+                        cnt = 1;
+                    }
+                }
+                int tot;
+                const int o = block_scan<NT>(cnt, tot, S.scan);
+                if (nleaf + tot > C::NLEAF) {
+                    ok = false;
+                    break;
+                }
+                if (e < n) {
+                    S.loff[e] = (unsigned short)(nleaf + o);
+                    for (int c = 0; c < cnt; ++c) S.lnode[nleaf + o + c] = (unsigned short)e;
+                }
+                nleaf += tot;
+            }
+            if (!ok) {
+                if (tid == 0) defer[atomicAdd(ndefer, 1)] = defer_cell ? q : entry;
+            } else {
+                for (int v = tid; v < npool; v += NT) {
+                    A.g[0][v] = 0.0;
+                    A.g[1][v] = 0.0;
+                    A.g[2][v] = 0.0;
+                }
+                __syncthreads();
+                // ---- vol and vLiq of the cell, as the forward pass adds them ------------------
+                double Vs = 0.0, Ls = 0.0;
+                for (int L = tid; L < nleaf; L += NT) {
+                    TreeLeaf T;
+                    tree_leaf<C>(S, cur, L, span, bx, by, T);
+                    const int lcls = S.cls[T.l];
+                    auto on_liq = [&](double tv) { Vs += tv; Ls += tv; };
+                    auto on_gas = [&](double tv) { Vs += tv; };
+                    if (lcls != 0) {
+                        const double tv = tet_vol3(T.Q[0], T.Q[1], T.Q[2], T.Q[3]);
+                        if (lcls > 0) on_liq(tv);
+                        else on_gas(tv);
+                        continue;
+                    }
+                    const double n0 = S.nb[T.l][0], n1 = S.nb[T.l][1], n2 = S.nb[T.l][2], Dc = S.nb[T.l][3];
+                    double d[4], dd[4];
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        d[v] = ((n0 * T.Q[v].x + n1 * T.Q[v].y) + n2 * T.Q[v].z) - Dc;
+                        dd[v] = d[v];
+                    }
+                    const int cs = d2case_snap(dd);
+                    if (cs == 0 || cs == 15) {
+                        const double tv = tet_vol3(T.Q[0], T.Q[1], T.Q[2], T.Q[3]);
+                        if (cs == 0) on_liq(tv);
+                        else on_gas(tv);
+                    } else {
+                        cut_dispatch<true>(cs, T.Q, d, on_liq, on_gas);
+                    }
+                }
+#pragma unroll
+                for (int off = 32; off >= 1; off >>= 1) {
+                    Vs += __shfl_xor(Vs, off);
+                    Ls += __shfl_xor(Ls, off);
+                }
+                if constexpr (NT > 64) {
+                    if ((tid & 63) == 0) {
+                        S.red[0][tid >> 6] = Vs;
+                        S.red[1][tid >> 6] = Ls;
+                    }
+                    __syncthreads();
+                    if (tid == 0) {
+                        Vs = S.red[0][0];
+                        Ls = S.red[1][0];
+                        for (int w = 1; w < NT / 64; ++w) {
+                            Vs += S.red[0][w];
+                            Ls += S.red[1][w];
+                        }
+                    }
+                }
+                if (tid == 0) {
+                    A.VL[0] = Vs;
+                    A.VL[1] = Ls;
+                }
+                __syncthreads();
+                const double V = A.VL[0], Lq = A.VL[1];
+                const double gL = lv / V, gV = -lv * Lq / (V * V);  // VF' = vLiq / vol
+                // ---- leaves in reverse ---------------------------------------------------------
+                for (int L = tid; L < nleaf; L += NT) {
+                    TreeLeaf T;
+                    tree_leaf<C>(S, cur, L, span, bx, by, T);
+                    const int lcls = S.cls[T.l];
+                    V3 gQ[4] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+                    if (lcls != 0) {
+                        tet_vol3_grad(T.Q[0], T.Q[1], T.Q[2], T.Q[3], lcls > 0 ? gL + gV : gV, gQ[0], gQ[1], gQ[2],
+                                      gQ[3]);
+                    } else {
+                        const double n0 = S.nb[T.l][0], n1 = S.nb[T.l][1], n2 = S.nb[T.l][2], Dc = S.nb[T.l][3];
+                        double d[4], dd[4];
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            d[v] = ((n0 * T.Q[v].x + n1 * T.Q[v].y) + n2 * T.Q[v].z) - Dc;
+                            dd[v] = d[v];
+                        }
+                        const int cs = d2case_snap(dd);
+                        double gd[4] = {0.0, 0.0, 0.0, 0.0};
+                        if (cs == 0 || cs == 15) {
+                            tet_vol3_grad(T.Q[0], T.Q[1], T.Q[2], T.Q[3], cs == 0 ? gL + gV : gV, gQ[0], gQ[1], gQ[2],
+                                          gQ[3]);
+                        } else {
+                            cut_dispatch_adj(cs, T.Q, d, gL, gV, gQ, gd);
+                        }
+                        // d_v = n . Q_v - D
+                        double gn0 = 0.0, gn1 = 0.0, gn2 = 0.0;
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            gQ[v].x += gd[v] * n0;
+                            gQ[v].y += gd[v] * n1;
+                            gQ[v].z += gd[v] * n2;
+                            gn0 += gd[v] * T.Q[v].x;
+                            gn1 += gd[v] * T.Q[v].y;
+                            gn2 += gd[v] * T.Q[v].z;
+                        }
+                        const double gDc = -(((gd[0] + gd[1]) + gd[2]) + gd[3]);
+                        if (gn0 != 0.0) atomicAdd(&A.gnb[T.l][0], gn0);
+                        if (gn1 != 0.0) atomicAdd(&A.gnb[T.l][1], gn1);
+                        if (gn2 != 0.0) atomicAdd(&A.gnb[T.l][2], gn2);
+                        if (gDc != 0.0) atomicAdd(&A.gnb[T.l][3], gDc);
+                    }
+                    // back through the node's last z cut, or straight to the node's vertices
+                    if (T.zcut) {
+#pragma unroll
+                        for (int kk = 0; kk < 4; ++kk) {
+                            const int lvx = nib(T.qv, kk);
+                            if (lvx < 4) {
+                                pool_add(A, tn_id(T.N, lvx), gQ[kk]);
+                            } else {
+                                const unsigned eg = CUT_EDGE[T.cs][lvx - 4];
+                                const unsigned ia = tn_id(T.N, eg & 15), ib = tn_id(T.N, eg >> 4);
+                                const V3 pa = pool_v(S, ia), pb = pool_v(S, ib);
+                                V3 ga{0.0, 0.0, 0.0}, gb{0.0, 0.0, 0.0};
+                                double gda, gdb;
+                                lerp3_adj(pa, pb, pa.z - T.P, pb.z - T.P, gQ[kk], ga, gb, gda, gdb);
+                                ga.z += gda;
+                                gb.z += gdb;
+                                pool_add(A, ia, ga);
+                                pool_add(A, ib, gb);
+                            }
+                        }
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) pool_add(A, tn_id(T.N, v), gQ[v]);
+                    }
+                }
+                __syncthreads();
+                // ---- mesh-plane levels in reverse ----------------------------------------------
+                for (int lev = nl - 1; lev >= 0; --lev) {
+                    const int v0 = A.lvl[lev], v1 = A.lvl[lev + 1];
+                    for (int v = v0 + tid; v < v1; v += NT) {
+                        const V3 g{A.g[0][v], A.g[1][v], A.g[2][v]};
+                        if (g.x == 0.0 && g.y == 0.0 && g.z == 0.0) continue;
+                        const unsigned pk = A.par[v];
+                        const unsigned ia = pk & 2047u, ib = (pk >> 11) & 2047u;
+                        const int axis = (int)((pk >> 22) & 3u), mx = (int)(pk >> 24);
+                        const double P = S.plane[axis][mx];
+                        const V3 pa = pool_v(S, ia), pb = pool_v(S, ib);
+                        const double da = (axis == 0 ? pa.x : (axis == 1 ? pa.y : pa.z)) - P;
+                        const double db = (axis == 0 ? pb.x : (axis == 1 ? pb.y : pb.z)) - P;
+                        V3 ga{0.0, 0.0, 0.0}, gb{0.0, 0.0, 0.0};
+                        double gda, gdb;
+                        lerp3_adj(pa, pb, da, db, g, ga, gb, gda, gdb);
+                        if (axis == 0) { ga.x += gda; gb.x += gdb; }
+                        else if (axis == 1) { ga.y += gda; gb.y += gdb; }
+                        else { ga.z += gda; gb.z += gdb; }
+                        pool_add(A, ia, ga);
+                        pool_add(A, ib, gb);
+                    }
+                    __syncthreads();
+                }
+                // ---- the back-traced corners and the box cells' planes --------------------------
+                if (tid < 24) {
+                    const int c = tid / 3, a = tid - 3 * c;
+                    const double gg = A.g[a][c];
+                    if (gg != 0.0) {
+                        const int64_t vq = vtx_id(m, i + (c & 1), j + ((c >> 1) & 1), k + (c >> 2));
+                        atomicAdd(gpv + 3 * vq + a, gg);
+                    }
+                }
+                for (int l = tid; l < ncell; l += NT) {
+                    const int ox = l % bx, oy = (l / bx) % by, oz = l / (bx * by);
+                    const int64_t c = m.cell(lo[0] + ox, lo[1] + oy, lo[2] + oz);
+#pragma unroll
+                    for (int a = 0; a < 4; ++a) {
+                        const double gg = A.gnb[l][a];
+                        if (gg != 0.0) atomicAdd(gpl[a] + c, gg);
+                    }
+                }
+            }
+        }
+        if (tid == 0) S.next = claim;
+        __syncthreads();
+        cidx = S.next;
+    }
+}
+
 // Corners of band cells: the adjoint of the back-traced position through project!
 __global__ void k_adj_backtrace(DevMesh m, DevParams p, const int16_t* band, const double* uf, const double* vf,
                                 const double* wf, double dt, const double* gpv, double* guf, double* gvf,
@@ -555,13 +1048,29 @@ int launch_transport_vjp(mist_ctx* ctx, double dt) {
     const double* ny = (const double*)ctx->f[MIST_FIELD_NY];
     const double* nz = (const double*)ctx->f[MIST_FIELD_NZ];
     const double* D = (const double*)ctx->f[MIST_FIELD_PLIC_D];
-    // the cells' vol and vLiq: the tree pass, the general walk for the cells it leaves
-    if ((st = launch_tree_momentum(ctx, ctx->adj_sums))) return st;
+    // the band cells' cut trees in reverse: bulk cells (every leaf on one side of its plane)
+    // have a zero adjoint and stay out of the cut list; one work-group per cell, the wider pass
+    // for trees that outgrow the first, the general walk for the rest
+    if ((st = launch_cut_list(ctx))) return st;
+    if (!ctx->tree_blocks_adj) {
+        int per = 0, dev = 0, cus = 0;
+        MIST_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_adj_tree<TreeMain>, TreeMain::NT, 0));
+        MIST_HIP_TRY(hipGetDevice(&dev));
+        MIST_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        ctx->tree_blocks_adj = std::max(1, per) * std::max(1, cus);
+    }
+    const CutCell* cut = (const CutCell*)ctx->cut_list;
+    ADJL(k_adj_tree<TreeMain>, ctx->tree_blocks_adj, TreeMain::NT, m, (const double*)ctx->pv, cut,
+         (const int*)nullptr, (const int*)(ctx->counters + 8), nx, ny, nz, D, (const double*)lam, ctx->adj_pv, gpl[0],
+         gpl[1], gpl[2], gpl[3], ctx->iface_list, ctx->counters + 4, 0, ctx->counters + 5);
+    ADJL(k_adj_tree<TreeWide>, 32, TreeWide::NT, m, (const double*)ctx->pv, cut,
+         (const int*)ctx->iface_list, (const int*)(ctx->counters + 4), nx, ny, nz, D, (const double*)lam, ctx->adj_pv,
+         gpl[0], gpl[1], gpl[2], gpl[3], ctx->defer_list, ctx->counters + 6, 1, ctx->counters + 7);
     ADJL(k_adj_gen_sums, 32, 64, m, (const double*)ctx->pv, (const int*)ctx->defer_list,
          (const int*)(ctx->counters + 6), nx, ny, nz, D, ctx->adj_sums, ctx->counters + 2);
-    ADJL(k_adj_cut, adj_grid(ncap, 2, 16384), 64, m, (const double*)ctx->pv, (const int*)ctx->band_list,
-         (const int*)ctx->counters, nx, ny, nz, D, (const double*)lam, (const double*)ctx->adj_sums, ctx->adj_pv,
-         gpl[0], gpl[1], gpl[2], gpl[3], ctx->counters + 2);
+    ADJL(k_adj_cut, 32, 64, m, (const double*)ctx->pv, (const int*)ctx->defer_list, (const int*)(ctx->counters + 6),
+         nx, ny, nz, D, (const double*)lam, (const double*)ctx->adj_sums, ctx->adj_pv, gpl[0], gpl[1], gpl[2], gpl[3],
+         ctx->counters + 2);
     // 3. the back-trace in reverse, into the face-velocity adjoints
     ADJL(k_adj_backtrace, adj_grid(nv, 128), 128, m, ctx->dp, (const int16_t*)ctx->f[MIST_FIELD_BAND],
          (const double*)ctx->f[MIST_FIELD_UF], (const double*)ctx->f[MIST_FIELD_VF],

@@ -69,6 +69,7 @@ struct mist_ctx {
     int tree_blocks = 0;         // resident blocks of the main cut pass on the device
     int tree_cus = 0;            // compute units of the device
     int tree_blocks_mom = 0;     // resident blocks of the main cut pass with momentum sums
+    int tree_blocks_adj = 0;     // resident blocks of the reverse cut pass
     double* ns_sums = nullptr;   // 6 sums per interior cell of the momentum predictor's cut
     double* h_red = nullptr;     // pinned
     uint64_t vf_version = 1, normals_version = 0, band_version = 0;
@@ -105,6 +106,7 @@ int launch_plic(mist_ctx* ctx);
 int launch_project_vertices(mist_ctx* ctx, double dt);
 int launch_transport(mist_ctx* ctx, double dt);
 int launch_tree_momentum(mist_ctx* ctx, double* sums);
+int launch_cut_list(mist_ctx* ctx);
 int launch_init_vof(mist_ctx* ctx, int shape, const double* args);
 int launch_props(mist_ctx* ctx);
 int launch_pressure_rhs(mist_ctx* ctx);
