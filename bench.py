@@ -295,18 +295,40 @@ def run_adjoint(args, torch):
         L2, g = run.loss_and_grad(a1, target)
     torch.cuda.synchronize()
     t_grad = (time.perf_counter() - t0) / reps
-    h = 1e-6
-    fd = np.array([(run.loss(a1 + h * e, target) - run.loss(a1 - h * e, target)) / (2 * h) for e in np.eye(16)])
-    rel = np.abs(g - fd) / np.maximum(np.abs(fd), 1e-3 * np.abs(fd).max())
+    # the reverse sweep alone (recomputing each segment from its checkpoint, then walking it
+    # back), after a forward that stored the checkpoints
+    m = ctx.mesh
+    o = m.nghost + 3
+    sl = (slice(o, o + m.Nx), slice(o, o + m.Ny), slice(o, o + m.Nz))
+    t_bwd = 0.0
+    for _ in range(reps):
+        VF = run.forward(a1)
+        diff = np.zeros_like(VF)
+        diff[sl] = VF[sl] - target[sl]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run.backward(diff)
+        torch.cuda.synchronize()
+        t_bwd += (time.perf_counter() - t0) / reps
+    # central differences at h = 1e-6 and 1e-7: the map is piecewise smooth, and a kink (an
+    # ELVIRA choice or a cut case switching) within h of the point makes a direction disagree
+    checks = []
+    for h in (1e-6, 1e-7):
+        fd = np.array([(run.loss(a1 + h * e, target) - run.loss(a1 - h * e, target)) / (2 * h) for e in np.eye(16)])
+        rel = np.abs(g - fd) / np.maximum(np.abs(fd), 1e-3 * np.abs(fd).max())
+        checks.append({"h": h, "within_1e-6": int((rel <= 1e-6).sum()), "directions": 16,
+                       "max_rel": float(rel.max()), "median_rel": float(np.median(rel))})
     ctx.close()
     out = {"workload": f"{N}^2 x 1, circle r=0.2 at (0.5,0.5), 16-mode stream function (seed 1 control, seed 2 "
                        f"target), {nsteps} steps, dt={dt:.3e}, checkpoints every {every}",
            "loss": L, "forward_ms": 1e3 * t_fwd, "loss_and_gradient_ms": 1e3 * t_grad,
-           "gradient_cost_vs_forward": t_grad / t_fwd,
-           "fd_check": {"h": h, "within_1e-6": int((rel <= 1e-6).sum()), "directions": 16,
-                        "max_rel": float(rel.max()), "median_rel": float(np.median(rel))}}
+           "gradient_cost_vs_forward": t_grad / t_fwd, "reverse_sweep_ms": 1e3 * t_bwd,
+           "reverse_sweep_vs_forward": t_bwd / t_fwd,
+           "fd_check": checks[0], "fd_check_h1e-7": checks[1]}
     log(f"[bench] adjoint: forward {out['forward_ms']:.1f} ms, loss+gradient {out['loss_and_gradient_ms']:.1f} ms "
-        f"({out['gradient_cost_vs_forward']:.2f}x), FD {out['fd_check']['within_1e-6']}/16 within 1e-6")
+        f"({out['gradient_cost_vs_forward']:.2f}x), reverse sweep {out['reverse_sweep_ms']:.1f} ms "
+        f"({out['reverse_sweep_vs_forward']:.2f}x); FD within 1e-6: {checks[0]['within_1e-6']}/16 at h=1e-6, "
+        f"{checks[1]['within_1e-6']}/16 at h=1e-7")
     return out
 
 

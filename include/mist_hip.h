@@ -217,9 +217,10 @@ int mist_ns_step(mist_ctx* ctx, double* t, double clamp_to, double* dt_used, dou
  * first step of a sweep). Branches (cut cases, clamps, ELVIRA's choice, cell searches) are
  * those of the forward evaluation. The first call allocates the work arrays. */
 int mist_transport_vjp(mist_ctx* ctx, double dt);
-/* Step checkpoints: copy the whole VF array (with its ghosts) to device slot `slot`
- * (0 <= slot < 4096, allocated on first use) and back; a restore also recomputes the band,
- * normals and PLIC constants of the restored VF, the state a step starts from. */
+/* Step checkpoints: copy the state a step starts from (VF with its ghosts, the band flags
+ * and band-cell list, the normals, PLIC constants and ELVIRA's choice; those of them that are
+ * stale are computed first) to device slot `slot` (0 <= slot < 4096, allocated on first use,
+ * about 5 cell fields per slot) and back. */
 int mist_checkpoint_save(mist_ctx* ctx, int slot);
 int mist_checkpoint_restore(mist_ctx* ctx, int slot);
 

@@ -59,11 +59,20 @@ __global__ void k_adj_neumann_axis(double* g, AdjExt e, int d, int mn, int mx) {
     }
 }
 
+// Up to four arrays of the same extents, treated alike by one launch.
+struct Arrays4 {
+    double* a[4];
+    int n;
+};
+
 // reverse of k_borders_axis (the statements of each column undone in reverse order)
-__global__ void k_adj_borders_axis(double* g, AdjExt e, int d, int mn, int mx, int mno, int mxo, int ng) {
+__global__ void k_adj_borders_axis(Arrays4 G, AdjExt e, int d, int mn, int mx, int mno, int mxo, int ng) {
     const int d1 = (d + 1) % 3, d2 = (d + 2) % 3;
     const int64_t tot = (int64_t)e.n[d1] * e.n[d2];
-    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < tot * G.n;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        double* const g = G.a[r / tot];
+        const int64_t q = r % tot;
         int id[3], is[3];
         id[d1] = is[d1] = e.lo[d1] + (int)(q % e.n[d1]);
         id[d2] = is[d2] = e.lo[d2] + (int)(q / e.n[d1]);
@@ -86,8 +95,14 @@ __global__ void k_adj_borders_axis(double* g, AdjExt e, int d, int mn, int mx, i
     }
 }
 
-__global__ void k_adj_fill(double* a, int64_t n, double v) {
-    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) a[q] = v;
+// zero three arrays of lengths n0, n1, n2
+__global__ void k_adj_zero3(double* a0, int64_t n0, double* a1, int64_t n1, double* a2, int64_t n2) {
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n0 + n1 + n2; q += st) {
+        if (q < n0) a0[q] = 0.0;
+        else if (q < n0 + n1) a1[q - n0] = 0.0;
+        else a2[q - n0 - n1] = 0.0;
+    }
 }
 
 // interior cells outside the band: VF' = VF
@@ -968,13 +983,20 @@ __global__ void k_adj_elvira(DevMesh m, DevParams p, const double* VF, const int
         if (g[0] == 0.0 && g[1] == 0.0 && g[2] == 0.0) continue;
         const double vf = VF[m.vof(i, j, k)];
         if (!(vf >= p.VFlo && vf <= p.VFhi)) continue;  // axis normals: constant
-        const int cand = best[q];
-        if (cand < 0) continue;
-        elvira_cand_adj(
-            m, i, j, k, cand, [&](int di, int dj, int dk) { return VF[m.vof(i + di, j + dj, k + dk)]; }, g,
-            [&](int di, int dj, int dk, double w) {
-                if (w != 0.0) atomicAdd(lam_new + m.vof(i + di, j + dj, k + dk), w);
-            });
+        const int code = best[q];
+        if (code < 0) continue;
+        // the chosen candidate; where a second one gave the very same normal (a kink, whose
+        // side a perturbation's sign picks) the mean of the two, the slope a central
+        // difference sees
+        const int cand = code & 255, tie = (code >> 8) - 1;
+        const double h = tie >= 0 ? 0.5 : 1.0;
+        const double gh[3] = {h * g[0], h * g[1], h * g[2]};
+        auto at = [&](int di, int dj, int dk) { return VF[m.vof(i + di, j + dj, k + dk)]; };
+        auto acc = [&](int di, int dj, int dk, double w) {
+            if (w != 0.0) atomicAdd(lam_new + m.vof(i + di, j + dj, k + dk), w);
+        };
+        elvira_cand_adj(m, i, j, k, cand, at, gh, acc);
+        if (tie >= 0) elvira_cand_adj(m, i, j, k, tie, at, gh, acc);
     }
 }
 
@@ -998,8 +1020,8 @@ static AdjExt adj_ext(mist_ctx* ctx, int field) {
     return e;
 }
 
-// reverse of launch_borders on a double array with the extents of `field`
-static int adj_borders(mist_ctx* ctx, double* g, int field) {
+// reverse of launch_borders on up to four double arrays with the extents of `field`
+static int adj_borders(mist_ctx* ctx, Arrays4 G, int field) {
     const AdjExt e = adj_ext(ctx, field);
     const int64_t per[3] = {ctx->p.xper, ctx->p.yper, ctx->p.zper};
     const int mx[3] = {ctx->m.Nx, ctx->m.Ny, ctx->m.Nz};
@@ -1007,7 +1029,7 @@ static int adj_borders(mist_ctx* ctx, double* g, int field) {
     for (int d = 2; d >= 0; --d) {
         if (!per[d]) continue;
         const int64_t tot = (int64_t)e.n[(d + 1) % 3] * e.n[(d + 2) % 3];
-        ADJL(k_adj_borders_axis, adj_grid(tot, 256), 256, g, e, d, 1, mx[d], 1 - ng, mx[d] + ng, ng);
+        ADJL(k_adj_borders_axis, adj_grid(tot * G.n, 256), 256, G, e, d, 1, mx[d], 1 - ng, mx[d] + ng, ng);
     }
     return MIST_OK;
 }
@@ -1036,11 +1058,10 @@ int launch_transport_vjp(mist_ctx* ctx, double dt) {
     // pre-images of this state's band cells under the current face velocities
     if ((st = launch_project_vertices(ctx, dt))) return st;
     // 1. ghost fills of VF' in reverse: periodic copies, then Neumann
-    if ((st = adj_borders(ctx, lam, MIST_FIELD_VOF))) return st;
+    if ((st = adj_borders(ctx, Arrays4{{lam, nullptr, nullptr, nullptr}, 1}, MIST_FIELD_VOF))) return st;
     if ((st = adj_neumann(ctx, lam, MIST_FIELD_VOF))) return st;
-    ADJL(k_adj_fill, adj_grid(nvof, 256), 256, lam_new, nvof, 0.0);
-    ADJL(k_adj_fill, adj_grid(4 * ncell_ext, 256), 256, ctx->adj_plane, 4 * ncell_ext, 0.0);
-    ADJL(k_adj_fill, adj_grid(3 * nv, 256), 256, ctx->adj_pv, 3 * nv, 0.0);
+    ADJL(k_adj_zero3, adj_grid(nvof + 4 * ncell_ext + 3 * nv, 256), 256, lam_new, nvof, ctx->adj_plane,
+         4 * ncell_ext, ctx->adj_pv, 3 * nv);
     // 2. identity outside the band; the band cells' cut in reverse
     ADJL(k_adj_identity, adj_grid(ncap, 256), 256, m, (int)ctx->p.nband, (const int16_t*)ctx->f[MIST_FIELD_BAND],
          (const double*)lam, lam_new);
@@ -1077,11 +1098,10 @@ int launch_transport_vjp(mist_ctx* ctx, double dt) {
          (const double*)ctx->f[MIST_FIELD_WF], dt, (const double*)ctx->adj_pv, (double*)ctx->f[MIST_FIELD_UF_BAR],
          (double*)ctx->f[MIST_FIELD_VFACE_BAR], (double*)ctx->f[MIST_FIELD_WF_BAR]);
     // 4. computePLIC! (after D's periodic copies) and ELVIRA (after the normals' copies)
-    if ((st = adj_borders(ctx, gpl[3], MIST_FIELD_PLIC_D))) return st;
+    if ((st = adj_borders(ctx, Arrays4{{gpl[3], nullptr, nullptr, nullptr}, 1}, MIST_FIELD_PLIC_D))) return st;
     ADJL(k_adj_plic, adj_grid(ncell_ext, 256), 256, m, ctx->dp, (const double*)ctx->f[MIST_FIELD_VOF], nx, ny, nz,
          (const double*)gpl[3], gpl[0], gpl[1], gpl[2], lam_new);
-    for (int a = 0; a < 3; ++a)
-        if ((st = adj_borders(ctx, gpl[a], MIST_FIELD_NX + a))) return st;
+    if ((st = adj_borders(ctx, Arrays4{{gpl[0], gpl[1], gpl[2], nullptr}, 3}, MIST_FIELD_NX))) return st;
     ADJL(k_adj_elvira, adj_grid(ncap, 256), 256, m, ctx->dp, (const double*)ctx->f[MIST_FIELD_VOF],
          (const int*)ctx->elv_best, (const double*)gpl[0], (const double*)gpl[1], (const double*)gpl[2], lam_new);
     MIST_HIP_TRY(hipMemcpyAsync(lam, lam_new, nvof * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));

@@ -1,5 +1,6 @@
 // mist_capi.hip -- the C ABI of libmist_hip.so (declared in include/mist_hip.h): context
 // life cycle, field transfer, and one entry point per reference stage.
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -757,25 +758,83 @@ int mist_transport_vjp(mist_ctx* ctx, double dt) {
     return MIST_OK;
 }
 
+// A checkpoint holds the state a step starts from: VF with its ghosts, the band flags and the
+// band-cell list with its length, the normals, the PLIC constants and ELVIRA's choice per
+// interface cell, each part at a 256-byte aligned offset of one allocation per slot.
+struct CkptPart {
+    void* dev;
+    size_t bytes;
+};
+static int ckpt_parts(mist_ctx* ctx, CkptPart (&parts)[9], size_t& total) {
+    const size_t ncell = (size_t)ctx->m.Nx * ctx->m.Ny * ctx->m.Nz;
+    parts[0] = {ctx->f[MIST_FIELD_VOF], field_bytes(ctx, MIST_FIELD_VOF)};
+    parts[1] = {ctx->f[MIST_FIELD_BAND], field_bytes(ctx, MIST_FIELD_BAND)};
+    parts[2] = {ctx->f[MIST_FIELD_NX], field_bytes(ctx, MIST_FIELD_NX)};
+    parts[3] = {ctx->f[MIST_FIELD_NY], field_bytes(ctx, MIST_FIELD_NY)};
+    parts[4] = {ctx->f[MIST_FIELD_NZ], field_bytes(ctx, MIST_FIELD_NZ)};
+    parts[5] = {ctx->f[MIST_FIELD_PLIC_D], field_bytes(ctx, MIST_FIELD_PLIC_D)};
+    parts[6] = {ctx->elv_best, ncell * sizeof(int)};
+    parts[7] = {ctx->band_list, ncell * sizeof(int)};
+    parts[8] = {ctx->counters, sizeof(int)};  // the band-cell count
+    total = 0;
+    for (const CkptPart& q : parts) total += (q.bytes + 255) & ~(size_t)255;
+    return MIST_OK;
+}
+
+// The parts of a checkpoint copied by one launch: 16-byte words, then the tail bytes (every
+// part starts 256-byte aligned).
+struct CopyTab {
+    const char* src[9];
+    char* dst[9];
+    unsigned long long bytes[9];
+};
+__global__ void k_copy_parts(CopyTab t) {
+    const size_t tid = blockIdx.x * (size_t)blockDim.x + threadIdx.x, st = (size_t)gridDim.x * blockDim.x;
+    for (int p = 0; p < 9; ++p) {
+        const size_t n16 = t.bytes[p] / 16;
+        const uint4* s = (const uint4*)t.src[p];
+        uint4* d = (uint4*)t.dst[p];
+        for (size_t q = tid; q < n16; q += st) d[q] = s[q];
+        for (size_t b = 16 * n16 + tid; b < t.bytes[p]; b += st) t.dst[p][b] = t.src[p][b];
+    }
+}
+
 static int ckpt_copy(mist_ctx* ctx, int slot, bool save) {
     CHECK_CTX();
     if (slot < 0 || slot >= 4096) return fail(ctx, MIST_ERR_PARAM, "checkpoint slot must be 0..4095");
     MIST_HIP_TRY(hipSetDevice(ctx->device));
-    const size_t bytes = field_bytes(ctx, MIST_FIELD_VOF);
+    CkptPart parts[9];
+    size_t total = 0;
+    RET_IF(ckpt_parts(ctx, parts, total));
     if ((size_t)slot >= ctx->ckpt.size()) ctx->ckpt.resize((size_t)slot + 1, nullptr);
     if (!ctx->ckpt[slot]) {
         if (!save) return fail(ctx, MIST_ERR_STATE, "checkpoint slot " + std::to_string(slot) + " is empty");
-        if (hipMalloc(&ctx->ckpt[slot], bytes) != hipSuccess) return fail(ctx, MIST_ERR_OOM, "checkpoint allocation");
+        if (hipMalloc(&ctx->ckpt[slot], total) != hipSuccess) return fail(ctx, MIST_ERR_OOM, "checkpoint allocation");
     }
     if (save) {
-        MIST_HIP_TRY(hipMemcpyAsync(ctx->ckpt[slot], ctx->f[MIST_FIELD_VOF], bytes, hipMemcpyDeviceToDevice, ctx->stream));
-    } else {
-        MIST_HIP_TRY(hipMemcpyAsync(ctx->f[MIST_FIELD_VOF], ctx->ckpt[slot], bytes, hipMemcpyDeviceToDevice, ctx->stream));
-        ctx->vf_version++;
-        // the rest of the state a step starts from: band, normals and PLIC constants of VF
-        RET_IF(mist_compute_band(ctx));
+        // the band, normals and PLIC constants of this VF, where they are stale
         RET_IF(mist_compute_normal(ctx));
-        RET_IF(mist_compute_plic(ctx));
+        if (ctx->plic_version != ctx->vf_version) RET_IF(mist_compute_plic(ctx));
+        if (!ctx->band_list_valid) RET_IF(mist_compute_band(ctx));
+    }
+    char* base = (char*)ctx->ckpt[slot];
+    size_t off = 0;
+    CopyTab tab;
+    for (int p = 0; p < 9; ++p) {
+        char* slotp = base + off;
+        tab.src[p] = save ? (const char*)parts[p].dev : slotp;
+        tab.dst[p] = save ? slotp : (char*)parts[p].dev;
+        tab.bytes[p] = parts[p].bytes;
+        off += (parts[p].bytes + 255) & ~(size_t)255;
+    }
+    const int grid = (int)std::min<size_t>(2048, std::max<size_t>(1, total / (16 * 256)));
+    hipLaunchKernelGGL(k_copy_parts, dim3(grid), dim3(256), 0, ctx->stream, tab);
+    MIST_HIP_TRY(hipGetLastError());
+    if (!save) {
+        ctx->vf_version++;
+        ctx->normals_version = ctx->vf_version;
+        ctx->plic_version = ctx->vf_version;
+        ctx->band_list_valid = true;
     }
     return MIST_OK;
 }

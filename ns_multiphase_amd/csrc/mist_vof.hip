@@ -537,12 +537,18 @@ __global__ __launch_bounds__(ELV_THREADS, 4) void k_elvira(DevMesh m, DevParams 
                 const int ol = __shfl_xor(bl, off, 32);
                 if (ol < 64 && (bl == 64 || oe < be || (oe == be && ol < bl))) { be = oe; bl = ol; }
             }
+            // another candidate with the very same normal: the choice between the two is a kink
+            // of the map (a perturbation's sign decides it), recorded for the adjoint
+            const bool same = bl < 64 && lane < 27 && lane != bl && sOk[lane] && sC[lane][0] == sC[bl][0] &&
+                              sC[lane][1] == sC[bl][1] && sC[lane][2] == sC[bl][2];
+            const unsigned long long tmask = __ballot(same);
+            const int tie = tmask ? __ffsll((long long)tmask) - 1 : -1;
             if (lane == 0) {
                 if (bl < 64) {
                     const int64_t c = m.cell(i, j, k);
                     nxo[c] = sC[bl][0]; nyo[c] = sC[bl][1]; nzo[c] = sC[bl][2];
                 }
-                best[list[cidx]] = bl < 64 ? bl : -1;
+                best[list[cidx]] = bl < 64 ? (bl | ((tie + 1) << 8)) : -1;
             }
         }
         __syncthreads();
