@@ -297,4 +297,80 @@ struct CutCell {
 };
 static_assert(sizeof(CutCell) == 24, "the context allocates 24 bytes per cut-list entry");
 
+// A leaf of the tree of a cell: its four vertices, the cell it lies in (offset l in the index
+// box) and how it was made from its node (the node's last z cut, src/VFgeom.jl:853-862, or the
+// node itself). The same arithmetic as the leaf loop of k_sl_tree.
+struct TreeLeaf {
+    V3 Q[4];
+    TreeNode N;
+    int l;
+    bool zcut;
+    double P;       // the z-plane of the cut
+    int cs;         // its case
+    unsigned qv;    // the leaf's vertices among the node's 4 + cut vertices
+};
+
+template <class C>
+__device__ __forceinline__ void tree_leaf(const TreeShared<C>& S, int cur, int L, const int (&span)[3], int bx, int by,
+                                          TreeLeaf& T) {
+    const int e = S.lnode[L];
+    const int c = L - S.loff[e];
+    const TreeNode N = S.node[cur][e];
+    T.N = N;
+    const int o0 = boff(N.bits, 0, 2), o1 = boff(N.bits, 1, 2), o2 = boff(N.bits, 2, 2), o3 = boff(N.bits, 3, 2);
+    const int mx = max(max(o0, o1), max(o2, o3));
+    unsigned ob;
+    T.zcut = mx > min(min(o0, o1), min(o2, o3));
+    if (T.zcut) {
+        const double P = S.plane[2][mx];
+        double dd[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) dd[v] = S.pz[tn_id(N, v)] - P;
+        const int cs = d2case_snap(dd);
+        const unsigned qv = CUT_VTET[cs][c];
+        const bool s2 = (CUT_SIDE2[cs] >> c) & 1;
+        T.P = P;
+        T.cs = cs;
+        T.qv = qv;
+        int e1v = 0, e2v = 0;
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            const int lv = nib(qv, kk);
+            const bool cutv = lv >= 4;
+            const unsigned eg = CUT_EDGE[cs][cutv ? lv - 4 : 0];
+            const int v1 = cutv ? (int)(eg & 15) : lv, v2 = cutv ? (int)(eg >> 4) : lv;
+            const V3 A = pool_v(S, tn_id(N, v1)), B = pool_v(S, tn_id(N, v2));
+            const double d1 = A.z - P, d2 = B.z - P;
+            const double mu = cutv ? jmin(1.0, jmax(0.0, sdiv(-d1, d2 - d1))) : 0.0;
+            T.Q[kk] = lerp3(A, B, mu);
+            if (kk == 0) { e1v = v1; e2v = v2; }
+        }
+        const int lv0 = nib(qv, 0);
+        if (lv0 < 4) {
+            ob = (N.bits >> (6 * lv0)) & 63u;
+            const unsigned oa = ob >> 4;
+            ob = (ob & 15u) | ((s2 ? (unsigned)mx : min(oa, (unsigned)(mx - 1))) << 4);
+        } else {
+            ob = (unsigned)(s2 ? mx : mx - 1) << 4;
+#pragma unroll
+            for (int ax = 0; ax < 2; ++ax) {
+                const int ea = boff(N.bits, e1v, ax), eb = boff(N.bits, e2v, ax);
+                const unsigned oo = ea == eb ? (unsigned)ea
+                                             : walk_clamp(ax == 0 ? T.Q[0].x : T.Q[0].y, boff(N.bits, lv0 - 4, ax),
+                                                          ea, eb, S.plane[ax], span[ax]);
+                ob |= oo << (2 * ax);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) T.Q[v] = pool_v(S, tn_id(N, v));
+        ob = N.bits & 63u;
+        T.P = 0.0;
+        T.cs = 0;
+        T.qv = 0;
+    }
+    const unsigned ox = ob & 3u, oy = (ob >> 2) & 3u, oz = (ob >> 4) & 3u;
+    T.l = (int)ox + bx * ((int)oy + by * (int)oz);
+}
+
 }  // namespace mist
