@@ -105,7 +105,7 @@ int launch_normals(mist_ctx* ctx);
 int launch_plic(mist_ctx* ctx);
 int launch_project_vertices(mist_ctx* ctx, double dt);
 int launch_transport(mist_ctx* ctx, double dt);
-int launch_tree_momentum(mist_ctx* ctx, double* sums);
+int launch_tree_momentum(mist_ctx* ctx, bool corr, double dt);
 int launch_cut_list(mist_ctx* ctx);
 int launch_init_vof(mist_ctx* ctx, int shape, const double* args);
 int launch_props(mist_ctx* ctx);

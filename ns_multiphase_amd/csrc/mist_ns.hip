@@ -12,6 +12,7 @@
 
 #include "mist_internal.h"
 #include "mist_kernels.h"
+#include "mist_nsgeom.h"
 
 namespace mist {
 
@@ -27,140 +28,10 @@ __device__ __forceinline__ int64_t fzi(const DevMesh& m, int i, int j, int k) { 
 
 __device__ __forceinline__ bool in_band(const int16_t* band, int64_t c, int nband) { return abs((int)band[c]) <= nband; }
 
-// ------------------------------------------------------------------------------------
-// SLfluxVol (src/VFgeom.jl:428-509): the volume swept through face `dir` (1, 2, 3) of cell
-// (i,j,k) in one step, 5 tets over its 4 back-traced and 4 fixed corners. The back-traced
-// corners are the projected vertices pv of the band cells. `verts` receives the 8 vertices.
-// ------------------------------------------------------------------------------------
-__device__ inline double sl_flux_vol(const DevMesh& m, const double* pv, int dir, int i, int j, int k,
-                                     double verts[8][3]) {
-    const bool even = ((i + j + k) % 2) == 0;
-    const int odd = even ? 0 : 1;
-#pragma unroll
-    for (int s = 0; s < 8; ++s) {
-        int I, J, K;
-        bool proj;
-        if (dir == 1) {  // pairs over (y, z) corners: projected first for even faces
-            const int n = s >> 1;
-            I = i; J = j + (n & 1); K = k + (n >> 1);
-            proj = ((s & 1) == 0) == even;
-        } else if (dir == 2) {
-            I = i + ((s & 1) ^ odd); J = j; K = k + (s >> 2);
-            proj = (s & 2) == 0;
-        } else {
-            I = i + ((s & 1) ^ odd); J = j + ((s >> 1) & 1); K = k;
-            proj = s < 4;
-        }
-        if (proj) {
-            const int64_t q = vtx_id(m, I, J, K);
-            verts[s][0] = pv[3 * q + 0]; verts[s][1] = pv[3 * q + 1]; verts[s][2] = pv[3 * q + 2];
-        } else {
-            verts[s][0] = m.x[I]; verts[s][1] = m.y[J]; verts[s][2] = m.z[K];
-        }
-    }
-    double v = 0.0;  // tets_vol (src/VFgeom.jl:683)
-#pragma unroll
-    for (int t = 0; t < 5; ++t) {
-        const unsigned q = TET5[t];
-        v += tet_vol(verts[nib(q, 0)], verts[nib(q, 1)], verts[nib(q, 2)], verts[nib(q, 3)]);
-    }
-    return v * (even ? 1.0 : -1.0);
-}
-
-// correction_tets_{x,y,z} (src/VFgeom.jl:581-640): the coordinate of the apex e along the face
-// normal that gives the pyramid (a,b,d,e) + (b,c,d,e) the volume `vol`; e starts at the
-// centroid of a, b, c, d. The terms are added left to right as the reference writes them.
-__device__ inline void correction_apex(int dir, const double* a, const double* b, const double* cc, const double* d,
-                                       double vol, double e[3]) {
-    for (int q = 0; q < 3; ++q) e[q] = 0.25 * (((a[q] + b[q]) + cc[q]) + d[q]);
-    const double a1 = a[0], a2 = a[1], a3 = a[2], b1 = b[0], b2 = b[1], b3 = b[2];
-    const double c1 = cc[0], c2 = cc[1], c3 = cc[2], d1 = d[0], d2 = d[1], d3 = d[2];
-    const double e1 = e[0], e2 = e[1], e3 = e[2];
-    double num = 6.0 * vol;
-    num = num + a1 * b2 * d3; num = num - a1 * b3 * d2; num = num - a2 * b1 * d3; num = num + a2 * b3 * d1;
-    num = num + a3 * b1 * d2; num = num - a3 * b2 * d1;
-    double den;
-    if (dir == 1) {
-        num = num - a1 * b2 * e3; num = num + a1 * b3 * e2; num = num + a2 * b1 * e3; num = num - a3 * b1 * e2;
-        num = num + b1 * c2 * d3; num = num - b1 * c3 * d2; num = num - b2 * c1 * d3; num = num + b2 * c3 * d1;
-        num = num + b3 * c1 * d2; num = num - b3 * c2 * d1; num = num + a1 * d2 * e3; num = num - a1 * d3 * e2;
-        num = num - a2 * d1 * e3; num = num + a3 * d1 * e2; num = num - b1 * c2 * e3; num = num + b1 * c3 * e2;
-        num = num + b2 * c1 * e3; num = num - b3 * c1 * e2; num = num - c1 * d2 * e3; num = num + c1 * d3 * e2;
-        num = num + c2 * d1 * e3; num = num - c3 * d1 * e2;
-        den = a2 * b3; den = den - a3 * b2; den = den - a2 * d3; den = den + a3 * d2; den = den + b2 * c3;
-        den = den - b3 * c2; den = den + c2 * d3; den = den - c3 * d2;
-        e[0] = num / den;
-    } else if (dir == 2) {
-        num = num - a1 * b2 * e3; num = num + a2 * b1 * e3; num = num - a2 * b3 * e1; num = num + a3 * b2 * e1;
-        num = num + b1 * c2 * d3; num = num - b1 * c3 * d2; num = num - b2 * c1 * d3; num = num + b2 * c3 * d1;
-        num = num + b3 * c1 * d2; num = num - b3 * c2 * d1; num = num + a1 * d2 * e3; num = num - a2 * d1 * e3;
-        num = num + a2 * d3 * e1; num = num - a3 * d2 * e1; num = num - b1 * c2 * e3; num = num + b2 * c1 * e3;
-        num = num - b2 * c3 * e1; num = num + b3 * c2 * e1; num = num - c1 * d2 * e3; num = num + c2 * d1 * e3;
-        num = num - c2 * d3 * e1; num = num + c3 * d2 * e1;
-        den = a1 * b3; den = den - a3 * b1; den = den - a1 * d3; den = den + a3 * d1; den = den + b1 * c3;
-        den = den - b3 * c1; den = den + c1 * d3; den = den - c3 * d1;
-        e[1] = (-num) / den;
-    } else {
-        num = num + a1 * b3 * e2; num = num - a2 * b3 * e1; num = num - a3 * b1 * e2; num = num + a3 * b2 * e1;
-        num = num + b1 * c2 * d3; num = num - b1 * c3 * d2; num = num - b2 * c1 * d3; num = num + b2 * c3 * d1;
-        num = num + b3 * c1 * d2; num = num - b3 * c2 * d1; num = num - a1 * d3 * e2; num = num + a2 * d3 * e1;
-        num = num + a3 * d1 * e2; num = num - a3 * d2 * e1; num = num + b1 * c3 * e2; num = num - b2 * c3 * e1;
-        num = num - b3 * c1 * e2; num = num + b3 * c2 * e1; num = num + c1 * d3 * e2; num = num - c2 * d3 * e1;
-        num = num - c3 * d1 * e2; num = num + c3 * d2 * e1;
-        den = a1 * b2; den = den - a2 * b1; den = den - a1 * d2; den = den + a2 * d1; den = den + b1 * c2;
-        den = den - b2 * c1; den = den + c1 * d2; den = den - c2 * d1;
-        e[2] = num / den;
-    }
-}
-
-// Tet `t` (0-based) of band cell (i,j,k) in transport!: 0..4 the back-traced 5-tet
-// decomposition (cell2tets!, src/VFgeom.jl:345-420), 5..16 the correction tets of
-// add_correction_tets (src/VFgeom.jl:513-664): faces x-, x+, y-, y+, z-, z+, two tets each.
-__device__ inline void ns_tet(const DevMesh& m, const double* pv, const double* uf, const double* vf,
-                              const double* wf, double dt, int i, int j, int k, int t, double tet[4][3]) {
-    const bool even = ((i + j + k) % 2) == 0;
-    if (t < 5) {
-        const unsigned tv = TET5[t];
-        for (int n = 0; n < 4; ++n) {
-            const int c = vert_corner(nib(tv, n), even);
-            const int64_t vq = vtx_id(m, i + (c & 1), j + ((c >> 1) & 1), k + (c >> 2));
-            tet[n][0] = pv[3 * vq + 0]; tet[n][1] = pv[3 * vq + 1]; tet[n][2] = pv[3 * vq + 2];
-        }
-        return;
-    }
-    const int f = (t - 5) >> 1, half = (t - 5) & 1;
-    const int dir = 1 + (f >> 1), up = f & 1;
-    const int fi = i + (dir == 1 ? up : 0), fj = j + (dir == 2 ? up : 0), fk = k + (dir == 3 ? up : 0);
-    double verts[8][3];
-    const double flux_vol = sl_flux_vol(m, pv, dir, fi, fj, fk, verts);
-    double divg_vol;
-    const int64_t fc = m.cell(fi, fj, fk);
-    if (dir == 1) divg_vol = ((dt * m.dy) * m.dz) * uf[fc];
-    else if (dir == 2) divg_vol = ((dt * m.dx) * m.dz) * vf[fc];
-    else divg_vol = ((dt * m.dx) * m.dy) * wf[fc];
-    const bool feven = ((fi + fj + fk) % 2) == 0;
-    const double vol = feven ? (divg_vol - flux_vol) : -(divg_vol - flux_vol);
-    int ia, ib, ic, id;
-    if (dir == 1) {
-        if (feven) { ia = 4; ib = 6; ic = 2; id = 0; }
-        else { ia = 7; ib = 5; ic = 1; id = 3; }
-    } else if (dir == 2) { ia = 1; ib = 5; ic = 4; id = 0; }
-    else { ia = 1; ib = 0; ic = 2; id = 3; }
-    double e[3];
-    correction_apex(dir, verts[ia], verts[ib], verts[ic], verts[id], vol, e);
-    const int p0 = half ? ib : ia, p1 = half ? ic : ib;
-    for (int q = 0; q < 3; ++q) {
-        tet[0][q] = verts[p0][q];
-        tet[1][q] = verts[p1][q];
-        tet[2][q] = verts[id][q];
-        tet[3][q] = e[q];
-    }
-}
-
 // Tets FIRST .. FIRST+T-1 of the cells of `list` through the general cut walk with the six
 // momentum sums, one lane per (cell, tet), 64/T cells per block. Without FINAL the cell's
-// sums go to sums[6 q ..]; with FINAL they are added to the sums already there (the tree
-// pass's five tets) and the cell's VF (in place: the cuts read only normals and D) and
+// sums go to sums[6 q ..]; with FINAL they are added to the sums already there (those of tets
+// 0 .. FIRST-1) and the cell's VF (in place: the cuts read only normals and D) and
 // us, vs, ws = (rho u vol) / (rho vol) are written (src/Transport.jl:107-150). The tetsign of
 // the reference multiplies every tet of a cell alike and cancels in these ratios.
 template <int T, int FIRST, bool FINAL>
@@ -199,7 +70,7 @@ __global__ __launch_bounds__(64) void k_ns_walk(DevMesh m, const double* pv, con
             double S[6];
             double* o = sums + 6 * (int64_t)q;
 #pragma unroll
-            for (int a = 0; a < 6; ++a) S[a] = FINAL ? o[a] : 0.0;
+            for (int a = 0; a < 6; ++a) S[a] = (FINAL && FIRST > 0) ? o[a] : 0.0;
             for (int n = 0; n < T; ++n)
 #pragma unroll
                 for (int a = 0; a < 6; ++a) S[a] += s_acc[a][lane + n];
@@ -215,24 +86,6 @@ __global__ __launch_bounds__(64) void k_ns_walk(DevMesh m, const double* pv, con
             }
         }
         __syncthreads();
-    }
-}
-
-// Band cells of the semi-Lagrangian scheme (no correction tets): VF and us, vs, ws from the
-// five tets' sums.
-__global__ void k_ns_finalize(DevMesh m, const int* list, const int* count, const double* sums, double* VF,
-                              double* us, double* vs, double* ws) {
-    const int nb = *count;
-    for (int cidx = blockIdx.x * blockDim.x + threadIdx.x; cidx < nb; cidx += gridDim.x * blockDim.x) {
-        const int q = list[cidx];
-        int i, j, k;
-        interior_of(m, q, i, j, k);
-        const double* S = sums + 6 * (int64_t)q;
-        VF[m.vof(i, j, k)] = S[1] / S[0];
-        const int64_t c = m.cell(i, j, k);
-        us[c] = S[2] / S[5];
-        vs[c] = S[3] / S[5];
-        ws[c] = S[4] / S[5];
     }
 }
 
@@ -594,20 +447,22 @@ int launch_ns_advect(mist_ctx* ctx, double dt) {
     const double* vf = (const double*)FLD(MIST_FIELD_VF);
     const double* wf = (const double*)FLD(MIST_FIELD_WF);
     hipEvent_t ev = prof_begin(ctx, MIST_PROF_NS_CUT);
-    // the five back-traced tets: cut trees per work-group, then the general walk for the rest
-    if ((st = launch_tree_momentum(ctx, sums))) return st;
-    NSL((k_ns_walk<5, 0, false>), 32, 64, ctx->m, (const double*)ctx->pv, (const int*)ctx->defer_list,
-        (const int*)(ctx->counters + 6), nx, ny, nz, D, lm, uf, vf, wf, dt, sums, FLD(MIST_FIELD_VOF),
-        FLD(MIST_FIELD_US), FLD(MIST_FIELD_VS), FLD(MIST_FIELD_WS), ctx->counters + 2);
-    // the twelve correction tets (finite-difference scheme), then VF and us, vs, ws
-    if (fd)
-        NSL((k_ns_walk<12, 5, true>), ns_grid(ncap, 5, 8192), 64, ctx->m, (const double*)ctx->pv,
-            (const int*)ctx->band_list, (const int*)ctx->counters, nx, ny, nz, D, lm, uf, vf, wf, dt, sums,
-            FLD(MIST_FIELD_VOF), FLD(MIST_FIELD_US), FLD(MIST_FIELD_VS), FLD(MIST_FIELD_WS), ctx->counters + 2);
-    else
-        NSL(k_ns_finalize, ns_grid(ncap, 256, 4096), 256, ctx->m, (const int*)ctx->band_list,
-            (const int*)ctx->counters, (const double*)sums, FLD(MIST_FIELD_VOF), FLD(MIST_FIELD_US),
-            FLD(MIST_FIELD_VS), FLD(MIST_FIELD_WS));
+    // the five back-traced tets and, under the FD scheme, the twelve correction tets of every
+    // band cell: cut trees per work-group writing VF and us, vs, ws; the general walk for the
+    // cells they leave
+    if ((st = launch_tree_momentum(ctx, fd, dt))) return st;
+    if (fd) {
+        NSL((k_ns_walk<5, 0, false>), 32, 64, ctx->m, (const double*)ctx->pv, (const int*)ctx->defer_list,
+            (const int*)(ctx->counters + 6), nx, ny, nz, D, lm, uf, vf, wf, dt, sums, FLD(MIST_FIELD_VOF),
+            FLD(MIST_FIELD_US), FLD(MIST_FIELD_VS), FLD(MIST_FIELD_WS), ctx->counters + 2);
+        NSL((k_ns_walk<12, 5, true>), 32, 64, ctx->m, (const double*)ctx->pv, (const int*)ctx->defer_list,
+            (const int*)(ctx->counters + 6), nx, ny, nz, D, lm, uf, vf, wf, dt, sums, FLD(MIST_FIELD_VOF),
+            FLD(MIST_FIELD_US), FLD(MIST_FIELD_VS), FLD(MIST_FIELD_WS), ctx->counters + 2);
+    } else {
+        NSL((k_ns_walk<5, 0, true>), 32, 64, ctx->m, (const double*)ctx->pv, (const int*)ctx->defer_list,
+            (const int*)(ctx->counters + 6), nx, ny, nz, D, lm, uf, vf, wf, dt, sums, FLD(MIST_FIELD_VOF),
+            FLD(MIST_FIELD_US), FLD(MIST_FIELD_VS), FLD(MIST_FIELD_WS), ctx->counters + 2);
+    }
     prof_end(ctx, MIST_PROF_NS_CUT, ev, ctx->counters + 0, 0);
     ev = prof_begin(ctx, MIST_PROF_NS_FIELDS);
     NSL(k_ns_convect, ns_grid(ncap, 256), 256, ctx->m, (int)ctx->p.nband, fd ? 0 : 1,

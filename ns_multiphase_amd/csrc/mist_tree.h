@@ -289,6 +289,21 @@ __device__ __forceinline__ int wave_slot(bool flag, int* count) {
     return flag ? base + __popcll(mask & ((1ull << lane) - 1ull)) : -1;
 }
 
+// The momentum predictor's part of a cut-tree pass: the leaf cells' velocities and phase
+// densities, the face velocities and dt that build the correction tets, the predicted cell
+// velocities it writes, and the flag of a cell search that left the mesh.
+struct TreeNS {
+    LeafMomentum lm;
+    const double* uf;
+    const double* vf;
+    const double* wf;
+    double dt;
+    double* us;
+    double* vs;
+    double* ws;
+    int* ovf;
+};
+
 // A band cell to cut: interior id, smallest cell index `lo` of its projected corners, and the
 // 6-bit offsets (x, y, z: 2 bits each) of corner c from lo at bits 6c.
 struct CutCell {

@@ -9,6 +9,7 @@
 #include "mist_internal.h"
 #include "mist_kernels.h"
 #include "mist_tree.h"
+#include "mist_nsgeom.h"
 
 namespace mist {
 
@@ -717,18 +718,23 @@ __global__ __launch_bounds__(256) void k_sl_classify(DevMesh m, const double* pv
 }
 
 // MOM: the six sums of the momentum predictor (vol, vLiq, rho u vol, rho v vol, rho w vol,
-// rho vol with the leaf cell's centre velocity, src/VFgeom.jl:791-827) go to sums[6 q .. 6 q + 5]
-// instead of VF.
-template <class C, bool MOM>
+// rho vol with the leaf cell's centre velocity, src/VFgeom.jl:791-827) give VF and the predicted
+// velocities us, vs, ws of the cell. CORR: the twelve flux-correction tets of the
+// finite-difference scheme are cut in the same tree as the five back-traced ones; a cell whose
+// tets do not fit one index box of 4 cells per axis goes to `defer`.
+template <class C, bool MOM, bool CORR = false>
 __global__ __launch_bounds__(C::NT) void k_sl_tree(DevMesh m, const double* pv, const CutCell* cut, const int* idx,
                                                 const int* count, const double* nx, const double* ny,
                                                 const double* nz, const double* D, double* VF, int* defer,
-                                                int* ndefer, int defer_cell, int* queue, LeafMomentum lm,
-                                                double* sums) {
+                                                int* ndefer, int defer_cell, int* queue, TreeNS ns) {
+    static_assert(MOM || !CORR, "the correction tets belong to the momentum predictor");
     constexpr int NT = C::NT;
+    const LeafMomentum& lm = ns.lm;
     __shared__ TreeShared<C> S;
     __shared__ double s_vel[MOM ? 64 : 1][3];
     __shared__ double s_red[MOM ? 4 : 1][NT / 64];
+    __shared__ int s_cind[CORR ? 48 : 1][3];  // cells of the correction tets' vertices
+    __shared__ int s_box[2][3];               // index box over corners and correction vertices
     const int tid = threadIdx.x;
     const int nb = *count;
     // the next cell is claimed while the current one is cut (the atomic's latency is hidden)
@@ -745,14 +751,57 @@ __global__ __launch_bounds__(C::NT) void k_sl_tree(DevMesh m, const double* pv, 
         int i, j, k;
         interior_of(m, q, i, j, k);
         const bool even = ((i + j + k) % 2) == 0;
-        const int lo[3] = {cc.lo0, cc.lo1, cc.lo2};
+        int lo[3] = {cc.lo0, cc.lo1, cc.lo2};
         int span[3] = {0, 0, 0};
 #pragma unroll
         for (int c = 0; c < 8; ++c)
 #pragma unroll
             for (int a = 0; a < 3; ++a) span[a] = max(span[a], (int)((cc.offs >> (6 * c + 2 * a)) & 3ull));
+        int shift[3] = {0, 0, 0};  // of the corners' offsets when the box widens
+        bool box_ok = true;
+        if constexpr (CORR) {
+            // the twelve correction pyramids (add_correction_tets, src/VFgeom.jl:513-664): their
+            // vertices go to pool ids 8 + 4t + n, their cells (pt2index! from the cell, as
+            // transport! walks them) widen the index box
+            if (tid == 0) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    s_box[0][a] = lo[a];
+                    s_box[1][a] = lo[a] + span[a];
+                }
+            }
+            __syncthreads();
+            if (tid < 12) {
+                double tet[4][3];
+                ns_tet(m, pv, ns.uf, ns.vf, ns.wf, ns.dt, i, j, k, 5 + tid, tet);
+                int o = 0;
+#pragma unroll
+                for (int n = 0; n < 4; ++n) {
+                    S.px[8 + 4 * tid + n] = tet[n][0];
+                    S.py[8 + 4 * tid + n] = tet[n][1];
+                    S.pz[8 + 4 * tid + n] = tet[n][2];
+                    int I[3];
+                    pt2index(m, I, tet[n], i, j, k, &o);
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        s_cind[4 * tid + n][a] = I[a];
+                        atomicMin(&s_box[0][a], I[a]);
+                        atomicMax(&s_box[1][a], I[a]);
+                    }
+                }
+                if (o) atomicOr(ns.ovf, 1);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                shift[a] = lo[a] - s_box[0][a];
+                lo[a] = s_box[0][a];
+                span[a] = s_box[1][a] - s_box[0][a];
+                box_ok = box_ok && span[a] <= 3;
+            }
+        }
         const int bx = span[0] + 1, by = span[1] + 1, bz = span[2] + 1;
-        const int ncell = bx * by * bz;
+        const int ncell = box_ok ? bx * by * bz : 0;
         // ---- projected corners, mesh planes and the box cells' PLIC planes, all at once ------
         if (tid < 8) {
             const int64_t vq = vtx_id(m, i + (tid & 1), j + ((tid >> 1) & 1), k + (tid >> 2));
@@ -764,7 +813,7 @@ __global__ __launch_bounds__(C::NT) void k_sl_tree(DevMesh m, const double* pv, 
             const int a = tid / 5, off = tid - 5 * a;
             const int la = a == 0 ? lo[0] : (a == 1 ? lo[1] : lo[2]);
             const int sa = a == 0 ? span[0] : (a == 1 ? span[1] : span[2]);
-            if (off <= sa + 1) S.plane[a][off] = (a == 0 ? m.x : (a == 1 ? m.y : m.z))[la + off];
+            if (box_ok && off <= sa + 1) S.plane[a][off] = (a == 0 ? m.x : (a == 1 ? m.y : m.z))[la + off];
         }
         for (int l = tid; l < ncell; l += NT) {
             const int ox = l % bx, oy = (l / bx) % by, oz = l / (bx * by);
@@ -782,7 +831,7 @@ __global__ __launch_bounds__(C::NT) void k_sl_tree(DevMesh m, const double* pv, 
                 s_vel[l][2] = lm.w[c];
             }
         }
-        // ---- the five tets (src/VFgeom.jl:235-256) ---------------------------------------
+        // ---- the five tets (src/VFgeom.jl:235-256), then the correction tets -------------------
         if (tid < 5) {
             const unsigned tv = TET5[tid];
             unsigned ids[4], bits = 0;
@@ -790,14 +839,31 @@ __global__ __launch_bounds__(C::NT) void k_sl_tree(DevMesh m, const double* pv, 
             for (int n = 0; n < 4; ++n) {
                 const int c = vert_corner(nib(tv, n), even);
                 ids[n] = (unsigned)c;
-                bits |= (unsigned)((cc.offs >> (6 * c)) & 63ull) << (6 * n);
+                unsigned ob = 0;
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+                    ob |= (unsigned)(((cc.offs >> (6 * c + 2 * a)) & 3ull) + shift[a]) << (2 * a);
+                bits |= ob << (6 * n);
             }
             S.node[0][tid] = TreeNode{ids[0] | (ids[1] << 16), ids[2] | (ids[3] << 16), bits};
         }
+        if constexpr (CORR) {
+            if (tid >= 5 && tid < 17 && box_ok) {
+                const int t = tid - 5;
+                unsigned bits = 0;
+#pragma unroll
+                for (int n = 0; n < 4; ++n)
+#pragma unroll
+                    for (int a = 0; a < 3; ++a)
+                        bits |= (unsigned)(s_cind[4 * t + n][a] - lo[a]) << (6 * n + 2 * a);
+                const unsigned id0 = 8 + 4 * t;
+                S.node[0][tid] = TreeNode{id0 | ((id0 + 1) << 16), (id0 + 2) | ((id0 + 3) << 16), bits};
+            }
+        }
         __syncthreads();
         // ---- mesh-plane levels -------------------------------------------------------------
-        int n = 5, npool = 8, cur = 0;
-        bool ok = true;
+        int n = CORR ? 17 : 5, npool = CORR ? 56 : 8, cur = 0;
+        bool ok = box_ok;
         for (int axis = 0; axis < 3 && ok; ++axis) {
             const int minspan = axis == 2 ? 2 : 1;
             for (int rep = 0; rep < 3 && ok; ++rep) {
@@ -991,11 +1057,12 @@ __global__ __launch_bounds__(C::NT) void k_sl_tree(DevMesh m, const double* pv, 
                 }
             }
             if (tid == 0) {
-                if constexpr (MOM) {
-                    double* o = sums + 6 * (int64_t)q;
-                    o[0] = Vs; o[1] = Ls; o[2] = Ms[0]; o[3] = Ms[1]; o[4] = Ms[2]; o[5] = Ms[3];
-                } else {
-                    VF[m.vof(i, j, k)] = Ls / Vs;
+                VF[m.vof(i, j, k)] = Ls / Vs;
+                if constexpr (MOM) {  // the tetsign of the reference cancels in these ratios
+                    const int64_t c = m.cell(i, j, k);
+                    ns.us[c] = Ms[0] / Ms[3];
+                    ns.vs[c] = Ms[1] / Ms[3];
+                    ns.ws[c] = Ms[2] / Ms[3];
                 }
             }
         }
@@ -1370,17 +1437,18 @@ int launch_transport(mist_ctx* ctx, double dt) {
     LAUNCH(k_sl_classify, grid_for(ncap, 256, 8192), 256, ctx->m, (const double*)ctx->pv, (const int*)ctx->band_list,
            (const int*)(ctx->counters + 0), nx, ny, nz, D, VF, ctx->counters + 2, cut, ctx->counters + 8,
            ctx->defer_list, ctx->counters + 6, 0);
-    const LeafMomentum none{nullptr, nullptr, nullptr, 0.0, 0.0};
+    const TreeNS none{{nullptr, nullptr, nullptr, 0.0, 0.0}, nullptr, nullptr, nullptr, 0.0,
+                      nullptr, nullptr, nullptr, nullptr};
     LAUNCH((k_sl_tree<TreeMain, false>), ctx->tree_blocks, TreeMain::NT, ctx->m, (const double*)ctx->pv,
            (const CutCell*)cut, (const int*)nullptr, (const int*)(ctx->counters + 8), nx, ny, nz, D, VF,
-           ctx->iface_list, ctx->counters + 4, 0, ctx->counters + 5, none, (double*)nullptr);
+           ctx->iface_list, ctx->counters + 4, 0, ctx->counters + 5, none);
     prof_end(ctx, MIST_PROF_SL_CUT, ev, ctx->counters + 0, 0);
     // cells whose cut tree outgrows the main pass: the same walk with larger capacities; then
     // the general walk for those and for cells whose corners span more than four cells
     ev = prof_begin(ctx, MIST_PROF_SL_GENERAL);
     LAUNCH((k_sl_tree<TreeWide, false>), 32, TreeWide::NT, ctx->m, (const double*)ctx->pv, (const CutCell*)cut,
            (const int*)ctx->iface_list, (const int*)(ctx->counters + 4), nx, ny, nz, D, VF, ctx->defer_list,
-           ctx->counters + 6, 1, ctx->counters + 7, none, (double*)nullptr);
+           ctx->counters + 6, 1, ctx->counters + 7, none);
     LAUNCH(k_sl_cut, 32, 64, ctx->m, (const double*)ctx->pv, (const int*)ctx->defer_list,
            (const int*)(ctx->counters + 6), nx, ny, nz, D, VF, ctx->counters + 2);
     prof_end(ctx, MIST_PROF_SL_GENERAL, ev, ctx->counters + 6, 0);
@@ -1389,19 +1457,21 @@ int launch_transport(mist_ctx* ctx, double dt) {
     return launch_borders(ctx, MIST_FIELD_VOF);
 }
 
-// The five back-traced tets of every band cell with the six momentum sums into `sums`
-// (indexed by interior cell id): the classify / main tree / wide tree pipeline of
-// launch_transport without the bulk shortcut; cells left for the general walk are listed in
+// The momentum predictor's cut pass (src/Transport.jl:107-150): every band cell's five
+// back-traced tets, with `corr` also its twelve correction tets, through the cut trees with the
+// six momentum sums, writing VF and us, vs, ws (the classify / main tree / wide tree pipeline of
+// launch_transport without the bulk shortcut). Cells left for the general walk are listed in
 // defer_list (count counters[6]) for the caller.
-int launch_tree_momentum(mist_ctx* ctx, double* sums) {
+int launch_tree_momentum(mist_ctx* ctx, bool corr, double dt) {
     const int64_t ncap = (int64_t)ctx->m.Nx * ctx->m.Ny * ctx->m.Nz;
     LAUNCH(k_fill_int, 1, 64, ctx->counters + 4, 5, 0);
     if (!ctx->tree_blocks_mom) {
         int per = 0, dev = 0, cus = 0;
-        MIST_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_sl_tree<TreeMain, true>, TreeMain::NT, 0));
+        MIST_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_sl_tree<TreeMain, true, true>, TreeMain::NT, 0));
         MIST_HIP_TRY(hipGetDevice(&dev));
         MIST_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        ctx->tree_blocks_mom = std::max(1, per) * std::max(1, cus);
+        ctx->tree_cus = std::max(1, cus);
+        ctx->tree_blocks_mom = std::max(1, per) * ctx->tree_cus;
     }
     const double* nx = (const double*)ctx->f[MIST_FIELD_NX];
     const double* ny = (const double*)ctx->f[MIST_FIELD_NY];
@@ -1409,17 +1479,31 @@ int launch_tree_momentum(mist_ctx* ctx, double* sums) {
     const double* D = (const double*)ctx->f[MIST_FIELD_PLIC_D];
     double* VF = (double*)ctx->f[MIST_FIELD_VOF];
     CutCell* cut = (CutCell*)ctx->cut_list;
-    const LeafMomentum lm{(const double*)ctx->f[MIST_FIELD_U], (const double*)ctx->f[MIST_FIELD_V],
-                          (const double*)ctx->f[MIST_FIELD_W], ctx->p.rho_liq, ctx->p.rho_gas};
+    const TreeNS ns{{(const double*)ctx->f[MIST_FIELD_U], (const double*)ctx->f[MIST_FIELD_V],
+                     (const double*)ctx->f[MIST_FIELD_W], ctx->p.rho_liq, ctx->p.rho_gas},
+                    (const double*)ctx->f[MIST_FIELD_UF], (const double*)ctx->f[MIST_FIELD_VF],
+                    (const double*)ctx->f[MIST_FIELD_WF], dt, (double*)ctx->f[MIST_FIELD_US],
+                    (double*)ctx->f[MIST_FIELD_VS], (double*)ctx->f[MIST_FIELD_WS], ctx->counters + 2};
     LAUNCH(k_sl_classify, grid_for(ncap, 256, 8192), 256, ctx->m, (const double*)ctx->pv, (const int*)ctx->band_list,
            (const int*)(ctx->counters + 0), nx, ny, nz, D, VF, ctx->counters + 2, cut, ctx->counters + 8,
            ctx->defer_list, ctx->counters + 6, 1);
-    LAUNCH((k_sl_tree<TreeMain, true>), ctx->tree_blocks_mom, TreeMain::NT, ctx->m, (const double*)ctx->pv,
-           (const CutCell*)cut, (const int*)nullptr, (const int*)(ctx->counters + 8), nx, ny, nz, D, VF,
-           ctx->iface_list, ctx->counters + 4, 0, ctx->counters + 5, lm, sums);
-    LAUNCH((k_sl_tree<TreeWide, true>), 32, TreeWide::NT, ctx->m, (const double*)ctx->pv, (const CutCell*)cut,
-           (const int*)ctx->iface_list, (const int*)(ctx->counters + 4), nx, ny, nz, D, VF, ctx->defer_list,
-           ctx->counters + 6, 1, ctx->counters + 7, lm, sums);
+    // the wide pass gets one block per compute unit: without the bulk shortcut, and with the
+    // correction tets, more cells outgrow the main pass
+    if (corr) {
+        LAUNCH((k_sl_tree<TreeMain, true, true>), ctx->tree_blocks_mom, TreeMain::NT, ctx->m, (const double*)ctx->pv,
+               (const CutCell*)cut, (const int*)nullptr, (const int*)(ctx->counters + 8), nx, ny, nz, D, VF,
+               ctx->iface_list, ctx->counters + 4, 0, ctx->counters + 5, ns);
+        LAUNCH((k_sl_tree<TreeWide, true, true>), ctx->tree_cus, TreeWide::NT, ctx->m, (const double*)ctx->pv,
+               (const CutCell*)cut, (const int*)ctx->iface_list, (const int*)(ctx->counters + 4), nx, ny, nz, D, VF,
+               ctx->defer_list, ctx->counters + 6, 1, ctx->counters + 7, ns);
+    } else {
+        LAUNCH((k_sl_tree<TreeMain, true, false>), ctx->tree_blocks_mom, TreeMain::NT, ctx->m, (const double*)ctx->pv,
+               (const CutCell*)cut, (const int*)nullptr, (const int*)(ctx->counters + 8), nx, ny, nz, D, VF,
+               ctx->iface_list, ctx->counters + 4, 0, ctx->counters + 5, ns);
+        LAUNCH((k_sl_tree<TreeWide, true, false>), ctx->tree_cus, TreeWide::NT, ctx->m, (const double*)ctx->pv,
+               (const CutCell*)cut, (const int*)ctx->iface_list, (const int*)(ctx->counters + 4), nx, ny, nz, D, VF,
+               ctx->defer_list, ctx->counters + 6, 1, ctx->counters + 7, ns);
+    }
     return MIST_OK;
 }
 
