@@ -69,7 +69,6 @@ struct MGState {
     double* c64[3] = {nullptr, nullptr, nullptr};  // fine FP64 operator
     double* p64 = nullptr;
     double* b64 = nullptr;
-    double* r64 = nullptr;
     double* part = nullptr;
     double* h_part = nullptr;  // pinned
     int nblocks = 0;
@@ -306,18 +305,6 @@ __global__ void k_to_level(LevelGeom g, const double* a, T* b) {
     }
 }
 
-// b_T = (T) r64, e_T = 0 (r64 in plain order, b and e in the storage of g)
-template <class T>
-__global__ void k_start_cycle(LevelGeom g, const double* r64, T* b, T* e) {
-    const int64_t n = g.size();
-    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
-        int i, j, k;
-        lcell(g, q, i, j, k);
-        b[q] = (T)r64[i + (int64_t)g.n[0] * (j + (int64_t)g.n[1] * k)];
-        e[q] = T(0);
-    }
-}
-
 // p64 += e (p64 in plain order, e in the storage of g)
 template <class T>
 __global__ void k_add_correction(LevelGeom g, const T* e, double* p) {
@@ -354,6 +341,37 @@ __device__ double mg_block_sum(double v, bool maxabs) {
     const double r = s[0];
     __syncthreads();
     return r;
+}
+
+// The outer loop's FP64 residual r = b - A p (plain order, flux form) handed straight to the
+// V-cycle of precision T: bT = (T) r and eT = 0 in the storage of gT, max |r| of the block into
+// part[blockIdx.x]; r itself is not stored.
+template <class T>
+__global__ __launch_bounds__(MG_RED) void k_outer_resid(LevelGeom g, LevelGeom gT, const double* cx, const double* cy,
+                                                       const double* cz, const double* p, const double* b, T* bT,
+                                                       T* eT, double* part) {
+    const double* C[3] = {cx, cy, cz};
+    const int64_t tot = g.size();
+    double vmax = 0.0;
+    for (int64_t cell = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; cell < tot;
+         cell += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(cell % g.n[0]);
+        const int64_t row = cell / g.n[0];
+        const int j = (int)(row % g.n[1]), k = (int)(row / g.n[1]);
+        int64_t nb[6];
+        double cf[6];
+        stencil<double, false>(g, const_cast<double* const*>(C), i, j, k, cell, nb, cf);
+        const double pc = p[cell];
+        const double Ap = ((((cf[0] * (p[nb[0]] - pc) + cf[1] * (p[nb[1]] - pc)) + cf[2] * (p[nb[2]] - pc)) +
+                            cf[3] * (p[nb[3]] - pc)) + cf[4] * (p[nb[4]] - pc)) + cf[5] * (p[nb[5]] - pc);
+        const double r = b[cell] - Ap;
+        const int64_t qs = lidx(gT, i, j, k);
+        bT[qs] = (T)r;
+        eT[qs] = T(0);
+        vmax = fmax(vmax, fabs(r));
+    }
+    const double m = mg_block_sum(vmax, true);
+    if (threadIdx.x == 0) part[blockIdx.x] = m;
 }
 
 template <class T>
@@ -731,12 +749,12 @@ static int build_state(mist_ctx* ctx) {
     }
     s->nfine = g0.size();
     const size_t fb = (size_t)s->nfine * sizeof(double);
-    double** f64[6] = {&s->c64[0], &s->c64[1], &s->c64[2], &s->p64, &s->b64, &s->r64};
+    double** f64[5] = {&s->c64[0], &s->c64[1], &s->c64[2], &s->p64, &s->b64};
     for (double** q : f64) {
         if (hipMalloc((void**)q, fb) != hipSuccess) { mg_free(s); ctx->err = "multigrid: device allocation failed"; return MIST_ERR_OOM; }
         s->allocs.push_back(*q);
     }
-    s->nblocks = 1024;
+    s->nblocks = 4096;
     if (hipMalloc((void**)&s->part, (s->nblocks + 8) * sizeof(double)) != hipSuccess) { mg_free(s); ctx->err = "multigrid: device allocation failed"; return MIST_ERR_OOM; }
     s->allocs.push_back(s->part);
     if (hipHostMalloc((void**)&s->h_part, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess) { mg_free(s); ctx->err = "multigrid: pinned allocation failed"; return MIST_ERR_OOM; }
@@ -811,28 +829,33 @@ static int setup_hierarchy(mist_ctx* ctx, std::vector<LevelBuf<T>>& L) {
     return MIST_OK;
 }
 
-// FP64 residual r = b - A p of the outer loop and max|r| into s->part[nblocks]
-static int outer_residual(mist_ctx* ctx) {
+// FP64 residual of the outer loop handed to the V-cycle of precision T (b of level 0, e = 0)
+// and max|r| into s->part[nblocks]
+template <class T>
+static int outer_residual(mist_ctx* ctx, std::vector<LevelBuf<T>>& L) {
     MGState* s = ctx->mg;
     const int64_t n = s->nfine;
+    const int g = std::min<int>(s->nblocks, gridn(n, MG_RED));
     hipEvent_t ev = prof_begin(ctx, MIST_PROF_MG_RESID);
-    MGL(k_resid<double>, gridn(n), 256, s->g[0], s->c64[0], s->c64[1], s->c64[2], (const double*)s->p64,
-        (const double*)s->b64, s->r64);
+    MGL(k_outer_resid<T>, g, MG_RED, s->g[0], s->gq[sizeof(T) == sizeof(double) ? 0 : 1][0], (const double*)s->c64[0],
+        (const double*)s->c64[1], (const double*)s->c64[2], (const double*)s->p64, (const double*)s->b64, L[0].b,
+        L[0].e, s->part);
     prof_end(ctx, MIST_PROF_MG_RESID, ev, nullptr, n);
-    return dev_reduce<double>(ctx, s->r64, n, 1);
+    MGL(k_sum_final, 1, MG_RED, (const double*)s->part, g, 1, s->part + s->nblocks);
+    return MIST_OK;
 }
 
-// One defect-correction iteration: e = V-cycle(r) in precision T, p += e, new residual.
+// One defect-correction iteration: e = V-cycle(r) in precision T, p += e, the new residual
+// handed to the next V-cycle.
 template <class T>
 static int outer_iteration(mist_ctx* ctx, std::vector<LevelBuf<T>>& L) {
     MGState* s = ctx->mg;
     const int64_t n = s->nfine;
     const LevelGeom& g0 = s->gq[sizeof(T) == sizeof(double) ? 0 : 1][0];
-    MGL(k_start_cycle<T>, gridn(n), 256, g0, (const double*)s->r64, L[0].b, L[0].e);
     int st = vcycle(ctx, L, 0);
     if (st) return st;
     MGL(k_add_correction<T>, gridn(n), 256, g0, (const T*)L[0].e, s->p64);
-    return outer_residual(ctx);
+    return outer_residual(ctx, L);
 }
 
 template <class T>
@@ -845,7 +868,7 @@ static int solve_t(mist_ctx* ctx, std::vector<LevelBuf<T>>& L, double tol, int64
     int64_t cyc = 0;
     double res = 0.0;
     int status = MIST_OK;
-    if ((st = outer_residual(ctx))) return st;
+    if ((st = outer_residual(ctx, L))) return st;
     for (;;) {
         MIST_HIP_TRY(hipMemcpyAsync(s->h_part, s->part + s->nblocks, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         MIST_HIP_TRY(hipStreamSynchronize(ctx->stream));
