@@ -1003,12 +1003,13 @@ int launch_transport_vjp(mist_ctx* ctx, double dt) {
         MIST_HIP_TRY(hipGetDevice(&dev));
         MIST_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         ctx->tree_blocks_adj = std::max(1, per) * std::max(1, cus);
+        if (!ctx->tree_cus) ctx->tree_cus = std::max(1, cus);
     }
     const CutCell* cut = (const CutCell*)ctx->cut_list;
     ADJL(k_adj_tree<TreeMain>, ctx->tree_blocks_adj, TreeMain::NT, m, (const double*)ctx->pv, cut,
          (const int*)nullptr, (const int*)(ctx->counters + 8), nx, ny, nz, D, (const double*)lam, ctx->adj_pv, gpl[0],
          gpl[1], gpl[2], gpl[3], ctx->iface_list, ctx->counters + 4, 0, ctx->counters + 5);
-    ADJL(k_adj_tree<TreeWide>, 32, TreeWide::NT, m, (const double*)ctx->pv, cut,
+    ADJL(k_adj_tree<TreeWide>, ctx->tree_cus, TreeWide::NT, m, (const double*)ctx->pv, cut,
          (const int*)ctx->iface_list, (const int*)(ctx->counters + 4), nx, ny, nz, D, (const double*)lam, ctx->adj_pv,
          gpl[0], gpl[1], gpl[2], gpl[3], ctx->defer_list, ctx->counters + 6, 1, ctx->counters + 7);
     ADJL(k_adj_gen_sums, 32, 64, m, (const double*)ctx->pv, (const int*)ctx->defer_list,

@@ -1443,10 +1443,11 @@ int launch_transport(mist_ctx* ctx, double dt) {
            (const CutCell*)cut, (const int*)nullptr, (const int*)(ctx->counters + 8), nx, ny, nz, D, VF,
            ctx->iface_list, ctx->counters + 4, 0, ctx->counters + 5, none);
     prof_end(ctx, MIST_PROF_SL_CUT, ev, ctx->counters + 0, 0);
-    // cells whose cut tree outgrows the main pass: the same walk with larger capacities; then
-    // the general walk for those and for cells whose corners span more than four cells
+    // cells whose cut tree outgrows the main pass: the same walk with larger capacities, one
+    // block per compute unit; then the general walk for those and for cells whose corners span
+    // more than four cells
     ev = prof_begin(ctx, MIST_PROF_SL_GENERAL);
-    LAUNCH((k_sl_tree<TreeWide, false>), 32, TreeWide::NT, ctx->m, (const double*)ctx->pv, (const CutCell*)cut,
+    LAUNCH((k_sl_tree<TreeWide, false>), ctx->tree_cus, TreeWide::NT, ctx->m, (const double*)ctx->pv, (const CutCell*)cut,
            (const int*)ctx->iface_list, (const int*)(ctx->counters + 4), nx, ny, nz, D, VF, ctx->defer_list,
            ctx->counters + 6, 1, ctx->counters + 7, none);
     LAUNCH(k_sl_cut, 32, 64, ctx->m, (const double*)ctx->pv, (const int*)ctx->defer_list,
