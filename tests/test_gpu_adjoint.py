@@ -124,3 +124,53 @@ def test_gradient_matches_oracle_differences(mist, oracle):
     print(f"adjoint vs oracle FD: {int((rel <= 1e-6).sum())}/16 within 1e-6, max rel {rel.max():.2e}")
     assert int((rel <= 1e-6).sum()) >= 12
     assert np.linalg.norm(g - fd) <= 1e-3 * np.linalg.norm(fd)
+
+
+def test_one_step_vjp_stretched_preimages(mist):
+    """The reverse step where pre-images stretch over more than four cells (3D, CFL 4): the
+    reverse tree passes and the depth-first reverse walk run; a velocity direction against
+    central differences of the library's step."""
+    from test_oracle_pins import cube_param
+    from test_gpu_parity import _sheared_faces
+    ctx = mist.Context(cube_param(16, CFL=4.0, solveNS=False))
+    m = ctx.mesh
+    ctx.init_vof("sphere", 0.3, 0.47, 0.52, 0.5)
+    ctx.vof_init()
+    vf0 = ctx.get("VF")
+    base = _sheared_faces(m)
+    pi = np.pi
+    X, Y, Z = np.meshgrid(m.x[:-1], m.ym, m.zm, indexing="ij")
+    d_u = np.sin(pi * X) ** 2 * np.cos(2 * pi * Y) * np.sin(2 * pi * Z)
+    X, Y, Z = np.meshgrid(m.xm, m.y[:-1], m.zm, indexing="ij")
+    d_v = np.sin(pi * Y) ** 2 * np.cos(2 * pi * Z) * np.sin(3 * pi * X)
+    X, Y, Z = np.meshgrid(m.xm, m.ym, m.z[:-1], indexing="ij")
+    d_w = np.sin(pi * Z) ** 2 * np.sin(2 * pi * X) * np.cos(3 * pi * Y)
+    dirs = (d_u, d_v, d_w)
+    dt = 0.15
+
+    def start(h):
+        ctx.set("VF", vf0)
+        ctx.vof_init()
+        for name, b, d in zip(("uf", "vf", "wf"), base, dirs):
+            ctx.set(name, b + h * d)
+
+    def step(h):
+        start(h)
+        ctx.transport(dt)
+        ctx.compute_band()
+        return ctx.get("VF")
+
+    VF1 = step(0.0)
+    lam = np.zeros_like(VF1)
+    lam[m.nghost + 3:m.nghost + 3 + m.Nx, m.nghost + 3:m.nghost + 3 + m.Ny, m.nghost + 3:m.nghost + 3 + m.Nz] = \
+        np.random.default_rng(11).standard_normal((m.Nx, m.Ny, m.Nz))
+    start(0.0)
+    ctx.set("VF_bar", lam)
+    for name in ("uf_bar", "vf_bar", "wf_bar"):
+        ctx.set(name, np.zeros(ctx.get(name).shape))
+    ctx.transport_vjp(dt)
+    g = sum(float((ctx.get(n) * d).sum()) for n, d in zip(("uf_bar", "vf_bar", "wf_bar"), dirs))
+    h = 1e-7
+    fd = (float((lam * step(h)).sum()) - float((lam * step(-h)).sum())) / (2 * h)
+    print(f"stretched one-step VJP {g:.10e} vs central difference {fd:.10e}")
+    assert g == pytest.approx(fd, rel=1e-5)

@@ -129,11 +129,13 @@ def test_deformation_steps(mist, oracle, case):
     _vof_state_equal(g, o, g.mesh)
 
 
-def test_transport_planar_uniform_flow_exact(mist):
-    """One SL step of a planar interface in uniform flow equals the exact shifted plane."""
+@pytest.mark.parametrize("solve_ns", [False, True])
+def test_transport_planar_uniform_flow_exact(mist, solve_ns):
+    """One SL step of a planar interface in uniform flow equals the exact shifted plane (the
+    VOF transport, and the momentum predictor's VF with its correction tets)."""
     from test_oracle_pins import cube_param, planar_vf
     N = 12
-    ctx = mist.Context(cube_param(N, CFL=0.5))
+    ctx = mist.Context(cube_param(N, CFL=0.5, solveNS=solve_ns))
     m = ctx.mesh
     n = np.array((-0.5, 1.0, 0.75))
     D0 = float(n @ np.array([0.5, 0.5, 0.5]))
@@ -239,3 +241,42 @@ def test_error_paths(mist):
     h = ctypes.c_void_p()
     st = mist.lib().mist_ctx_create(ctypes.byref(bad), 0, ctypes.byref(h))
     assert st == 1 and b"tesselation" in mist.lib().mist_last_error(None)
+
+
+def _sheared_faces(mesh, amp=1.5):
+    """Face velocities of a strongly sheared field with no flow through the walls and no
+    symmetric zeros at grid vertices (phase-shifted): at dt = 0.15 on 16^3 the corners move up to
+    about 3.3 cells and hundreds of pre-images spread over more than 4 cells along an axis (so
+    the wide tree pass and the general walk both run)."""
+    pi = np.pi
+    X, Y, Z = np.meshgrid(mesh.x[:-1], mesh.ym, mesh.zm, indexing="ij")
+    uf = amp * np.sin(pi * X) ** 2 * np.sin(4 * pi * Y + 0.3) * np.sin(4 * pi * Z + 1.1)
+    X, Y, Z = np.meshgrid(mesh.xm, mesh.y[:-1], mesh.zm, indexing="ij")
+    vf = amp * np.sin(pi * Y) ** 2 * np.sin(4 * pi * Z + 0.7) * np.sin(4 * pi * X + 1.9)
+    X, Y, Z = np.meshgrid(mesh.xm, mesh.ym, mesh.z[:-1], indexing="ij")
+    wf = amp * np.sin(pi * Z) ** 2 * np.sin(4 * pi * X + 2.3) * np.sin(4 * pi * Y + 0.5)
+    return uf, vf, wf
+
+
+def test_transport_stretched_preimages_match_oracle(mist, oracle):
+    """One transport step whose pre-images stretch over more than four cells (CFL 4: nband 4,
+    nghost 5): cells outgrow the first tree pass and take the wide pass and the general walk;
+    VF, band, normals and PLIC constants still match the oracle."""
+    from test_oracle_pins import cube_param
+    param = cube_param(16, CFL=4.0, solveNS=False)
+    g, o = _pair(mist, oracle, param, ("sphere", 0.3, 0.47, 0.52, 0.5))
+    g.vof_init()
+    o.vof_init()
+    uf, vf, wf = _sheared_faces(g.mesh)
+    for c in (g, o):
+        c.set("uf", uf)
+        c.set("vf", vf)
+        c.set("wf", wf)
+    g.profile(True)
+    dt = 0.15
+    g.transport(dt)
+    o.transport(dt)
+    assert g.profile_read()["sl_general"][2] > 0  # cells were left to the general walk
+    g.compute_band()
+    o.compute_band()
+    _vof_state_equal(g, o, g.mesh)
