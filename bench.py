@@ -81,7 +81,7 @@ def counted_flops(N):
 # scaled by 2.0 and WRITE_SIZE by 1.0 (what the counters read for a plain 256 MiB device copy);
 # see profiles/pmc_traffic_deformation3d_128.md. k_elvira's figure is almost all WRITE_SIZE: its
 # spilled registers (scratch stores), not field traffic.
-TRAFFIC_PER_UNIT = {"k_elvira": 32992.0, "k_sl_tree": 859.0}
+TRAFFIC_PER_UNIT = {"k_elvira": 33014.0, "k_sl_tree": 863.0}
 
 
 def run_vof(args, torch):
