@@ -280,3 +280,37 @@ def test_transport_stretched_preimages_match_oracle(mist, oracle):
     g.compute_band()
     o.compute_band()
     _vof_state_equal(g, o, g.mesh)
+
+
+def test_deformation_anisotropic_periodic_match_oracle(mist, oracle):
+    """Steps of the 3D deformation on non-cubic cells (16 x 16 x 20 cells over 1 x 1.25 x 0.75:
+    dx, dy, dz = 0.0625, 0.078, 0.0375) with the x axis periodic and CFL 0.8: VF, band, normals
+    and PLIC constants match the oracle after every step."""
+    param, _ = W.deformation3d(16, Nz=20, Ly=1.25, Lz=0.75, xper=True, CFL=0.8, max_dt=0.02)
+    g, o = _pair(mist, oracle, param, ("sphere", 0.2, 0.43, 0.61, 0.37))
+    g.vof_init()
+    o.vof_init()
+    tg = to = 0.0
+    for _ in range(6):
+        tg, dtg = g.vof_step(tg)
+        to, dto = o.vof_step(to)
+        assert dtg == dto and tg == to
+        _vof_state_equal(g, o, g.mesh)
+
+
+@pytest.mark.parametrize("fill", [0.0, 1.0])
+def test_transport_without_interface(mist, oracle, fill):
+    """An all-gas or all-liquid domain: the band, interface and cut lists are empty, the kernels
+    launch over nothing and VF stays exactly as it was."""
+    param, _ = W.deformation3d(12)
+    g = mist.Context(param)
+    o = oracle.OracleContext(param)
+    VF = np.full(g.get("VF").shape, fill)
+    for c in (g, o):
+        c.set("VF", VF)
+        c.vof_init()
+    t = 0.0
+    for _ in range(3):
+        t, _ = g.vof_step(t)
+    assert np.array_equal(g.get("VF"), VF)
+    assert np.array_equal(g.get("band"), o.get("band"))
