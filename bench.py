@@ -104,6 +104,10 @@ def run_vof(args, torch):
     ctx.vof_init()
     ctx.profile(True)
     V0 = ctx.reduce("sum", "VF")
+    m = ctx.mesh
+    o = m.nghost + 3
+    inner = (slice(o, o + m.Nx), slice(o, o + m.Ny), slice(o, o + m.Nz))
+    VF0 = ctx.get("VF")[inner].copy()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     t = 0.0
@@ -113,7 +117,10 @@ def run_vof(args, torch):
     elapsed = time.perf_counter() - t0
     prof = ctx.profile_read()
     V1 = ctx.reduce("sum", "VF")
-    return ctx, param, steps, warm, elapsed, prof, (V0, V1, t)
+    # shape error sum |VF(t) - VF(0)| / N^3 (3D form of test/deformation_error.jl:22); the
+    # deformation returns the sphere at t = T
+    shape_err = float(abs(ctx.get("VF")[inner] - VF0).sum()) / (m.Nx * m.Ny * m.Nz)
+    return ctx, param, steps, warm, elapsed, prof, (V0, V1, t, shape_err)
 
 
 def cpu_baseline(args, param, ic):
@@ -143,13 +150,44 @@ def cpu_baseline(args, param, ic):
         o.vof_step(t, param.tFinal)
         cpu_time += time.perf_counter() - t0
     log(f"  cpu: {len(samples)} oracle steps in {cpu_time:.1f} s")
+    # the same restatement on one core: two steps from the last sampled state
+    pyoracle.set_threads(1)
+    one_time = 0.0
+    for _ in range(2):
+        for name in ("VF", "band", "u", "v", "w"):
+            o.set(name, g.get(name))
+        t0 = time.perf_counter()
+        o.vof_step(t, param.tFinal)
+        one_time += time.perf_counter() - t0
+    pyoracle.set_threads(threads)
+    log(f"  cpu: 2 oracle steps on one core in {one_time:.1f} s")
     g.close()
     cells = param.Nx * param.Ny * param.Nz
     return {"value": cells * len(samples) / cpu_time, "unit": "cell-updates/s", "cores": threads,
             "kind": "port",
             "sample": f"{len(samples)} oracle steps (C++ FP64 restatement, OpenMP), each from the device "
                       f"state at steps 0, {samples[1] if len(samples) > 1 else 0}, ... of the {param.Nx}^3 "
-                      f"deformation period, {cpu_time:.1f} s of CPU time"}
+                      f"deformation period, {cpu_time:.1f} s of CPU time",
+            "single_core": {"value": cells * 2 / one_time, "unit": "cell-updates/s", "cores": 1,
+                            "sample": f"2 oracle steps on one thread from the device state at step {samples[-1]}, "
+                                      f"{one_time:.1f} s"}}
+
+
+def copy_bandwidth(torch, nbytes, reps=20):
+    """Device-to-device copy rate in GB/s (bytes read + written) of an nbytes buffer."""
+    a = torch.empty(nbytes // 4, device="cuda", dtype=torch.float32).fill_(1.0)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    sec = e0.elapsed_time(e1) / 1e3 / reps
+    del a, b
+    return 2.0 * nbytes / sec / 1e9
 
 
 def run_mg(args, torch):
@@ -422,7 +460,7 @@ def main():
     from ns_multiphase_amd import workloads as W
 
     log(f"[bench] SL-VOF deformation3d N={args.N}")
-    ctx, param, steps, warm, elapsed, prof, (V0, V1, tend) = run_vof(args, torch)
+    ctx, param, steps, warm, elapsed, prof, (V0, V1, tend, shape_err) = run_vof(args, torch)
     cells = param.Nx * param.Ny * param.Nz
     value = cells * steps / elapsed
     log(f"[bench] {steps} steps in {elapsed:.3f} s -> {value:.3e} cell-updates/s (t_end={tend:.4f})")
@@ -467,6 +505,14 @@ def main():
                                       "bytes_per_launch": bytes_launch}
             mg[prec] = entry
         mg["speedup_mixed_vs_fp64"] = mg["fp64"]["solve_ms"] / mg["mixed"]["solve_ms"]
+        # measured copy bandwidths beside the HBM peak: 2 x 64 MiB stays in the 256 MB Infinity
+        # Cache (where the 128^3 levels live), 2 x 2 GiB does not
+        mg["copy_GBs"] = {"cache_resident_2x64MiB": copy_bandwidth(torch, 64 << 20),
+                          "hbm_2x2GiB": copy_bandwidth(torch, 2 << 30)}
+        for prec in ("fp64", "mixed"):
+            if "rbgs_fine" in mg[prec]:
+                rf = mg[prec]["rbgs_fine"]
+                rf["frac_cache_copy"] = rf["achieved_GBs"] / mg["copy_GBs"]["cache_resident_2x64MiB"]
         log(f"[bench] mg fp64 {mg['fp64']['solve_ms']:.2f} ms, mixed {mg['mixed']['solve_ms']:.2f} ms")
 
     bubble = None
@@ -503,6 +549,7 @@ def main():
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"3D deformation {args.N}^3, sphere r=0.15, T=3, CFL 0.5, dt=1/512, RK4, 5 tets",
                    "steps_timed_from_t0": steps, "volume_rel_change": (V1 - V0) / V0,
+                   "t_end": tend, "shape_error": shape_err,
                    "mg_workload": f"variable-density MMS {args.mg_N}^3, rho 1000/1, tol 1e-10"},
         "roofline": roof, "cpu_baseline": cpu, "mg": mg, "rising_bubble": bubble, "adjoint": adjoint,
         "large": large, "kernels": kernels,
