@@ -706,7 +706,7 @@ __global__ __launch_bounds__(C::NT) void k_adj_tree(DevMesh m, const double* pv,
                         if (cs == 0) on_liq(tv);
                         else on_gas(tv);
                     } else {
-                        cut_dispatch<true>(cs, T.Q, d, on_liq, on_gas);
+                        cut_canonical(cs, T.Q, d, on_liq, on_gas);
                     }
                 }
 #pragma unroll

@@ -267,7 +267,7 @@ struct IC {
     static constexpr int value = V;
 };
 template <int B, int E, class F>
-__device__ __forceinline__ void sfor(F&& f) {
+__host__ __device__ __forceinline__ void sfor(F&& f) {
     if constexpr (B < E) {
         f(IC<B>{});
         sfor<B + 1, E>(f);
@@ -300,7 +300,7 @@ __host__ __device__ constexpr int vtet_of(int c, int n, int k) {
 // sub-tets in the reference's order (src/VFgeom.jl:784-827): liquid tets from ntets down
 // to nntet, then gas tets 1..nntet-1. on_liq(tv) / on_gas(tv) receive each sub-tet volume.
 template <int CS, bool GAS, class FL, class FG>
-__device__ __forceinline__ void cut_case(const V3 (&P)[4], const double (&dm)[4], FL&& on_liq, FG&& on_gas) {
+__host__ __device__ __forceinline__ void cut_case(const V3 (&P)[4], const double (&dm)[4], FL&& on_liq, FG&& on_gas) {
     constexpr int NV = CutTab::nvert[CS], NT = CutTab::ntets[CS], NN = CutTab::nntet[CS];
     V3 Q[8];
     Q[0] = P[0]; Q[1] = P[1]; Q[2] = P[2]; Q[3] = P[3];
@@ -325,8 +325,8 @@ __device__ __forceinline__ void cut_case(const V3 (&P)[4], const double (&dm)[4]
 }
 
 template <bool GAS, class FL, class FG>
-__device__ __forceinline__ void cut_dispatch(int cs, const V3 (&P)[4], const double (&dm)[4], FL&& on_liq,
-                                             FG&& on_gas) {
+__host__ __device__ __forceinline__ void cut_dispatch(int cs, const V3 (&P)[4], const double (&dm)[4], FL&& on_liq,
+                                                      FG&& on_gas) {
     switch (cs) {
 #define MIST_CASE(k) \
     case k: cut_case<k, GAS>(P, dm, on_liq, on_gas); break;
@@ -335,6 +335,85 @@ __device__ __forceinline__ void cut_dispatch(int cs, const V3 (&P)[4], const dou
 #undef MIST_CASE
         default: break;
     }
+}
+
+// The liquid and gas volumes of a leaf cut by its PLIC plane (cut_dispatch<true> summed per
+// side) through two case bodies instead of fourteen, so the lanes of a wave that cut leaves of
+// different cases run at most two bodies. Every case with three cut vertices is the reference's
+// case 1 (vertex 1 alone above the plane) on the vertices renumbered with the lone vertex first;
+// every case with four is case 3 (vertices 1 and 2 above) with the two above first. The pieces
+// are the reference's, so the volumes agree with cut_dispatch to rounding (the sub-tets' vertex
+// order changes); a renumbering of odd parity flips the sign of every piece, and when the lone
+// vertex is below the plane its piece is the liquid one.
+// Per case: renumbering (4 nibbles, the order the switch of cut_canonical copies the vertices
+// in), parity (bit 16), lone vertex below (bit 17), four cut vertices (bit 18).
+__host__ __device__ __forceinline__ unsigned canon_case(int cs) {
+    switch (cs) {
+        case 1: return 0x3210u;
+        case 2: return 0x3201u | (1u << 16);
+        case 4: return 0x3102u;
+        case 8: return 0x2103u | (1u << 16);
+        case 14: return 0x3210u | (1u << 17);
+        case 13: return 0x3201u | (1u << 16) | (1u << 17);
+        case 11: return 0x3102u | (1u << 17);
+        case 7: return 0x2103u | (1u << 16) | (1u << 17);
+        case 3: return 0x3210u | (1u << 18);
+        case 5: return 0x3120u | (1u << 16) | (1u << 18);
+        case 6: return 0x3021u | (1u << 18);
+        case 9: return 0x2130u | (1u << 18);
+        case 10: return 0x2031u | (1u << 16) | (1u << 18);
+        case 12: return 0x1032u | (1u << 18);
+        default: return 0x3210u;
+    }
+}
+
+template <class FL, class FG>
+__host__ __device__ __forceinline__ void cut_canonical(int cs, const V3 (&P)[4], const double (&dm)[4], FL&& on_liq,
+                                                       FG&& on_gas) {
+    if (cs == 0 || cs == 15) {  // one piece, the leaf itself
+        const double tv = tet_vol3(P[0], P[1], P[2], P[3]);
+        on_liq(cs == 0 ? tv : 0.0);
+        on_gas(cs == 0 ? 0.0 : tv);
+        return;
+    }
+    const unsigned c = canon_case(cs);
+    V3 Q[4];
+    double e[4];
+    // the renumbering as register copies per case (an index taken from the table would make the
+    // compiler load the vertices from a private array in scratch memory)
+    auto perm = [&](auto A, auto B, auto C, auto D) {
+        constexpr int a = decltype(A)::value, b = decltype(B)::value, cc = decltype(C)::value,
+                      d = decltype(D)::value;
+        Q[0] = P[a]; Q[1] = P[b]; Q[2] = P[cc]; Q[3] = P[d];
+        e[0] = dm[a]; e[1] = dm[b]; e[2] = dm[cc]; e[3] = dm[d];
+    };
+    switch (cs) {
+        case 2: case 13: perm(IC<1>{}, IC<0>{}, IC<2>{}, IC<3>{}); break;
+        case 4: case 11: perm(IC<2>{}, IC<0>{}, IC<1>{}, IC<3>{}); break;
+        case 8: case 7: perm(IC<3>{}, IC<0>{}, IC<1>{}, IC<2>{}); break;
+        case 5: perm(IC<0>{}, IC<2>{}, IC<1>{}, IC<3>{}); break;
+        case 6: perm(IC<1>{}, IC<2>{}, IC<0>{}, IC<3>{}); break;
+        case 9: perm(IC<0>{}, IC<3>{}, IC<1>{}, IC<2>{}); break;
+        case 10: perm(IC<1>{}, IC<3>{}, IC<0>{}, IC<2>{}); break;
+        case 12: perm(IC<2>{}, IC<3>{}, IC<0>{}, IC<1>{}); break;
+        default: perm(IC<0>{}, IC<1>{}, IC<2>{}, IC<3>{}); break;
+    }
+    double L = 0.0, G = 0.0;
+    auto lq = [&](double v) { L += v; };
+    auto gs = [&](double v) { G += v; };
+    if (c & (1u << 18)) cut_case<3, true>(Q, e, lq, gs);
+    else cut_case<1, true>(Q, e, lq, gs);
+    if (c & (1u << 17)) {
+        const double t = L;
+        L = G;
+        G = t;
+    }
+    if (c & (1u << 16)) {
+        L = -L;
+        G = -G;
+    }
+    on_liq(L);
+    on_gas(G);
 }
 
 // computePLIC2VF (src/VF.jl:219-278) of the cell [x0,x1]x[y0,y1]x[z0,z1] for the plane

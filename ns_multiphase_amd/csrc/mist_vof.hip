@@ -1016,7 +1016,7 @@ __global__ __launch_bounds__(C::NT) void k_sl_tree(DevMesh m, const double* pv, 
                     if (cs == 0) on_liq(tv);
                     else on_gas(tv);
                 } else {
-                    cut_dispatch<true>(cs, Q, d, on_liq, on_gas);
+                    cut_canonical(cs, Q, d, on_liq, on_gas);
                 }
             }
             // ---- block sums in a fixed order ----------------------------------------------------

@@ -1,7 +1,9 @@
 // adjoint_harness.hip -- TEST INFRASTRUCTURE: host entry points over the library's adjoint
 // pieces (ns_multiphase_amd/csrc/mist_adjoint.h) and the forward functions they reverse,
 // so the tests can compare each adjoint with central differences of its forward function on
-// the CPU. Built by tests/test_adjoint_pieces.py with hipcc; never part of the library.
+// the CPU; and over the two forms of the leaf cut (the fourteen case bodies of cut_dispatch and
+// the two of cut_canonical) so the tests can compare them. Built by tests/test_adjoint_pieces.py
+// with hipcc; never part of the library.
 #include <cmath>
 #include <vector>
 
@@ -95,6 +97,23 @@ void h_leaf_adj(const double* x, const double* n, double D, double gL, double gV
         for (int q = 0; q < 3; ++q) gx[3 * v + q] = g4[v][q];
     for (int q = 0; q < 3; ++q) gn[q] = gnn[q];
     *gD = gd;
+}
+
+// liquid and total volume of tet x cut by n.x = D: out[0..1] by cut_dispatch, out[2..3] by
+// cut_canonical, out[4] the snapped case
+void h_cut_compare(const double* x, const double* n, double D, double* out) {
+    V3 P[4];
+    double d[4], dd[4];
+    for (int v = 0; v < 4; ++v) {
+        P[v] = V3{x[3 * v], x[3 * v + 1], x[3 * v + 2]};
+        d[v] = ((n[0] * P[v].x + n[1] * P[v].y) + n[2] * P[v].z) - D;
+        dd[v] = d[v];
+    }
+    const int cs = d2case_snap(dd);
+    double L = 0.0, G = 0.0, Lc = 0.0, Gc = 0.0;
+    cut_dispatch<true>(cs, P, d, [&](double v) { L += v; }, [&](double v) { G += v; });
+    cut_canonical(cs, P, d, [&](double v) { Lc += v; }, [&](double v) { Gc += v; });
+    out[0] = L; out[1] = L + G; out[2] = Lc; out[3] = Lc + Gc; out[4] = cs;
 }
 
 double h_alpha(double a, double b, double c, double VF) { return compute_alpha(a, b, c, VF); }

@@ -43,6 +43,7 @@ def H():
     L.h_project_adj.argtypes = [dp, dp, dp, dp] + [ctypes.c_int] * 3 + [ctypes.c_double, dp, dp, dp, dp]
     L.h_elvira_cand.argtypes = [dp] + [ctypes.c_int] * 4 + [dp]
     L.h_elvira_cand_adj.argtypes = [dp] + [ctypes.c_int] * 4 + [dp, dp]
+    L.h_cut_compare.argtypes = [dp, dp, ctypes.c_double, dp]
     return L
 
 
@@ -164,3 +165,27 @@ def test_elvira_candidate_adjoint(H, cand):
         return float(gn @ out)
     num = (F(1e-7) - F(-1e-7)) / 2e-7
     assert float((gVF * d).sum()) == pytest.approx(num, rel=1e-6, abs=1e-9)
+
+
+def test_canonical_leaf_cut_matches_reference_cases(H):
+    """cut_canonical (two case bodies on renumbered vertices) against cut_dispatch (the
+    reference's fourteen case bodies): liquid and total volumes agree to rounding for every cut
+    case, in both orientations of the tet."""
+    rng = np.random.default_rng(21)
+    seen = set()
+    for trial in range(3000):
+        x = rng.random((4, 3))
+        if trial % 2:
+            x = x[[1, 0, 2, 3]]  # the other orientation
+        n = rng.standard_normal(3)
+        n /= np.linalg.norm(n)
+        D = float(n @ x[rng.integers(0, 4)] + 0.3 * rng.standard_normal() * 0.2)
+        out = np.zeros(5)
+        H.h_cut_compare(P(x), P(n), D, P(out))
+        cs = int(out[4])
+        seen.add(cs)
+        # rounding of a determinant of unit-box coordinates: absolute, not relative to a sliver's volume
+        tol = 1e-13 * abs(out[1]) + 1e-15
+        assert abs(out[2] - out[0]) <= tol, (cs, out)
+        assert abs(out[3] - out[1]) <= tol, (cs, out)
+    assert seen >= set(range(1, 15)), sorted(seen)
