@@ -113,7 +113,10 @@ def test_vof_init_band_normals_plic(mist, oracle):
 @pytest.mark.parametrize("case", ["3d", "2d"])
 def test_deformation_steps(mist, oracle, case):
     if case == "3d":
-        param, ic = W.deformation3d(32)
+        # off the y-z diagonal: on it the symmetric field gives mirrored ELVIRA candidates with
+        # equal errors, ordered by last-bit libm differences between device and host
+        param, _ = W.deformation3d(32)
+        ic = ("sphere", 0.15, 0.35, 0.36, 0.34)
         nsteps = 12
     else:
         param, ic = W.deformation2d(64)
