@@ -555,6 +555,7 @@ __global__ __launch_bounds__(C::NT) void k_adj_tree(DevMesh m, const double* pv,
     const int tid = threadIdx.x;
     const int nb = *count;
     double* const gpl[4] = {gnx, gny, gnz, gD};
+    tab_load<C>(S);
     if (tid == 0) S.next = atomicAdd(queue, 1);
     __syncthreads();
     int cidx = S.next;
@@ -653,7 +654,7 @@ __global__ __launch_bounds__(C::NT) void k_adj_tree(DevMesh m, const double* pv,
                         double dd[4];
 #pragma unroll
                         for (int v = 0; v < 4; ++v) dd[v] = S.pz[tn_id(N, v)] - P;
-                        cnt = CUT_NTETS[d2case_snap(dd)];
+                        cnt = S.tab.ntets[d2case_snap(dd)];
                     } else {
                         cnt = 1;
                     }
@@ -786,7 +787,7 @@ __global__ __launch_bounds__(C::NT) void k_adj_tree(DevMesh m, const double* pv,
                             if (lvx < 4) {
                                 pool_add(A, tn_id(T.N, lvx), gQ[kk]);
                             } else {
-                                const unsigned eg = CUT_EDGE[T.cs][lvx - 4];
+                                const unsigned eg = S.tab.edge[T.cs][lvx - 4];
                                 const unsigned ia = tn_id(T.N, eg & 15), ib = tn_id(T.N, eg >> 4);
                                 const V3 pa = pool_v(S, ia), pb = pool_v(S, ib);
                                 V3 ga{0.0, 0.0, 0.0}, gb{0.0, 0.0, 0.0};

@@ -49,8 +49,17 @@ struct TreeNode {
     unsigned bits;    // cell offset from lo of vertex n along axis q at bit 6n + 2q
 };
 
+// The cut tables in LDS: every lane indexes them with its own case, and LDS answers such a
+// gather with less latency than the constant-memory path.
+struct CutTabS {
+    unsigned short vtet[16][6];
+    unsigned char edge[16][4];
+    unsigned char ntets[16], nvert[16], side2[16];
+};
+
 template <class C>
 struct TreeShared {
+    CutTabS tab;
     double px[C::PV], py[C::PV], pz[C::PV];
     TreeNode node[2][C::NL];
     unsigned short loff[C::NL];      // first leaf of each node of the last list
@@ -97,6 +106,18 @@ __device__ __forceinline__ V3 pool_v(const TreeShared<C>& S, unsigned id) {
 template <class C>
 __device__ __forceinline__ double pool_c(const TreeShared<C>& S, unsigned id, int axis) {
     return axis == 0 ? S.px[id] : (axis == 1 ? S.py[id] : S.pz[id]);
+}
+
+// Copies the cut tables into the block's LDS; a barrier must follow before their first use.
+template <class C>
+__device__ __forceinline__ void tab_load(TreeShared<C>& S) {
+    for (int q = threadIdx.x; q < 16 * 6; q += C::NT) S.tab.vtet[q / 6][q % 6] = CUT_VTET[q / 6][q % 6];
+    for (int q = threadIdx.x; q < 16 * 4; q += C::NT) S.tab.edge[q / 4][q % 4] = CUT_EDGE[q / 4][q % 4];
+    if (threadIdx.x < 16) {
+        S.tab.ntets[threadIdx.x] = CUT_NTETS[threadIdx.x];
+        S.tab.nvert[threadIdx.x] = CUT_NVERT[threadIdx.x];
+        S.tab.side2[threadIdx.x] = CUT_SIDE2[threadIdx.x];
+    }
 }
 
 // Exclusive prefix sum over the block; *total receives the block sum.
@@ -180,8 +201,8 @@ __device__ bool tree_level(TreeShared<C>& S, int axis, int minspan, int cur, int
                     dd[q] = d[q];
                 }
                 cs = d2case_snap(dd);  // the reference snaps a copy
-                nch = CUT_NTETS[cs];
-                nv = CUT_NVERT[cs];
+                nch = S.tab.ntets[cs];
+                nv = S.tab.nvert[cs];
             } else {
                 nch = 1;
             }
@@ -197,7 +218,7 @@ __device__ bool tree_level(TreeShared<C>& S, int axis, int minspan, int cur, int
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 if (q < nv) {
-                    const unsigned eg = CUT_EDGE[cs][q];
+                    const unsigned eg = S.tab.edge[cs][q];
                     const int v1 = eg & 15, v2 = eg >> 4;
                     const double d1 = sel4(d, v1), d2 = sel4(d, v2);
                     const double mu = jmin(1.0, jmax(0.0, sdiv(-d1, d2 - d1)));
@@ -224,9 +245,9 @@ __device__ bool tree_level(TreeShared<C>& S, int axis, int minspan, int cur, int
                 }
             }
             const unsigned amask = 3u << (2 * axis);
-            const unsigned side2 = CUT_SIDE2[cs];
+            const unsigned side2 = S.tab.side2[cs];
             for (int c = 0; c < nch; ++c) {
-                const unsigned qv = CUT_VTET[cs][c];
+                const unsigned qv = S.tab.vtet[cs][c];
                 const bool s2 = (side2 >> c) & 1;
                 unsigned ids[4], bits = 0;
 #pragma unroll
@@ -342,8 +363,8 @@ __device__ __forceinline__ void tree_leaf(const TreeShared<C>& S, int cur, int L
 #pragma unroll
         for (int v = 0; v < 4; ++v) dd[v] = S.pz[tn_id(N, v)] - P;
         const int cs = d2case_snap(dd);
-        const unsigned qv = CUT_VTET[cs][c];
-        const bool s2 = (CUT_SIDE2[cs] >> c) & 1;
+        const unsigned qv = S.tab.vtet[cs][c];
+        const bool s2 = (S.tab.side2[cs] >> c) & 1;
         T.P = P;
         T.cs = cs;
         T.qv = qv;
@@ -352,7 +373,7 @@ __device__ __forceinline__ void tree_leaf(const TreeShared<C>& S, int cur, int L
         for (int kk = 0; kk < 4; ++kk) {
             const int lv = nib(qv, kk);
             const bool cutv = lv >= 4;
-            const unsigned eg = CUT_EDGE[cs][cutv ? lv - 4 : 0];
+            const unsigned eg = S.tab.edge[cs][cutv ? lv - 4 : 0];
             const int v1 = cutv ? (int)(eg & 15) : lv, v2 = cutv ? (int)(eg >> 4) : lv;
             const V3 A = pool_v(S, tn_id(N, v1)), B = pool_v(S, tn_id(N, v2));
             const double d1 = A.z - P, d2 = B.z - P;

@@ -737,6 +737,7 @@ __global__ __launch_bounds__(C::NT) void k_sl_tree(DevMesh m, const double* pv, 
     __shared__ int s_box[2][3];               // index box over corners and correction vertices
     const int tid = threadIdx.x;
     const int nb = *count;
+    tab_load<C>(S);
     // the next cell is claimed while the current one is cut (the atomic's latency is hidden)
     if (tid == 0) S.next = atomicAdd(queue, 1);
     __syncthreads();
@@ -894,7 +895,7 @@ __global__ __launch_bounds__(C::NT) void k_sl_tree(DevMesh m, const double* pv, 
                     double dd[4];
 #pragma unroll
                     for (int v = 0; v < 4; ++v) dd[v] = S.pz[tn_id(N, v)] - P;
-                    cnt = CUT_NTETS[d2case_snap(dd)];
+                    cnt = S.tab.ntets[d2case_snap(dd)];
                 } else {
                     cnt = 1;
                 }
@@ -935,14 +936,14 @@ __global__ __launch_bounds__(C::NT) void k_sl_tree(DevMesh m, const double* pv, 
 #pragma unroll
                     for (int v = 0; v < 4; ++v) dd[v] = S.pz[tn_id(N, v)] - P;
                     const int cs = d2case_snap(dd);
-                    const unsigned qv = CUT_VTET[cs][c];
-                    const bool s2 = (CUT_SIDE2[cs] >> c) & 1;
+                    const unsigned qv = S.tab.vtet[cs][c];
+                    const bool s2 = (S.tab.side2[cs] >> c) & 1;
                     int e1v = 0, e2v = 0;  // edge of the first vertex
 #pragma unroll
                     for (int kk = 0; kk < 4; ++kk) {
                         const int lv = nib(qv, kk);
                         const bool cutv = lv >= 4;
-                        const unsigned eg = CUT_EDGE[cs][cutv ? lv - 4 : 0];
+                        const unsigned eg = S.tab.edge[cs][cutv ? lv - 4 : 0];
                         const int v1 = cutv ? (int)(eg & 15) : lv, v2 = cutv ? (int)(eg >> 4) : lv;
                         const V3 A = pool_v(S, tn_id(N, v1)), B = pool_v(S, tn_id(N, v2));
                         const double d1 = A.z - P, d2 = B.z - P;
