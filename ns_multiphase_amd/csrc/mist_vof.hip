@@ -351,9 +351,11 @@ __global__ __launch_bounds__(ELV_THREADS, 4) void k_elvira(DevMesh m, DevParams 
     __shared__ int sCount[16], sBase[16];
     __shared__ double sE[ELV_PAIRS];
     __shared__ double sErr[27];
+    __shared__ unsigned char sNLiq[16];  // liquid sub-tets per case: ntets - nntet + 1
     const int tid = threadIdx.x;
     const int ncell = *count;
     const double cellvol = (m.dx * m.dy) * m.dz;
+    if (tid < 16) sNLiq[tid] = (unsigned char)(CUT_NTETS[tid] - CUT_NNTET[tid] + 1);
     for (int cidx = blockIdx.x; cidx < ncell; cidx += gridDim.x) {
         int i, j, k;
         interior_of(m, list[cidx], i, j, k);
@@ -511,7 +513,7 @@ __global__ __launch_bounds__(ELV_THREADS, 4) void k_elvira(DevMesh m, DevParams 
                 } else if (kind == 0) {
                     for (int t = 0; t < 5; ++t) {
                         const int cs = sCase[tid][t];
-                        const int nt = CUT_NTETS[cs] - CUT_NNTET[cs] + 1;
+                        const int nt = sNLiq[cs];
                         for (int w = 0; w < nt; ++w) vf_ += sTerm[tid][t][w];
                     }
                 }
