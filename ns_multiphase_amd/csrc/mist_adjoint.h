@@ -441,4 +441,138 @@ __host__ __device__ inline void elvira_cand_adj(const DevMesh& m, int i, int j, 
     }
 }
 
+// ---- the leaf cut on V3 vertices (the reverse of cut_case and cut_canonical) ----------
+// Gradient of tet_vol3 (the volume -det(p0-p3, p1-p3, p2-p3) / 6) with weight w, into g0..g3.
+__host__ __device__ __forceinline__ void tet_vol3_grad(const V3& p0, const V3& p1, const V3& p2, const V3& p3,
+                                                       double w, V3& g0, V3& g1, V3& g2, V3& g3) {
+    const double a1 = p0.x - p3.x, a2 = p0.y - p3.y, a3 = p0.z - p3.z;
+    const double b1 = p1.x - p3.x, b2 = p1.y - p3.y, b3 = p1.z - p3.z;
+    const double c1 = p2.x - p3.x, c2 = p2.y - p3.y, c3 = p2.z - p3.z;
+    const double s = -w / 6.0;
+    const V3 ga{s * (b2 * c3 - c2 * b3), s * (c1 * b3 - b1 * c3), s * (b1 * c2 - c1 * b2)};
+    const V3 gb{s * (c2 * a3 - a2 * c3), s * (a1 * c3 - c1 * a3), s * (c1 * a2 - a1 * c2)};
+    const V3 gc{s * (a2 * b3 - b2 * a3), s * (b1 * a3 - a1 * b3), s * (a1 * b2 - b1 * a2)};
+    g0.x += ga.x; g0.y += ga.y; g0.z += ga.z;
+    g1.x += gb.x; g1.y += gb.y; g1.z += gb.z;
+    g2.x += gc.x; g2.y += gc.y; g2.z += gc.z;
+    g3.x -= (ga.x + gb.x) + gc.x; g3.y -= (ga.y + gb.y) + gc.y; g3.z -= (ga.z + gb.z) + gc.z;
+}
+
+// lerp_adj on V3: x = lerp3(a, b, mu(da, db)); adds gx's share to ga, gb, returns gda, gdb.
+__host__ __device__ __forceinline__ void lerp3_adj(const V3& a, const V3& b, double da, double db, const V3& gx,
+                                                   V3& ga, V3& gb, double& gda, double& gdb) {
+    const double A[3] = {a.x, a.y, a.z}, B[3] = {b.x, b.y, b.z}, G[3] = {gx.x, gx.y, gx.z};
+    double sa[3] = {0.0, 0.0, 0.0}, sb[3] = {0.0, 0.0, 0.0};
+    lerp_adj(A, B, da, db, G, sa, sb, gda, gdb);
+    ga.x += sa[0]; ga.y += sa[1]; ga.z += sa[2];
+    gb.x += sb[0]; gb.y += sb[1]; gb.z += sb[2];
+}
+
+// Reverse of cut_case<CS, true> (the PLIC cut of a leaf): the liquid sub-tets carry gL + gV,
+// the gas ones gV; the cut vertices pass their adjoints to their edge ends and to the vertex
+// distances dm. Adds to gP and gd.
+template <int CS>
+__host__ __device__ __forceinline__ void cut_case_adj(const V3 (&P)[4], const double (&dm)[4], double gL,
+                                                      double gV, V3 (&gP)[4], double (&gd)[4]) {
+    constexpr int NV = CutTab::nvert[CS], NT = CutTab::ntets[CS], NN = CutTab::nntet[CS];
+    V3 Q[8];
+    Q[0] = P[0]; Q[1] = P[1]; Q[2] = P[2]; Q[3] = P[3];
+    sfor<0, NV>([&](auto I) {
+        constexpr int n = decltype(I)::value;
+        constexpr int v1 = CutTab::edge[CS][n] & 15, v2 = CutTab::edge[CS][n] >> 4;
+        const double mu = jmin(1.0, jmax(0.0, sdiv(-dm[v1], dm[v2] - dm[v1])));
+        Q[4 + n] = lerp3(Q[v1], Q[v2], mu);
+    });
+    V3 g[8];
+    sfor<0, 8>([&](auto I) { g[decltype(I)::value] = V3{0.0, 0.0, 0.0}; });
+    sfor<0, NT - NN + 1>([&](auto I) {
+        constexpr int n = NT - decltype(I)::value;
+        constexpr int a = vtet_of(CS, n - 1, 0), b = vtet_of(CS, n - 1, 1), c = vtet_of(CS, n - 1, 2),
+                      d = vtet_of(CS, n - 1, 3);
+        tet_vol3_grad(Q[a], Q[b], Q[c], Q[d], gL + gV, g[a], g[b], g[c], g[d]);
+    });
+    sfor<1, NN>([&](auto I) {
+        constexpr int n = decltype(I)::value;
+        constexpr int a = vtet_of(CS, n - 1, 0), b = vtet_of(CS, n - 1, 1), c = vtet_of(CS, n - 1, 2),
+                      d = vtet_of(CS, n - 1, 3);
+        tet_vol3_grad(Q[a], Q[b], Q[c], Q[d], gV, g[a], g[b], g[c], g[d]);
+    });
+    sfor<0, NV>([&](auto I) {
+        constexpr int n = NV - 1 - decltype(I)::value;
+        constexpr int v1 = CutTab::edge[CS][n] & 15, v2 = CutTab::edge[CS][n] >> 4;
+        double ga, gb;
+        lerp3_adj(Q[v1], Q[v2], dm[v1], dm[v2], g[4 + n], g[v1], g[v2], ga, gb);
+        gd[v1] += ga;
+        gd[v2] += gb;
+    });
+    sfor<0, 4>([&](auto I) {
+        constexpr int v = decltype(I)::value;
+        gP[v].x += g[v].x; gP[v].y += g[v].y; gP[v].z += g[v].z;
+    });
+}
+
+__host__ __device__ __forceinline__ void cut_dispatch_adj(int cs, const V3 (&P)[4], const double (&dm)[4],
+                                                          double gL, double gV, V3 (&gP)[4], double (&gd)[4]) {
+    switch (cs) {
+#define MIST_CASE(k) \
+    case k: cut_case_adj<k>(P, dm, gL, gV, gP, gd); break;
+        MIST_CASE(0) MIST_CASE(1) MIST_CASE(2) MIST_CASE(3) MIST_CASE(4) MIST_CASE(5) MIST_CASE(6) MIST_CASE(7)
+        MIST_CASE(8) MIST_CASE(9) MIST_CASE(10) MIST_CASE(11) MIST_CASE(12) MIST_CASE(13) MIST_CASE(14) MIST_CASE(15)
+#undef MIST_CASE
+        default: break;
+    }
+}
+
+// Reverse of cut_canonical: the same renumbering onto case 1 or case 3, whose liquid and gas
+// pieces carry the weights of the leaf's own liquid (gL + gV) and gas (gV) volumes after the
+// swap of a lone vertex below the plane and the sign of an odd renumbering; the adjoints of the
+// renumbered vertices and distances go back to gP and gd. Cases 0 and 15 (the leaf whole) give
+// the tet's volume gradient.
+__host__ __device__ __forceinline__ void cut_canonical_adj(int cs, const V3 (&P)[4], const double (&dm)[4],
+                                                           double gL, double gV, V3 (&gP)[4], double (&gd)[4]) {
+    if (cs == 0 || cs == 15) {
+        tet_vol3_grad(P[0], P[1], P[2], P[3], cs == 0 ? gL + gV : gV, gP[0], gP[1], gP[2], gP[3]);
+        return;
+    }
+    const unsigned c = canon_case(cs);
+    const double s = (c & (1u << 16)) ? -1.0 : 1.0;
+    // weights of the canonical case's liquid and gas pieces
+    double wl = s * (gL + gV), wg = s * gV;
+    if (c & (1u << 17)) {
+        const double t = wl;
+        wl = wg;
+        wg = t;
+    }
+    auto run = [&](auto A, auto B, auto C, auto D) {
+        constexpr int a = decltype(A)::value, b = decltype(B)::value, cc = decltype(C)::value,
+                      d = decltype(D)::value;
+        const V3 Q[4] = {P[a], P[b], P[cc], P[d]};
+        const double e[4] = {dm[a], dm[b], dm[cc], dm[d]};
+        V3 gQ[4] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+        double ge[4] = {0.0, 0.0, 0.0, 0.0};
+        // cut_case_adj weighs liquid pieces by its gL + gV and gas pieces by its gV
+        if (c & (1u << 18)) cut_case_adj<3>(Q, e, wl - wg, wg, gQ, ge);
+        else cut_case_adj<1>(Q, e, wl - wg, wg, gQ, ge);
+        gP[a].x += gQ[0].x; gP[a].y += gQ[0].y; gP[a].z += gQ[0].z;
+        gP[b].x += gQ[1].x; gP[b].y += gQ[1].y; gP[b].z += gQ[1].z;
+        gP[cc].x += gQ[2].x; gP[cc].y += gQ[2].y; gP[cc].z += gQ[2].z;
+        gP[d].x += gQ[3].x; gP[d].y += gQ[3].y; gP[d].z += gQ[3].z;
+        gd[a] += ge[0];
+        gd[b] += ge[1];
+        gd[cc] += ge[2];
+        gd[d] += ge[3];
+    };
+    switch (cs) {
+        case 2: case 13: run(IC<1>{}, IC<0>{}, IC<2>{}, IC<3>{}); break;
+        case 4: case 11: run(IC<2>{}, IC<0>{}, IC<1>{}, IC<3>{}); break;
+        case 8: case 7: run(IC<3>{}, IC<0>{}, IC<1>{}, IC<2>{}); break;
+        case 5: run(IC<0>{}, IC<2>{}, IC<1>{}, IC<3>{}); break;
+        case 6: run(IC<1>{}, IC<2>{}, IC<0>{}, IC<3>{}); break;
+        case 9: run(IC<0>{}, IC<3>{}, IC<1>{}, IC<2>{}); break;
+        case 10: run(IC<1>{}, IC<3>{}, IC<0>{}, IC<2>{}); break;
+        case 12: run(IC<2>{}, IC<3>{}, IC<0>{}, IC<1>{}); break;
+        default: run(IC<0>{}, IC<1>{}, IC<2>{}, IC<3>{}); break;
+    }
+}
+
 }  // namespace mist

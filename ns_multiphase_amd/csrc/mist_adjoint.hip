@@ -439,87 +439,6 @@ __global__ __launch_bounds__(64) void k_adj_cut(DevMesh m, const double* pv, con
 }
 
 // ---- reverse of the cut tree ----------------------------------------------------------
-// Gradient of tet_vol3 (the volume -det(p0-p3, p1-p3, p2-p3) / 6) with weight w, into g0..g3.
-__device__ __forceinline__ void tet_vol3_grad(const V3& p0, const V3& p1, const V3& p2, const V3& p3, double w,
-                                              V3& g0, V3& g1, V3& g2, V3& g3) {
-    const double a1 = p0.x - p3.x, a2 = p0.y - p3.y, a3 = p0.z - p3.z;
-    const double b1 = p1.x - p3.x, b2 = p1.y - p3.y, b3 = p1.z - p3.z;
-    const double c1 = p2.x - p3.x, c2 = p2.y - p3.y, c3 = p2.z - p3.z;
-    const double s = -w / 6.0;
-    const V3 ga{s * (b2 * c3 - c2 * b3), s * (c1 * b3 - b1 * c3), s * (b1 * c2 - c1 * b2)};
-    const V3 gb{s * (c2 * a3 - a2 * c3), s * (a1 * c3 - c1 * a3), s * (c1 * a2 - a1 * c2)};
-    const V3 gc{s * (a2 * b3 - b2 * a3), s * (b1 * a3 - a1 * b3), s * (a1 * b2 - b1 * a2)};
-    g0.x += ga.x; g0.y += ga.y; g0.z += ga.z;
-    g1.x += gb.x; g1.y += gb.y; g1.z += gb.z;
-    g2.x += gc.x; g2.y += gc.y; g2.z += gc.z;
-    g3.x -= (ga.x + gb.x) + gc.x; g3.y -= (ga.y + gb.y) + gc.y; g3.z -= (ga.z + gb.z) + gc.z;
-}
-
-// lerp_adj on V3: x = lerp3(a, b, mu(da, db)); adds gx's share to ga, gb, returns gda, gdb.
-__device__ __forceinline__ void lerp3_adj(const V3& a, const V3& b, double da, double db, const V3& gx, V3& ga, V3& gb,
-                                          double& gda, double& gdb) {
-    const double A[3] = {a.x, a.y, a.z}, B[3] = {b.x, b.y, b.z}, G[3] = {gx.x, gx.y, gx.z};
-    double sa[3] = {0.0, 0.0, 0.0}, sb[3] = {0.0, 0.0, 0.0};
-    lerp_adj(A, B, da, db, G, sa, sb, gda, gdb);
-    ga.x += sa[0]; ga.y += sa[1]; ga.z += sa[2];
-    gb.x += sb[0]; gb.y += sb[1]; gb.z += sb[2];
-}
-
-// Reverse of cut_case<CS, true> (the PLIC cut of a leaf): the liquid sub-tets carry gL + gV,
-// the gas ones gV; the cut vertices pass their adjoints to their edge ends and to the vertex
-// distances dm. Adds to gP and gd.
-template <int CS>
-__device__ __forceinline__ void cut_case_adj(const V3 (&P)[4], const double (&dm)[4], double gL, double gV,
-                                             V3 (&gP)[4], double (&gd)[4]) {
-    constexpr int NV = CutTab::nvert[CS], NT = CutTab::ntets[CS], NN = CutTab::nntet[CS];
-    V3 Q[8];
-    Q[0] = P[0]; Q[1] = P[1]; Q[2] = P[2]; Q[3] = P[3];
-    sfor<0, NV>([&](auto I) {
-        constexpr int n = decltype(I)::value;
-        constexpr int v1 = CutTab::edge[CS][n] & 15, v2 = CutTab::edge[CS][n] >> 4;
-        const double mu = jmin(1.0, jmax(0.0, sdiv(-dm[v1], dm[v2] - dm[v1])));
-        Q[4 + n] = lerp3(Q[v1], Q[v2], mu);
-    });
-    V3 g[8];
-    sfor<0, 8>([&](auto I) { g[decltype(I)::value] = V3{0.0, 0.0, 0.0}; });
-    sfor<0, NT - NN + 1>([&](auto I) {
-        constexpr int n = NT - decltype(I)::value;
-        constexpr int a = vtet_of(CS, n - 1, 0), b = vtet_of(CS, n - 1, 1), c = vtet_of(CS, n - 1, 2),
-                      d = vtet_of(CS, n - 1, 3);
-        tet_vol3_grad(Q[a], Q[b], Q[c], Q[d], gL + gV, g[a], g[b], g[c], g[d]);
-    });
-    sfor<1, NN>([&](auto I) {
-        constexpr int n = decltype(I)::value;
-        constexpr int a = vtet_of(CS, n - 1, 0), b = vtet_of(CS, n - 1, 1), c = vtet_of(CS, n - 1, 2),
-                      d = vtet_of(CS, n - 1, 3);
-        tet_vol3_grad(Q[a], Q[b], Q[c], Q[d], gV, g[a], g[b], g[c], g[d]);
-    });
-    sfor<0, NV>([&](auto I) {
-        constexpr int n = NV - 1 - decltype(I)::value;
-        constexpr int v1 = CutTab::edge[CS][n] & 15, v2 = CutTab::edge[CS][n] >> 4;
-        double ga, gb;
-        lerp3_adj(Q[v1], Q[v2], dm[v1], dm[v2], g[4 + n], g[v1], g[v2], ga, gb);
-        gd[v1] += ga;
-        gd[v2] += gb;
-    });
-    sfor<0, 4>([&](auto I) {
-        constexpr int v = decltype(I)::value;
-        gP[v].x += g[v].x; gP[v].y += g[v].y; gP[v].z += g[v].z;
-    });
-}
-
-__device__ __forceinline__ void cut_dispatch_adj(int cs, const V3 (&P)[4], const double (&dm)[4], double gL,
-                                                 double gV, V3 (&gP)[4], double (&gd)[4]) {
-    switch (cs) {
-#define MIST_CASE(k) \
-    case k: cut_case_adj<k>(P, dm, gL, gV, gP, gd); break;
-        MIST_CASE(0) MIST_CASE(1) MIST_CASE(2) MIST_CASE(3) MIST_CASE(4) MIST_CASE(5) MIST_CASE(6) MIST_CASE(7)
-        MIST_CASE(8) MIST_CASE(9) MIST_CASE(10) MIST_CASE(11) MIST_CASE(12) MIST_CASE(13) MIST_CASE(14) MIST_CASE(15)
-#undef MIST_CASE
-        default: break;
-    }
-}
-
 template <class C>
 struct TreeAdjShared {
     unsigned par[C::PV];   // how each pool vertex was made (par_pack)
@@ -756,12 +675,7 @@ __global__ __launch_bounds__(C::NT) void k_adj_tree(DevMesh m, const double* pv,
                         }
                         const int cs = d2case_snap(dd);
                         double gd[4] = {0.0, 0.0, 0.0, 0.0};
-                        if (cs == 0 || cs == 15) {
-                            tet_vol3_grad(T.Q[0], T.Q[1], T.Q[2], T.Q[3], cs == 0 ? gL + gV : gV, gQ[0], gQ[1], gQ[2],
-                                          gQ[3]);
-                        } else {
-                            cut_dispatch_adj(cs, T.Q, d, gL, gV, gQ, gd);
-                        }
+                        cut_canonical_adj(cs, T.Q, d, gL, gV, gQ, gd);
                         // d_v = n . Q_v - D
                         double gn0 = 0.0, gn1 = 0.0, gn2 = 0.0;
 #pragma unroll

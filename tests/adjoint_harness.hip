@@ -2,8 +2,8 @@
 // pieces (ns_multiphase_amd/csrc/mist_adjoint.h) and the forward functions they reverse,
 // so the tests can compare each adjoint with central differences of its forward function on
 // the CPU; and over the two forms of the leaf cut (the fourteen case bodies of cut_dispatch and
-// the two of cut_canonical) so the tests can compare them. Built by tests/test_adjoint_pieces.py
-// with hipcc; never part of the library.
+// the two of cut_canonical), forward and reverse, so the tests can compare them. Built by
+// tests/test_adjoint_pieces.py with hipcc; never part of the library.
 #include <cmath>
 #include <vector>
 
@@ -114,6 +114,37 @@ void h_cut_compare(const double* x, const double* n, double D, double* out) {
     cut_dispatch<true>(cs, P, d, [&](double v) { L += v; }, [&](double v) { G += v; });
     cut_canonical(cs, P, d, [&](double v) { Lc += v; }, [&](double v) { Gc += v; });
     out[0] = L; out[1] = L + G; out[2] = Lc; out[3] = Lc + Gc; out[4] = cs;
+}
+
+// adjoint of gL * liquid + gV * total volume of the cut of h_cut_compare, with respect to
+// x (12), n (3) and D (1): out[0..15] through cut_dispatch_adj, out[16..31] through
+// cut_canonical_adj
+void h_cut_adj_compare(const double* x, const double* n, double D, double gL, double gV, double* out) {
+    V3 P[4];
+    double d[4], dd[4];
+    for (int v = 0; v < 4; ++v) {
+        P[v] = V3{x[3 * v], x[3 * v + 1], x[3 * v + 2]};
+        d[v] = ((n[0] * P[v].x + n[1] * P[v].y) + n[2] * P[v].z) - D;
+        dd[v] = d[v];
+    }
+    const int cs = d2case_snap(dd);
+    for (int form = 0; form < 2; ++form) {
+        V3 gP[4] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+        double gd[4] = {0.0, 0.0, 0.0, 0.0};
+        if (form == 0) cut_dispatch_adj(cs, P, d, gL, gV, gP, gd);
+        else cut_canonical_adj(cs, P, d, gL, gV, gP, gd);
+        double* o = out + 16 * form;
+        for (int q = 0; q < 16; ++q) o[q] = 0.0;
+        for (int v = 0; v < 4; ++v) {
+            o[3 * v + 0] = gP[v].x + gd[v] * n[0];
+            o[3 * v + 1] = gP[v].y + gd[v] * n[1];
+            o[3 * v + 2] = gP[v].z + gd[v] * n[2];
+            o[12] += gd[v] * P[v].x;
+            o[13] += gd[v] * P[v].y;
+            o[14] += gd[v] * P[v].z;
+            o[15] -= gd[v];
+        }
+    }
 }
 
 double h_alpha(double a, double b, double c, double VF) { return compute_alpha(a, b, c, VF); }

@@ -44,6 +44,7 @@ def H():
     L.h_elvira_cand.argtypes = [dp] + [ctypes.c_int] * 4 + [dp]
     L.h_elvira_cand_adj.argtypes = [dp] + [ctypes.c_int] * 4 + [dp, dp]
     L.h_cut_compare.argtypes = [dp, dp, ctypes.c_double, dp]
+    L.h_cut_adj_compare.argtypes = [dp, dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp]
     return L
 
 
@@ -189,3 +190,40 @@ def test_canonical_leaf_cut_matches_reference_cases(H):
         assert abs(out[2] - out[0]) <= tol, (cs, out)
         assert abs(out[3] - out[1]) <= tol, (cs, out)
     assert seen >= set(range(1, 15)), sorted(seen)
+
+
+def test_canonical_leaf_cut_adjoint(H):
+    """cut_canonical_adj (the reverse the tree kernels run) against cut_dispatch_adj (the reverse
+    of the reference's fourteen case bodies) for every cut case in both orientations, and both
+    against central differences of the canonical forward cut, away from case switches."""
+    rng = np.random.default_rng(22)
+    seen = set()
+    fd_checked = 0
+    for trial in range(1500):
+        x = rng.random((4, 3))
+        if trial % 2:
+            x = x[[1, 0, 2, 3]]
+        n = rng.standard_normal(3)
+        n /= np.linalg.norm(n)
+        D = float(n @ x[rng.integers(0, 4)] + 0.06 * rng.standard_normal())
+        gL, gV = rng.standard_normal(2)
+        out = np.zeros(32)
+        H.h_cut_adj_compare(P(x), P(n), D, gL, gV, P(out))
+        f5 = np.zeros(5)
+        H.h_cut_compare(P(x), P(n), D, P(f5))
+        cs = int(f5[4])
+        seen.add(cs)
+        scale = np.abs(out[:16]).max() + 1e-12
+        assert np.abs(out[16:] - out[:16]).max() <= 1e-12 * scale, (cs, out[:16], out[16:])
+        d = x @ n - D
+        if fd_checked < 300 and np.abs(d).min() > 1e-3:
+            def F(z):
+                o = np.zeros(5)
+                H.h_cut_compare(P(z[:12]), P(z[12:15]), float(z[15]), P(o))
+                return gL * o[2] + gV * o[3]
+            z = np.concatenate([x.ravel(), n, [D]])
+            num = fd(F, z, 1e-6)
+            assert np.allclose(out[16:], num, rtol=1e-6, atol=1e-8), (cs, out[16:], num)
+            fd_checked += 1
+    assert seen >= set(range(0, 16)), sorted(seen)
+    assert fd_checked >= 100
