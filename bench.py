@@ -81,7 +81,7 @@ def counted_flops(N):
 # scaled by 2.0 and WRITE_SIZE by 1.0 (what the counters read for a plain 256 MiB device copy);
 # see profiles/pmc_traffic_deformation3d_128.md. k_elvira's figure is almost all WRITE_SIZE: its
 # spilled registers (scratch stores), not field traffic.
-TRAFFIC_PER_UNIT = {"k_elvira": 33014.0, "k_sl_tree": 863.0}
+TRAFFIC_PER_UNIT = {"k_elvira": 32985.0, "k_sl_tree": 850.0}
 
 
 def run_vof(args, torch):
@@ -305,12 +305,14 @@ def cpu_bubble(args, param, ic, nsamples=2):
 
 def run_adjoint(args, torch):
     """Gradient of the SL-VOF loss with respect to 16 stream-function amplitudes: 256^2, 64
-    steps, checkpoints every 8; timed against the plain forward loss, and checked per
-    amplitude against central differences (h = 1e-6) of the forward loss."""
+    steps, the checkpoint interval chosen by an 8 GiB device-memory budget (every state at this
+    size); timed against the plain forward loss, and checked per amplitude against central
+    differences (h = 1e-6) of the forward loss. The loss and gradient with checkpoints every 8
+    steps (each segment run forward again in the reverse sweep) is timed beside it."""
     import numpy as np
     from ns_multiphase_amd import adjoint as A
     from ns_multiphase_amd import solver
-    N, nsteps, every = args.adj_N, 64, 8
+    N, nsteps = args.adj_N, 64
     p = A.adjoint_params(N)
     ctx = solver.Context(p)
     ctx.init_vof("circle", 0.2, 0.5, 0.5)
@@ -319,9 +321,12 @@ def run_adjoint(args, torch):
     a1 = np.random.default_rng(1).normal(0.0, 0.1, 16)
     a2 = np.random.default_rng(2).normal(0.0, 0.1, 16)
     dt = min(A.stable_dt(ctx.mesh, basis, a1), A.stable_dt(ctx.mesh, basis, a2))
-    run = A.SLVOFRun(ctx, vf0, basis, dt, nsteps, every)
+    run = A.SLVOFRun(ctx, vf0, basis, dt, nsteps, None, mem_budget=8 << 30)
+    every = run.every
+    run8 = A.SLVOFRun(ctx, vf0, basis, dt, nsteps, 8)
     target = run.forward(a2, checkpoints=False)
     run.loss_and_grad(a1, target)  # warm-up
+    run8.loss_and_grad(a1, target)
     reps = 3
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -334,8 +339,13 @@ def run_adjoint(args, torch):
         L2, g = run.loss_and_grad(a1, target)
     torch.cuda.synchronize()
     t_grad = (time.perf_counter() - t0) / reps
-    # the reverse sweep alone (recomputing each segment from its checkpoint, then walking it
-    # back), after a forward that stored the checkpoints
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        _, g8 = run8.loss_and_grad(a1, target)
+    torch.cuda.synchronize()
+    t_grad8 = (time.perf_counter() - t0) / reps
+    # the reverse sweep alone (recomputing each segment from its checkpoint, when the interval
+    # is above 1, then walking it back), after a forward that stored the checkpoints
     m = ctx.mesh
     o = m.nghost + 3
     sl = (slice(o, o + m.Nx), slice(o, o + m.Ny), slice(o, o + m.Nz))
@@ -360,12 +370,18 @@ def run_adjoint(args, torch):
     ctx.close()
     out = {"workload": f"{N}^2 x 1, circle r=0.2 at (0.5,0.5), 16-mode stream function (seed 1 control, seed 2 "
                        f"target), {nsteps} steps, dt={dt:.3e}, checkpoints every {every}",
+           "checkpoint_every": every,
+           "checkpoint_MB": (run.nseg + every - 1) * A.checkpoint_bytes(m) / 1e6,
            "loss": L, "forward_ms": 1e3 * t_fwd, "loss_and_gradient_ms": 1e3 * t_grad,
            "gradient_cost_vs_forward": t_grad / t_fwd, "reverse_sweep_ms": 1e3 * t_bwd,
            "reverse_sweep_vs_forward": t_bwd / t_fwd,
+           "loss_and_gradient_ms_every8": 1e3 * t_grad8, "gradient_cost_vs_forward_every8": t_grad8 / t_fwd,
+           "every8_vs_every_max_rel": float(np.abs(g8 - g).max() / np.abs(g).max()),
            "fd_check": checks[0], "fd_check_h1e-7": checks[1]}
-    log(f"[bench] adjoint: forward {out['forward_ms']:.1f} ms, loss+gradient {out['loss_and_gradient_ms']:.1f} ms "
-        f"({out['gradient_cost_vs_forward']:.2f}x), reverse sweep {out['reverse_sweep_ms']:.1f} ms "
+    log(f"[bench] adjoint: checkpoints every {every} ({out['checkpoint_MB']:.0f} MB), forward {out['forward_ms']:.1f} ms, "
+        f"loss+gradient {out['loss_and_gradient_ms']:.1f} ms "
+        f"({out['gradient_cost_vs_forward']:.2f}x; {out['loss_and_gradient_ms_every8']:.1f} ms = "
+        f"{out['gradient_cost_vs_forward_every8']:.2f}x with checkpoints every 8), reverse sweep {out['reverse_sweep_ms']:.1f} ms "
         f"({out['reverse_sweep_vs_forward']:.2f}x); FD within 1e-6: {checks[0]['within_1e-6']}/16 at h=1e-6, "
         f"{checks[1]['within_1e-6']}/16 at h=1e-7")
     return out

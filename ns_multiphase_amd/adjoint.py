@@ -92,16 +92,52 @@ def fold_vof_ghosts(mesh, param, lam):
     return g
 
 
+MAX_SLOTS = 4096  # checkpoint slots one library context holds (mist_checkpoint_save)
+
+
+def checkpoint_bytes(mesh) -> int:
+    """Device bytes of one checkpoint slot (mist_checkpoint_save): VF with its ghosts (FP64),
+    the band flags (int16), the normals and PLIC constants (4 x FP64), ELVIRA's choice and the
+    band-cell list (2 x int32 per interior cell), each part 256-byte aligned."""
+    def al(n):
+        return (n + 255) // 256 * 256
+    ncell_ext = int(np.prod(mesh.cell_shape()))
+    ncell = mesh.Nx * mesh.Ny * mesh.Nz
+    return (al(8 * int(np.prod(mesh.vof_shape()))) + al(2 * ncell_ext) + 4 * al(8 * ncell_ext)
+            + 2 * al(4 * ncell) + al(4))
+
+
+def checkpoint_interval(mesh, nsteps: int, mem_budget: int) -> int:
+    """The smallest interval k whose slots fit in `mem_budget` bytes of device memory: ceil(nsteps
+    / k) checkpoints plus k - 1 states of the segment being walked back. k = 1 keeps every state,
+    and the reverse sweep then runs no transport step again."""
+    per = checkpoint_bytes(mesh)
+    for k in range(1, max(1, nsteps) + 1):
+        slots = -(-nsteps // k) + k - 1
+        if slots <= MAX_SLOTS and slots * per <= mem_budget:
+            return k
+    raise ValueError(f"{nsteps} steps do not fit a checkpoint budget of {mem_budget} bytes "
+                     f"({per} bytes per slot)")
+
+
 class SLVOFRun:
     """Forward runs and checkpointed reverse sweeps of `nsteps` transport steps of fixed dt
-    under steady face velocities, on one library context. Checkpoints every `every` steps are
-    kept on the device; a reverse sweep recomputes each segment forward from its checkpoint
-    (storing its `every` states) and walks it back with `mist_transport_vjp`."""
+    under steady face velocities, on one library context. The state every `every` steps is kept
+    on the device; a reverse sweep recomputes each segment forward from its checkpoint (storing
+    its other states) and walks it back with `mist_transport_vjp`. With `every=None` the
+    interval is the smallest whose slots fit `mem_budget` bytes (checkpoint_interval)."""
 
-    def __init__(self, ctx, vf0, basis, dt, nsteps, every=8):
+    def __init__(self, ctx, vf0, basis, dt, nsteps, every=None, mem_budget=8 << 30):
         self.ctx, self.vf0, self.basis = ctx, vf0, basis
-        self.dt, self.nsteps, self.every = float(dt), int(nsteps), int(every)
+        self.dt, self.nsteps = float(dt), int(nsteps)
+        if every is None:
+            every = checkpoint_interval(ctx.mesh, self.nsteps, int(mem_budget))
+        self.every = int(every)
+        if self.every < 1:
+            raise ValueError("every must be >= 1")
         self.nseg = (self.nsteps + self.every - 1) // self.every
+        if self.nseg + self.every - 1 > MAX_SLOTS:
+            raise ValueError(f"{self.nseg + self.every - 1} checkpoint slots needed, at most {MAX_SLOTS}")
         zeros = np.zeros(ctx.get("wf").shape)
         self._wf0 = zeros
 
@@ -132,17 +168,20 @@ class SLVOFRun:
         c.set("VF_bar", lam_T)
         for name in ("uf_bar", "vf_bar", "wf_bar"):
             c.set(name, np.zeros(c.get(name).shape))
-        base = self.nseg  # slots after the checkpoints hold a segment's states
+        base = self.nseg  # slots after the checkpoints hold a segment's later states
         for seg in range(self.nseg - 1, -1, -1):
             s0 = seg * self.every
             s1 = min(self.nsteps, s0 + self.every)
-            c.checkpoint_restore(seg)  # VF, band, normals and PLIC constants of state s0
-            for s in range(s0, s1):
-                c.checkpoint_save(base + (s - s0))
-                if s < s1 - 1:
+
+            def slot(s):
+                return seg if s == s0 else base + (s - s0 - 1)
+            if s1 - s0 > 1:
+                c.checkpoint_restore(seg)  # VF, band, normals and PLIC constants of state s0
+                for s in range(s0 + 1, s1):
                     self._step()
+                    c.checkpoint_save(slot(s))
             for s in range(s1 - 1, s0 - 1, -1):
-                c.checkpoint_restore(base + (s - s0))
+                c.checkpoint_restore(slot(s))
                 c.transport_vjp(self.dt)
         guf, gvf = c.get("uf_bar"), c.get("vf_bar")
         return np.array([(guf * b[0]).sum() + (gvf * b[1]).sum() for b in self.basis])

@@ -93,6 +93,26 @@ def test_gradient_matches_central_differences(mist):
     assert np.linalg.norm(g - fd) <= 1e-3 * np.linalg.norm(fd)
 
 
+def test_checkpoint_intervals_give_one_gradient(mist):
+    """Every state kept (interval 1), segments of 8, ragged segments of 6 (the last one shorter)
+    and the interval a memory budget of 40 slots picks: the same gradient to rounding (the
+    reverse kernels add with atomics, so the order of the additions may differ)."""
+    p, ctx, run, amps = _setup(mist, 64, 20, 1)
+    tgt_amps = np.random.default_rng(2).normal(0.0, 0.1, 16)
+    target = run.forward(tgt_amps, False)
+    L1, g1 = run.loss_and_grad(amps, target)
+    budget = 40 * A.checkpoint_bytes(ctx.mesh)
+    k_budget = A.checkpoint_interval(ctx.mesh, 20, budget)
+    assert k_budget == 1
+    tight = A.checkpoint_interval(ctx.mesh, 20, 8 * A.checkpoint_bytes(ctx.mesh))
+    assert tight == 4  # ceil(20 / 4) + 3 = 8 slots; 3 needs 9
+    for every in (8, 6, tight):
+        r = A.SLVOFRun(ctx, run.vf0, run.basis, run.dt, run.nsteps, every)
+        L, g = r.loss_and_grad(amps, target)
+        assert L == pytest.approx(L1, rel=1e-13)
+        assert np.abs(g - g1).max() <= 1e-10 * np.abs(g1).max(), (every, g, g1)
+
+
 def test_gradient_matches_oracle_differences(mist, oracle):
     """The same gradient against central differences of the oracle's loss (the reference's
     algorithm on the CPU) at 64 x 64, 16 steps."""

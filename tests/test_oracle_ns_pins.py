@@ -430,3 +430,24 @@ def test_fold_vof_ghosts_is_the_adjoint_of_the_fill(oracle):
     fx = c.get("VF")
     lam = rng.standard_normal(x.shape)
     assert float((A.fold_vof_ghosts(c.mesh, p, lam) * x).sum()) == pytest.approx(float((lam * fx).sum()), rel=1e-12)
+
+
+def test_checkpoint_interval_policy():
+    """The checkpoint interval a memory budget picks: the smallest k whose ceil(n / k)
+    checkpoints plus k - 1 segment states fit the budget; the slot size is the parts of
+    mist_checkpoint_save at 256-byte alignment."""
+    from ns_multiphase_amd import adjoint as A
+    from ns_multiphase_amd.params import create_mesh
+    m = create_mesh(A.adjoint_params(64))
+    per = A.checkpoint_bytes(m)
+    vof = int(np.prod(m.vof_shape()))
+    cells = int(np.prod(m.cell_shape()))
+    assert per >= 8 * vof + 2 * cells + 32 * cells + 8 * 64 * 64 + 4
+    assert per <= 8 * vof + 2 * cells + 32 * cells + 8 * 64 * 64 + 4 + 9 * 256
+    assert A.checkpoint_interval(m, 64, 10 << 30) == 1
+    assert A.checkpoint_interval(m, 64, 64 * per) == 1
+    assert A.checkpoint_interval(m, 64, 63 * per) == 2  # 32 + 1 slots
+    assert A.checkpoint_interval(m, 64, 15 * per) == 8  # the fewest slots: 8 + 7
+    with pytest.raises(ValueError):
+        A.checkpoint_interval(m, 64, 14 * per)
+    assert A.checkpoint_interval(m, 5000, 1 << 40) == 2  # 4096 slots at most
