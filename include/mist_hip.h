@@ -51,8 +51,10 @@ enum { MIST_PROJ_RK4 = 0, MIST_PROJ_EULER = 1, MIST_PROJ_MIDPOINT = 2 };    /* p
 enum { MIST_TESS_5_TETS = 0 };                                              /* tesselation */
 enum { MIST_CASE_GENERIC = 0, MIST_CASE_ZALESAK = 1 };                      /* test_case */
 enum { MIST_MG_FP64 = 0, MIST_MG_MIXED = 1 };                               /* new: V-cycle precision */
-/* new: coarse operator. MIST_COARSE_RESTRICTED_VF (coarse densities from restricted VF,
- * src/Multigrid.jl:732) is reserved and rejected with MIST_ERR_PARAM: it is not built. */
+/* new: coarse operator. MIST_COARSE_FACE_AVERAGE: coarse face coefficients are the mean of
+ * the fine ones. MIST_COARSE_RESTRICTED_VF: the reference's coarse densities, compute_dens!
+ * of the 8-cell mean of the finer level's VF (src/Multigrid.jl:725-737); its V-cycle can
+ * diverge at high density ratios (1000:1), where the solve then reports MIST_ERR_NOCONV. */
 enum { MIST_COARSE_FACE_AVERAGE = 0, MIST_COARSE_RESTRICTED_VF = 1 };
 
 /*
@@ -86,7 +88,7 @@ typedef struct mist_params {
     int64_t mg_precision;   /* MIST_MG_FP64 or MIST_MG_MIXED */
     int64_t mg_nu1, mg_nu2; /* smoothing sweeps, reference 10 and 10 */
     int64_t mg_max_cycles;  /* outer cycle limit, reference 10000 */
-    int64_t mg_coarse_op;   /* MIST_COARSE_FACE_AVERAGE (the only one built) */
+    int64_t mg_coarse_op;   /* MIST_COARSE_FACE_AVERAGE (default) or MIST_COARSE_RESTRICTED_VF */
     int64_t mg_coarse_sweeps; /* RB-GS sweeps on the coarsest level */
 } mist_params;
 

@@ -55,7 +55,7 @@ int check_params(const mist_params* p, std::string& err) {
     if (p->normal_method != MIST_NORMAL_ELVIRA) { err = "Unknown method to copute interface normal"; return MIST_ERR_PARAM; }
     if (p->projection_method < 0 || p->projection_method > 2) { err = "Unknown projection_method in project!"; return MIST_ERR_PARAM; }
     if (!(p->CFL > 0)) { err = "CFL must be > 0"; return MIST_ERR_PARAM; }
-    if (p->mg_coarse_op != MIST_COARSE_FACE_AVERAGE) { err = "mg_coarse_op: only the face-average coarse operator is built"; return MIST_ERR_PARAM; }
+    if (p->mg_coarse_op != MIST_COARSE_FACE_AVERAGE && p->mg_coarse_op != MIST_COARSE_RESTRICTED_VF) { err = "unknown mg_coarse_op"; return MIST_ERR_PARAM; }
     if (p->mg_nu1 < 0 || p->mg_nu2 < 0 || p->mg_coarse_sweeps < 0) { err = "multigrid sweep counts must be >= 0"; return MIST_ERR_PARAM; }
     return MIST_OK;
 }

@@ -6,7 +6,9 @@
 // Gauss-Seidel (src/Pressure.jl:1096-1133, colour (i+j+k)%2, red first, omega 1), 8-cell
 // average restriction (src/Multigrid.jl:183-197), trilinear prolongation from coarse centres
 // (src/Multigrid.jl:5-54). Coarse face coefficients are the mean of the fine faces covering
-// them. The outer loop evaluates the FP64 residual in flux form and applies one V-cycle in
+// them (MIST_COARSE_FACE_AVERAGE), or the reference's face densities of the liquid fraction
+// restricted to the level (MIST_COARSE_RESTRICTED_VF, src/Multigrid.jl:725-737). The outer
+// loop evaluates the FP64 residual in flux form and applies one V-cycle in
 // precision T (double, or float for the mixed mode) to it per iteration, until
 // max|b - A p| <= tol; the solution is shifted to zero mean.
 //
@@ -69,6 +71,8 @@ struct MGState {
     double* c64[3] = {nullptr, nullptr, nullptr};  // fine FP64 operator
     double* p64 = nullptr;
     double* b64 = nullptr;
+    std::vector<double*> vfl;  // restricted liquid fraction per level (plain order; 1..)
+    double dt = 0.0;           // of the solve being set up
     double* part = nullptr;
     double* h_part = nullptr;  // pinned
     int nblocks = 0;
@@ -269,6 +273,58 @@ __global__ void k_coarsen_coef(LevelGeom F, LevelGeom C, int d, const T* cf, T* 
                 s += cf[lidx(F, x[0], x[1], x[2])];
             }
         cc[q] = s * scale;
+    }
+}
+
+// The liquid fraction of level C (plain order): the mean of the finer level's over each coarse
+// cell, summed in i, then j, then k order and scaled by 1/8 (restrict!,
+// src/Multigrid.jl:183-197). From the context's VF (interior cells) when `src` is null.
+__global__ void k_restrict_vf(DevMesh m, LevelGeom F, LevelGeom C, const double* vf_field, const double* src,
+                              double* dst) {
+    const int f0 = C.coarsened[0] ? 2 : 1, f1 = C.coarsened[1] ? 2 : 1, f2 = C.coarsened[2] ? 2 : 1;
+    const double inv = 1.0 / (double)(f0 * f1 * f2);
+    const int64_t tot = C.size();
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
+        const int I = (int)(q % C.n[0]);
+        const int J = (int)((q / C.n[0]) % C.n[1]);
+        const int K = (int)(q / ((int64_t)C.n[0] * C.n[1]));
+        double s = 0.0;
+        for (int c = 0; c < f2; ++c)
+            for (int b = 0; b < f1; ++b)
+                for (int a = 0; a < f0; ++a) {
+                    const int i = I * f0 + a, j = J * f1 + b, k = K * f2 + c;
+                    s += src ? src[i + (int64_t)F.n[0] * (j + (int64_t)F.n[1] * k)]
+                             : vf_field[m.vof(i + 1, j + 1, k + 1)];
+                }
+        dst[q] = s * inv;
+    }
+}
+
+// Face coefficients of level C along d from its liquid fraction vf (plain order), in the
+// level's storage: compute_dens! (src/Tools.jl:1212-1232) gives the face density
+// rho_liq v + rho_gas (1 - v), v = clamp((vf_c + vf_{c-1}) / 2, 0, 1), and the coefficient is
+// dt / (den h^2) as on the fine level; wall faces 0, periodic faces wrapped.
+template <class T>
+__global__ void k_coef_from_vf(LevelGeom C, int d, double dt, double rl, double rg, double h2, const double* vf,
+                               T* c) {
+    const int64_t tot = C.size();
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
+        int Q[3];
+        lcell(C, q, Q[0], Q[1], Q[2]);
+        int lo[3] = {Q[0], Q[1], Q[2]};
+        bool wall = false;
+        if (Q[d] > 0) lo[d] = Q[d] - 1;
+        else if (C.per[d]) lo[d] = C.n[d] - 1;
+        else wall = true;
+        if (wall) {
+            c[q] = T(0);
+            continue;
+        }
+        const double a = vf[Q[0] + (int64_t)C.n[0] * (Q[1] + (int64_t)C.n[1] * Q[2])];
+        const double b = vf[lo[0] + (int64_t)C.n[0] * (lo[1] + (int64_t)C.n[1] * lo[2])];
+        const double v = fmin(1.0, fmax(0.0, (a + b) / 2));
+        const double den = rl * v + rg * (1 - v);
+        c[q] = (T)(dt / (den * h2));
     }
 }
 
@@ -734,6 +790,12 @@ static int build_state(mist_ctx* ctx) {
     }
     s->d.resize(s->g.size());
     s->s.resize(s->g.size());
+    s->vfl.assign(s->g.size(), nullptr);
+    if (p.mg_coarse_op == MIST_COARSE_RESTRICTED_VF)
+        for (size_t l = 1; l < s->g.size(); ++l) {
+            if (hipMalloc((void**)&s->vfl[l], (size_t)s->g[l].size() * sizeof(double)) != hipSuccess) { mg_free(s); ctx->err = "multigrid: device allocation failed"; return MIST_ERR_OOM; }
+            s->allocs.push_back(s->vfl[l]);
+        }
     int st;
     for (size_t l = 0; l < s->g.size(); ++l) {
         if ((st = alloc_level(ctx, s, s->g[l], s->d[l]))) { mg_free(s); return st; }
@@ -823,6 +885,20 @@ static int setup_hierarchy(mist_ctx* ctx, std::vector<LevelBuf<T>>& L) {
     MGState* s = ctx->mg;
     const std::vector<LevelGeom>& G = s->gq[sizeof(T) == sizeof(double) ? 0 : 1];
     for (int d = 0; d < 3; ++d) MGL(k_to_level<T>, gridn(s->nfine), 256, G[0], (const double*)s->c64[d], L[0].c[d]);
+    if (ctx->p.mg_coarse_op == MIST_COARSE_RESTRICTED_VF) {
+        double h[3] = {ctx->m.dx, ctx->m.dy, ctx->m.dz};
+        for (size_t l = 1; l < L.size(); ++l) {
+            MGL(k_restrict_vf, gridn(G[l].size()), 256, ctx->m, s->g[l - 1], s->g[l],
+                (const double*)ctx->f[MIST_FIELD_VOF], l == 1 ? (const double*)nullptr : (const double*)s->vfl[l - 1],
+                s->vfl[l]);
+            for (int d = 0; d < 3; ++d) {
+                if (G[l].coarsened[d]) h[d] *= 2.0;
+                MGL(k_coef_from_vf<T>, gridn(G[l].size()), 256, G[l], d, s->dt, ctx->p.rho_liq, ctx->p.rho_gas,
+                    h[d] * h[d], (const double*)s->vfl[l], L[l].c[d]);
+            }
+        }
+        return MIST_OK;
+    }
     for (size_t l = 1; l < L.size(); ++l)
         for (int d = 0; d < 3; ++d)
             MGL(k_coarsen_coef<T>, gridn(G[l].size()), 256, G[l - 1], G[l], d, (const T*)L[l - 1].c[d], L[l].c[d]);
@@ -908,6 +984,7 @@ int pressure_solve(mist_ctx* ctx, double dt, double tol, int precision, int64_t*
     if (st) return st;
     MGState* s = ctx->mg;
     const int64_t n = s->nfine;
+    s->dt = dt;
     MGL(k_fine_setup, gridn(n), 256, ctx->m, s->g[0], dt, (const double*)ctx->f[MIST_FIELD_DENX],
         (const double*)ctx->f[MIST_FIELD_DENY], (const double*)ctx->f[MIST_FIELD_DENZ],
         (const double*)ctx->f[MIST_FIELD_RHS], (const double*)ctx->f[MIST_FIELD_P], s->c64[0], s->c64[1], s->c64[2],

@@ -25,6 +25,7 @@ static int check_params(const mist_params* p, std::string& err) {
     if (p->tesselation != MIST_TESS_5_TETS) { err = "incompatible tesselation called"; return MIST_ERR_PARAM; }
     if (p->normal_method != MIST_NORMAL_ELVIRA) { err = "Unknown method to copute interface normal"; return MIST_ERR_PARAM; }
     if (p->projection_method < 0 || p->projection_method > 2) { err = "Unknown projection_method in project!"; return MIST_ERR_PARAM; }
+    if (p->mg_coarse_op != MIST_COARSE_FACE_AVERAGE && p->mg_coarse_op != MIST_COARSE_RESTRICTED_VF) { err = "unknown mg_coarse_op"; return MIST_ERR_PARAM; }
     return MIST_OK;
 }
 
@@ -168,6 +169,17 @@ int ora_pressure_rhs(void* h) { pressure_rhs(*(Ctx*)h); return MIST_OK; }
 int ora_corrector(void* h, double dt) { corrector(*(Ctx*)h, dt); return MIST_OK; }
 int ora_pressure_solve(void* h, double dt, double tol, int64_t precision, int64_t* cycles, double* resid) {
     return pressure_solve(*(Ctx*)h, dt, tol, (int)precision, cycles, resid);
+}
+// the FP64 face coefficients of multigrid level `level` (0 = fine) along `axis` that a
+// pressure solve with this dt would build from the context's densities and VF, plain order
+// (n[0] x n[1] x n[2] values, at most `cap`)
+int ora_mg_level_coef(void* h, double dt, int level, int axis, double* out, int64_t cap, int64_t n[3]) {
+    std::vector<double> c;
+    const int st = mg_level_coefficients(*(Ctx*)h, dt, level, axis, c, n);
+    if (st) return st;
+    if ((int64_t)c.size() > cap) return MIST_ERR_PARAM;
+    std::memcpy(out, c.data(), c.size() * sizeof(double));
+    return MIST_OK;
 }
 int64_t ora_max_cut_level(void* h) { return ((Ctx*)h)->max_cut_level; }
 

@@ -141,6 +141,7 @@ int transport_vof(Ctx& c, double dt);
 void compute_dens(Ctx& c);
 void pressure_rhs(Ctx& c);
 int pressure_solve(Ctx& c, double dt, double tol, int precision, int64_t* cycles, double* resid);
+int mg_level_coefficients(Ctx& c, double dt, int level, int axis, std::vector<double>& out, int64_t n[3]);
 void corrector(Ctx& c, double dt);
 void compute_props(Ctx& c);
 double sl_flux_vol(const Ctx& c, int dir, int64_t i, int64_t j, int64_t k, double dt, double verts[8][3]);

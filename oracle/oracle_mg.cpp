@@ -7,7 +7,9 @@
 // (src/Multigrid.jl:678-764): red-black Gauss-Seidel smoothing (src/Pressure.jl:1096-1133,
 // colour (i+j+k)%2 with red first, omega = 1), 8-cell average restriction
 // (src/Multigrid.jl:183-197), trilinear prolongation from coarse centres
-// (src/Multigrid.jl:5-54), correction added to the fine level. The outer loop evaluates
+// (src/Multigrid.jl:5-54), correction added to the fine level. Coarse operators: the mean of
+// the fine face coefficients (MIST_COARSE_FACE_AVERAGE), or the reference's face densities
+// of the restricted liquid fraction (MIST_COARSE_RESTRICTED_VF). The outer loop evaluates
 // the FP64 residual in flux form and runs one V-cycle (in T = double or float) on it per
 // iteration until max|r| <= tol; the result is shifted to zero mean (src/Pressure.jl:1160).
 #include <algorithm>
@@ -140,6 +142,52 @@ void coarsen_coefficients(const Level<T>& F, Level<T>& C) {
     }
 }
 
+// The liquid fraction of level C: the mean of the finer level's over each coarse cell
+// (restrict!, src/Multigrid.jl:183-197: the 8 fine values summed in i, then j, then k order,
+// divided by 8), vf of the finer level in plain order.
+template <class T>
+void restrict_vf(const Level<T>& F, const Level<T>& C, const std::vector<double>& vf, std::vector<double>& out) {
+    const int64_t f0 = C.coarsened[0] ? 2 : 1, f1 = C.coarsened[1] ? 2 : 1, f2 = C.coarsened[2] ? 2 : 1;
+    const double inv = 1.0 / (double)(f0 * f1 * f2);
+    out.assign((size_t)C.size(), 0.0);
+    for (int64_t K = 0; K < C.n[2]; ++K)
+        for (int64_t J = 0; J < C.n[1]; ++J)
+            for (int64_t I = 0; I < C.n[0]; ++I) {
+                double s = 0.0;
+                for (int64_t c = 0; c < f2; ++c)
+                    for (int64_t b = 0; b < f1; ++b)
+                        for (int64_t a = 0; a < f0; ++a) s += vf[(size_t)F.id(I * f0 + a, J * f1 + b, K * f2 + c)];
+                out[(size_t)C.id(I, J, K)] = s * inv;
+            }
+}
+
+// Face coefficients of a coarse level from its liquid fraction (mg_vc_lin!,
+// src/Multigrid.jl:725-737): compute_dens! (src/Tools.jl:1212-1232) gives the face density
+// rho_liq vf + rho_gas (1 - vf) with vf = clamp((VF_c + VF_{c-1}) / 2, 0, 1), and the
+// coefficient is dt / (den h^2) with the level's spacing h, as on the fine level. Wall faces
+// are dropped and periodic faces wrap, as on the fine level.
+template <class T>
+void coefficients_from_vf(Level<T>& C, const std::vector<double>& vf, double dt, double rl, double rg,
+                          const double h[3]) {
+    for (int d = 0; d < 3; ++d) {
+        C.c[d].assign((size_t)C.size(), T(0));
+        const double h2 = h[d] * h[d];
+        for (int64_t K = 0; K < C.n[2]; ++K)
+            for (int64_t J = 0; J < C.n[1]; ++J)
+                for (int64_t I = 0; I < C.n[0]; ++I) {
+                    const int64_t Q[3] = {I, J, K};
+                    int64_t lo[3] = {I, J, K};
+                    if (Q[d] > 0) lo[d] = Q[d] - 1;
+                    else if (C.per[d]) lo[d] = C.n[d] - 1;
+                    else continue;
+                    const double v = std::min(1.0, std::max(0.0, (vf[(size_t)C.id(I, J, K)] +
+                                                                  vf[(size_t)C.id(lo[0], lo[1], lo[2])]) / 2));
+                    const double den = rl * v + rg * (1 - v);
+                    C.c[d][(size_t)C.id(I, J, K)] = (T)(dt / (den * h2));
+                }
+    }
+}
+
 template <class T>
 void prolong_add(const Level<T>& C, Level<T>& F) {
 #pragma omp parallel for collapse(2) schedule(static)
@@ -192,13 +240,11 @@ void vcycle(std::vector<Level<T>>& lv, size_t l, int nu1, int nu2, int coarse_sw
     smooth(L, nu2);
 }
 
+// The fine FP64 operator and the hierarchy of precision T with its coarse operators.
 template <class T>
-int solve(Ctx& c, double dt, double tol, int64_t* cycles_out, double* resid_out) {
+int build_levels(Ctx& c, double dt, Level<double>& fine, std::vector<Level<T>>& lv) {
     const Mesh& m = c.m;
     const mist_params& p = c.p;
-    // Level geometry
-    std::vector<Level<T>> lv;
-    Level<double> fine;  // FP64 operator for the outer residual
     fine.n[0] = m.Nx; fine.n[1] = m.Ny; fine.n[2] = m.Nz;
     fine.per[0] = p.xper && m.Nx > 1; fine.per[1] = p.yper && m.Ny > 1; fine.per[2] = p.zper && m.Nz > 1;
     const double h2[3] = {m.dx * m.dx, m.dy * m.dy, m.dz * m.dz};
@@ -242,13 +288,38 @@ int solve(Ctx& c, double dt, double tol, int64_t* cycles_out, double* resid_out)
             lv[l].n[d] = nf > 1 ? nf / 2 : 1;
             lv[l].per[d] = lv[l - 1].per[d] && lv[l].n[d] > 1;
         }
-        coarsen_coefficients(lv[l - 1], lv[l]);
+        if (p.mg_coarse_op != MIST_COARSE_RESTRICTED_VF) coarsen_coefficients(lv[l - 1], lv[l]);
+    }
+    if (p.mg_coarse_op == MIST_COARSE_RESTRICTED_VF) {
+        std::vector<double> vf((size_t)fine.size()), vc;
+        for (int64_t k = 0; k < m.Nz; ++k)
+            for (int64_t j = 0; j < m.Ny; ++j)
+                for (int64_t i = 0; i < m.Nx; ++i) vf[(size_t)fine.id(i, j, k)] = c.VF(i + 1, j + 1, k + 1);
+        double h[3] = {m.dx, m.dy, m.dz};
+        for (size_t l = 1; l < lv.size(); ++l) {
+            restrict_vf(lv[l - 1], lv[l], vf, vc);
+            for (int d = 0; d < 3; ++d)
+                if (lv[l].coarsened[d]) h[d] *= 2.0;
+            coefficients_from_vf(lv[l], vc, dt, p.rho_liq, p.rho_gas, h);
+            vf.swap(vc);
+        }
     }
     for (auto& L : lv) {
         L.e.assign((size_t)L.size(), T(0));
         L.b.assign((size_t)L.size(), T(0));
         L.r.assign((size_t)L.size(), T(0));
     }
+    return MIST_OK;
+}
+
+template <class T>
+int solve(Ctx& c, double dt, double tol, int64_t* cycles_out, double* resid_out) {
+    const Mesh& m = c.m;
+    const mist_params& p = c.p;
+    std::vector<Level<T>> lv;
+    Level<double> fine;  // FP64 operator for the outer residual
+    const int bst = build_levels(c, dt, fine, lv);
+    if (bst) return bst;
     // right-hand side (mean removed) and initial guess
     fine.b.resize((size_t)fine.size());
     fine.e.resize((size_t)fine.size());
@@ -267,7 +338,16 @@ int solve(Ctx& c, double dt, double tol, int64_t* cycles_out, double* resid_out)
     for (;;) {
         residual(fine);
         res = 0.0;
-        for (double v : fine.r) res = std::max(res, std::fabs(v));
+        bool nan = false;
+        for (double v : fine.r) {
+            nan |= v != v;
+            res = std::max(res, std::fabs(v));
+        }
+        if (nan) {  // a diverged iteration: reported, not taken for convergence
+            res = NAN;
+            st = MIST_ERR_NOCONV;
+            break;
+        }
         if (res <= tol) break;
         if (cycles >= maxc) { st = MIST_ERR_NOCONV; break; }
         for (size_t q = 0; q < fine.r.size(); ++q) { lv[0].b[q] = (T)fine.r[q]; lv[0].e[q] = T(0); }
@@ -289,6 +369,18 @@ int solve(Ctx& c, double dt, double tol, int64_t* cycles_out, double* resid_out)
 }
 
 }  // namespace
+
+int mg_level_coefficients(Ctx& c, double dt, int level, int axis, std::vector<double>& out, int64_t n[3]) {
+    Level<double> fine;
+    std::vector<Level<double>> lv;
+    const int st = build_levels(c, dt, fine, lv);
+    if (st) return st;
+    if (level < 0 || (size_t)level >= lv.size() || axis < 0 || axis > 2) return MIST_ERR_PARAM;
+    const Level<double>& L = lv[(size_t)level];
+    for (int d = 0; d < 3; ++d) n[d] = L.n[d];
+    out = L.c[axis];
+    return MIST_OK;
+}
 
 int pressure_solve(Ctx& c, double dt, double tol, int precision, int64_t* cycles, double* resid) {
     if (precision == MIST_MG_MIXED) return solve<float>(c, dt, tol, cycles, resid);

@@ -60,6 +60,7 @@ def lib():
             "ora_corrector": (i32, [vp, d]),
             "ora_pressure_solve": (i32, [vp, d, d, i64, ip, dp]),
             "ora_max_cut_level": (i64, [vp]),
+            "ora_mg_level_coef": (i32, [vp, d, i32, i32, dp, i64, ip]),
             "ora_vof_init": (i32, [vp]),
             "ora_vof_step": (i32, [vp, dp, d, dp]),
             "ora_reduce": (i32, [vp, i32, i32, dp]),
@@ -208,6 +209,17 @@ class OracleContext:
         if check:
             self._check(st)
         return cyc.value, res.value
+
+    def mg_level_coef(self, dt, level, axis):
+        """FP64 face coefficients of multigrid level `level` (0 = fine) along `axis` that a
+        pressure solve with this dt builds, as an (n0, n1, n2) array indexed [i, j, k]."""
+        ce = self.extents("P")[1]
+        cap = int(ce[0] * ce[1] * ce[2])
+        out = np.zeros(cap)
+        n = (ctypes.c_int64 * 3)()
+        self._check(lib().ora_mg_level_coef(self.h, dt, int(level), int(axis), _dp(out), cap, n))
+        m = int(n[0] * n[1] * n[2])
+        return out[:m].reshape((int(n[2]), int(n[1]), int(n[0]))).transpose(2, 1, 0)
 
     def vof_init(self):
         self._check(lib().ora_vof_init(self.h))

@@ -63,7 +63,7 @@ def test_context_without_device_fails_cleanly():
     ("tesselation", 7, b"incompatible tesselation called"),
     ("normal_method", 3, b"Unknown method to copute interface normal"),
     ("projection_method", 5, b"Unknown projection_method in project!"),
-    ("mg_coarse_op", 1, b"only the face-average coarse operator is built"),
+    ("mg_coarse_op", 2, b"unknown mg_coarse_op"),
     ("mg_nu1", -1, b"sweep counts must be >= 0"),
     ("CFL", 0.0, b"CFL must be > 0"),
     ("Nx", 0, b"Nx, Ny, Nz must be >= 1"),

@@ -194,6 +194,40 @@ def test_pressure_mms_matches_oracle(mist, oracle, precision):
     assert np.abs(Pg - ex).max() <= 2e-3  # O(h^2) discretisation error at N = 32
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_pressure_restricted_vf_coarse_operator_matches_oracle(mist, oracle, precision):
+    """The reference's coarse operator (face densities of the restricted VF) on the 10:1 MMS at
+    32^3. After exactly two V-cycles the device's residual equals the oracle's (the face-average
+    operator leaves a different one, so the coarse coefficients are what is compared); the full
+    solve takes the same cycles to the same residual and the pressure of the face-average
+    solve. At 1000:1 the iteration diverges on both sides and the device reports it."""
+    N, dt = 32, 1e-3
+
+    def run(ctx_of, op, prec, **kw):
+        c = ctx_of(W.mms_params(N, rho_heavy=10.0, mg_coarse_op=op, **kw))
+        W.mms_setup(c, dt)
+        cyc, res = c.pressure_solve(dt, 1e-10, prec, check=False)
+        return c, cyc, res
+    _, _, r2g = run(mist.Context, "restricted-vf", precision, mg_max_cycles=2)
+    _, _, r2o = run(oracle.OracleContext, "restricted-vf", precision, mg_max_cycles=2)
+    _, _, r2f = run(oracle.OracleContext, "face-average", precision, mg_max_cycles=2)
+    tol2 = 1e-6 if precision == "fp64" else 1e-3  # FP32 V-cycles: rounding of 1e-7 relative per sweep
+    assert abs(r2g - r2o) <= tol2 * r2o, (r2g, r2o)
+    assert abs(r2f - r2o) > 1e-2 * r2o, (r2f, r2o)
+    g, cg, rg = run(mist.Context, "restricted-vf", precision)
+    o, co, ro = run(oracle.OracleContext, "restricted-vf", precision)
+    f, cf, rf = run(mist.Context, "face-average", precision)
+    assert rg <= 1e-10 and ro <= 1e-10 and rf <= 1e-10
+    assert abs(cg - co) <= 1
+    Pg = W.interior(g.mesh, g.get("P"))
+    assert np.abs(Pg - W.interior(o.mesh, o.get("P"))).max() <= 1e-6
+    assert np.abs(Pg - W.interior(f.mesh, f.get("P"))).max() <= 1e-6
+    d = mist.Context(W.mms_params(16, mg_coarse_op="restricted-vf", mg_max_cycles=40))
+    W.mms_setup(d, dt)
+    with pytest.raises(mist.MistError, match="did not converge"):
+        d.pressure_solve(dt, 1e-10, precision)
+
+
 def test_pressure_fp64_vs_mixed_same_residual(mist):
     N = 64
     dt = 1e-3
