@@ -46,7 +46,8 @@ enum {
 enum { MIST_PS_SEMI_LAGRANGIAN = 0, MIST_PS_FINITE_DIFFERENCE = 1 };      /* pressure_scheme */
 enum { MIST_NORMAL_ELVIRA = 0 };                                           /* normalMethod */
 enum { MIST_VEL_NONE = 0, MIST_VEL_DEFORMATION = 1, MIST_VEL_DEFORMATION3D = 2,
-       MIST_VEL_ROTATION = 3, MIST_VEL_DIVFLOW = 4, MIST_VEL_DIVFLOW3D = 5 }; /* VFVelocity */
+       MIST_VEL_ROTATION = 3, MIST_VEL_DIVFLOW = 4, MIST_VEL_DIVFLOW3D = 5,
+       MIST_VEL_AIRBLAST = 6 };                                             /* VFVelocity */
 enum { MIST_PROJ_RK4 = 0, MIST_PROJ_EULER = 1, MIST_PROJ_MIDPOINT = 2 };    /* projection_method */
 enum { MIST_TESS_5_TETS = 0 };                                              /* tesselation */
 enum { MIST_CASE_GENERIC = 0, MIST_CASE_ZALESAK = 1 };                      /* test_case */

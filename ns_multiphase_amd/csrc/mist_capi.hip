@@ -363,7 +363,7 @@ int mist_update_borders(mist_ctx* ctx, int field) {
 
 int mist_define_velocity(mist_ctx* ctx, double t) {
     CHECK_CTX();
-    if (ctx->p.vf_velocity < 0 || ctx->p.vf_velocity > MIST_VEL_DIVFLOW3D || ctx->p.vf_velocity == MIST_VEL_NONE)
+    if (ctx->p.vf_velocity < 0 || ctx->p.vf_velocity > MIST_VEL_AIRBLAST || ctx->p.vf_velocity == MIST_VEL_NONE)
         return fail(ctx, MIST_ERR_PARAM, "Unknown VFVelocity");
     RET_IF(mist::launch_define_velocity(ctx, t));
     return MIST_OK;

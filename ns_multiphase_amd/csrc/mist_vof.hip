@@ -37,6 +37,7 @@ __device__ __forceinline__ void vel_fun(int kind, double T, double x, double y, 
         case MIST_VEL_ROTATION: u = twopi * (0.5 - y); v = twopi * (x - 0.5); w = 0.0; break;
         case MIST_VEL_DIVFLOW: u = 2.0 * x; v = -1.0 * (2.5 - y); w = 0.0; break;
         case MIST_VEL_DIVFLOW3D: u = 200.0 * x; v = -100.0 * (5.0 - y); w = -50.0 * (5.0 - z); break;
+        case MIST_VEL_AIRBLAST: u = 50.0 * (1.0 - exp(-100.0 * fabs(y - 0.5))); v = 0.0; w = 0.0; break;
         default: u = v = w = 0.0;
     }
 }

@@ -19,7 +19,7 @@ import numpy as np
 PRESSURE_SCHEMES = {"semi-lagrangian": 0, "finite-difference": 1}
 NORMAL_METHODS = {"ELVIRA": 0}
 VF_VELOCITIES = {"Nothing": 0, "Deformation": 1, "Deformation3D": 2, "rotation": 3,
-                 "divFlow": 4, "divFlow3D": 5}
+                 "divFlow": 4, "divFlow3D": 5, "airblast": 6}
 PROJECTION_METHODS = {"RK4": 0, "Euler": 1, "Midpoint": 2}
 TESSELATIONS = {"5_tets": 0}
 MG_PRECISIONS = {"fp64": 0, "mixed": 1}

@@ -314,6 +314,9 @@ static void vel_fun(int kind, double T, double x, double y, double z, double t, 
         case MIST_VEL_DIVFLOW3D:
             *u = 200.0 * x; *v = -100.0 * (5.0 - y); *w = -50.0 * (5.0 - z);
             break;
+        case MIST_VEL_AIRBLAST:
+            *u = 50.0 * (1.0 - std::exp(-100.0 * std::fabs(y - 0.5))); *v = 0.0; *w = 0.0;
+            break;
         default:
             *u = *v = *w = 0.0;
     }

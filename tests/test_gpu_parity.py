@@ -132,6 +132,24 @@ def test_deformation_steps(mist, oracle, case):
     _vof_state_equal(g, o, g.mesh)
 
 
+@pytest.mark.parametrize("variant", ["Euler", "Midpoint", "Zalesak"])
+def test_deformation_backtrace_variants(mist, oracle, variant):
+    """The other back-traces of project! (src/Tools.jl:392-418): explicit Euler, the midpoint
+    rule, and the closed-form rotation about (0.5, 0.5) of test_case "Zalesak", each on 16
+    steps of the 2D deformation (48^2, circle off the centre) against the oracle."""
+    kw = {"test_case": "Zalesak"} if variant == "Zalesak" else {"projection_method": variant}
+    param, _ = W.deformation2d(48, **kw)
+    g, o = _pair(mist, oracle, param, ("circle", 0.15, 0.42, 0.7))
+    g.vof_init()
+    o.vof_init()
+    tg = to = 0.0
+    for s in range(16):
+        tg, dtg = g.vof_step(tg)
+        to, dto = o.vof_step(to)
+        assert dtg == dto and tg == to
+    _vof_state_equal(g, o, g.mesh)
+
+
 @pytest.mark.parametrize("solve_ns", [False, True])
 def test_transport_planar_uniform_flow_exact(mist, solve_ns):
     """One SL step of a planar interface in uniform flow equals the exact shifted plane (the
@@ -155,6 +173,22 @@ def test_transport_planar_uniform_flow_exact(mist, solve_ns):
     sl = slice(4 + g + 2, N - 3 + g + 3)
     exact = planar_vf(m, n, D0 + float(n @ U) * dt)
     assert np.abs(ctx.get("VF")[sl, sl, sl] - exact[sl, sl, sl]).max() <= 1e-12
+
+
+@pytest.mark.parametrize("kind", ["Deformation", "rotation", "divFlow", "divFlow3D", "airblast"])
+def test_prescribed_velocities_match_oracle(mist, oracle, kind):
+    """Every prescribed field of defineVelocity! (src/Tools.jl:648-693) on centres and faces,
+    ghosts included, against the oracle (device sin/cos/exp may differ from the host libm by an
+    ulp), and compute_dt of the result."""
+    from test_oracle_pins import cube_param
+    p = cube_param(12, Nz=6, Lz=0.5, VFVelocity=kind, vel_period=2.0)
+    g, o = mist.Context(p), oracle.OracleContext(p)
+    g.define_velocity(0.3)
+    o.define_velocity(0.3)
+    for name in ("u", "v", "w", "uf", "vf", "wf"):
+        a, b = g.get(name), o.get(name)
+        assert np.abs(a - b).max() <= 1e-14 * max(1.0, np.abs(b).max()), name
+    assert g.compute_dt() == pytest.approx(o.compute_dt(), rel=1e-14)
 
 
 def test_reduce_and_dt(mist, oracle):

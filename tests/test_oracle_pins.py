@@ -330,3 +330,27 @@ def test_band_flags(oracle):
     for s in [(a, b_, d) for a in (-1, 0, 1) for b_ in (-1, 0, 1) for d in (-1, 0, 1)]:
         nb = pad[1 + s[0]:1 + s[0] + N, 1 + s[1]:1 + s[1] + N, 1 + s[2]:1 + s[2] + N]
         assert np.all(np.abs(bi) <= np.minimum(nb + 1, 3))
+
+
+def test_airblast_velocity_properties(oracle):
+    """The "airblast" field u = 50 (1 - exp(-100 |y - 0.5|)) (src/Tools.jl:676-679): even about
+    the line y = 0.5, rising away from it and saturating at 50 (e^-100|y-0.5| drops below an
+    ulp of 1 within a quarter unit), no v or w, a function of y alone."""
+    p = cube_param(16, Nz=4, Lz=0.25, VFVelocity="airblast")
+    c = oracle.OracleContext(p)
+    c.define_velocity(0.7)
+    m = c.mesh
+    uf, u = c.get("uf"), c.get("u")
+    assert np.all(c.get("v") == 0.0) and np.all(c.get("w") == 0.0) and np.all(c.get("vf") == 0.0)
+    prof = u[3, :, 2]
+    ym = np.asarray(m.ym)
+    half = ym - 0.5
+    assert np.all(np.diff(prof[half > 0]) >= 0) and np.all(np.diff(prof[half < 0]) <= 0)
+    assert np.diff(prof[half > 0])[0] > 0  # strictly rising next to the line
+    for j in range(len(ym)):
+        jj = int(np.argmin(np.abs(half + half[j])))  # mirror cell about y = 0.5
+        assert abs(prof[j] - prof[jj]) <= 1e-12
+    assert np.all(prof <= 50.0) and prof.max() > 50.0 - 1e-6
+    assert np.all(uf[:, :, 2] == uf[3, :, 2][None, :])  # depends on y only
+    # y = 0.5 lies on a face row of y when Ny is even: the centre profile brackets zero
+    assert prof[np.argmin(np.abs(half))] == pytest.approx(50.0 * (1 - np.exp(-100 * np.abs(half).min())), rel=1e-14)
