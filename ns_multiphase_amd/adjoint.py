@@ -138,14 +138,19 @@ class SLVOFRun:
         self.nseg = (self.nsteps + self.every - 1) // self.every
         if self.nseg + self.every - 1 > MAX_SLOTS:
             raise ValueError(f"{self.nseg + self.every - 1} checkpoint slots needed, at most {MAX_SLOTS}")
-        zeros = np.zeros(ctx.get("wf").shape)
-        self._wf0 = zeros
+        # the modes as rows of two matrices, so that the faces of a set of amplitudes and the
+        # projection of the face adjoints onto the modes are one matrix-vector product each
+        self._shape = basis[0][0].shape
+        self._Bu = np.stack([b[0].ravel() for b in basis])
+        self._Bv = np.stack([b[1].ravel() for b in basis])
+        self._zero = {name: np.zeros(tuple(int(v) for v in ctx.extents(name)[1]))
+                      for name in ("wf", "uf_bar", "vf_bar", "wf_bar")}
 
     def set_velocity(self, amps):
-        uf, vf = faces_of(self.basis, amps)
-        self.ctx.set("uf", uf)
-        self.ctx.set("vf", vf)
-        self.ctx.set("wf", self._wf0)
+        a = np.asarray(amps, np.float64)
+        self.ctx.set("uf", (a @ self._Bu).reshape(self._shape))
+        self.ctx.set("vf", (a @ self._Bv).reshape(self._shape))
+        self.ctx.set("wf", self._zero["wf"])
 
     def _step(self):
         self.ctx.transport(self.dt)
@@ -167,7 +172,7 @@ class SLVOFRun:
         c = self.ctx
         c.set("VF_bar", lam_T)
         for name in ("uf_bar", "vf_bar", "wf_bar"):
-            c.set(name, np.zeros(c.get(name).shape))
+            c.set(name, self._zero[name])
         base = self.nseg  # slots after the checkpoints hold a segment's later states
         for seg in range(self.nseg - 1, -1, -1):
             s0 = seg * self.every
@@ -184,7 +189,7 @@ class SLVOFRun:
                 c.checkpoint_restore(slot(s))
                 c.transport_vjp(self.dt)
         guf, gvf = c.get("uf_bar"), c.get("vf_bar")
-        return np.array([(guf * b[0]).sum() + (gvf * b[1]).sum() for b in self.basis])
+        return self._Bu @ guf.ravel() + self._Bv @ gvf.ravel()
 
     def loss_and_grad(self, amps, target):
         """L = 1/2 sum over interior cells (VF - target)^2 and dL/d(amps)."""
