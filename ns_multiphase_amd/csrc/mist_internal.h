@@ -81,6 +81,7 @@ struct mist_ctx {
     KernelProfile prof;
     double bc_wall[6][3] = {};   // wall velocity of sides x-, x+, y-, y+, z-, z+ (reflect BC)
     int* elv_best = nullptr;     // ELVIRA candidate kept per interior cell (-1: none)
+    double* elv_dist = nullptr;  // computeDist of the 27 candidates per interface cell (allocated on first use)
     double* adj_lam = nullptr;   // adjoint work: dL/dVF of the step input (VF extents)
     double* adj_plane = nullptr; // adjoint work: planes nx, ny, nz, D (4 cell-extent arrays)
     double* adj_pv = nullptr;    // adjoint work: back-traced corners (3 per grid vertex)

@@ -319,9 +319,73 @@ __global__ void k_normals_prep(DevMesh m, DevParams p, const double* VF, double*
     }
 }
 
+// The normal of ELVIRA candidate c of cell (i, j, k) (ELVIRA_calc!, src/ELVIRA.jl:62-100:
+// family c / 9 = X-Y, X-Z or Y-Z stencils, outer stencil pair (c % 9) / 3, inner c % 3) from the
+// 3x3x3 liquid fractions at(di, dj, dk), normalised.
+template <class At>
+__device__ __forceinline__ void elvira_normal(const DevMesh& m, int i, int j, int k, int c, At&& at, double& cnx,
+                                              double& cny, double& cnz) {
+    const int fam = c / 9, a = (c % 9) / 3, b = c % 3;
+    // stencil pairs [[-1,0],[-1,1],[0,1]] of src/ELVIRA.jl:72
+    auto P0 = [](int q) { return q == 2 ? 0 : -1; };
+    auto P1 = [](int q) { return q == 0 ? 0 : 1; };
+    if (fam == 0) {  // X-Y stencils (src/ELVIRA.jl:71-78)
+        const int i1 = P0(b), i2 = P1(b), j1 = P0(a), j2 = P1(a);
+        const double dz = m.dz;
+        const double sx2 = (at(i2, 0, -1) * dz + at(i2, 0, 0) * dz) + at(i2, 0, 1) * dz;
+        const double sx1 = (at(i1, 0, -1) * dz + at(i1, 0, 0) * dz) + at(i1, 0, 1) * dz;
+        cnx = -(sx2 - sx1) / (m.xm[i + i2] - m.xm[i + i1]);
+        const double sy2 = (at(0, j2, -1) * dz + at(0, j2, 0) * dz) + at(0, j2, 1) * dz;
+        const double sy1 = (at(0, j1, -1) * dz + at(0, j1, 0) * dz) + at(0, j1, 1) * dz;
+        cny = -(sy2 - sy1) / (m.ym[j + j2] - m.ym[j + j1]);
+        cnz = at(0, 0, 1) > at(0, 0, -1) ? -1.0 : 1.0;
+    } else if (fam == 1) {  // X-Z stencils (src/ELVIRA.jl:81-88)
+        const int i1 = P0(b), i2 = P1(b), k1 = P0(a), k2 = P1(a);
+        const double dy = m.dy;
+        const double sx2 = (at(i2, -1, 0) * dy + at(i2, 0, 0) * dy) + at(i2, 1, 0) * dy;
+        const double sx1 = (at(i1, -1, 0) * dy + at(i1, 0, 0) * dy) + at(i1, 1, 0) * dy;
+        cnx = -(sx2 - sx1) / (m.xm[i + i2] - m.xm[i + i1]);
+        cny = at(0, 1, 0) > at(0, -1, 0) ? -1.0 : 1.0;
+        const double sz2 = (at(0, -1, k2) * dy + at(0, 0, k2) * dy) + at(0, 1, k2) * dy;
+        const double sz1 = (at(0, -1, k1) * dy + at(0, 0, k1) * dy) + at(0, 1, k1) * dy;
+        cnz = -(sz2 - sz1) / (m.zm[k + k2] - m.zm[k + k1]);
+    } else {  // Y-Z stencils (src/ELVIRA.jl:91-98)
+        const int j1 = P0(b), j2 = P1(b), k1 = P0(a), k2 = P1(a);
+        const double dx = m.dx;
+        cnx = at(1, 0, 0) > at(-1, 0, 0) ? -1.0 : 1.0;
+        const double sy2 = (at(-1, j2, 0) * dx + at(0, j2, 0) * dx) + at(1, j2, 0) * dx;
+        const double sy1 = (at(-1, j1, 0) * dx + at(0, j1, 0) * dx) + at(1, j1, 0) * dx;
+        cny = -(sy2 - sy1) / (m.ym[j + j2] - m.ym[j + j1]);
+        const double sz2 = (at(-1, 0, k2) * dx + at(0, 0, k2) * dx) + at(1, 0, k2) * dx;
+        const double sz1 = (at(-1, 0, k1) * dx + at(0, 0, k1) * dx) + at(1, 0, k1) * dx;
+        cnz = -(sz2 - sz1) / (m.zm[k + k2] - m.zm[k + k1]);
+    }
+    const double norm = sqrt((cnx * cnx + cny * cny) + cnz * cnz);
+    cnx = cnx / norm; cny = cny / norm; cnz = cnz / norm;
+}
+
+// computeDist (src/VF.jl:109-156) of every ELVIRA candidate of every interface cell, into
+// dist[27 cidx + c]. Its own kernel: the closed-form root's registers would otherwise spill
+// in the scoring kernel's loops.
+__global__ __launch_bounds__(256) void k_elvira_dist(DevMesh m, DevParams p, const double* VF, const int* list,
+                                                     const int* count, int fam_mask, double* dist) {
+    const int64_t n = (int64_t)(*count) * 27;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        const int cidx = (int)(q / 27), c = (int)(q - 27 * (int64_t)cidx);
+        if (!((fam_mask >> (c / 9)) & 1)) continue;
+        int i, j, k;
+        interior_of(m, list[cidx], i, j, k);
+        auto at = [&](int di, int dj, int dk) { return VF[m.vof(i + di, j + dj, k + dk)]; };
+        double cnx, cny, cnz;
+        elvira_normal(m, i, j, k, c, at, cnx, cny, cnz);
+        dist[q] = compute_dist(m, p, i, j, k, cnx, cny, cnz, at(0, 0, 0));
+    }
+}
+
 // ELVIRA_calc! + check_normal! (src/ELVIRA.jl:62-137) with one interface cell per work-group
 // of 256 threads. Threads 0..26 load the 3x3x3 liquid fractions and build the 27 candidate
-// normals and their PLIC constants once (computeDist). Each candidate's error is the sum over
+// normals once; their PLIC constants (computeDist) come from k_elvira_dist, which evaluates the
+// same normals bit for bit. Each candidate's error is the sum over
 // the 27 neighbours, in the reference's order, of (VF - computePLIC2VF)^2, where
 // computePLIC2VF adds, tet by tet, the liquid sub-tet volumes of the cut (src/VF.jl:219-278).
 //
@@ -338,8 +402,8 @@ constexpr int ELV_THREADS = 256;
 constexpr int ELV_PAIRS = 27 * 9;  // candidate-neighbour pairs of one z-plane of neighbours
 
 __global__ __launch_bounds__(ELV_THREADS, 4) void k_elvira(DevMesh m, DevParams p, const double* VF, const int* list,
-                                                        const int* count, int fam_mask, double* nxo, double* nyo,
-                                                        double* nzo, int* best) {
+                                                        const int* count, int fam_mask, const double* dist,
+                                                        double* nxo, double* nyo, double* nzo, int* best) {
     __shared__ double sVF[27];
     __shared__ double sC[27][4];
     __shared__ int sOk[27];
@@ -380,49 +444,13 @@ __global__ __launch_bounds__(ELV_THREADS, 4) void k_elvira(DevMesh m, DevParams 
             sFull[tid] = f;
         }
         if (tid < 27) {
-            const int fam = tid / 9, a = (tid % 9) / 3, b = tid % 3;
-            int ok = (fam_mask >> fam) & 1;
+            const int ok = (fam_mask >> (tid / 9)) & 1;
             if (ok) {
                 auto at = [&](int di, int dj, int dk) { return sVF[(di + 1) + 3 * (dj + 1) + 9 * (dk + 1)]; };
-                // stencil pairs [[-1,0],[-1,1],[0,1]] of src/ELVIRA.jl:72
-                auto P0 = [](int q) { return q == 2 ? 0 : -1; };
-                auto P1 = [](int q) { return q == 0 ? 0 : 1; };
                 double cnx, cny, cnz;
-                if (fam == 0) {  // X-Y stencils (src/ELVIRA.jl:71-78)
-                    const int i1 = P0(b), i2 = P1(b), j1 = P0(a), j2 = P1(a);
-                    const double dz = m.dz;
-                    const double sx2 = (at(i2, 0, -1) * dz + at(i2, 0, 0) * dz) + at(i2, 0, 1) * dz;
-                    const double sx1 = (at(i1, 0, -1) * dz + at(i1, 0, 0) * dz) + at(i1, 0, 1) * dz;
-                    cnx = -(sx2 - sx1) / (m.xm[i + i2] - m.xm[i + i1]);
-                    const double sy2 = (at(0, j2, -1) * dz + at(0, j2, 0) * dz) + at(0, j2, 1) * dz;
-                    const double sy1 = (at(0, j1, -1) * dz + at(0, j1, 0) * dz) + at(0, j1, 1) * dz;
-                    cny = -(sy2 - sy1) / (m.ym[j + j2] - m.ym[j + j1]);
-                    cnz = at(0, 0, 1) > at(0, 0, -1) ? -1.0 : 1.0;
-                } else if (fam == 1) {  // X-Z stencils (src/ELVIRA.jl:81-88)
-                    const int i1 = P0(b), i2 = P1(b), k1 = P0(a), k2 = P1(a);
-                    const double dy = m.dy;
-                    const double sx2 = (at(i2, -1, 0) * dy + at(i2, 0, 0) * dy) + at(i2, 1, 0) * dy;
-                    const double sx1 = (at(i1, -1, 0) * dy + at(i1, 0, 0) * dy) + at(i1, 1, 0) * dy;
-                    cnx = -(sx2 - sx1) / (m.xm[i + i2] - m.xm[i + i1]);
-                    cny = at(0, 1, 0) > at(0, -1, 0) ? -1.0 : 1.0;
-                    const double sz2 = (at(0, -1, k2) * dy + at(0, 0, k2) * dy) + at(0, 1, k2) * dy;
-                    const double sz1 = (at(0, -1, k1) * dy + at(0, 0, k1) * dy) + at(0, 1, k1) * dy;
-                    cnz = -(sz2 - sz1) / (m.zm[k + k2] - m.zm[k + k1]);
-                } else {  // Y-Z stencils (src/ELVIRA.jl:91-98)
-                    const int j1 = P0(b), j2 = P1(b), k1 = P0(a), k2 = P1(a);
-                    const double dx = m.dx;
-                    cnx = at(1, 0, 0) > at(-1, 0, 0) ? -1.0 : 1.0;
-                    const double sy2 = (at(-1, j2, 0) * dx + at(0, j2, 0) * dx) + at(1, j2, 0) * dx;
-                    const double sy1 = (at(-1, j1, 0) * dx + at(0, j1, 0) * dx) + at(1, j1, 0) * dx;
-                    cny = -(sy2 - sy1) / (m.ym[j + j2] - m.ym[j + j1]);
-                    const double sz2 = (at(-1, 0, k2) * dx + at(0, 0, k2) * dx) + at(1, 0, k2) * dx;
-                    const double sz1 = (at(-1, 0, k1) * dx + at(0, 0, k1) * dx) + at(1, 0, k1) * dx;
-                    cnz = -(sz2 - sz1) / (m.zm[k + k2] - m.zm[k + k1]);
-                }
-                const double norm = sqrt((cnx * cnx + cny * cny) + cnz * cnz);
-                cnx = cnx / norm; cny = cny / norm; cnz = cnz / norm;
+                elvira_normal(m, i, j, k, tid, at, cnx, cny, cnz);
                 sC[tid][0] = cnx; sC[tid][1] = cny; sC[tid][2] = cnz;
-                sC[tid][3] = compute_dist(m, p, i, j, k, cnx, cny, cnz, at(0, 0, 0));
+                sC[tid][3] = dist[27 * (int64_t)cidx + tid];
             }
             sOk[tid] = ok;
         }
@@ -1387,9 +1415,13 @@ int launch_normals(mist_ctx* ctx) {
            ctx->iface_list, ctx->counters + 1);
     const int fam = (ctx->m.Nz > 1 ? 1 : 0) | (ctx->m.Ny > 1 ? 2 : 0) | (ctx->m.Nx > 1 ? 4 : 0);
     const int64_t ncap = (int64_t)ctx->m.Nx * ctx->m.Ny * ctx->m.Nz;
-    hipEvent_t ev = prof_begin(ctx, MIST_PROF_ELVIRA);
+    if (!ctx->elv_dist) MIST_HIP_TRY(hipMalloc(&ctx->elv_dist, (size_t)ncap * 27 * sizeof(double)));
+    hipEvent_t ev = prof_begin(ctx, MIST_PROF_ELVIRA);  // both kernels of ELVIRA
+    LAUNCH(k_elvira_dist, grid_for(ncap * 27, 256, 8192), 256, ctx->m, ctx->dp, (const double*)ctx->f[MIST_FIELD_VOF],
+           (const int*)ctx->iface_list, (const int*)(ctx->counters + 1), fam, ctx->elv_dist);
     LAUNCH(k_elvira, grid_for(ncap, 1, 16384), ELV_THREADS, ctx->m, ctx->dp, (const double*)ctx->f[MIST_FIELD_VOF],
-           (const int*)ctx->iface_list, (const int*)(ctx->counters + 1), fam, nx, ny, nz, ctx->elv_best);
+           (const int*)ctx->iface_list, (const int*)(ctx->counters + 1), fam, (const double*)ctx->elv_dist, nx, ny, nz,
+           ctx->elv_best);
     prof_end(ctx, MIST_PROF_ELVIRA, ev, ctx->counters + 1, 0);
     int st;
     if ((st = launch_borders(ctx, MIST_FIELD_NX))) return st;
