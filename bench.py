@@ -76,12 +76,11 @@ def counted_flops(N):
             "N": N, "interface_cells": nif, "band_cells": nband}
 
 
-# HBM bytes per unit (interface cell for k_elvira, band cell for the cut pass) from
-# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per dispatch on 20 steps of this workload, FETCH_SIZE
-# scaled by 2.0 and WRITE_SIZE by 1.0 (what the counters read for a plain 256 MiB device copy);
-# see profiles/pmc_traffic_deformation3d_128.md. k_elvira's figure is almost all WRITE_SIZE: its
-# spilled registers (scratch stores), not field traffic.
-TRAFFIC_PER_UNIT = {"k_elvira": 32985.0, "k_sl_tree": 850.0}
+# HBM bytes per unit (interface cell for ELVIRA's two kernels together, band cell for the cut
+# pass) from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per dispatch on 20 steps of this workload,
+# FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0 (what the counters read for a plain 256 MiB
+# device copy); see profiles/pmc_traffic_deformation3d_128.md.
+TRAFFIC_PER_UNIT = {"k_elvira_dist + k_elvira": 1712.0, "k_sl_tree": 849.0}
 
 
 def run_vof(args, torch):
@@ -484,7 +483,8 @@ def main():
     # rooflines of the two FP64 kernels of the step; the dominant one is `roofline`
     fc = counted_flops(min(args.N, 64))
     roofs = {}
-    for key, kern, per, unit_name in (("elvira", "k_elvira", fc["elvira_per_interface_cell"], "interface_cells"),
+    for key, kern, per, unit_name in (("elvira", "k_elvira_dist + k_elvira", fc["elvira_per_interface_cell"],
+                                       "interface_cells"),
                                       ("sl_cut", "k_sl_tree", fc["cut_per_band_cell"], "band_cells")):
         ms, n, units = prof[key]
         if n <= 0 or ms <= 0:
