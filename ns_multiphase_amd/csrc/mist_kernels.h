@@ -28,19 +28,23 @@ __device__ __forceinline__ int interior_id(const DevMesh& m, int i, int j, int k
     return (i - 1) + m.Nx * ((j - 1) + m.Ny * (k - 1));
 }
 
-// Wave-aggregated append of `flag` items to list[*count].
-__device__ __forceinline__ void append(bool flag, int item, int* list, int* count) {
+// Work-group-aggregated append: the waves take their slices of the group's range in LDS and
+// one thread adds the group's total to *count, so a sparse list over a large grid costs one
+// global atomic per group instead of one per wave. Every thread of the group must call it the
+// same number of times (it synchronises the group); s is 2 ints of LDS.
+__device__ __forceinline__ void append_group(bool flag, int item, int* list, int* count, int* s) {
     const unsigned long long mask = __ballot(flag);
-    if (mask == 0) return;
     const int lane = __lane_id();
-    const int leader = __ffsll((long long)mask) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(count, __popcll(mask));
-    base = __shfl(base, leader);
-    if (flag) {
-        const unsigned long long lower = mask & ((1ull << lane) - 1ull);
-        list[base + __popcll(lower)] = item;
-    }
+    if (threadIdx.x == 0) s[0] = 0;
+    __syncthreads();
+    int woff = 0;
+    if (lane == 0 && mask) woff = atomicAdd(&s[0], __popcll(mask));
+    woff = __shfl(woff, 0);
+    __syncthreads();
+    if (threadIdx.x == 0) s[1] = s[0] ? atomicAdd(count, s[0]) : 0;
+    __syncthreads();
+    if (flag) list[s[1] + woff + __popcll(mask & ((1ull << lane) - 1ull))] = item;
+    __syncthreads();  // s is reused by the next call
 }
 
 // ------------------------------------------------------------------------------------

@@ -260,8 +260,9 @@ __global__ void k_band_sweep(DevMesh m, int nstep, int16_t* band) {
 }
 
 __global__ void k_band_list(DevMesh m, int nband, const int16_t* band, int* list, int* count) {
+    __shared__ int s_app[2];
     const int n = m.Nx * m.Ny * m.Nz;
-    const int nb = (n + blockDim.x - 1) / blockDim.x * blockDim.x;  // whole waves take part
+    const int nb = (n + blockDim.x - 1) / blockDim.x * blockDim.x;  // whole work-groups take part
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nb; q += gridDim.x * blockDim.x) {
         bool f = false;
         if (q < n) {
@@ -269,7 +270,7 @@ __global__ void k_band_list(DevMesh m, int nband, const int16_t* band, int* list
             interior_of(m, q, i, j, k);
             f = abs((int)band[m.cell(i, j, k)]) <= nband;
         }
-        append(f, q, list, count);
+        append_group(f, q, list, count, s_app);
     }
 }
 
@@ -283,8 +284,9 @@ __global__ void k_fill_int(int* a, int n, int v) {
 // ------------------------------------------------------------------------------------
 __global__ void k_normals_prep(DevMesh m, DevParams p, const double* VF, double* nx, double* ny, double* nz,
                                int* list, int* count) {
+    __shared__ int s_app[2];
     const int64_t n = (int64_t)m.cnx * m.cny * m.cnz;
-    const int64_t nb = (n + blockDim.x - 1) / blockDim.x * blockDim.x;
+    const int64_t nb = (n + blockDim.x - 1) / blockDim.x * blockDim.x;  // whole work-groups take part
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nb; q += (int64_t)gridDim.x * blockDim.x) {
         bool iface = false;
         int item = 0;
@@ -315,7 +317,7 @@ __global__ void k_normals_prep(DevMesh m, DevParams p, const double* VF, double*
             }
             nx[q] = a; ny[q] = b; nz[q] = c;
         }
-        append(iface, item, list, count);
+        append_group(iface, item, list, count, s_app);
     }
 }
 
