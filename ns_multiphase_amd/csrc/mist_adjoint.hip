@@ -37,7 +37,7 @@ struct AdjExt {
     }
 };
 
-// reverse of k_neumann_axis: ghost[mn-1] = A[mn], ghost[mx+1] = A[mx]
+// reverse of the one-layer Neumann copy along one axis: ghost[mn-1] = A[mn], ghost[mx+1] = A[mx]
 __global__ void k_adj_neumann_axis(double* g, AdjExt e, int d, int mn, int mx) {
     const int d1 = (d + 1) % 3, d2 = (d + 2) % 3;
     const int64_t tot = (int64_t)e.n[d1] * e.n[d2];

@@ -149,12 +149,13 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_interior(DevMesh m, cons
     if (threadIdx.x == 0) part[blockIdx.x] = b;
 }
 
+// block b reduces the n partials part[b + q * stride] into out[b]
 __global__ __launch_bounds__(RED_THREADS) void k_reduce_partials(const double* part, int n, int stride, int kind,
                                                                  double* out) {
     double r = red_init(kind);
-    for (int q = threadIdx.x; q < n; q += blockDim.x) r = red_op(kind, r, part[q * stride]);
+    for (int q = threadIdx.x; q < n; q += blockDim.x) r = red_op(kind, r, part[blockIdx.x + q * stride]);
     const double b = block_reduce(kind, r);
-    if (threadIdx.x == 0) *out = b;
+    if (threadIdx.x == 0) out[blockIdx.x] = b;
 }
 
 // ------------------------------------------------------------------------------------
@@ -168,22 +169,32 @@ struct Ext3 {
     }
 };
 
+// Neumann! on the three axes in one launch. Applied axis after axis over the full extents of
+// the other axes, the one-layer copies leave in every first-layer ghost the value of the cell
+// with each first-layer ghost coordinate moved to the adjacent interior index (0 -> 1,
+// mx+1 -> mx) and deeper coordinates kept; those sources are never written, so one thread per
+// ghost position of the six faces copies its value directly (edge and corner positions get the
+// same value from each face that contains them).
 template <class T>
-__global__ void k_neumann_axis(T* A, Ext3 e, int d, int mn, int mx) {
-    const int d1 = (d + 1) % 3, d2 = (d + 2) % 3;
-    const int64_t tot = (int64_t)e.n[d1] * e.n[d2];
+__global__ void k_neumann3(T* A, Ext3 e, int mx0, int mx1, int mx2) {
+    const int mx[3] = {mx0, mx1, mx2};
+    const int64_t f0 = (int64_t)e.n[1] * e.n[2], f1 = (int64_t)e.n[2] * e.n[0], f2 = (int64_t)e.n[0] * e.n[1];
+    const int64_t tot = f0 + f1 + f2;
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
+        const int d = q < f0 ? 0 : (q < f0 + f1 ? 1 : 2);
+        const int64_t r = q - (d == 0 ? 0 : (d == 1 ? f0 : f0 + f1));
+        const int d1 = (d + 1) % 3, d2 = (d + 2) % 3;
         int id[3];
-        id[d1] = e.lo[d1] + (int)(q % e.n[d1]);
-        id[d2] = e.lo[d2] + (int)(q / e.n[d1]);
-        id[d] = mn;
-        const T lo_v = A[e.at(id[0], id[1], id[2])];
-        id[d] = mx;
-        const T hi_v = A[e.at(id[0], id[1], id[2])];
-        id[d] = mn - 1;
-        A[e.at(id[0], id[1], id[2])] = lo_v;
-        id[d] = mx + 1;
-        A[e.at(id[0], id[1], id[2])] = hi_v;
+        id[d1] = e.lo[d1] + (int)(r % e.n[d1]);
+        id[d2] = e.lo[d2] + (int)(r / e.n[d1]);
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            id[d] = side ? mx[d] + 1 : 0;
+            int s[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) s[a] = id[a] == 0 ? 1 : (id[a] == mx[a] + 1 ? mx[a] : id[a]);
+            A[e.at(id[0], id[1], id[2])] = A[e.at(s[0], s[1], s[2])];
+        }
     }
 }
 
@@ -1335,8 +1346,7 @@ int launch_maxabs_uvw(mist_ctx* ctx, double out[3]) {
     const int g = std::min(ctx->red_blocks, grid_for(n, RED_THREADS));
     LAUNCH(k_maxabs3, g, RED_THREADS, (const double*)ctx->f[MIST_FIELD_U], (const double*)ctx->f[MIST_FIELD_V],
            (const double*)ctx->f[MIST_FIELD_W], n, ctx->red);
-    for (int c = 0; c < 3; ++c)
-        LAUNCH(k_reduce_partials, 1, RED_THREADS, ctx->red + c, g, 3, (int)MIST_REDUCE_MAX, ctx->red + 3 * ctx->red_blocks + c);
+    LAUNCH(k_reduce_partials, 3, RED_THREADS, ctx->red, g, 3, (int)MIST_REDUCE_MAX, ctx->red + 3 * ctx->red_blocks);
     MIST_HIP_TRY(hipMemcpyAsync(ctx->h_red, ctx->red + 3 * ctx->red_blocks, 3 * sizeof(double), hipMemcpyDeviceToHost,
                                 ctx->stream));
     MIST_HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -1361,14 +1371,11 @@ int launch_reduce(mist_ctx* ctx, int kind, int field, double* out) {
 
 int launch_neumann(mist_ctx* ctx, int field) {
     const Ext3 e = ext_of(ctx, field);
-    const int mn[3] = {1, 1, 1}, mx[3] = {ctx->m.Nx, ctx->m.Ny, ctx->m.Nz};
-    for (int d = 0; d < 3; ++d) {
-        const int64_t tot = (int64_t)e.n[(d + 1) % 3] * e.n[(d + 2) % 3];
-        if (field == MIST_FIELD_BAND)
-            LAUNCH(k_neumann_axis<int16_t>, grid_for(tot, 256), 256, (int16_t*)ctx->f[field], e, d, mn[d], mx[d]);
-        else
-            LAUNCH(k_neumann_axis<double>, grid_for(tot, 256), 256, (double*)ctx->f[field], e, d, mn[d], mx[d]);
-    }
+    const int64_t tot = (int64_t)e.n[1] * e.n[2] + (int64_t)e.n[2] * e.n[0] + (int64_t)e.n[0] * e.n[1];
+    if (field == MIST_FIELD_BAND)
+        LAUNCH(k_neumann3<int16_t>, grid_for(tot, 256), 256, (int16_t*)ctx->f[field], e, ctx->m.Nx, ctx->m.Ny, ctx->m.Nz);
+    else
+        LAUNCH(k_neumann3<double>, grid_for(tot, 256), 256, (double*)ctx->f[field], e, ctx->m.Nx, ctx->m.Ny, ctx->m.Nz);
     return MIST_OK;
 }
 
