@@ -80,7 +80,7 @@ def counted_flops(N):
 # pass) from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per dispatch on 20 steps of this workload,
 # FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0 (what the counters read for a plain 256 MiB
 # device copy); see profiles/pmc_traffic_deformation3d_128.md.
-TRAFFIC_PER_UNIT = {"k_elvira_dist + k_elvira": 1712.0, "k_sl_tree": 849.0}
+TRAFFIC_PER_UNIT = {"k_elvira_dist + k_elvira": 1725.0, "k_sl_tree": 847.0}
 
 
 def run_vof(args, torch):
