@@ -1,5 +1,5 @@
 // mist_kernels.h -- device helpers shared by the kernels of mist_vof.hip and mist_ns.hip:
-// cell indexing, wave-aggregated appends, the back-trace of a point, and cutTet as an
+// cell indexing, work-group-aggregated appends, the back-trace of a point, and cutTet as an
 // explicit depth-first walk carrying either the two volume sums of the VOF transport or the
 // six sums of the momentum predictor.
 #pragma once
