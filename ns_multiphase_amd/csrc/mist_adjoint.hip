@@ -770,7 +770,7 @@ __global__ __launch_bounds__(C::NT) void k_adj_tree(DevMesh m, const double* pv,
 }
 
 // Corners of band cells: the adjoint of the back-traced position through project!
-__global__ void k_adj_backtrace(DevMesh m, DevParams p, const int16_t* band, const double* uf, const double* vf,
+__global__ __launch_bounds__(128) void k_adj_backtrace(DevMesh m, DevParams p, const int16_t* band, const double* uf, const double* vf,
                                 const double* wf, double dt, const double* gpv, double* guf, double* gvf,
                                 double* gwf) {
     const int64_t n = (int64_t)(m.Nx + 1) * (m.Ny + 1) * (m.Nz + 1);

@@ -141,7 +141,7 @@ __global__ void k_ns_convect(DevMesh m, int nband, int sl, const int16_t* band, 
 
 // compute_curvature! (src/ELVIRA.jl:143-192), with fill!(Curve, 0) for the cells it skips:
 // the height function of the 3x3x7 column along the strictly dominant normal component.
-__global__ void k_curvature(DevMesh m, const double* VF, const double* nx, const double* ny, const double* nz,
+__global__ __launch_bounds__(256) void k_curvature(DevMesh m, const double* VF, const double* nx, const double* ny, const double* nz,
                             double* Curve) {
     const int64_t n = (int64_t)m.cnx * m.cny * m.cnz;
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
