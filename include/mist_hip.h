@@ -133,8 +133,15 @@ const char* mist_last_error(const mist_ctx* ctx);   /* ctx may be NULL: last glo
 /* Extents of a field: lo = lowest (1-based reference) index per axis, n = length per axis,
  * elem = element size in bytes. */
 int mist_field_extents(const mist_ctx* ctx, int field, int64_t lo[3], int64_t n[3], int64_t* elem);
-/* Device pointer of a field (for zero-copy interop; owned by the context). */
+/* Device pointer of a field (for zero-copy interop; owned by the context). Fetching it marks
+ * what the field feeds as stale (as mist_field_modified does); a caller that writes through
+ * the pointer after later stage calls must call mist_field_modified after the write. */
 int mist_field_device_ptr(mist_ctx* ctx, int field, void** dptr);
+/* Declare that `field` was written through its device pointer: the normals, PLIC constants
+ * and band list derived from it are recomputed when next needed (VF: all three; a normal
+ * component: normals and constants; D: constants; band: the band-cell list). No reference
+ * counterpart (the reference's arrays carry no derived state). */
+int mist_field_modified(mist_ctx* ctx, int field);
 
 int mist_upload(mist_ctx* ctx, int field, const void* host, size_t nbytes);
 int mist_download(mist_ctx* ctx, int field, void* host, size_t nbytes);
@@ -226,6 +233,14 @@ int mist_transport_vjp(mist_ctx* ctx, double dt);
  * about 5 cell fields per slot) and back. */
 int mist_checkpoint_save(mist_ctx* ctx, int slot);
 int mist_checkpoint_restore(mist_ctx* ctx, int slot);
+
+/* ELVIRA's choice for the current VF (ELVIRA!'s argmin, src/ELVIRA.jl:106-137), for checks
+ * against another implementation: out[(i-1) + Nx*((j-1) + Ny*(k-1))] = the candidate kept in
+ * interface cell (i, j, k), numbered c = 9*family + 3*outer pair + inner pair in the loop
+ * order of ELVIRA_calc! (families X-Y, X-Z, Y-Z; src/ELVIRA.jl:71-98), -1 in other cells.
+ * n must be Nx*Ny*Nz. No reference counterpart: the reference keeps the choice implicit in
+ * the normals. Synchronises. */
+int mist_elvira_choice(mist_ctx* ctx, int32_t* out, size_t n);
 
 /* Batched PLIC primitives (first slice; host arrays of length m).
  * computeDist (src/VF.jl:109-156) for cells (i,j,k) with normals n and fractions vf. */

@@ -17,6 +17,22 @@ static int fail(mist_ctx* ctx, int code, const std::string& msg) {
     return code;
 }
 
+// VF changed: the cached normals, PLIC constants and band list of the old VF no longer apply
+// (the version counters make the normals and constants stale; the band list is flagged).
+static inline void vf_changed(mist_ctx* ctx) {
+    ctx->vf_version++;
+    ctx->band_list_valid = false;
+}
+
+// What a write to `field` (upload, or through the device pointer) invalidates.
+static void field_written(mist_ctx* ctx, int field) {
+    if (field == MIST_FIELD_VOF) vf_changed(ctx);
+    if (field == MIST_FIELD_NX || field == MIST_FIELD_NY || field == MIST_FIELD_NZ) ctx->normals_version = 0;
+    if (field == MIST_FIELD_NX || field == MIST_FIELD_NY || field == MIST_FIELD_NZ || field == MIST_FIELD_PLIC_D)
+        ctx->plic_version = 0;
+    if (field == MIST_FIELD_BAND) ctx->band_list_valid = false;
+}
+
 #define CHECK_CTX()                                                   \
     do {                                                              \
         if (!ctx) return fail(nullptr, MIST_ERR_PARAM, "null context"); \
@@ -292,11 +308,14 @@ int mist_field_device_ptr(mist_ctx* ctx, int field, void** dptr) {
     if (field < 0 || field >= MIST_FIELD_COUNT || !dptr) return fail(ctx, MIST_ERR_PARAM, "bad field");
     *dptr = ctx->f[field];
     // the caller may write through the pointer: cached derived state is no longer trusted
-    if (field == MIST_FIELD_VOF) ctx->vf_version++;
-    if (field == MIST_FIELD_NX || field == MIST_FIELD_NY || field == MIST_FIELD_NZ) ctx->normals_version = 0;
-    if (field == MIST_FIELD_NX || field == MIST_FIELD_NY || field == MIST_FIELD_NZ || field == MIST_FIELD_PLIC_D)
-        ctx->plic_version = 0;
-    if (field == MIST_FIELD_BAND) ctx->band_list_valid = false;
+    field_written(ctx, field);
+    return MIST_OK;
+}
+
+int mist_field_modified(mist_ctx* ctx, int field) {
+    CHECK_CTX();
+    if (field < 0 || field >= MIST_FIELD_COUNT) return fail(ctx, MIST_ERR_PARAM, "bad field");
+    field_written(ctx, field);
     return MIST_OK;
 }
 
@@ -308,11 +327,7 @@ int mist_upload(mist_ctx* ctx, int field, const void* host, size_t nbytes) {
     MIST_HIP_TRY(hipSetDevice(ctx->device));
     MIST_HIP_TRY(hipMemcpyAsync(ctx->f[field], host, nbytes, hipMemcpyHostToDevice, ctx->stream));
     MIST_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    if (field == MIST_FIELD_VOF) ctx->vf_version++;
-    if (field == MIST_FIELD_NX || field == MIST_FIELD_NY || field == MIST_FIELD_NZ) ctx->normals_version = 0;
-    if (field == MIST_FIELD_NX || field == MIST_FIELD_NY || field == MIST_FIELD_NZ || field == MIST_FIELD_PLIC_D)
-        ctx->plic_version = 0;
-    if (field == MIST_FIELD_BAND) ctx->band_list_valid = false;
+    field_written(ctx, field);
     return MIST_OK;
 }
 
@@ -332,7 +347,7 @@ int mist_init_vof(mist_ctx* ctx, int shape, const double* args, int nargs) {
         return fail(ctx, MIST_ERR_PARAM, "bad initial-condition shape or arguments");
     MIST_HIP_TRY(hipSetDevice(ctx->device));
     RET_IF(mist::launch_init_vof(ctx, shape, args));
-    ctx->vf_version++;
+    vf_changed(ctx);
     return MIST_OK;
 }
 
@@ -348,7 +363,7 @@ int mist_neumann(mist_ctx* ctx, int field) {
     if (field < 0 || field >= MIST_FIELD_COUNT || face_field(field) || field == MIST_FIELD_MASK)
         return fail(ctx, MIST_ERR_PARAM, "Neumann! applies to cell fields");
     RET_IF(mist::launch_neumann(ctx, field));
-    if (field == MIST_FIELD_VOF) ctx->vf_version++;
+    if (field == MIST_FIELD_VOF) vf_changed(ctx);
     return MIST_OK;
 }
 
@@ -357,7 +372,7 @@ int mist_update_borders(mist_ctx* ctx, int field) {
     if (field < 0 || field >= MIST_FIELD_COUNT || face_field(field) || field == MIST_FIELD_MASK)
         return fail(ctx, MIST_ERR_PARAM, "update_borders! applies to cell fields");
     RET_IF(mist::launch_borders(ctx, field));
-    if (field == MIST_FIELD_VOF && (ctx->p.xper || ctx->p.yper || ctx->p.zper)) ctx->vf_version++;
+    if (field == MIST_FIELD_VOF && (ctx->p.xper || ctx->p.yper || ctx->p.zper)) vf_changed(ctx);
     return MIST_OK;
 }
 
@@ -443,7 +458,7 @@ static int transport_body(mist_ctx* ctx, double dt) {
         StageTimer tm(ctx, MIST_STAGE_TRANSPORT);
         RET_IF(mist::launch_transport(ctx, dt));
     }
-    ctx->vf_version++;
+    vf_changed(ctx);
     RET_IF(mist_compute_normal(ctx));
     RET_IF(mist_compute_plic(ctx));
     ctx->band_list_valid = false;  // band no longer matches VF; the caller recomputes it
@@ -458,7 +473,7 @@ static int transport_ns_body(mist_ctx* ctx, double dt) {
         StageTimer tm(ctx, MIST_STAGE_TRANSPORT);
         RET_IF(mist::launch_ns_advect(ctx, dt));
     }
-    ctx->vf_version++;
+    vf_changed(ctx);
     RET_IF(mist_compute_normal(ctx));
     {
         StageTimer tm(ctx, MIST_STAGE_TRANSPORT);
@@ -487,7 +502,7 @@ int mist_vof_init(mist_ctx* ctx) {
     MIST_HIP_TRY(hipSetDevice(ctx->device));
     RET_IF(mist::launch_neumann(ctx, MIST_FIELD_VOF));
     RET_IF(mist::launch_borders(ctx, MIST_FIELD_VOF));
-    ctx->vf_version++;
+    vf_changed(ctx);
     RET_IF(mist_compute_band(ctx));
     RET_IF(mist_compute_normal(ctx));
     RET_IF(mist_compute_plic(ctx));
@@ -514,6 +529,30 @@ int mist_vof_step(mist_ctx* ctx, double* t, double clamp_to, double* dt_used) {
 int mist_compute_props(mist_ctx* ctx) {
     CHECK_CTX();
     RET_IF(mist::launch_props(ctx));
+    return MIST_OK;
+}
+
+int mist_elvira_choice(mist_ctx* ctx, int32_t* out, size_t n) {
+    CHECK_CTX();
+    const mist::DevMesh& m = ctx->m;
+    const size_t ncell = (size_t)m.Nx * m.Ny * m.Nz;
+    if (!out || n != ncell) return fail(ctx, MIST_ERR_PARAM, "elvira choice: expected Nx*Ny*Nz entries");
+    MIST_HIP_TRY(hipSetDevice(ctx->device));
+    RET_IF(mist_compute_normal(ctx));  // the choice of the current VF
+    std::vector<int> best(ncell);
+    std::vector<double> vf((size_t)m.vnx * m.vny * m.vnz);
+    MIST_HIP_TRY(hipMemcpyAsync(best.data(), ctx->elv_best, ncell * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    MIST_HIP_TRY(hipMemcpyAsync(vf.data(), ctx->f[MIST_FIELD_VOF], vf.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    MIST_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (int k = 1; k <= m.Nz; ++k)
+        for (int j = 1; j <= m.Ny; ++j)
+            for (int i = 1; i <= m.Nx; ++i) {
+                const size_t q = (size_t)(i - 1) + (size_t)m.Nx * ((j - 1) + (size_t)m.Ny * (k - 1));
+                const double v = vf[(size_t)m.vof(i, j, k)];
+                // the kernel writes only interface cells; the low byte is the candidate
+                out[q] = (v >= ctx->p.VFlo && v <= ctx->p.VFhi && best[q] >= 0) ? (best[q] & 255) : -1;
+            }
     return MIST_OK;
 }
 
@@ -586,7 +625,7 @@ int mist_ns_init(mist_ctx* ctx) {
     RET_IF(mist::launch_borders(ctx, MIST_FIELD_V));
     RET_IF(mist::launch_borders(ctx, MIST_FIELD_W));
     RET_IF(mist::launch_borders(ctx, MIST_FIELD_VOF));
-    ctx->vf_version++;
+    vf_changed(ctx);
     RET_IF(mist::launch_interp_face(ctx, MIST_FIELD_U, MIST_FIELD_V, MIST_FIELD_W));
     RET_IF(mist::launch_props(ctx));
     RET_IF(mist_compute_band(ctx));

@@ -82,6 +82,8 @@ def lib():
         "mist_transport_vjp": (i32, [vp, d]),
         "mist_checkpoint_save": (i32, [vp, i32]),
         "mist_checkpoint_restore": (i32, [vp, i32]),
+        "mist_elvira_choice": (i32, [vp, ctypes.POINTER(ctypes.c_int32), ctypes.c_size_t]),
+        "mist_field_modified": (i32, [vp, i32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -158,6 +160,18 @@ class Context:
         p = ctypes.c_void_p()
         self._check(lib().mist_field_device_ptr(self.h, FIELD[name], ctypes.byref(p)))
         return p.value
+
+    def field_modified(self, name):
+        """Declare a write through ``device_ptr(name)``: derived state is recomputed."""
+        self._check(lib().mist_field_modified(self.h, FIELD[name]))
+
+    def elvira_choice(self):
+        """ELVIRA candidate kept per interior cell for the current VF, indexed [i, j, k]
+        (-1 outside interface cells)."""
+        m = self.mesh
+        out = np.zeros(m.Nx * m.Ny * m.Nz, np.int32)
+        self._check(lib().mist_elvira_choice(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), out.size))
+        return out.reshape(m.Nz, m.Ny, m.Nx).transpose(2, 1, 0)
 
     # ---- stages -----------------------------------------------------------------------
     def init_vof(self, shape, *args):

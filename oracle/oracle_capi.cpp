@@ -393,6 +393,43 @@ int ora_elvira_cell(void* h, int64_t i, int64_t j, int64_t k, double out[3]) {
     return best;
 }
 
+// Scores of the ELVIRA candidates of one cell (error, normal; INFINITY where a family is
+// skipped), for the tie checks of the tests.
+int ora_elvira_scores(void* h, int64_t i, int64_t j, int64_t k, double err[27], double nrm[27 * 3]) {
+    int best = -1;
+    double a = 0, b = 0, d = 0;
+    double cn[27][3] = {};
+    elvira_calc(*(Ctx*)h, i, j, k, &a, &b, &d, &best, err, cn);
+    std::memcpy(nrm, cn, sizeof(cn));
+    return best;
+}
+
+// Tie hints (kind 0: ELVIRA candidates, kind 1: height-function directions), one array of
+// Nx*Ny*Nz entries (x fastest) per later compute_normal / compute_curvature call.
+int ora_push_hints(void* h, int kind, const int32_t* hint, int64_t n) {
+    Ctx& c = *(Ctx*)h;
+    if (n != c.p.Nx * c.p.Ny * c.p.Nz || kind < 0 || kind > 1) { c.err = "bad hint array"; return MIST_ERR_PARAM; }
+    (kind == 0 ? c.elv_hints : c.curv_hints).emplace_back(hint, hint + n);
+    return MIST_OK;
+}
+void ora_clear_hints(void* h) {
+    Ctx& c = *(Ctx*)h;
+    c.elv_hints.clear();
+    c.curv_hints.clear();
+    std::memset(c.tie_stats, 0, sizeof(c.tie_stats));
+    c.tie_gap[0] = c.tie_gap[1] = 0.0;
+}
+// {cells whose hint differed, adopted, refused, pending hint arrays} and the largest adopted
+// relative score gap (ELVIRA only).
+int ora_tie_stats(void* h, int kind, int64_t out[4], double* gap) {
+    Ctx& c = *(Ctx*)h;
+    if (kind < 0 || kind > 1) return MIST_ERR_PARAM;
+    for (int q = 0; q < 3; ++q) out[q] = c.tie_stats[kind][q];
+    out[3] = (int64_t)(kind == 0 ? c.elv_hints.size() : c.curv_hints.size());
+    *gap = c.tie_gap[kind];
+    return MIST_OK;
+}
+
 void ora_set_count_flops(int on) { g_count_flops = on != 0; }
 void ora_get_flops(uint64_t out[2]) { out[0] = t_flops.flops; out[1] = t_flops.special; }
 void ora_set_threads(int n) {

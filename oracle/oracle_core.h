@@ -17,6 +17,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <deque>
 #include <string>
 #include <vector>
 
@@ -101,6 +102,16 @@ struct Ctx {
     int64_t max_cut_level = 0;
     int64_t mg_cycles = 0;
     double mg_resid = 0;
+    // Tie hints of a parity check against another implementation. Each compute_normal call
+    // takes the front array of elv_hints (if any): the ELVIRA candidate the other side kept
+    // per interior cell (x fastest, -1 none); each compute_curvature call takes the front of
+    // curv_hints: the height-function direction it used (0 x, 1 y, 2 z, -1 none). A hinted
+    // choice replaces the oracle's own only where the oracle's scores tie with it to rounding
+    // (see elvira_tie / direction_tie); tie_stats[kind] = {cells whose hint differed, adopted,
+    // refused}, tie_gap[kind] = the largest relative score gap adopted.
+    std::deque<std::vector<int32_t>> elv_hints, curv_hints;
+    int64_t tie_stats[2][3] = {};
+    double tie_gap[2] = {};
 };
 
 // ---- geometry primitives (src/VFgeom.jl, src/VF.jl, src/Tools.jl) -----------------
@@ -123,7 +134,8 @@ double vel_wface(const Ctx& c, const double pt[3], int64_t i, int64_t j, int64_t
 CutSums cut_tet(Ctx& c, double tet[4][3], int64_t ind[4][3], bool xdone, bool ydone,
                 bool zdone, int lvl, CutScratch& s);
 void elvira_calc(const Ctx& c, int64_t i, int64_t j, int64_t k, double* onx, double* ony,
-                 double* onz, int* best_candidate);
+                 double* onz, int* best_candidate, double cand_err[27] = nullptr,
+                 double cand_n[27][3] = nullptr);
 
 // ---- stages ----------------------------------------------------------------------
 Arr3<double>* field_f64(Ctx& c, int field);

@@ -271,7 +271,7 @@ static inline double VFat(const Ctx& c, int64_t i, int64_t j, int64_t k) { retur
 
 static void check_normal(const Ctx& c, double& err, double nx_, double ny_, double nz_, int64_t i,
                          int64_t j, int64_t k, double* onx, double* ony, double* onz, int cand,
-                         int* best) {
+                         int* best, double* cand_err, double (*cand_n)[3]) {
     double err_ = 0.0;
     const double norm = std::sqrt((nx_ * nx_ + ny_ * ny_) + nz_ * nz_);
     nx_ = nx_ / norm; ny_ = ny_ / norm; nz_ = nz_ / norm;
@@ -285,6 +285,8 @@ static void check_normal(const Ctx& c, double& err, double nx_, double ny_, doub
                 err_ += e * e;
                 FL(3);
             }
+    if (cand_err) cand_err[cand] = err_;
+    if (cand_n) { cand_n[cand][0] = nx_; cand_n[cand][1] = ny_; cand_n[cand][2] = nz_; }
     if (err_ < err) {
         err = err_;
         *onx = nx_; *ony = ny_; *onz = nz_;
@@ -293,9 +295,11 @@ static void check_normal(const Ctx& c, double& err, double nx_, double ny_, doub
 }
 
 void elvira_calc(const Ctx& c, int64_t i, int64_t j, int64_t k, double* onx, double* ony, double* onz,
-                 int* best) {
+                 int* best, double cand_err[27], double cand_n[27][3]) {
     const Mesh& m = c.m;
     double err = INFINITY;
+    if (cand_err)
+        for (int q = 0; q < 27; ++q) cand_err[q] = INFINITY;
     static const int PAIRS[3][2] = {{-1, 0}, {-1, 1}, {0, 1}};
     const double dx = m.dx, dy = m.dy, dz = m.dz;
     int cand = 0;
@@ -312,7 +316,7 @@ void elvira_calc(const Ctx& c, int64_t i, int64_t j, int64_t k, double* onx, dou
                 const double ny_ = -(sy2 - sy1) / (m.ym(j2) - m.ym(j1));
                 const double nz_ = VFat(c, i, j, k + 1) > VFat(c, i, j, k - 1) ? -1.0 : 1.0;
                 FL(26);
-                check_normal(c, err, nx_, ny_, nz_, i, j, k, onx, ony, onz, cand, best);
+                check_normal(c, err, nx_, ny_, nz_, i, j, k, onx, ony, onz, cand, best, cand_err, cand_n);
             }
     } else {
         cand += 9;
@@ -330,7 +334,7 @@ void elvira_calc(const Ctx& c, int64_t i, int64_t j, int64_t k, double* onx, dou
                 const double sz1 = (VFat(c, i, j - 1, k1) * dy + VFat(c, i, j, k1) * dy) + VFat(c, i, j + 1, k1) * dy;
                 const double nz_ = -(sz2 - sz1) / (m.zm(k2) - m.zm(k1));
                 FL(26);
-                check_normal(c, err, nx_, ny_, nz_, i, j, k, onx, ony, onz, cand, best);
+                check_normal(c, err, nx_, ny_, nz_, i, j, k, onx, ony, onz, cand, best, cand_err, cand_n);
             }
     } else {
         cand += 9;
@@ -348,7 +352,7 @@ void elvira_calc(const Ctx& c, int64_t i, int64_t j, int64_t k, double* onx, dou
                 const double sz1 = (VFat(c, i - 1, j, k1) * dx + VFat(c, i, j, k1) * dx) + VFat(c, i + 1, j, k1) * dx;
                 const double nz_ = -(sz2 - sz1) / (m.zm(k2) - m.zm(k1));
                 FL(26);
-                check_normal(c, err, nx_, ny_, nz_, i, j, k, onx, ony, onz, cand, best);
+                check_normal(c, err, nx_, ny_, nz_, i, j, k, onx, ony, onz, cand, best, cand_err, cand_n);
             }
     }
 }

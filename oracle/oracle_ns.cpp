@@ -277,16 +277,49 @@ bool add_correction_tets(const Ctx& c, int64_t i, int64_t j, int64_t k, double d
 // compute_curvature! (src/ELVIRA.jl:143-192): height function over the 3x3x7 column along
 // the dominant normal direction; 0 where no component strictly dominates.
 // ------------------------------------------------------------------------------------
-static void curvature_cell(Ctx& c, int64_t i, int64_t j, int64_t k) {
+// The direction the reference's strict comparisons pick: 0 x, 1 y, 2 z, -1 none dominates.
+static int dominant_direction(double anx, double any, double anz) {
+    if (anx > any && anx > anz) return 0;
+    if (any > anx && any > anz) return 1;
+    if (anz > any && anz > anx) return 2;
+    return -1;
+}
+
+// A hinted height-function direction ties with the oracle's own when the normal components
+// that decide between them are equal to rounding (1e-12 relative): on a symmetric interface
+// |nx| = |ny| in exact arithmetic, and the last bits of the two implementations' normals
+// then pick the direction. Direction d is admissible when |n_d| is the largest component to
+// that tolerance; "none" when the two largest are equal to it.
+static bool direction_tie(int hint, const double a[3]) {
+    const double mx = std::max(a[0], std::max(a[1], a[2]));
+    const double tol = 1e-12 * mx;
+    if (hint >= 0 && hint < 3) return mx - a[hint] <= tol;
+    if (hint != -1) return false;
+    int top = 0;
+    for (int d = 1; d < 3; ++d)
+        if (a[d] > a[top]) top = d;
+    for (int d = 0; d < 3; ++d)
+        if (d != top && a[top] - a[d] <= tol) return true;
+    return false;
+}
+
+static void curvature_cell(Ctx& c, int64_t i, int64_t j, int64_t k, const std::vector<int32_t>* hint,
+                           int64_t stats[3]) {
     const Mesh& m = c.m;
     const double anx = std::fabs(c.nx(i, j, k)), any = std::fabs(c.ny(i, j, k)), anz = std::fabs(c.nz(i, j, k));
     double hf[3][3] = {};  // hf[a][b], a and b the two tangential offsets + 1
     double h1, h2;         // spacings of the first / second tangential axis
-    int dom;
-    if (anx > any && anx > anz) dom = 0;
-    else if (any > anx && any > anz) dom = 1;
-    else if (anz > any && anz > anx) dom = 2;
-    else return;
+    int dom = dominant_direction(anx, any, anz);
+    if (hint) {
+        const int h = (*hint)[(size_t)((i - 1) + m.Nx * ((j - 1) + m.Ny * (k - 1)))];
+        if (h != dom) {
+            const double a[3] = {anx, any, anz};
+            ++stats[0];
+            if (direction_tie(h, a)) { ++stats[1]; dom = h; }
+            else ++stats[2];
+        }
+    }
+    if (dom < 0) return;
     if (dom == 0) {  // hf[jj,kk], loop kk, jj, ii
         for (int kk = -1; kk <= 1; ++kk)
             for (int jj = -1; jj <= 1; ++jj)
@@ -322,9 +355,16 @@ static void curvature_cell(Ctx& c, int64_t i, int64_t j, int64_t k) {
 void compute_curvature(Ctx& c) {
     const Mesh& m = c.m;
     std::fill(c.Curve.a.begin(), c.Curve.a.end(), 0.0);
+    std::vector<int32_t> hint;
+    if (!c.curv_hints.empty()) {
+        hint = std::move(c.curv_hints.front());
+        c.curv_hints.pop_front();
+    }
+    int64_t stats[3] = {};
     for (int64_t k = m.kmin_; k <= m.kmax_; ++k)
         for (int64_t j = m.jmin_; j <= m.jmax_; ++j)
-            for (int64_t i = m.imin_; i <= m.imax_; ++i) curvature_cell(c, i, j, k);
+            for (int64_t i = m.imin_; i <= m.imax_; ++i) curvature_cell(c, i, j, k, hint.empty() ? nullptr : &hint, stats);
+    for (int q = 0; q < 3; ++q) c.tie_stats[1][q] += stats[q];
 }
 
 // mask_maker! (src/Tools.jl:1582-1598): faces next to positive curvature; never reset.

@@ -172,15 +172,34 @@ void compute_band(Ctx& c) {
 // ------------------------------------------------------------------------------------
 // ELVIRA! (src/ELVIRA.jl:6-55)
 // ------------------------------------------------------------------------------------
+// A hinted ELVIRA candidate is a tie of the oracle's own choice when its error exceeds the
+// smallest one by no more than rounding: 1e-9 relative (a genuinely worse candidate differs
+// at O(1) relative), plus 1e-30 absolute for planar interfaces whose errors are rounding
+// noise themselves. The errors of mirrored candidates on a symmetric field are equal in exact
+// arithmetic and differ in their last bits between implementations (libm cbrt/acos/cos in
+// computeDist, the association of the transport's volume sums).
+static bool elvira_tie(double e_hint, double e_best) {
+    if (!(e_hint == e_hint) || !(e_hint < INFINITY)) return false;
+    return e_hint <= e_best * (1.0 + 1e-9) + 1e-30;
+}
+
 void compute_normal(Ctx& c) {
     const Mesh& m = c.m;
     std::fill(c.nx.a.begin(), c.nx.a.end(), 0.0);
     std::fill(c.ny.a.begin(), c.ny.a.end(), 0.0);
     std::fill(c.nz.a.begin(), c.nz.a.end(), 0.0);
     const double VFlo = c.p.VFlo, VFhi = c.p.VFhi;
+    std::vector<int32_t> hint;
+    if (!c.elv_hints.empty()) {
+        hint = std::move(c.elv_hints.front());
+        c.elv_hints.pop_front();
+    }
+    int64_t h_diff = 0, h_adopt = 0, h_refuse = 0;
+    double h_gap = 0.0;
     const FlopCount before = t_flops;  // the operation counter of the calling thread
     uint64_t fl = 0, fs = 0;
-#pragma omp parallel for collapse(2) schedule(dynamic, 1) reduction(+ : fl, fs)
+#pragma omp parallel for collapse(2) schedule(dynamic, 1) reduction(+ : fl, fs, h_diff, h_adopt, h_refuse) \
+    reduction(max : h_gap)
     for (int64_t k = m.kmin_; k <= m.kmax_; ++k)
         for (int64_t j = m.jmin_; j <= m.jmax_; ++j) {
             t_flops = FlopCount{};
@@ -188,7 +207,24 @@ void compute_normal(Ctx& c) {
                 const double vf = c.VF(i, j, k);
                 if (vf >= VFlo && vf <= VFhi) {
                     double a = 0, b = 0, d = 0;
-                    elvira_calc(c, i, j, k, &a, &b, &d, nullptr);
+                    if (hint.empty()) {
+                        elvira_calc(c, i, j, k, &a, &b, &d, nullptr);
+                    } else {
+                        int best = -1;
+                        double err[27], cn[27][3];
+                        elvira_calc(c, i, j, k, &a, &b, &d, &best, err, cn);
+                        const int h = hint[(size_t)((i - 1) + m.Nx * ((j - 1) + m.Ny * (k - 1)))];
+                        if (h != best) {
+                            ++h_diff;
+                            if (h >= 0 && h < 27 && best >= 0 && elvira_tie(err[h], err[best])) {
+                                ++h_adopt;
+                                h_gap = std::max(h_gap, (err[h] - err[best]) / std::max(err[best], 1e-300));
+                                a = cn[h][0]; b = cn[h][1]; d = cn[h][2];
+                            } else {
+                                ++h_refuse;
+                            }
+                        }
+                    }
                     c.nx(i, j, k) = a; c.ny(i, j, k) = b; c.nz(i, j, k) = d;
                 } else if (vf < VFlo) {
                     if (c.VF(i - 1, j, k) > VFhi) c.nx(i, j, k) = 1;
@@ -211,6 +247,10 @@ void compute_normal(Ctx& c) {
         }
     t_flops.flops = before.flops + fl;
     t_flops.special = before.special + fs;
+    c.tie_stats[0][0] += h_diff;
+    c.tie_stats[0][1] += h_adopt;
+    c.tie_stats[0][2] += h_refuse;
+    c.tie_gap[0] = std::max(c.tie_gap[0], h_gap);
     update_borders(c, c.nx);
     update_borders(c, c.ny);
     update_borders(c, c.nz);

@@ -87,6 +87,10 @@ def lib():
             "ora_ns_step": (i32, [vp, dp, d, dp, d, i64, ip, dp]),
             "ora_sl_flux_vol": (d, [vp, i32, i64, i64, i64, d, dp]),
             "ora_correction_tets": (i32, [vp, i64, i64, i64, d, dp]),
+            "ora_elvira_scores": (i32, [vp, i64, i64, i64, dp, dp]),
+            "ora_push_hints": (i32, [vp, i32, ctypes.POINTER(ctypes.c_int32), i64]),
+            "ora_clear_hints": (None, [vp]),
+            "ora_tie_stats": (i32, [vp, i32, ip, dp]),
             "ora_set_count_flops": (None, [i32]),
             "ora_get_flops": (None, [ctypes.POINTER(ctypes.c_uint64)]),
             "ora_set_threads": (None, [i32]),
@@ -332,6 +336,33 @@ class OracleContext:
         out = np.zeros(3)
         best = lib().ora_elvira_cell(self.h, i, j, k, _dp(out))
         return out, best
+
+    def elvira_scores(self, i, j, k):
+        """(errors[27], normals[27, 3], kept candidate) of ELVIRA at cell (i, j, k)."""
+        err = np.zeros(27)
+        nrm = np.zeros(27 * 3)
+        best = lib().ora_elvira_scores(self.h, i, j, k, _dp(err), _dp(nrm))
+        return err, nrm.reshape(27, 3), best
+
+    # ---- tie hints of the parity checks ---------------------------------------------
+    def push_hints(self, kind, hints):
+        """Queue one hint array for the next compute_normal ("elvira": candidate indices) or
+        compute_curvature ("curvature": height-function directions), shaped [i, j, k] over
+        the interior cells."""
+        a = np.ascontiguousarray(np.asarray(hints, np.int32).transpose(2, 1, 0)).reshape(-1)
+        k = {"elvira": 0, "curvature": 1}[kind]
+        self._check(lib().ora_push_hints(self.h, k, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), a.size))
+
+    def clear_hints(self):
+        lib().ora_clear_hints(self.h)
+
+    def tie_stats(self, kind):
+        """{differed, adopted, refused, pending, gap} of the tie hints of one kind."""
+        out = np.zeros(4, np.int64)
+        gap = ctypes.c_double()
+        self._check(lib().ora_tie_stats(self.h, {"elvira": 0, "curvature": 1}[kind], _ip(out), ctypes.byref(gap)))
+        return dict(differed=int(out[0]), adopted=int(out[1]), refused=int(out[2]), pending=int(out[3]),
+                    gap=gap.value)
 
 
 def compute_alpha(m, VF):

@@ -194,3 +194,32 @@ def test_one_step_vjp_stretched_preimages(mist):
     fd = (float((lam * step(h)).sum()) - float((lam * step(-h)).sum())) / (2 * h)
     print(f"stretched one-step VJP {g:.10e} vs central difference {fd:.10e}")
     assert g == pytest.approx(fd, rel=1e-5)
+
+
+def test_vjp_after_upload_uses_the_new_band(mist):
+    """A VF uploaded after a step's band was built makes that band stale: the reverse step then
+    recomputes the band (and normals, PLIC constants) of the new VF, and equals the reverse
+    step of a context initialised on that VF."""
+    p = A.adjoint_params(48)
+    a, b = mist.Context(p), mist.Context(p)
+    a.init_vof("circle", 0.2, 0.47, 0.52)
+    a.vof_init()
+    b.init_vof("circle", 0.15, 0.55, 0.4)
+    b.vof_init()
+    vf2 = b.get("VF")  # with its ghost fills
+    rng = np.random.default_rng(7)
+    faces = {n: 0.3 * rng.standard_normal(a.get(n).shape) for n in ("uf", "vf", "wf")}
+    lam = np.zeros_like(vf2)
+    _interior(a.mesh, lam)[...] = rng.standard_normal((a.mesh.Nx, a.mesh.Ny, a.mesh.Nz))
+    a.set("VF", vf2)  # the band list of the first circle is stale from here
+    for c in (a, b):
+        for n, f in faces.items():
+            c.set(n, f)
+        c.set("VF_bar", lam)
+        for n in ("uf_bar", "vf_bar", "wf_bar"):
+            c.set(n, np.zeros(c.get(n).shape))
+        c.transport_vjp(1e-3)
+    # adjoints shared by several cells are added with atomics: equal up to their order
+    for n in ("VF_bar", "uf_bar", "vf_bar", "wf_bar"):
+        x, y = a.get(n), b.get(n)
+        assert np.abs(x - y).max() <= 1e-12 * max(1.0, np.abs(y).max()), n
