@@ -1,6 +1,7 @@
 // oracle_capi.cpp -- TEST INFRASTRUCTURE ONLY (the oracle): extern "C" entry points used by
 // tests/ (through ctypes) and by bench.py's cpu_baseline leg. They mirror the product's
 // C ABI (include/mist_hip.h) on host arrays, plus single-cell primitives for the pins.
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -173,6 +174,38 @@ int ora_pressure_solve(void* h, double dt, double tol, int64_t precision, int64_
 // the FP64 face coefficients of multigrid level `level` (0 = fine) along `axis` that a
 // pressure solve with this dt would build from the context's densities and VF, plain order
 // (n[0] x n[1] x n[2] values, at most `cap`)
+// The SL-SL pressure solve (mg_cycler / res_iteration); smooth_iters = res_iteration
+// iterations over the solve.
+int ora_pressure_solve_sl(void* h, double dt, double tol, int exact_restore, int64_t* cycles, double* resid,
+                          int64_t* smooth_iters) {
+    Ctx& c = *(Ctx*)h;
+    if (c.p.xper || c.p.yper || c.p.zper) { c.err = "SL pressure: periodic axes are not supported"; return MIST_ERR_PARAM; }
+    int st = pressure_solve_sl(c, dt, tol, c.p.mg_max_cycles, exact_restore != 0, cycles, resid, smooth_iters);
+    if (st) c.err = "P-solve did not converge";
+    return st;
+}
+void ora_set_sl_exact_restore(void* h, int on) { ((Ctx*)h)->sl_exact_restore = on != 0; }
+// A! of the current P, uf, vf, wf, densities and band: LHS into divg, the corrected faces
+// into gx, gy, gz (cell extents; any may be null).
+int ora_sl_operator(void* h, double dt, double* gx, double* gy, double* gz) {
+    Ctx& c = *(Ctx*)h;
+    Arr3<double> a, b, d;
+    sl_operator(c, dt, a, b, d);
+    const size_t bytes = a.a.size() * sizeof(double);
+    if (gx) std::memcpy(gx, a.a.data(), bytes);
+    if (gy) std::memcpy(gy, b.a.data(), bytes);
+    if (gz) std::memcpy(gz, d.a.data(), bytes);
+    return MIST_OK;
+}
+int ora_sl_jacobian(void* h, double dt, int exact_restore, double* out) {
+    Ctx& c = *(Ctx*)h;
+    Arr3<double> J;
+    sl_jacobian(c, dt, exact_restore != 0, J);
+    std::memcpy(out, J.a.data(), J.a.size() * sizeof(double));
+    return MIST_OK;
+}
+void ora_anderson_coefficients(const double* gram, int n, double* alpha) { anderson_coefficients(gram, n, alpha); }
+
 int ora_mg_level_coef(void* h, double dt, int level, int axis, double* out, int64_t cap, int64_t n[3]) {
     std::vector<double> c;
     const int st = mg_level_coefficients(*(Ctx*)h, dt, level, axis, c, n);
@@ -412,8 +445,23 @@ int ora_push_hints(void* h, int kind, const int32_t* hint, int64_t n) {
     (kind == 0 ? c.elv_hints : c.curv_hints).emplace_back(hint, hint + n);
     return MIST_OK;
 }
+// The refused hints recorded so far (at most 64): per entry kind, i, j, k, hint, own choice
+// (ints) and the two scores.
+int64_t ora_tie_refusals(void* h, int64_t* ints, double* scores, int64_t cap) {
+    const Ctx& c = *(Ctx*)h;
+    const int64_t n = std::min<int64_t>(cap, (int64_t)c.refusals.size());
+    for (int64_t q = 0; q < n; ++q) {
+        const Ctx::Refusal& r = c.refusals[(size_t)q];
+        const int64_t v[6] = {r.kind, r.i, r.j, r.k, r.hint, r.own};
+        for (int t = 0; t < 6; ++t) ints[6 * q + t] = v[t];
+        scores[2 * q] = r.e_hint;
+        scores[2 * q + 1] = r.e_own;
+    }
+    return n;
+}
 void ora_clear_hints(void* h) {
     Ctx& c = *(Ctx*)h;
+    c.refusals.clear();
     c.elv_hints.clear();
     c.curv_hints.clear();
     std::memset(c.tie_stats, 0, sizeof(c.tie_stats));

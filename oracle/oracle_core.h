@@ -112,6 +112,16 @@ struct Ctx {
     std::deque<std::vector<int32_t>> elv_hints, curv_hints;
     int64_t tie_stats[2][3] = {};
     double tie_gap[2] = {};
+    struct Refusal {
+        int kind;
+        int64_t i, j, k;
+        int hint, own;
+        double e_hint, e_own;  // ELVIRA errors; for directions the |n| components of hint and own
+    };
+    std::vector<Refusal> refusals;  // the first 64 refused hints
+    // SL pressure: jacob_single with P and the corrected faces restored exactly per cell
+    // (not the reference; see jacob_single in oracle_fas.cpp)
+    bool sl_exact_restore = false;
 };
 
 // ---- geometry primitives (src/VFgeom.jl, src/VF.jl, src/Tools.jl) -----------------
@@ -154,6 +164,11 @@ void compute_dens(Ctx& c);
 void pressure_rhs(Ctx& c);
 int pressure_solve(Ctx& c, double dt, double tol, int precision, int64_t* cycles, double* resid);
 int mg_level_coefficients(Ctx& c, double dt, int level, int axis, std::vector<double>& out, int64_t n[3]);
+int pressure_solve_sl(Ctx& c, double dt, double tol, int64_t max_cycles, bool exact_restore, int64_t* cycles,
+                      double* resid, int64_t* smooth_iters);
+void sl_operator(Ctx& c, double dt, Arr3<double>& gx, Arr3<double>& gy, Arr3<double>& gz);
+void sl_jacobian(Ctx& c, double dt, bool exact_restore, Arr3<double>& out);
+void anderson_coefficients(const double* gram, int n, double* alpha);
 void corrector(Ctx& c, double dt);
 void compute_props(Ctx& c);
 double sl_flux_vol(const Ctx& c, int dir, int64_t i, int64_t j, int64_t k, double dt, double verts[8][3]);
@@ -166,6 +181,7 @@ void apply_bc(Ctx& c, Arr3<double>& u, Arr3<double>& v, Arr3<double>& w);
 void interpolate_face(Ctx& c, const Arr3<double>& u, const Arr3<double>& v, const Arr3<double>& w);
 void interpolate_center(Ctx& c);
 void divergence(Ctx& c, double dt);
+double divg_cell(const Ctx& c, int64_t i, int64_t j, int64_t k, double dt);
 int transport_ns(Ctx& c, double dt);
 int ns_step(Ctx& c, double* t, double clamp_to, double* dt_used, double tol, int precision, int64_t* cycles,
             double* resid);

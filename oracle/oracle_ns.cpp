@@ -315,8 +315,14 @@ static void curvature_cell(Ctx& c, int64_t i, int64_t j, int64_t k, const std::v
         if (h != dom) {
             const double a[3] = {anx, any, anz};
             ++stats[0];
-            if (direction_tie(h, a)) { ++stats[1]; dom = h; }
-            else ++stats[2];
+            if (direction_tie(h, a)) {
+                ++stats[1];
+                dom = h;
+            } else {
+                ++stats[2];
+                if (c.refusals.size() < 64)
+                    c.refusals.push_back({1, i, j, k, h, dom, h >= 0 && h < 3 ? a[h] : NAN, dom >= 0 ? a[dom] : NAN});
+            }
         }
     }
     if (dom < 0) return;
@@ -468,37 +474,42 @@ void interpolate_center(Ctx& c) {
             }
 }
 
-// divergence! / divg_cell (src/Velocity.jl:59-122)
-void divergence(Ctx& c, double dt) {
+// divg_cell (src/Velocity.jl:72-122): the SL pre-image volume in band cells under the
+// semi-Lagrangian scheme, finite-volume fluxes elsewhere (SL flux volumes at band-edge faces).
+double divg_cell(const Ctx& c, int64_t i, int64_t j, int64_t k, double dt) {
     const Mesh& m = c.m;
     const bool sl = c.p.pressure_scheme == MIST_PS_SEMI_LAGRANGIAN;
     const int64_t nb = c.p.nband;
-    std::fill(c.divg.a.begin(), c.divg.a.end(), 0.0);
     const double vol = (m.dx * m.dy) * m.dz;
+    double verts[8][3];
+    if (sl && std::abs((int)c.band(i, j, k)) <= nb) {
+        const double tetsign = cell2verts(c, i, j, k, verts);
+        for (int n = 0; n < 8; ++n) project(c, verts[n], i, j, k, dt);
+        double tets[5][4][3];
+        verts2tets5(verts, tets);
+        double tv = 0.0;
+        for (int t = 0; t < 5; ++t) tv += tet_vol(tets[t]);
+        const double v1 = tetsign * tv;
+        return (vol - v1) / vol / dt;
+    }
+    auto bandat = [&](int64_t a, int64_t b, int64_t cc) { return sl && std::abs((int)c.band(a, b, cc)) <= nb; };
+    const double Fxm = bandat(i - 1, j, k) ? sl_flux_vol(c, 1, i, j, k, dt, verts) / dt : (m.dy * m.dz) * c.uf(i, j, k);
+    const double Fxp = bandat(i + 1, j, k) ? sl_flux_vol(c, 1, i + 1, j, k, dt, verts) / dt : (m.dy * m.dz) * c.uf(i + 1, j, k);
+    const double Fym = bandat(i, j - 1, k) ? sl_flux_vol(c, 2, i, j, k, dt, verts) / dt : (m.dx * m.dz) * c.vf(i, j, k);
+    const double Fyp = bandat(i, j + 1, k) ? sl_flux_vol(c, 2, i, j + 1, k, dt, verts) / dt : (m.dx * m.dz) * c.vf(i, j + 1, k);
+    const double Fzm = bandat(i, j, k - 1) ? sl_flux_vol(c, 3, i, j, k, dt, verts) / dt : (m.dx * m.dy) * c.wf(i, j, k);
+    const double Fzp = bandat(i, j, k + 1) ? sl_flux_vol(c, 3, i, j, k + 1, dt, verts) / dt : (m.dx * m.dy) * c.wf(i, j, k + 1);
+    return (((((Fxp - Fxm) + Fyp) - Fym) + Fzp) - Fzm) / vol;
+}
+
+// divergence! (src/Velocity.jl:59-70): divg_cell on the interior, 0 elsewhere.
+void divergence(Ctx& c, double dt) {
+    const Mesh& m = c.m;
+    std::fill(c.divg.a.begin(), c.divg.a.end(), 0.0);
+#pragma omp parallel for collapse(2) schedule(dynamic, 1)
     for (int64_t k = m.kmin_; k <= m.kmax_; ++k)
         for (int64_t j = m.jmin_; j <= m.jmax_; ++j)
-            for (int64_t i = m.imin_; i <= m.imax_; ++i) {
-                double verts[8][3];
-                if (sl && std::abs((int)c.band(i, j, k)) <= nb) {
-                    const double tetsign = cell2verts(c, i, j, k, verts);
-                    for (int n = 0; n < 8; ++n) project(c, verts[n], i, j, k, dt);
-                    double tets[5][4][3];
-                    verts2tets5(verts, tets);
-                    double tv = 0.0;
-                    for (int t = 0; t < 5; ++t) tv += tet_vol(tets[t]);
-                    const double v1 = tetsign * tv;
-                    c.divg(i, j, k) = (vol - v1) / vol / dt;
-                    continue;
-                }
-                auto bandat = [&](int64_t a, int64_t b, int64_t cc) { return sl && std::abs((int)c.band(a, b, cc)) <= nb; };
-                const double Fxm = bandat(i - 1, j, k) ? sl_flux_vol(c, 1, i, j, k, dt, verts) / dt : (m.dy * m.dz) * c.uf(i, j, k);
-                const double Fxp = bandat(i + 1, j, k) ? sl_flux_vol(c, 1, i + 1, j, k, dt, verts) / dt : (m.dy * m.dz) * c.uf(i + 1, j, k);
-                const double Fym = bandat(i, j - 1, k) ? sl_flux_vol(c, 2, i, j, k, dt, verts) / dt : (m.dx * m.dz) * c.vf(i, j, k);
-                const double Fyp = bandat(i, j + 1, k) ? sl_flux_vol(c, 2, i, j + 1, k, dt, verts) / dt : (m.dx * m.dz) * c.vf(i, j + 1, k);
-                const double Fzm = bandat(i, j, k - 1) ? sl_flux_vol(c, 3, i, j, k, dt, verts) / dt : (m.dx * m.dy) * c.wf(i, j, k);
-                const double Fzp = bandat(i, j, k + 1) ? sl_flux_vol(c, 3, i, j, k + 1, dt, verts) / dt : (m.dx * m.dy) * c.wf(i, j, k + 1);
-                c.divg(i, j, k) = (((((Fxp - Fxm) + Fyp) - Fym) + Fzp) - Fzm) / vol;
-            }
+            for (int64_t i = m.imin_; i <= m.imax_; ++i) c.divg(i, j, k) = divg_cell(c, i, j, k, dt);
 }
 
 // ------------------------------------------------------------------------------------
@@ -664,8 +675,13 @@ int ns_step(Ctx& c, double* t, double clamp_to, double* dt_used, double tol, int
     compute_band(c);
     compute_props(c);
     interpolate_face(c, c.us, c.vs, c.ws);
-    pressure_rhs(c);
-    st = pressure_solve(c, dt, tol, precision, cycles, resid);
+    if (c.p.pressure_scheme == MIST_PS_SEMI_LAGRANGIAN) {
+        // pressure_solver! with RHS = nothing: the SL operator's FAS cycles (src/Pressure.jl:5-30)
+        st = pressure_solve_sl(c, dt, tol, c.p.mg_max_cycles, c.sl_exact_restore, cycles, resid, nullptr);
+    } else {
+        pressure_rhs(c);
+        st = pressure_solve(c, dt, tol, precision, cycles, resid);
+    }
     if (st && st != MIST_ERR_NOCONV) return st;
     corrector(c, dt);
     interpolate_center(c);

@@ -222,6 +222,10 @@ void compute_normal(Ctx& c) {
                                 a = cn[h][0]; b = cn[h][1]; d = cn[h][2];
                             } else {
                                 ++h_refuse;
+#pragma omp critical(ora_refusals)
+                                if (c.refusals.size() < 64)
+                                    c.refusals.push_back({0, i, j, k, h, best, (h >= 0 && h < 27) ? err[h] : NAN,
+                                                          best >= 0 ? err[best] : NAN});
                             }
                         }
                     }

@@ -91,6 +91,12 @@ def lib():
             "ora_push_hints": (i32, [vp, i32, ctypes.POINTER(ctypes.c_int32), i64]),
             "ora_clear_hints": (None, [vp]),
             "ora_tie_stats": (i32, [vp, i32, ip, dp]),
+            "ora_tie_refusals": (i64, [vp, ip, dp, i64]),
+            "ora_pressure_solve_sl": (i32, [vp, d, d, i32, ip, dp, ip]),
+            "ora_set_sl_exact_restore": (None, [vp, i32]),
+            "ora_sl_operator": (i32, [vp, d, dp, dp, dp]),
+            "ora_sl_jacobian": (i32, [vp, d, i32, dp]),
+            "ora_anderson_coefficients": (None, [dp, i32, dp]),
             "ora_set_count_flops": (None, [i32]),
             "ora_get_flops": (None, [ctypes.POINTER(ctypes.c_uint64)]),
             "ora_set_threads": (None, [i32]),
@@ -213,6 +219,34 @@ class OracleContext:
         if check:
             self._check(st)
         return cyc.value, res.value
+
+    def pressure_solve_sl(self, dt, tol, exact_restore=False, check=True):
+        """The SL-SL pressure solve (FAS cycles of res_iteration on the semi-Lagrangian
+        operator); returns (cycles, final max|A(P)+tau|, res_iteration iterations)."""
+        cyc = ctypes.c_int64()
+        res = ctypes.c_double()
+        its = ctypes.c_int64()
+        st = lib().ora_pressure_solve_sl(self.h, dt, tol, 1 if exact_restore else 0, ctypes.byref(cyc),
+                                        ctypes.byref(res), ctypes.byref(its))
+        if check:
+            self._check(st)
+        return cyc.value, res.value, its.value
+
+    def set_sl_exact_restore(self, on=True):
+        lib().ora_set_sl_exact_restore(self.h, 1 if on else 0)
+
+    def sl_operator(self, dt):
+        """A!(P): returns (LHS, gradx, grady, gradz), cell extents indexed [i, j, k]."""
+        shape = self.extents("P")[1]
+        outs = [np.zeros(tuple(shape[::-1])) for _ in range(3)]
+        self._check(lib().ora_sl_operator(self.h, dt, *(_dp(a) for a in outs)))
+        return (self.get("divg"),) + tuple(a.transpose(2, 1, 0) for a in outs)
+
+    def sl_jacobian(self, dt, exact_restore=False):
+        shape = self.extents("P")[1]
+        out = np.zeros(tuple(shape[::-1]))
+        self._check(lib().ora_sl_jacobian(self.h, dt, 1 if exact_restore else 0, _dp(out)))
+        return out.transpose(2, 1, 0)
 
     def mg_level_coef(self, dt, level, axis):
         """FP64 face coefficients of multigrid level `level` (0 = fine) along `axis` that a
@@ -356,6 +390,14 @@ class OracleContext:
     def clear_hints(self):
         lib().ora_clear_hints(self.h)
 
+    def tie_refusals(self):
+        """The refused hints so far: (kind, i, j, k, hint, own, score of hint, score of own)."""
+        ints = np.zeros(6 * 64, np.int64)
+        sc = np.zeros(2 * 64)
+        n = lib().ora_tie_refusals(self.h, _ip(ints), _dp(sc), 64)
+        return [tuple(int(v) for v in ints[6 * q:6 * q + 6]) + (float(sc[2 * q]), float(sc[2 * q + 1]))
+                for q in range(n)]
+
     def tie_stats(self, kind):
         """{differed, adopted, refused, pending, gap} of the tie hints of one kind."""
         out = np.zeros(4, np.int64)
@@ -379,6 +421,15 @@ def d2case(d, inplace=True):
 def tet_vol(v):
     a = np.ascontiguousarray(np.asarray(v, np.float64).reshape(4, 3))
     return lib().ora_tet_vol(_dp(a))
+
+
+def anderson_coefficients(gram):
+    """par_anderson_accel's coefficients for a Gram matrix (n x n)."""
+    g = np.ascontiguousarray(gram, np.float64)
+    n = g.shape[0]
+    out = np.zeros(n)
+    lib().ora_anderson_coefficients(_dp(g), n, _dp(out))
+    return out
 
 
 def set_threads(n):

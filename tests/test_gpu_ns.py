@@ -173,11 +173,19 @@ def test_rising_bubble_stated_ic_tie_aware(mist, oracle, precision):
     _init_state_equal(g, o)
     tg = 0.0
     for step in range(3):
+        # the state a step starts from (VF and the PLIC constants the previous step left are
+        # both read by transport!): the device's fields, the oracle's own derived ones
         for name in ("VF", "u", "v", "w", "uf", "vf", "wf", "P", "mask"):
             o.set(name, g.get(name))
+        h0 = g.elvira_choice()
+        o.push_hints("elvira", h0)
+        o.compute_normal()
+        o.compute_plic()
         o.compute_band()
         o.compute_props()
-        h0 = g.elvira_choice()
+        assert np.array_equal(g.get("band"), o.get("band")), f"band before step {step}"
+        for name in ("nx", "ny", "nz", "D"):
+            _close(g.get(name), o.get(name), 1e-12, f"{name} before step {step}")
         t0 = tg
         tg, dtg, cg, rg = g.ns_step(tg, param.tol, precision)
         o.push_hints("elvira", h0)
