@@ -234,6 +234,15 @@ int mist_transport_vjp(mist_ctx* ctx, double dt);
 int mist_checkpoint_save(mist_ctx* ctx, int slot);
 int mist_checkpoint_restore(mist_ctx* ctx, int slot);
 
+/* computeDist of a batch, as mist_plic_dist_batch, without allocating: no hipMalloc, hipFree
+ * or allocating library call. No reference counterpart; it is for callers that cannot
+ * allocate device memory (tight serving loops, code that forbids allocation). The batch is
+ * staged in a per-GPU device array of 4 MiB, held until the process ends, so m may be at most
+ * 65536 (larger batches fail with MIST_ERR_PARAM). Every call on a GPU shares that GPU's
+ * array: calls that overlap on one GPU (from several host threads) race. */
+int mist_plic_dist_batch_noalloc(mist_ctx* ctx, const int64_t* ijk, const double* nrm,
+                                 const double* vf, double* dist_out, int64_t m);
+
 /* ELVIRA's choice for the current VF (ELVIRA!'s argmin, src/ELVIRA.jl:106-137), for checks
  * against another implementation: out[(i-1) + Nx*((j-1) + Ny*(k-1))] = the candidate kept in
  * interface cell (i, j, k), numbered c = 9*family + 3*outer pair + inner pair in the loop

@@ -4,12 +4,38 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "mist_internal.h"
 
 static std::string g_last_error;
+
+// Device-resident staging of mist_plic_dist_batch_noalloc: one array per GPU, held for the
+// life of the process. Per batch entry 64 bytes (3 int64 cell indices, 3 normal components,
+// VF, the result), so 4 MiB hold NOALLOC_BATCH_MAX entries. Host code must never pass the
+// array's name to a kernel or copy: its device address comes from hipGetSymbolAddress, taken
+// once per device in mist_ctx_create and kept in g_scratch_ptr.
+namespace {
+constexpr int64_t NOALLOC_BATCH_MAX = 65536;
+constexpr size_t SCRATCH_DOUBLES = (size_t)NOALLOC_BATCH_MAX * 8;
+constexpr int MAX_DEVICES = 64;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+std::mutex g_scratch_mu;
+}  // namespace
+static __device__ __align__(256) double scratch[SCRATCH_DOUBLES];
+
+// The device address of this device's scratch (the current device must be `device`).
+static hipError_t scratch_address(int device) {
+    if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e == hipSuccess) g_scratch_ptr[device] = ptr;
+    return e;
+}
 
 static int fail(mist_ctx* ctx, int code, const std::string& msg) {
     if (ctx) ctx->err = msg;
@@ -203,6 +229,10 @@ int mist_ctx_create(const mist_params* pin, int device, mist_ctx** out) {
         return code;
     };
     if (hipSetDevice(device) != hipSuccess) return bail(MIST_ERR_HIP, "hipSetDevice failed");
+    {
+        const hipError_t e = scratch_address(device);
+        if (e != hipSuccess) return bail(MIST_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    }
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return bail(MIST_ERR_HIP, "stream");
     if (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) return bail(MIST_ERR_HIP, "event");
 
@@ -767,6 +797,37 @@ int mist_plic_dist_batch(mist_ctx* ctx, const int64_t* ijk, const double* nrm, c
 int mist_plic_vf_batch(mist_ctx* ctx, const int64_t* ijk, const double* nrm, const double* dist, double* vf_out,
                        int64_t m) {
     return batch(ctx, ijk, nrm, dist, vf_out, m, false);
+}
+
+int mist_plic_dist_batch_noalloc(mist_ctx* ctx, const int64_t* ijk, const double* nrm, const double* vf,
+                                 double* dist_out, int64_t m) {
+    CHECK_CTX();
+    if (m < 0 || (m > 0 && (!ijk || !nrm || !vf || !dist_out))) return fail(ctx, MIST_ERR_PARAM, "bad batch");
+    if (m > NOALLOC_BATCH_MAX)
+        return fail(ctx, MIST_ERR_PARAM, "mist_plic_dist_batch_noalloc: batch of " + std::to_string(m) +
+                                             " entries exceeds the limit of " + std::to_string(NOALLOC_BATCH_MAX));
+    if (m == 0) return MIST_OK;
+    const mist::DevMesh& mm = ctx->m;
+    for (int64_t q = 0; q < m; ++q) {
+        const int64_t i = ijk[3 * q], j = ijk[3 * q + 1], k = ijk[3 * q + 2];
+        if (i < 1 - mm.ng || j < 1 - mm.ng || k < 1 - mm.ng || i > mm.Nx + mm.ng || j > mm.Ny + mm.ng || k > mm.Nz + mm.ng)
+            return fail(ctx, MIST_ERR_PARAM, "cell index out of range");
+    }
+    MIST_HIP_TRY(hipSetDevice(ctx->device));
+    double* base = (double*)g_scratch_ptr[ctx->device];  // set by mist_ctx_create on this device
+    if (!base) return fail(ctx, MIST_ERR_STATE, "scratch address of the device is unknown");
+    // staging inside the scratch array: [ijk 3m int64][nrm 3m][vf m][dist m]
+    int64_t* d_ijk = (int64_t*)base;
+    double* d_n = base + 3 * m;
+    double* d_s = base + 6 * m;
+    double* d_o = base + 7 * m;
+    MIST_HIP_TRY(hipMemcpyAsync(d_ijk, ijk, m * 3 * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
+    MIST_HIP_TRY(hipMemcpyAsync(d_n, nrm, m * 3 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    MIST_HIP_TRY(hipMemcpyAsync(d_s, vf, m * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    RET_IF(mist::launch_plic_dist_batch(ctx, d_ijk, d_n, d_s, d_o, m));
+    MIST_HIP_TRY(hipMemcpyAsync(dist_out, d_o, m * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    MIST_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return MIST_OK;
 }
 
 // ---- adjoint ---------------------------------------------------------------------------

@@ -67,6 +67,7 @@ def lib():
         "mist_vof_init": (i32, [vp]),
         "mist_plic_dist_batch": (i32, [vp, ip, dp, dp, dp, i64]),
         "mist_plic_vf_batch": (i32, [vp, ip, dp, dp, dp, i64]),
+        "mist_plic_dist_batch_noalloc": (i32, [vp, ip, dp, dp, dp, i64]),
         "mist_stage_times": (i32, [vp, dp, i32]),
         "mist_profile_enable": (i32, [vp, i32]),
         "mist_profile_read": (i32, [vp, i32, dp, ip, ip]),
@@ -242,6 +243,15 @@ class Context:
         vf = np.ascontiguousarray(vf, np.float64).reshape(-1)
         out = np.zeros(len(vf))
         self._check(lib().mist_plic_dist_batch(self.h, _ip(ijk), _dp(nrm), _dp(vf), _dp(out), len(vf)))
+        return out
+
+    def plic_dist_batch_noalloc(self, ijk, nrm, vf):
+        """``plic_dist_batch`` without device allocation (at most 65536 entries)."""
+        ijk = np.ascontiguousarray(ijk, np.int64).reshape(-1, 3)
+        nrm = np.ascontiguousarray(nrm, np.float64).reshape(-1, 3)
+        vf = np.ascontiguousarray(vf, np.float64).reshape(-1)
+        out = np.zeros(len(vf))
+        self._check(lib().mist_plic_dist_batch_noalloc(self.h, _ip(ijk), _dp(nrm), _dp(vf), _dp(out), len(vf)))
         return out
 
     def plic_vf_batch(self, ijk, nrm, dist):
