@@ -176,6 +176,19 @@ int mist_pressure_rhs(mist_ctx* ctx);                   /* RHS = div(u*)  src/Pr
 int mist_pressure_solve(mist_ctx* ctx, double dt, double tol, int64_t precision,
                         int64_t* cycles, double* final_resid);
 int mist_corrector(mist_ctx* ctx, double dt);           /* corrector!  src/Velocity.jl:3-20 */
+/* Semi-Lagrangian ("SL-SL") pressure equation, pressure_scheme = semi-Lagrangian:
+ *   A(P) = divg_cell of the corrected faces u* - dt/rho grad P, the SL pre-image volume in the
+ *   band (src/Pressure.jl:83-114, src/Velocity.jl:72-122). mist_pressure_solve then runs the
+ *   FAS cycles of mg_cycler/mg_fas! (src/Multigrid.jl:333-500) with res_iteration as smoother
+ *   (diagonal-Newton steps with Anderson mixing, src/Pressure.jl:777-884) to max|A(P)| < tol,
+ *   or res_iteration alone for mg_lvl = 1; precision is ignored (FP64). Non-periodic axes.
+ * mist_pressure_operator: A!(P) of the current P, face velocities, densities and band into
+ *   MIST_FIELD_DIVG (src/Pressure.jl:83-114).
+ * mist_pressure_jacobian: jacob_single (src/Pressure.jl:699-717), the diagonal of dA/dP by
+ *   central differences with delta 5, each cell's columns from the unperturbed P, into the host
+ *   array out of n = the cell-extent count (interior set). */
+int mist_pressure_operator(mist_ctx* ctx, double dt);
+int mist_pressure_jacobian(mist_ctx* ctx, double dt, double* out, size_t n);
 int mist_reduce(mist_ctx* ctx, int kind, int field, double* out); /* interior cells, FP64 */
 
 /* One step of run_solver with solveNS=false (src/NavierStokes_Parallel.jl:125-178):
@@ -211,9 +224,8 @@ int mist_surface_force(mist_ctx* ctx);
 int mist_ns_init(mist_ctx* ctx);
 /* One step of run_solver with solveNS (src/NavierStokes_Parallel.jl:125-168): dt, transport!,
  * computeBand!, compute_props!, interpolateFace!(us,vs,ws), the pressure solve (as
- * mist_pressure_solve with b = div(u*)), corrector!, interpolateCenter!, update_borders!, and
- * divergence!. Needs pressure_scheme = finite-difference (the semi-Lagrangian scheme's
- * nonlinear pressure equation is not built: MIST_ERR_PARAM). */
+ * mist_pressure_solve: b = div(u*) under the finite-difference scheme, the SL equation under
+ * the semi-Lagrangian one), corrector!, interpolateCenter!, update_borders!, and divergence!. */
 int mist_ns_step(mist_ctx* ctx, double* t, double clamp_to, double* dt_used, double tol, int64_t precision,
                  int64_t* cycles, double* final_resid);
 

@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmist_hip.so")
-SOURCES = ["mist_vof.hip", "mist_ns.hip", "mist_adjoint.hip", "mist_mg.hip", "mist_capi.hip"]
+SOURCES = ["mist_vof.hip", "mist_ns.hip", "mist_adjoint.hip", "mist_mg.hip", "mist_fas.hip", "mist_capi.hip"]
 ARCH = os.environ.get("MIST_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"

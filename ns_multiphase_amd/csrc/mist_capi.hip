@@ -187,6 +187,7 @@ int mist_ctx_destroy(mist_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     mist::mg_free(ctx->mg);
+    mist::fas_free(ctx->fas);
     for (int f = 0; f < MIST_FIELD_COUNT; ++f)
         if (ctx->f[f]) (void)hipFree(ctx->f[f]);
     void* bufs[] = {ctx->geom, ctx->trig, ctx->pv, ctx->band_list, ctx->iface_list, ctx->defer_list, ctx->cut_list,
@@ -671,9 +672,6 @@ int mist_ns_step(mist_ctx* ctx, double* t, double clamp_to, double* dt_used, dou
                  int64_t* cycles, double* final_resid) {
     CHECK_CTX();
     if (!t) return fail(ctx, MIST_ERR_PARAM, "null t");
-    if (ctx->p.pressure_scheme != MIST_PS_FINITE_DIFFERENCE)
-        return fail(ctx, MIST_ERR_PARAM, "mist_ns_step: the semi-Lagrangian pressure equation is not built; "
-                                         "use pressure_scheme = finite-difference");
     if (precision != MIST_MG_FP64 && precision != MIST_MG_MIXED) return fail(ctx, MIST_ERR_PARAM, "precision");
     if (!(tol > 0.0)) return fail(ctx, MIST_ERR_PARAM, "tol must be > 0");
     MIST_HIP_TRY(hipSetDevice(ctx->device));
@@ -688,13 +686,15 @@ int mist_ns_step(mist_ctx* ctx, double* t, double clamp_to, double* dt_used, dou
     RET_IF(mist_compute_band(ctx));
     RET_IF(mist::launch_props(ctx));
     RET_IF(mist::launch_interp_face(ctx, MIST_FIELD_US, MIST_FIELD_VS, MIST_FIELD_WS));
-    RET_IF(mist::launch_pressure_rhs(ctx));
+    const bool sl = ctx->p.pressure_scheme == MIST_PS_SEMI_LAGRANGIAN;
+    if (!sl) RET_IF(mist::launch_pressure_rhs(ctx));  // the SL equation has no right-hand side
     int64_t cyc = 0;
     double res = 0.0;
     int st;
     {
         StageTimer tm(ctx, MIST_STAGE_MG);
-        st = mist::pressure_solve(ctx, dt, tol, (int)precision, &cyc, &res);
+        st = sl ? mist::pressure_solve_sl(ctx, dt, tol, &cyc, &res, nullptr)
+                : mist::pressure_solve(ctx, dt, tol, (int)precision, &cyc, &res);
     }
     if (cycles) *cycles = cyc;
     if (final_resid) *final_resid = res;
@@ -726,12 +726,36 @@ int mist_pressure_solve(mist_ctx* ctx, double dt, double tol, int64_t precision,
     int st;
     {
         StageTimer tm(ctx, MIST_STAGE_MG);
-        st = mist::pressure_solve(ctx, dt, tol, (int)precision, &cyc, &res);
+        st = ctx->p.pressure_scheme == MIST_PS_SEMI_LAGRANGIAN ? mist::pressure_solve_sl(ctx, dt, tol, &cyc, &res, nullptr)
+                                                               : mist::pressure_solve(ctx, dt, tol, (int)precision, &cyc, &res);
     }
     if (cycles) *cycles = cyc;
     if (final_resid) *final_resid = res;
     if (st) g_last_error = ctx->err;
     return st;
+}
+
+int mist_pressure_operator(mist_ctx* ctx, double dt) {
+    CHECK_CTX();
+    if (ctx->p.pressure_scheme != MIST_PS_SEMI_LAGRANGIAN)
+        return fail(ctx, MIST_ERR_PARAM, "mist_pressure_operator: the semi-Lagrangian scheme's operator");
+    if (ctx->p.xper || ctx->p.yper || ctx->p.zper) return fail(ctx, MIST_ERR_PARAM, "semi-Lagrangian pressure: periodic axes are not supported");
+    if (!(dt > 0.0)) return fail(ctx, MIST_ERR_PARAM, "dt must be > 0");
+    MIST_HIP_TRY(hipSetDevice(ctx->device));
+    RET_IF(mist::pressure_operator_sl(ctx, dt));
+    return MIST_OK;
+}
+
+int mist_pressure_jacobian(mist_ctx* ctx, double dt, double* out, size_t n) {
+    CHECK_CTX();
+    if (ctx->p.pressure_scheme != MIST_PS_SEMI_LAGRANGIAN)
+        return fail(ctx, MIST_ERR_PARAM, "mist_pressure_jacobian: the semi-Lagrangian scheme's operator");
+    if (ctx->p.xper || ctx->p.yper || ctx->p.zper) return fail(ctx, MIST_ERR_PARAM, "semi-Lagrangian pressure: periodic axes are not supported");
+    if (!(dt > 0.0) || !out || n != (size_t)(ctx->fn[MIST_FIELD_P][0] * ctx->fn[MIST_FIELD_P][1] * ctx->fn[MIST_FIELD_P][2]))
+        return fail(ctx, MIST_ERR_PARAM, "mist_pressure_jacobian: dt > 0 and a cell-extent output array");
+    MIST_HIP_TRY(hipSetDevice(ctx->device));
+    RET_IF(mist::pressure_jacobian_sl(ctx, dt, out));
+    return MIST_OK;
 }
 
 int mist_corrector(mist_ctx* ctx, double dt) {

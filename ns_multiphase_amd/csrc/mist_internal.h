@@ -22,6 +22,8 @@
 namespace mist {
 struct MGState;
 void mg_free(MGState* s);
+struct FasState;
+void fas_free(FasState* s);
 }
 
 enum { MIST_STAGE_NORMAL = 0, MIST_STAGE_TRANSPORT, MIST_STAGE_PLIC, MIST_STAGE_BAND, MIST_STAGE_MG, MIST_STAGE_COUNT };
@@ -75,6 +77,7 @@ struct mist_ctx {
     uint64_t vf_version = 1, normals_version = 0, band_version = 0;
     bool band_list_valid = false;
     mist::MGState* mg = nullptr;
+    mist::FasState* fas = nullptr;  // SL pressure: multigrid levels (child contexts) and Anderson state
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double stage_ms[MIST_STAGE_COUNT] = {};
     bool timing = false;
@@ -100,6 +103,7 @@ int launch_trig_tables(mist_ctx* ctx, int64_t n);
 int launch_define_velocity(mist_ctx* ctx, double t);
 int launch_maxabs_uvw(mist_ctx* ctx, double out[3]);
 int launch_neumann(mist_ctx* ctx, int field);
+int launch_neumann_cells(mist_ctx* ctx, double* A);  // Neumann! of a cell-extent array of ctx's mesh
 int launch_borders(mist_ctx* ctx, int field);
 int launch_band(mist_ctx* ctx);
 int launch_normals(mist_ctx* ctx);
@@ -128,4 +132,8 @@ int launch_divergence(mist_ctx* ctx, double dt);
 int launch_transport_vjp(mist_ctx* ctx, double dt);
 // mist_mg.hip
 int pressure_solve(mist_ctx* ctx, double dt, double tol, int precision, int64_t* cycles, double* resid);
+// mist_fas.hip
+int pressure_solve_sl(mist_ctx* ctx, double dt, double tol, int64_t* cycles, double* resid, int64_t* smooth_iters);
+int pressure_operator_sl(mist_ctx* ctx, double dt);
+int pressure_jacobian_sl(mist_ctx* ctx, double dt, double* out);
 }  // namespace mist

@@ -1379,6 +1379,13 @@ int launch_neumann(mist_ctx* ctx, int field) {
     return MIST_OK;
 }
 
+int launch_neumann_cells(mist_ctx* ctx, double* A) {
+    const Ext3 e = ext_of(ctx, MIST_FIELD_P);
+    const int64_t tot = (int64_t)e.n[1] * e.n[2] + (int64_t)e.n[2] * e.n[0] + (int64_t)e.n[0] * e.n[1];
+    LAUNCH(k_neumann3<double>, grid_for(tot, 256), 256, A, e, ctx->m.Nx, ctx->m.Ny, ctx->m.Nz);
+    return MIST_OK;
+}
+
 int launch_borders(mist_ctx* ctx, int field) {
     const Ext3 e = ext_of(ctx, field);
     const int64_t per[3] = {ctx->p.xper, ctx->p.yper, ctx->p.zper};

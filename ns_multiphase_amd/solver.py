@@ -84,6 +84,8 @@ def lib():
         "mist_checkpoint_save": (i32, [vp, i32]),
         "mist_checkpoint_restore": (i32, [vp, i32]),
         "mist_elvira_choice": (i32, [vp, ctypes.POINTER(ctypes.c_int32), ctypes.c_size_t]),
+        "mist_pressure_operator": (i32, [vp, d]),
+        "mist_pressure_jacobian": (i32, [vp, d, dp, ctypes.c_size_t]),
         "mist_field_modified": (i32, [vp, i32]),
     }
     for name, (res, args) in sig.items():
@@ -213,6 +215,17 @@ class Context:
 
     def corrector(self, dt):
         self._check(lib().mist_corrector(self.h, dt))
+
+    def pressure_operator(self, dt):
+        """A!(P) of the semi-Lagrangian scheme into the "divg" field."""
+        self._check(lib().mist_pressure_operator(self.h, dt))
+
+    def pressure_jacobian(self, dt):
+        """jacob_single: the diagonal of dA/dP (cell extents, indexed [i, j, k])."""
+        shape = self.extents("P")[1]
+        out = np.zeros(tuple(shape[::-1]))
+        self._check(lib().mist_pressure_jacobian(self.h, dt, _dp(out), out.size))
+        return out.transpose(2, 1, 0)
 
     def pressure_solve(self, dt, tol, precision="fp64", check=True):
         cyc = ctypes.c_int64()

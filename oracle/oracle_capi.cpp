@@ -185,6 +185,7 @@ int ora_pressure_solve_sl(void* h, double dt, double tol, int exact_restore, int
     return st;
 }
 void ora_set_sl_exact_restore(void* h, int on) { ((Ctx*)h)->sl_exact_restore = on != 0; }
+void ora_set_sl_first_step_newton(void* h, int on) { ((Ctx*)h)->sl_first_step_newton = on != 0; }
 // A! of the current P, uf, vf, wf, densities and band: LHS into divg, the corrected faces
 // into gx, gy, gz (cell extents; any may be null).
 int ora_sl_operator(void* h, double dt, double* gx, double* gy, double* gz) {

@@ -119,9 +119,12 @@ struct Ctx {
         double e_hint, e_own;  // ELVIRA errors; for directions the |n| components of hint and own
     };
     std::vector<Refusal> refusals;  // the first 64 refused hints
-    // SL pressure: jacob_single with P and the corrected faces restored exactly per cell
-    // (not the reference; see jacob_single in oracle_fas.cpp)
-    bool sl_exact_restore = false;
+    // SL pressure, the build's decisions on two reference quirks (false gives the reference's
+    // literal form): jacob_single restores P and the corrected faces exactly per cell (see
+    // jacob_single in oracle_fas.cpp), and res_iteration's first iteration takes the
+    // diagonal-Newton step Gn_loc (the reference copies its zero-filled Gn argument into P)
+    bool sl_exact_restore = true;
+    bool sl_first_step_newton = true;
 };
 
 // ---- geometry primitives (src/VFgeom.jl, src/VF.jl, src/Tools.jl) -----------------

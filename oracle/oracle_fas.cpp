@@ -39,6 +39,7 @@ struct FasLevel {
 std::unique_ptr<Ctx> level_ctx(const Ctx& fine, int level) {
     auto c = std::make_unique<Ctx>();
     c->p = fine.p;
+    c->sl_first_step_newton = fine.sl_first_step_newton;
     c->m.build(c->p, level);
     const Ext ce = c->m.cell_ext();
     c->VF.init(c->m.vof_ext());
@@ -284,6 +285,9 @@ static int64_t res_iteration(FasLevel& L, Arr3<double>& P, double dt, double tol
                 P(i, j, k) = beta * Pn[r] + (1.0 - beta) * P(i, j, k);
                 ++r;
             });
+        } else if (c.sl_first_step_newton) {
+            size_t r = 0;
+            interior_for(m, [&](int64_t i, int64_t j, int64_t k) { P(i, j, k) = Gn_loc[r++]; });
         } else {
             // P = Gn[interior]: the Gn argument, zero-filled above and never written
             interior_for(m, [&](int64_t i, int64_t j, int64_t k) { P(i, j, k) = L.Gn(i, j, k); });

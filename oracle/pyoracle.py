@@ -94,6 +94,7 @@ def lib():
             "ora_tie_refusals": (i64, [vp, ip, dp, i64]),
             "ora_pressure_solve_sl": (i32, [vp, d, d, i32, ip, dp, ip]),
             "ora_set_sl_exact_restore": (None, [vp, i32]),
+            "ora_set_sl_first_step_newton": (None, [vp, i32]),
             "ora_sl_operator": (i32, [vp, d, dp, dp, dp]),
             "ora_sl_jacobian": (i32, [vp, d, i32, dp]),
             "ora_anderson_coefficients": (None, [dp, i32, dp]),
@@ -234,6 +235,9 @@ class OracleContext:
 
     def set_sl_exact_restore(self, on=True):
         lib().ora_set_sl_exact_restore(self.h, 1 if on else 0)
+
+    def set_sl_first_step_newton(self, on=True):
+        lib().ora_set_sl_first_step_newton(self.h, 1 if on else 0)
 
     def sl_operator(self, dt):
         """A!(P): returns (LHS, gradx, grady, gradz), cell extents indexed [i, j, k]."""

@@ -235,11 +235,11 @@ def test_rising_bubble_steps_match_oracle(mist, oracle, precision):
     assert g.reduce("max", "w") > 0.0
 
 
-def test_ns_step_rejects_semi_lagrangian_pressure(mist):
-    param, ic = W.rising_bubble(16, pressure_scheme="semi-lagrangian")
+def test_semi_lagrangian_pressure_rejects_periodic_axes(mist):
+    param, ic = W.rising_bubble(16, pressure_scheme="semi-lagrangian", zper=True)
     g = mist.Context(param)
-    with pytest.raises(mist.MistError, match="semi-Lagrangian"):
-        g.ns_step(0.0, 1e-10)
+    with pytest.raises(mist.MistError, match="periodic"):
+        g.pressure_solve(1e-3, 1e-10)
 
 
 @pytest.mark.parametrize("scheme", ["finite-difference", "semi-lagrangian"])
