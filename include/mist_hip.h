@@ -222,6 +222,15 @@ int mist_surface_force(mist_ctx* ctx);
  * Neumann!(VF), BC!(u,v,w), update_borders! of u, v, w and VF, interpolateFace!(u,v,w),
  * compute_props!, computeBand!, computeNormal!, computePLIC!, divergence! at the first dt. */
 int mist_ns_init(mist_ctx* ctx);
+/* The restart branch of run_solver's initialisation (src/NavierStokes_Parallel.jl:59-67 and
+ * :87-100), for a context whose VF, uf, vf, wf and P were uploaded from a restart file's arrays
+ * (fillArrays, src/ReadData.jl:3-53: cell extents with their ghosts) into an otherwise fresh
+ * context: VF beyond the cell extents is cleared (as fillArrays leaves it), u, v, w are rebuilt
+ * from the faces by interpolateCenter! (ghosts from us, vs, ws, zero in a fresh context), then
+ * update_borders!, compute_props!, computeBand!, computeNormal!, computePLIC!, compute_dt and
+ * divergence!. *dt (may be NULL) returns compute_dt's step. The time loop then continues with
+ * mist_vof_step / mist_ns_step from the restart time. */
+int mist_restart_init(mist_ctx* ctx, double* dt);
 /* One step of run_solver with solveNS (src/NavierStokes_Parallel.jl:125-168): dt, transport!,
  * computeBand!, compute_props!, interpolateFace!(us,vs,ws), the pressure solve (as
  * mist_pressure_solve: b = div(u*) under the finite-difference scheme, the SL equation under

@@ -668,6 +668,29 @@ int mist_ns_init(mist_ctx* ctx) {
     return MIST_OK;
 }
 
+int mist_restart_init(mist_ctx* ctx, double* dt) {
+    CHECK_CTX();
+    MIST_HIP_TRY(hipSetDevice(ctx->device));
+    // fillArrays fills the cell extents; the VF layers beyond them stay as initArrays left them
+    RET_IF(mist::launch_clear_vof_outer(ctx));
+    vf_changed(ctx);
+    RET_IF(mist::launch_interp_center(ctx));
+    RET_IF(mist::launch_borders(ctx, MIST_FIELD_U));
+    RET_IF(mist::launch_borders(ctx, MIST_FIELD_V));
+    RET_IF(mist::launch_borders(ctx, MIST_FIELD_W));
+    RET_IF(mist::launch_props(ctx));
+    RET_IF(mist_compute_band(ctx));
+    RET_IF(mist_compute_normal(ctx));
+    RET_IF(mist_compute_plic(ctx));
+    double mx[3];
+    RET_IF(mist::launch_maxabs_uvw(ctx, mx));
+    const double d = dt_from_max(ctx, mx);
+    RET_IF(mist::launch_divergence(ctx, d));
+    MIST_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (dt) *dt = d;
+    return MIST_OK;
+}
+
 int mist_ns_step(mist_ctx* ctx, double* t, double clamp_to, double* dt_used, double tol, int64_t precision,
                  int64_t* cycles, double* final_resid) {
     CHECK_CTX();

@@ -104,6 +104,7 @@ int launch_define_velocity(mist_ctx* ctx, double t);
 int launch_maxabs_uvw(mist_ctx* ctx, double out[3]);
 int launch_neumann(mist_ctx* ctx, int field);
 int launch_neumann_cells(mist_ctx* ctx, double* A);  // Neumann! of a cell-extent array of ctx's mesh
+int launch_clear_vof_outer(mist_ctx* ctx);           // VF outside the cell extents = 0
 int launch_borders(mist_ctx* ctx, int field);
 int launch_band(mist_ctx* ctx);
 int launch_normals(mist_ctx* ctx);

@@ -1379,6 +1379,21 @@ int launch_neumann(mist_ctx* ctx, int field) {
     return MIST_OK;
 }
 
+// The 3 extra ghost layers of VF beyond the cell extents set to 0.
+__global__ void k_clear_vof_outer(DevMesh m, double* VF) {
+    const int64_t n = (int64_t)m.vnx * m.vny * m.vnz;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        const int a = (int)(q % m.vnx), b = (int)((q / m.vnx) % m.vny), c = (int)(q / ((int64_t)m.vnx * m.vny));
+        if (a < 3 || b < 3 || c < 3 || a >= m.vnx - 3 || b >= m.vny - 3 || c >= m.vnz - 3) VF[q] = 0.0;
+    }
+}
+
+int launch_clear_vof_outer(mist_ctx* ctx) {
+    const int64_t n = (int64_t)ctx->m.vnx * ctx->m.vny * ctx->m.vnz;
+    LAUNCH(k_clear_vof_outer, grid_for(n, 256), 256, ctx->m, (double*)ctx->f[MIST_FIELD_VOF]);
+    return MIST_OK;
+}
+
 int launch_neumann_cells(mist_ctx* ctx, double* A) {
     const Ext3 e = ext_of(ctx, MIST_FIELD_P);
     const int64_t tot = (int64_t)e.n[1] * e.n[2] + (int64_t)e.n[2] * e.n[0] + (int64_t)e.n[0] * e.n[1];

@@ -85,6 +85,7 @@ def lib():
         "mist_checkpoint_restore": (i32, [vp, i32]),
         "mist_elvira_choice": (i32, [vp, ctypes.POINTER(ctypes.c_int32), ctypes.c_size_t]),
         "mist_pressure_operator": (i32, [vp, d]),
+        "mist_restart_init": (i32, [vp, dp]),
         "mist_pressure_jacobian": (i32, [vp, d, dp, ctypes.c_size_t]),
         "mist_field_modified": (i32, [vp, i32]),
     }
@@ -302,6 +303,13 @@ class Context:
 
     def ns_init(self):
         self._check(lib().mist_ns_init(self.h))
+
+    def restart_init(self):
+        """run_solver's restart initialisation after VF, uf, vf, wf, P were uploaded; returns
+        compute_dt's step."""
+        dt = ctypes.c_double()
+        self._check(lib().mist_restart_init(self.h, ctypes.byref(dt)))
+        return dt.value
 
     def ns_step(self, t, tol, precision="fp64", clamp_to=0.0, check=True):
         tt = ctypes.c_double(t)

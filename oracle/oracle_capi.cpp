@@ -303,6 +303,30 @@ int ora_ns_init(void* h) {
     return MIST_OK;
 }
 
+// The restart branch of run_solver's initialisation (src/NavierStokes_Parallel.jl:59-67,
+// :87-100) after fillArrays (src/ReadData.jl:3-53) filled VF, uf, vf, wf, P on the cell extents.
+int ora_restart_init(void* h, double* dt) {
+    Ctx& c = *(Ctx*)h;
+    const Mesh& m = c.m;
+    for (int64_t k = c.VF.e.lo[2]; k <= c.VF.hi(2); ++k)
+        for (int64_t j = c.VF.e.lo[1]; j <= c.VF.hi(1); ++j)
+            for (int64_t i = c.VF.e.lo[0]; i <= c.VF.hi(0); ++i)
+                if (i < m.imino_ || i > m.imaxo_ || j < m.jmino_ || j > m.jmaxo_ || k < m.kmino_ || k > m.kmaxo_)
+                    c.VF(i, j, k) = 0.0;
+    interpolate_center(c);
+    update_borders(c, c.u);
+    update_borders(c, c.v);
+    update_borders(c, c.w);
+    compute_props(c);
+    compute_band(c);
+    compute_normal(c);
+    compute_plic(c);
+    const double d = compute_dt(c);
+    divergence(c, d);
+    if (dt) *dt = d;
+    return MIST_OK;
+}
+
 int ora_ns_step(void* h, double* t, double clamp_to, double* dt_used, double tol, int64_t precision,
                 int64_t* cycles, double* resid) {
     return ns_step(*(Ctx*)h, t, clamp_to, dt_used, tol, (int)precision, cycles, resid);

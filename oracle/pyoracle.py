@@ -84,6 +84,7 @@ def lib():
             "ora_compute_curvature": (i32, [vp]),
             "ora_surface_force": (i32, [vp]),
             "ora_ns_init": (i32, [vp]),
+            "ora_restart_init": (i32, [vp, dp]),
             "ora_ns_step": (i32, [vp, dp, d, dp, d, i64, ip, dp]),
             "ora_sl_flux_vol": (d, [vp, i32, i64, i64, i64, d, dp]),
             "ora_correction_tets": (i32, [vp, i64, i64, i64, d, dp]),
@@ -304,6 +305,11 @@ class OracleContext:
 
     def ns_init(self):
         self._check(lib().ora_ns_init(self.h))
+
+    def restart_init(self):
+        dt = ctypes.c_double()
+        self._check(lib().ora_restart_init(self.h, ctypes.byref(dt)))
+        return dt.value
 
     def ns_step(self, t, tol, precision="fp64", clamp_to=0.0, check=True):
         tt = ctypes.c_double(t)

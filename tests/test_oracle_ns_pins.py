@@ -524,3 +524,33 @@ def test_coarse_operators_share_the_fixed_point(oracle):
     W.mms_setup(o, dt)
     cyc, res = o.pressure_solve(dt, 1e-10, "fp64", check=False)
     assert cyc == 40 and res > 1e-2
+
+
+def test_restart_continues_the_vof_run(oracle):
+    """run_solver's restart branch (src/NavierStokes_Parallel.jl:59-67) from the restart
+    arrays (VF, uf, vf, wf, P on the cell extents) continues the 2D deformation bit for bit:
+    the velocity is prescribed and the step is max_dt, so nothing else carries over."""
+    from ns_multiphase_amd import workloads as W
+    param, ic = W.deformation2d(32)
+    a = oracle.OracleContext(param)
+    a.init_vof(*ic)
+    a.vof_init()
+    t = 0.0
+    for _ in range(6):
+        t, _ = a.vof_step(t)
+    b = oracle.OracleContext(param)
+    for n in ("uf", "vf", "wf", "P"):
+        b.set(n, a.get(n))
+    vf = a.get("VF").copy()
+    inner = np.zeros_like(vf, bool)
+    inner[3:-3, 3:-3, 3:-3] = True
+    vf[~inner] = 0.0
+    b.set("VF", vf)
+    b.restart_init()
+    ta = tb = t
+    for _ in range(4):
+        ta, _ = a.vof_step(ta)
+        tb, _ = b.vof_step(tb)
+        assert ta == tb
+        assert np.array_equal(a.get("VF"), b.get("VF"))
+        assert np.array_equal(a.get("band"), b.get("band"))
