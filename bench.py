@@ -32,6 +32,11 @@ PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
 
+def cpu_threads():
+    """Host threads for the oracle legs: OMP_NUM_THREADS (16 on a one-GPU machine's share), at most 16."""
+    return min(int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1, 16)
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -142,8 +147,7 @@ def cpu_baseline(args, param, ic, info, time_cpu=True):
     import numpy as np
     import pyoracle
     from ns_multiphase_amd import solver
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
-    threads = min(threads, 16)
+    threads = cpu_threads()
     pyoracle.set_threads(threads)
     count_at = set(info["samples"][:: max(1, len(info["samples"]) // 4)][:4])
     samples = info["samples"] if time_cpu else sorted(count_at)
@@ -293,9 +297,37 @@ def run_mg(args, torch):
         prof = ctx.profile_read()
         P = W.interior(ctx.mesh, ctx.get("P"))
         err = float(np.abs(P - P.mean() - W.mms_exact(ctx.mesh)).max())
-        out[prec] = {"ms": 1e3 * min(times), "cycles": cyc, "resid": res, "linf_err": err, "prof": prof}
+        out[prec] = {"ms": 1e3 * min(times), "cycles": cyc, "resid": res, "linf_err": err, "prof": prof,
+                     "P": P - P.mean()}
         ctx.close()
     return out
+
+
+def oracle_mms(N, dt):
+    """The oracle's FP64 multigrid on the same MMS to the same residual (1e-10), on the host
+    cores: the pressure the device's solves are compared with."""
+    import numpy as np
+    from ns_multiphase_amd import workloads as W
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    threads = cpu_threads()
+    pyoracle.set_threads(threads)
+    o = pyoracle.OracleContext(W.mms_params(N))
+    W.mms_setup(o, dt)
+    t0 = time.perf_counter()
+    cyc, res = o.pressure_solve(dt, 1e-10, "fp64")
+    sec = time.perf_counter() - t0
+    P = W.interior(o.mesh, o.get("P"))
+    o.close()
+    return P - P.mean(), {"cycles": cyc, "final_resid": res, "seconds": sec, "cores": threads}
+
+
+def mms_vs_oracle(P_dev, P_ora, info):
+    """max|dp| between a device solve and the oracle's, both to max|r| <= 1e-10 with zero mean:
+    the two iterates differ by at most that residual through the inverse operator."""
+    import numpy as np
+    return {"max_abs_dP": float(np.abs(P_dev - P_ora).max()), "max_abs_P": float(np.abs(P_ora).max()),
+            "oracle": info}
 
 
 def run_bubble(args, torch):
@@ -398,8 +430,7 @@ def cpu_bubble(args, param, ic, nsamples=2):
     import pyoracle
     from ns_multiphase_amd import solver
     from ns_multiphase_amd.params import FIELD
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
-    threads = min(threads, 16)
+    threads = cpu_threads()
     pyoracle.set_threads(threads)
     g = solver.Context(param)
     g.init_vof(*ic)
@@ -463,7 +494,7 @@ def run_config1(args, torch):
     import pyoracle
     from ns_multiphase_amd import solver
     from ns_multiphase_amd import workloads as W
-    threads = min(int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1, 16)
+    threads = cpu_threads()
     pyoracle.set_threads(threads)
     param, ic = W.deformation2d(128)
     out = {"workload": "2D deformation 128^2, circle r=0.15 at (0.5,0.75), T=4, CFL 0.5, dt=1/256, RK4, 5 tets"}
@@ -595,6 +626,7 @@ def run_large(args, torch):
     out = {"workload": f"{N}^3 MMS (rho 1000/1, tol 1e-10) + 20 steps of 3D deformation {N}^3"}
     dt = 1e-3
     p = W.mms_params(N)
+    dev_P = {}
     for prec in ("fp64", "mixed"):
         ctx = solver.Context(p)
         W.mms_setup(ctx, dt)
@@ -609,9 +641,18 @@ def run_large(args, torch):
         Pn = W.interior(ctx.mesh, ctx.get("P"))
         err = float(np.abs(Pn - Pn.mean() - W.mms_exact(ctx.mesh)).max())
         out[f"mg_{prec}"] = {"solve_ms": ms, "cycles": cyc, "final_resid": res, "linf_err_vs_exact": err}
+        dev_P[prec] = Pn - Pn.mean()
         ctx.close()
         log(f"[bench] {N}^3 MMS {prec}: {ms:.1f} ms, {cyc} cycles, L-inf {err:.2e}")
     out["mg_speedup_mixed_vs_fp64"] = out["mg_fp64"]["solve_ms"] / out["mg_mixed"]["solve_ms"]
+    if not args.no_oracle:
+        P_ora, oinfo = oracle_mms(N, dt)
+        for prec in ("fp64", "mixed"):
+            out[f"mg_{prec}"]["vs_oracle"] = mms_vs_oracle(dev_P[prec], P_ora, oinfo)
+        del P_ora
+        log(f"[bench] {N}^3 MMS vs oracle: max|dP| fp64 {out['mg_fp64']['vs_oracle']['max_abs_dP']:.2e}, "
+            f"mixed {out['mg_mixed']['vs_oracle']['max_abs_dP']:.2e} (oracle {oinfo['seconds']:.1f} s)")
+    dev_P.clear()
     param, ic = W.deformation3d(N)
     ctx = solver.Context(param)
     ctx.init_vof(*ic)
@@ -708,6 +749,12 @@ def main():
                                       "frac_hbm": bytes_launch / avg / 1e9 / PEAK_HBM_GBS,
                                       "bytes_per_launch": bytes_launch}
             mg[prec] = entry
+        if not args.no_oracle:
+            P_ora, oinfo = oracle_mms(args.mg_N, 1e-3)
+            for prec in ("fp64", "mixed"):
+                mg[prec]["vs_oracle"] = mms_vs_oracle(mg_raw[prec]["P"], P_ora, oinfo)
+            log(f"[bench] mg vs oracle: max|dP| fp64 {mg['fp64']['vs_oracle']['max_abs_dP']:.2e}, "
+                f"mixed {mg['mixed']['vs_oracle']['max_abs_dP']:.2e} (oracle {oinfo['seconds']:.1f} s)")
         mg["speedup_mixed_vs_fp64"] = mg["fp64"]["solve_ms"] / mg["mixed"]["solve_ms"]
         # measured copy bandwidths beside the HBM peak: 2 x 64 MiB stays in the 256 MB Infinity
         # Cache (where the 128^3 levels live), 2 x 2 GiB does not

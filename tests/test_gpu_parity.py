@@ -74,7 +74,7 @@ def _pair(mist, oracle, param, ic):
     return g, o
 
 
-def _vof_state_equal(g, o, mesh, tol_vf=1e-12):
+def _vof_state_equal(g, o, mesh, tol_vf=1e-12, tol_d=1e-12):
     vg, vo = g.get("VF"), o.get("VF")
     assert np.abs(vg - vo).max() <= tol_vf
     bg, bo = g.get("band"), o.get("band")
@@ -82,7 +82,7 @@ def _vof_state_equal(g, o, mesh, tol_vf=1e-12):
     for name in ("nx", "ny", "nz"):
         assert np.abs(g.get(name) - o.get(name)).max() <= 1e-12, name
     dg, do = g.get("D"), o.get("D")
-    assert np.abs(dg - do).max() <= 1e-12
+    assert np.abs(dg - do).max() <= tol_d
     sg, so = g.reduce("sum", "VF"), o.reduce("sum", "VF")
     assert abs(sg - so) <= 1e-13 * abs(so)
 
@@ -125,7 +125,7 @@ def resync_vof_state(g, o):
     return h0
 
 
-def vof_steps_tie_aware(g, o, nsteps, check=None):
+def vof_steps_tie_aware(g, o, nsteps, check=None, tol_d=1e-12):
     """``nsteps`` steps of ``run_solver`` (solveNS = false) on the device ``g``, each checked
     against one oracle step from the same state.
 
@@ -146,7 +146,7 @@ def vof_steps_tie_aware(g, o, nsteps, check=None):
         o.push_hints("elvira", g.elvira_choice())
         to, dto = o.vof_step(t0)
         assert dtg == dto and tg == to
-        _vof_state_equal(g, o, g.mesh)
+        _vof_state_equal(g, o, g.mesh, tol_d=tol_d)
         if check:
             check(s)
     st = o.tie_stats("elvira")
@@ -166,6 +166,35 @@ def test_deformation3d_stated_sphere_tie_aware(mist, oracle):
     o.vof_init()
     _vof_state_equal(g, o, g.mesh)
     st = vof_steps_tie_aware(g, o, 12)
+    print("ELVIRA ties adopted:", st)
+
+
+def test_deformation3d_64_sampled_states_tie_aware(mist, oracle):
+    """The 3D deformation at 64^3 from the stated sphere, compared at states spread over the
+    768-step period: the device runs the trajectory, and from each sampled state one step is
+    checked against the oracle's step from the same state (VF 1e-12, band bit-exact, tie-aware
+    ELVIRA). The plane constant D is compared to 1e-11 after the step: computeDist maps a VF
+    difference of a few ulps near VFlo through the cube root of the volume, which amplifies it."""
+    param, ic = W.deformation3d(64)
+    g, o = _pair(mist, oracle, param, ic)
+    g.vof_init()
+    t, done = 0.0, 0
+    for target in (0, 160, 320, 480, 640):
+        while done < target:
+            t, _ = g.vof_step(t)
+            done += 1
+        print(f"sampled state: step {done}, t = {t:.6f}", flush=True)
+        h0 = resync_vof_state(g, o)
+        _vof_state_equal(g, o, g.mesh)
+        tg, dtg = g.vof_step(t)
+        o.push_hints("elvira", h0)
+        o.push_hints("elvira", g.elvira_choice())
+        to, dto = o.vof_step(t)
+        assert dtg == dto and tg == to
+        _vof_state_equal(g, o, g.mesh, tol_d=1e-11)
+        t, done = tg, done + 1
+    st = o.tie_stats("elvira")
+    assert st["refused"] == 0 and st["pending"] == 0, st
     print("ELVIRA ties adopted:", st)
 
 
