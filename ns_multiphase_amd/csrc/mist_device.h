@@ -484,19 +484,42 @@ __host__ __device__ __forceinline__ int find_cell(double p, int i, int lo, int h
     return i;
 }
 
-__host__ __device__ __forceinline__ double trilin(const double* A, const DevMesh& m, int i, int j, int k, double wx1,
-                                         double wx2, double wy1, double wy2, double wz1, double wz2) {
-    const int64_t sx = 1, sy = m.cnx, sz = (int64_t)m.cnx * m.cny;
-    const int64_t b = m.cell(i, j, k);
-    return wz1 * (wy1 * (wx1 * A[b + sx + sy + sz] + wx2 * A[b + sy + sz]) +
-                  wy2 * (wx1 * A[b + sx + sz] + wx2 * A[b + sz])) +
-           wz2 * (wy1 * (wx1 * A[b + sx + sy] + wx2 * A[b + sy]) +
-                  wy2 * (wx1 * A[b + sx] + wx2 * A[b]));
+// Face-velocity arrays read directly (the accessor of vel_face_acc and project_pt_acc).
+struct FacePtrs {
+    const double *u, *v, *w;
+    __host__ __device__ __forceinline__ double fu(int64_t q) const { return u[q]; }
+    __host__ __device__ __forceinline__ double fv(int64_t q) const { return v[q]; }
+    __host__ __device__ __forceinline__ double fw(int64_t q) const { return w[q]; }
+};
+
+template <int C, class FA>
+__host__ __device__ __forceinline__ double face_at(const FA& fa, int64_t q) {
+    if constexpr (C == 0) return fa.fu(q);
+    else if constexpr (C == 1) return fa.fv(q);
+    else return fa.fw(q);
 }
 
-// get_velocity_{u,v,w}face (src/Tools.jl:516-641), starting index (i,j,k).
-__host__ __device__ inline void vel_face(const DevMesh& m, const double* uf, const double* vf, const double* wf,
-                                const double pt[3], int i0, int j0, int k0, double v[3]) {
+template <int C, class FA>
+__host__ __device__ __forceinline__ double trilin_acc(const FA& fa, const DevMesh& m, int i, int j, int k, double wx1,
+                                                     double wx2, double wy1, double wy2, double wz1, double wz2) {
+    const int64_t sx = 1, sy = m.cnx, sz = (int64_t)m.cnx * m.cny;
+    const int64_t b = m.cell(i, j, k);
+    return wz1 * (wy1 * (wx1 * face_at<C>(fa, b + sx + sy + sz) + wx2 * face_at<C>(fa, b + sy + sz)) +
+                  wy2 * (wx1 * face_at<C>(fa, b + sx + sz) + wx2 * face_at<C>(fa, b + sz))) +
+           wz2 * (wy1 * (wx1 * face_at<C>(fa, b + sx + sy) + wx2 * face_at<C>(fa, b + sy)) +
+                  wy2 * (wx1 * face_at<C>(fa, b + sx) + wx2 * face_at<C>(fa, b)));
+}
+
+__host__ __device__ __forceinline__ double trilin(const double* A, const DevMesh& m, int i, int j, int k, double wx1,
+                                         double wx2, double wy1, double wy2, double wz1, double wz2) {
+    return trilin_acc<0>(FacePtrs{A, A, A}, m, i, j, k, wx1, wx2, wy1, wy2, wz1, wz2);
+}
+
+// get_velocity_{u,v,w}face (src/Tools.jl:516-641), starting index (i,j,k); the face
+// velocities come from the accessor fa (fu, fv, fw at a cell-extent index).
+template <class FA>
+__host__ __device__ inline void vel_face_acc(const DevMesh& m, const FA& fa, const double pt[3], int i0, int j0, int k0,
+                                             double v[3]) {
     const int lo = 1 - m.ng;
     int i, j, k;
     double wx1, wy1, wz1;
@@ -507,19 +530,24 @@ __host__ __device__ inline void vel_face(const DevMesh& m, const double* uf, con
     wx1 = sdiv(pt[0] - m.x[i], m.x[i + 1] - m.x[i]);
     wy1 = sdiv(pt[1] - m.ym[j], m.ym[j + 1] - m.ym[j]);
     wz1 = sdiv(pt[2] - m.zm[k], m.zm[k + 1] - m.zm[k]);
-    v[0] = trilin(uf, m, i, j, k, wx1, 1.0 - wx1, wy1, 1.0 - wy1, wz1, 1.0 - wz1);
+    v[0] = trilin_acc<0>(fa, m, i, j, k, wx1, 1.0 - wx1, wy1, 1.0 - wy1, wz1, 1.0 - wz1);
     // v on y faces
     i = find_cell(pt[0], i0, lo, m.Nx + m.ng, m.xm);
     j = find_cell(pt[1], j0, lo, m.Ny + m.ng, m.y);
     wx1 = sdiv(pt[0] - m.xm[i], m.xm[i + 1] - m.xm[i]);
     wy1 = sdiv(pt[1] - m.y[j], m.y[j + 1] - m.y[j]);
-    v[1] = trilin(vf, m, i, j, k, wx1, 1.0 - wx1, wy1, 1.0 - wy1, wz1, 1.0 - wz1);
+    v[1] = trilin_acc<1>(fa, m, i, j, k, wx1, 1.0 - wx1, wy1, 1.0 - wy1, wz1, 1.0 - wz1);
     // w on z faces
     j = find_cell(pt[1], j0, lo, m.Ny + m.ng, m.ym);
     k = find_cell(pt[2], k0, lo, m.Nz + m.ng, m.z);
     wy1 = sdiv(pt[1] - m.ym[j], m.ym[j + 1] - m.ym[j]);
     wz1 = sdiv(pt[2] - m.z[k], m.z[k + 1] - m.z[k]);
-    v[2] = trilin(wf, m, i, j, k, wx1, 1.0 - wx1, wy1, 1.0 - wy1, wz1, 1.0 - wz1);
+    v[2] = trilin_acc<2>(fa, m, i, j, k, wx1, 1.0 - wx1, wy1, 1.0 - wy1, wz1, 1.0 - wz1);
+}
+
+__host__ __device__ inline void vel_face(const DevMesh& m, const double* uf, const double* vf, const double* wf,
+                                const double pt[3], int i0, int j0, int k0, double v[3]) {
+    vel_face_acc(m, FacePtrs{uf, vf, wf}, pt, i0, j0, k0, v);
 }
 
 }  // namespace mist

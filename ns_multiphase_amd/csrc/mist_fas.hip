@@ -22,8 +22,8 @@
 //     copies its zero-filled Gn argument into P there (src/Pressure.jl:843), which restarts
 //     every smoother call from P = 0 and stalls the cycle;
 //   * the Jacobian's columns are independent: each cell's two perturbed evaluations start from
-//     the unperturbed P (the reference adds and subtracts delta in place). Cells whose stencils
-//     cannot meet are perturbed together (colours of spacing 2 nband + 2 per axis).
+//     the unperturbed P (the reference adds and subtracts delta in place); one thread per cell
+//     evaluates both at that cell.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -31,6 +31,7 @@
 #include <vector>
 
 #include "mist_kernels.h"
+#include "mist_nsgeom.h"
 
 namespace mist {
 
@@ -50,7 +51,7 @@ struct FasLevel {
     int64_t nint = 0;       // interior count
     double *uf0 = nullptr, *vf0 = nullptr, *wf0 = nullptr;  // predicted face velocities
     double *P_h = nullptr, *AP_f = nullptr, *AP_c = nullptr, *P_bar_H = nullptr, *P_H = nullptr;
-    double *tau = nullptr, *jac = nullptr, *Pt = nullptr, *A1 = nullptr;
+    double *tau = nullptr, *jac = nullptr;
     double* hist = nullptr;  // F slots then G slots, FAS_M each, interior size
     double* small = nullptr; // gram (FAS_M^2), alpha (FAS_M), scalars
 };
@@ -65,7 +66,7 @@ struct FasState {
 void fas_free(FasState* s) {
     if (!s) return;
     for (FasLevel& L : s->lv) {
-        double* bufs[] = {L.uf0, L.vf0, L.wf0, L.P_h, L.AP_f, L.AP_c, L.P_bar_H, L.P_H, L.tau, L.jac, L.Pt, L.A1,
+        double* bufs[] = {L.uf0, L.vf0, L.wf0, L.P_h, L.AP_f, L.AP_c, L.P_bar_H, L.P_H, L.tau, L.jac,
                           L.hist, L.small};
         for (double* b : bufs)
             if (b) (void)hipFree(b);
@@ -107,6 +108,117 @@ __global__ void k_fas_grad(DevMesh m, double dt, const double* uf0, const double
         if (jin && iin && k >= 0 && k <= m.Nz + 2)
             c = wf0[q] - sdiv(sdiv(dt, denz[q]) * (P[q] - P[q - (int64_t)m.cnx * m.cny]), m.dz);
         gx[q] = a; gy[q] = b; gz[q] = c;
+    }
+}
+
+// The corrected faces of A! with P + s at one interior cell c and at the ghost copies of it
+// that the one-layer Neumann fill makes (the cells whose coordinates clamp to c's: 0 -> 1,
+// N + 1 -> N). A face neither of whose cells is perturbed is read from the faces k_fas_grad
+// computed for the unperturbed P; the others are recomputed with k_fas_grad's arithmetic.
+struct PertFaces {
+    const DevMesh* m;
+    const double *gx, *gy, *gz;
+    const double *uf0, *vf0, *wf0, *denx, *deny, *denz, *P;
+    double dt, s;
+    int64_t img[8];
+    int nimg;
+    __device__ __forceinline__ bool pert(int64_t q) const {
+        for (int n = 0; n < nimg; ++n)
+            if (img[n] == q) return true;
+        return false;
+    }
+    __device__ __forceinline__ double Pp(int64_t q) const { return pert(q) ? P[q] + s : P[q]; }
+    __device__ double fu(int64_t q) const {
+        if (!pert(q) && !pert(q - 1)) return gx[q];
+        int i, j, k;
+        cell_of(*m, q, i, j, k);
+        if (!(k >= 0 && k <= m->Nz + 1 && j >= 0 && j <= m->Ny + 1 && i >= 0 && i <= m->Nx + 2)) return 0.0;
+        return uf0[q] - ((dt / denx[face_x(*m, i, j, k)]) * (Pp(q) - Pp(q - 1))) / m->dx;
+    }
+    __device__ double fv(int64_t q) const {
+        const int64_t sy = m->cnx;
+        if (!pert(q) && !pert(q - sy)) return gy[q];
+        int i, j, k;
+        cell_of(*m, q, i, j, k);
+        if (!(k >= 0 && k <= m->Nz + 1 && i >= 0 && i <= m->Nx + 1 && j >= 0 && j <= m->Ny + 2)) return 0.0;
+        return vf0[q] - sdiv(sdiv(dt, deny[face_y(*m, i, j, k)]) * (Pp(q) - Pp(q - sy)), m->dy);
+    }
+    __device__ double fw(int64_t q) const {
+        const int64_t sz = (int64_t)m->cnx * m->cny;
+        if (!pert(q) && !pert(q - sz)) return gz[q];
+        int i, j, k;
+        cell_of(*m, q, i, j, k);
+        if (!(j >= 0 && j <= m->Ny + 1 && i >= 0 && i <= m->Nx + 1 && k >= 0 && k <= m->Nz + 2)) return 0.0;
+        return wf0[q] - sdiv(sdiv(dt, denz[q]) * (Pp(q) - Pp(q - sz)), m->dz);
+    }
+};
+
+// The 8 corners of one cell back-traced with perturbed faces (the divergence of a cell reads
+// no other vertex).
+struct CornerVerts {
+    double v[8][3];
+    int i, j, k;
+    __device__ __forceinline__ double at(int I, int J, int K, int c) const {
+        return v[(I - i) + 2 * (J - j) + 4 * (K - k)][c];
+    }
+};
+
+// jacob_single (src/Pressure.jl:699-717), one thread per interior cell c: A!(P + delta e_c)
+// and A!(P - delta e_c) at c, each from the unperturbed state (the columns are independent),
+// J = (A+ - A-) /^ 2 delta. The divergence at c reads the faces of c and, through the
+// back-trace of c's corners, faces near them: PertFaces gives every face as A! would compute
+// it over the whole mesh with P perturbed at c, and the corners are traced as
+// k_project_vertices traces them (only vertices of band cells are read), so each column equals
+// a whole-mesh evaluation of A! with only c perturbed.
+__global__ void k_fas_jac_local(DevMesh m, DevParams p, int nband, const int16_t* band, double dt, const double* uf0,
+                                const double* vf0, const double* wf0, const double* denx, const double* deny,
+                                const double* denz, const double* P, const double* gx, const double* gy,
+                                const double* gz, double delta, double* J) {
+    const int64_t n = (int64_t)m.Nx * m.Ny * m.Nz;
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        const int i = 1 + (int)(r % m.Nx), j = 1 + (int)((r / m.Nx) % m.Ny), k = 1 + (int)(r / ((int64_t)m.Nx * m.Ny));
+        const int64_t q = m.cell(i, j, k);
+        PertFaces fa{&m, gx, gy, gz, uf0, vf0, wf0, denx, deny, denz, P, dt, 0.0, {}, 0};
+        {
+            int xs[2], ys[2], zs[2], nx = 0, ny = 0, nz = 0;
+            xs[nx++] = i;
+            if (i == 1) xs[nx++] = 0;
+            if (i == m.Nx) xs[nx++] = m.Nx + 1;
+            ys[ny++] = j;
+            if (j == 1) ys[ny++] = 0;
+            if (j == m.Ny) ys[ny++] = m.Ny + 1;
+            zs[nz++] = k;
+            if (k == 1) zs[nz++] = 0;
+            if (k == m.Nz) zs[nz++] = m.Nz + 1;
+            for (int c = 0; c < nz; ++c)
+                for (int b = 0; b < ny; ++b)
+                    for (int a = 0; a < nx; ++a) fa.img[fa.nimg++] = m.cell(xs[a], ys[b], zs[c]);
+        }
+        // the divergence reads back-traced vertices only in a band cell or beside one
+        const int64_t sy = m.cnx, sz = (int64_t)m.cnx * m.cny;
+        const bool trace = in_band(band, q, nband) || in_band(band, q - 1, nband) || in_band(band, q + 1, nband) ||
+                           in_band(band, q - sy, nband) || in_band(band, q + sy, nband) ||
+                           in_band(band, q - sz, nband) || in_band(band, q + sz, nband);
+        double A[2];
+        for (int sgn = 0; sgn < 2; ++sgn) {
+            fa.s = sgn == 0 ? delta : -delta;
+            CornerVerts cv;
+            cv.i = i; cv.j = j; cv.k = k;
+            for (int c = 0; c < 8; ++c) {
+                const int I = i + (c & 1), Jv = j + ((c >> 1) & 1), K = k + (c >> 2);
+                cv.v[c][0] = cv.v[c][1] = cv.v[c][2] = 0.0;
+                if (!trace) continue;
+                bool need = false;  // the vertex selection of k_project_vertices
+                for (int e = 0; e < 8; ++e)
+                    need |= abs((int)band[m.cell(I - 1 + (e & 1), Jv - 1 + ((e >> 1) & 1), K - 1 + (e >> 2))]) <= p.nband;
+                if (!need) continue;
+                double pt[3] = {m.x[I], m.y[Jv], m.z[K]};
+                project_pt_acc(m, p, fa, pt, min(I, m.Nx), min(Jv, m.Ny), min(K, m.Nz), dt);
+                cv.v[c][0] = pt[0]; cv.v[c][1] = pt[1]; cv.v[c][2] = pt[2];
+            }
+            A[sgn] = divg_cell_eval(m, nband, 1, band, cv, fa, dt, q, i, j, k);
+        }
+        J[q] = sdiv(A[0] - A[1], 2.0 * delta);
     }
 }
 
@@ -290,34 +402,6 @@ __global__ __launch_bounds__(FAS_T) void k_fas_sub_mean(double* P, int64_t n, co
         P[q] -= mean;
 }
 
-// Jacobian colours: Pt = P + s on the interior cells of colour (a, b, c) of spacing S.
-__global__ void k_fas_perturb(DevMesh m, const double* P, double* Pt, double s, int a, int b, int c, int S) {
-    const int64_t n = (int64_t)m.cnx * m.cny * m.cnz;
-    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
-        int i, j, k;
-        cell_of(m, q, i, j, k);
-        const bool in = i >= 1 && i <= m.Nx && j >= 1 && j <= m.Ny && k >= 1 && k <= m.Nz;
-        const bool col = in && (i - 1) % S == a && (j - 1) % S == b && (k - 1) % S == c;
-        Pt[q] = col ? P[q] + s : P[q];
-    }
-}
-
-// sign > 0: keep A(P + delta e) of the colour's cells; sign < 0: J = (A+ - A-) /^ 2 delta.
-__global__ void k_fas_jac(DevMesh m, const double* divg, double* A1, double* J, int a, int b, int c, int S, int sign,
-                          double delta) {
-    const int nx = (m.Nx - 1 - a) / S + 1, ny = (m.Ny - 1 - b) / S + 1, nz = (m.Nz - 1 - c) / S + 1;
-    if (a >= m.Nx || b >= m.Ny || c >= m.Nz) return;
-    const int64_t n = (int64_t)nx * ny * nz;
-    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
-        const int i = 1 + a + S * (int)(q % nx);
-        const int j = 1 + b + S * (int)((q / nx) % ny);
-        const int k = 1 + c + S * (int)(q / ((int64_t)nx * ny));
-        const int64_t cq = m.cell(i, j, k);
-        if (sign > 0) A1[cq] = divg[cq];
-        else J[cq] = sdiv(A1[cq] - divg[cq], 2.0 * delta);
-    }
-}
-
 struct Ext {
     int lo[3], n[3];
     __device__ __forceinline__ int64_t at(int i, int j, int k) const {
@@ -463,7 +547,7 @@ static int fas_setup(mist_ctx* ctx) {
         L.c = c;
         L.ncell = (int64_t)c->m.cnx * c->m.cny * c->m.cnz;
         L.nint = (int64_t)c->m.Nx * c->m.Ny * c->m.Nz;
-        double** cells[] = {&L.uf0, &L.vf0, &L.wf0, &L.P_h, &L.AP_f, &L.AP_c, &L.P_bar_H, &L.P_H, &L.tau, &L.jac, &L.Pt, &L.A1};
+        double** cells[] = {&L.uf0, &L.vf0, &L.wf0, &L.P_h, &L.AP_f, &L.AP_c, &L.P_bar_H, &L.P_H, &L.tau, &L.jac};
         for (double** b : cells) {
             if (hipMalloc(b, (size_t)L.ncell * sizeof(double)) != hipSuccess) { ctx->err = "FAS level allocation"; return MIST_ERR_OOM; }
             MIST_HIP_TRY(hipMemsetAsync(*b, 0, (size_t)L.ncell * sizeof(double), ctx->stream));
@@ -493,13 +577,20 @@ static int fas_setup(mist_ctx* ctx) {
         if (st_) { ctx->err = c->err; return st_; } \
     } while (0)
 
-static int fas_A(mist_ctx* ctx, FasLevel& L, double* P, double dt) {
+// The Neumann fill of P and the corrected faces of A! into the level's UF, VF, WF.
+static int fas_faces(mist_ctx* ctx, FasLevel& L, double* P, double dt) {
     mist_ctx* c = L.c;
     CHILD(launch_neumann_cells(c, P));
     FL(k_fas_grad, fgrid(L.ncell, 256, 65536), 256, c->m, dt, (const double*)L.uf0, (const double*)L.vf0,
        (const double*)L.wf0, (const double*)c->f[MIST_FIELD_DENX], (const double*)c->f[MIST_FIELD_DENY],
        (const double*)c->f[MIST_FIELD_DENZ], (const double*)P, (double*)c->f[MIST_FIELD_UF],
        (double*)c->f[MIST_FIELD_VF], (double*)c->f[MIST_FIELD_WF]);
+    return MIST_OK;
+}
+
+static int fas_A(mist_ctx* ctx, FasLevel& L, double* P, double dt) {
+    mist_ctx* c = L.c;
+    RET_IF_ST(fas_faces(ctx, L, P, dt));
     CHILD(launch_divergence(c, dt));
     return MIST_OK;
 }
@@ -517,22 +608,16 @@ static int fas_residual(mist_ctx* ctx, FasLevel& L, double* P, double* AP, bool 
     return MIST_OK;
 }
 
-// jacob_single (src/Pressure.jl:699-717) with independent columns, by colours.
+// jacob_single (src/Pressure.jl:699-717) with independent columns: the faces of the
+// unperturbed P, then every column evaluated locally by its own thread.
 static int fas_jacobian(mist_ctx* ctx, FasLevel& L, double* P, double dt) {
     mist_ctx* c = L.c;
-    const int S = 2 * (int)ctx->p.nband + 2;
-    const double delta = 5.0;
-    for (int a = 0; a < std::min(S, c->m.Nx); ++a)
-        for (int b = 0; b < std::min(S, c->m.Ny); ++b)
-            for (int d = 0; d < std::min(S, c->m.Nz); ++d) {
-                const int64_t ncol = (int64_t)((c->m.Nx - 1 - a) / S + 1) * ((c->m.Ny - 1 - b) / S + 1) * ((c->m.Nz - 1 - d) / S + 1);
-                for (int sign = 1; sign >= -1; sign -= 2) {
-                    FL(k_fas_perturb, fgrid(L.ncell, 256, 65536), 256, c->m, (const double*)P, L.Pt, sign * delta, a, b, d, S);
-                    RET_IF_ST(fas_A(ctx, L, L.Pt, dt));
-                    FL(k_fas_jac, fgrid(ncol, 256, 65536), 256, c->m, (const double*)c->f[MIST_FIELD_DIVG], L.A1, L.jac, a, b,
-                       d, S, sign, delta);
-                }
-            }
+    RET_IF_ST(fas_faces(ctx, L, P, dt));
+    FL(k_fas_jac_local, fgrid(L.nint, 256, 65536), 256, c->m, c->dp, (int)c->p.nband,
+       (const int16_t*)c->f[MIST_FIELD_BAND], dt, (const double*)L.uf0, (const double*)L.vf0, (const double*)L.wf0,
+       (const double*)c->f[MIST_FIELD_DENX], (const double*)c->f[MIST_FIELD_DENY],
+       (const double*)c->f[MIST_FIELD_DENZ], (const double*)P, (const double*)c->f[MIST_FIELD_UF],
+       (const double*)c->f[MIST_FIELD_VF], (const double*)c->f[MIST_FIELD_WF], 5.0, L.jac);
     return MIST_OK;
 }
 

@@ -52,8 +52,9 @@ __device__ __forceinline__ void append_group(bool flag, int item, int* list, int
 // a vertex does not depend on which cell asks for it (the interpolation index search
 // converges to the same cell from any start), so each vertex of a band cell is traced once.
 // ------------------------------------------------------------------------------------
-__host__ __device__ inline void project_pt(const DevMesh& m, const DevParams& p, const double* uf, const double* vf,
-                           const double* wf, double pt[3], int i, int j, int k, double dt) {
+template <class FA>
+__host__ __device__ inline void project_pt_acc(const DevMesh& m, const DevParams& p, const FA& fa, double pt[3], int i,
+                                               int j, int k, double dt) {
     if (p.zalesak) {
         const double pi2 = 2.0 * PI;
         const double x0 = pt[0], y0 = pt[1];
@@ -63,23 +64,28 @@ __host__ __device__ inline void project_pt(const DevMesh& m, const DevParams& p,
     }
     double v1[3], v2[3], v3[3], v4[3], q[3];
     if (p.projection == MIST_PROJ_RK4) {
-        vel_face(m, uf, vf, wf, pt, i, j, k, v1);
+        vel_face_acc(m, fa, pt, i, j, k, v1);
         for (int n = 0; n < 3; ++n) q[n] = pt[n] - (0.5 * dt) * v1[n];
-        vel_face(m, uf, vf, wf, q, i, j, k, v2);
+        vel_face_acc(m, fa, q, i, j, k, v2);
         for (int n = 0; n < 3; ++n) q[n] = pt[n] - (0.5 * dt) * v2[n];
-        vel_face(m, uf, vf, wf, q, i, j, k, v3);
+        vel_face_acc(m, fa, q, i, j, k, v3);
         for (int n = 0; n < 3; ++n) q[n] = pt[n] - dt * v3[n];
-        vel_face(m, uf, vf, wf, q, i, j, k, v4);
+        vel_face_acc(m, fa, q, i, j, k, v4);
         for (int n = 0; n < 3; ++n) pt[n] = pt[n] + ((-dt) / 6.0) * (((v1[n] + 2.0 * v2[n]) + 2.0 * v3[n]) + v4[n]);
     } else if (p.projection == MIST_PROJ_EULER) {
-        vel_face(m, uf, vf, wf, pt, i, j, k, v1);
+        vel_face_acc(m, fa, pt, i, j, k, v1);
         for (int n = 0; n < 3; ++n) pt[n] = pt[n] + (-dt) * v1[n];
     } else {
-        vel_face(m, uf, vf, wf, pt, i, j, k, v1);
+        vel_face_acc(m, fa, pt, i, j, k, v1);
         for (int n = 0; n < 3; ++n) q[n] = pt[n] - (0.5 * dt) * v1[n];
-        vel_face(m, uf, vf, wf, q, i, j, k, v2);
+        vel_face_acc(m, fa, q, i, j, k, v2);
         for (int n = 0; n < 3; ++n) pt[n] = pt[n] + (-dt) * v2[n];
     }
+}
+
+__host__ __device__ inline void project_pt(const DevMesh& m, const DevParams& p, const double* uf, const double* vf,
+                           const double* wf, double pt[3], int i, int j, int k, double dt) {
+    project_pt_acc(m, p, FacePtrs{uf, vf, wf}, pt, i, j, k, dt);
 }
 
 __device__ __forceinline__ int64_t vtx_id(const DevMesh& m, int I, int J, int K) {

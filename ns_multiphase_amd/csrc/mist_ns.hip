@@ -26,7 +26,6 @@ __device__ __forceinline__ int64_t fyi(const DevMesh& m, int i, int j, int k) {
 }
 __device__ __forceinline__ int64_t fzi(const DevMesh& m, int i, int j, int k) { return m.cell(i, j, k); }
 
-__device__ __forceinline__ bool in_band(const int16_t* band, int64_t c, int nband) { return abs((int)band[c]) <= nband; }
 
 // Tets FIRST .. FIRST+T-1 of the cells of `list` through the general cut walk with the six
 // momentum sums, one lane per (cell, tet), 64/T cells per block. Without FINAL the cell's
@@ -372,8 +371,8 @@ __global__ void k_interp_center(DevMesh m, const double* us, const double* vs, c
 __global__ void k_divergence(DevMesh m, int nband, int sl, const int16_t* band, const double* pv,
                              const double* uf, const double* vf, const double* wf, double dt, double* divg) {
     const int64_t n = (int64_t)m.cnx * m.cny * m.cnz;
-    const int64_t sy = m.cnx, sz = (int64_t)m.cnx * m.cny;
-    const double vol = (m.dx * m.dy) * m.dz;
+    const VertPtr pva{&m, pv};
+    const FacePtrs fa{uf, vf, wf};
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
         int i, j, k;
         cell_of(m, q, i, j, k);
@@ -381,36 +380,7 @@ __global__ void k_divergence(DevMesh m, int nband, int sl, const int16_t* band, 
             divg[q] = 0.0;
             continue;
         }
-        if (sl && in_band(band, q, nband)) {
-            const bool even = ((i + j + k) % 2) == 0;
-            double tv = 0.0;
-            for (int t = 0; t < 5; ++t) {
-                double tet[4][3];
-                const unsigned tq = TET5[t];
-                for (int v4 = 0; v4 < 4; ++v4) {
-                    const int c = vert_corner(nib(tq, v4), even);
-                    const int64_t vq = vtx_id(m, i + (c & 1), j + ((c >> 1) & 1), k + (c >> 2));
-                    tet[v4][0] = pv[3 * vq + 0]; tet[v4][1] = pv[3 * vq + 1]; tet[v4][2] = pv[3 * vq + 2];
-                }
-                tv += tet_vol(tet[0], tet[1], tet[2], tet[3]);
-            }
-            const double v1 = (even ? 1.0 : -1.0) * tv;
-            divg[q] = (vol - v1) / vol / dt;
-            continue;
-        }
-        double verts[8][3];
-        auto flux = [&](int d, int up, int64_t nb) {
-            const int fi = i + (d == 1 ? up : 0), fj = j + (d == 2 ? up : 0), fk = k + (d == 3 ? up : 0);
-            if (sl && in_band(band, nb, nband)) return sl_flux_vol(m, pv, d, fi, fj, fk, verts) / dt;
-            const int64_t fc = m.cell(fi, fj, fk);
-            if (d == 1) return (m.dy * m.dz) * uf[fc];
-            if (d == 2) return (m.dx * m.dz) * vf[fc];
-            return (m.dx * m.dy) * wf[fc];
-        };
-        const double Fxm = flux(1, 0, q - 1), Fxp = flux(1, 1, q + 1);
-        const double Fym = flux(2, 0, q - sy), Fyp = flux(2, 1, q + sy);
-        const double Fzm = flux(3, 0, q - sz), Fzp = flux(3, 1, q + sz);
-        divg[q] = (((((Fxp - Fxm) + Fyp) - Fym) + Fzp) - Fzm) / vol;
+        divg[q] = divg_cell_eval(m, nband, sl, band, pva, fa, dt, q, i, j, k);
     }
 }
 

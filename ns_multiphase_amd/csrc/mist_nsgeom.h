@@ -12,8 +12,18 @@ namespace mist {
 // (i,j,k) in one step, 5 tets over its 4 back-traced and 4 fixed corners. The back-traced
 // corners are the projected vertices pv of the band cells. `verts` receives the 8 vertices.
 // ------------------------------------------------------------------------------------
-__device__ inline double sl_flux_vol(const DevMesh& m, const double* pv, int dir, int i, int j, int k,
-                                     double verts[8][3]) {
+__device__ __forceinline__ bool in_band(const int16_t* band, int64_t c, int nband) { return abs((int)band[c]) <= nband; }
+
+// Back-traced vertices read from the array of all grid vertices (3 doubles per vertex).
+struct VertPtr {
+    const DevMesh* m;
+    const double* pv;
+    __device__ __forceinline__ double at(int I, int J, int K, int c) const { return pv[3 * vtx_id(*m, I, J, K) + c]; }
+};
+
+template <class PV>
+__device__ inline double sl_flux_vol_acc(const DevMesh& m, const PV& pv, int dir, int i, int j, int k,
+                                         double verts[8][3]) {
     const bool even = ((i + j + k) % 2) == 0;
     const int odd = even ? 0 : 1;
 #pragma unroll
@@ -32,8 +42,7 @@ __device__ inline double sl_flux_vol(const DevMesh& m, const double* pv, int dir
             proj = s < 4;
         }
         if (proj) {
-            const int64_t q = vtx_id(m, I, J, K);
-            verts[s][0] = pv[3 * q + 0]; verts[s][1] = pv[3 * q + 1]; verts[s][2] = pv[3 * q + 2];
+            verts[s][0] = pv.at(I, J, K, 0); verts[s][1] = pv.at(I, J, K, 1); verts[s][2] = pv.at(I, J, K, 2);
         } else {
             verts[s][0] = m.x[I]; verts[s][1] = m.y[J]; verts[s][2] = m.z[K];
         }
@@ -45,6 +54,51 @@ __device__ inline double sl_flux_vol(const DevMesh& m, const double* pv, int dir
         v += tet_vol(verts[nib(q, 0)], verts[nib(q, 1)], verts[nib(q, 2)], verts[nib(q, 3)]);
     }
     return v * (even ? 1.0 : -1.0);
+}
+
+__device__ inline double sl_flux_vol(const DevMesh& m, const double* pv, int dir, int i, int j, int k,
+                                     double verts[8][3]) {
+    return sl_flux_vol_acc(m, VertPtr{&m, pv}, dir, i, j, k, verts);
+}
+
+// divg_cell (src/Velocity.jl:79-122) of interior cell (i, j, k), linear index q. Under the
+// semi-Lagrangian scheme a band cell's value is 1 - (pre-image volume) / vol over dt and a
+// face beside a band cell carries its SLfluxVol; the other faces carry A u_face. The
+// back-traced vertices come from the accessor pv (at(I, J, K, c)), the face velocities from fa.
+template <class PV, class FA>
+__device__ inline double divg_cell_eval(const DevMesh& m, int nband, int sl, const int16_t* band, const PV& pv,
+                                        const FA& fa, double dt, int64_t q, int i, int j, int k) {
+    const int64_t sy = m.cnx, sz = (int64_t)m.cnx * m.cny;
+    const double vol = (m.dx * m.dy) * m.dz;
+    if (sl && in_band(band, q, nband)) {
+        const bool even = ((i + j + k) % 2) == 0;
+        double tv = 0.0;
+        for (int t = 0; t < 5; ++t) {
+            double tet[4][3];
+            const unsigned tq = TET5[t];
+            for (int v4 = 0; v4 < 4; ++v4) {
+                const int c = vert_corner(nib(tq, v4), even);
+                const int I = i + (c & 1), J = j + ((c >> 1) & 1), K = k + (c >> 2);
+                tet[v4][0] = pv.at(I, J, K, 0); tet[v4][1] = pv.at(I, J, K, 1); tet[v4][2] = pv.at(I, J, K, 2);
+            }
+            tv += tet_vol(tet[0], tet[1], tet[2], tet[3]);
+        }
+        const double v1 = (even ? 1.0 : -1.0) * tv;
+        return (vol - v1) / vol / dt;
+    }
+    double verts[8][3];
+    auto flux = [&](int d, int up, int64_t nb) {
+        const int fi = i + (d == 1 ? up : 0), fj = j + (d == 2 ? up : 0), fk = k + (d == 3 ? up : 0);
+        if (sl && in_band(band, nb, nband)) return sl_flux_vol_acc(m, pv, d, fi, fj, fk, verts) / dt;
+        const int64_t fc = m.cell(fi, fj, fk);
+        if (d == 1) return (m.dy * m.dz) * fa.fu(fc);
+        if (d == 2) return (m.dx * m.dz) * fa.fv(fc);
+        return (m.dx * m.dy) * fa.fw(fc);
+    };
+    const double Fxm = flux(1, 0, q - 1), Fxp = flux(1, 1, q + 1);
+    const double Fym = flux(2, 0, q - sy), Fyp = flux(2, 1, q + sy);
+    const double Fzm = flux(3, 0, q - sz), Fzp = flux(3, 1, q + sz);
+    return (((((Fxp - Fxm) + Fyp) - Fym) + Fzp) - Fzm) / vol;
 }
 
 // correction_tets_{x,y,z} (src/VFgeom.jl:581-640): the coordinate of the apex e along the face
