@@ -234,6 +234,7 @@ def cpu_baseline(args, param, ic, info, time_cpu=True):
     st = o.tie_stats("elvira")
     err["elvira_ties_adopted"] = st["adopted"]
     err["elvira_hints_refused"] = st["refused"]
+    err["elvira_max_tie_gap"] = st["gap"]  # largest relative error gap of an adopted tie
     g.close()
     log(f"  cpu: {ncpu} oracle steps in {cpu_time:.1f} s on {threads} threads, {none} on one thread in "
         f"{one_time:.1f} s; per step vs device: max|dVF| {err['max_abs_dVF']:.2e}, band equal {err['band_equal']}, "
@@ -476,6 +477,7 @@ def cpu_bubble(args, param, ic, nsamples=2):
         st = o.tie_stats(kind)
         err[f"{kind}_ties_adopted"] = st["adopted"]
         err[f"{kind}_hints_refused"] = st["refused"]
+        err[f"{kind}_max_tie_gap"] = st["gap"]
     g.close()
     log(f"  cpu bubble: {len(at)} oracle steps in {cpu_time:.1f} s; per step vs device: max|dVF| "
         f"{err['max_abs_dVF']:.2e}, max|dP| {err['max_abs_dP']:.2e} of {err['max_abs_P']:.2e}, "

@@ -56,10 +56,22 @@ struct FasLevel {
     double* small = nullptr; // gram (FAS_M^2), alpha (FAS_M), scalars
 };
 
+// Convergence state of one res_iteration call on the device: once `done` is set (max|AP| <
+// tol), the iterations already queued after it change nothing, so the host queues iterations
+// in batches and reads the state once per batch.
+struct FasCtrl {
+    int done;         // converged
+    int iters;        // iterations of this call (residual evaluations)
+    double res;       // the last max|AP|
+    long long total;  // iterations since the solve began
+};
+
 struct FasState {
     std::vector<FasLevel> lv;
     double* part = nullptr;   // FAS_BLOCKS x (FAS_M + 1) partial sums
     double* h_red = nullptr;  // pinned: max |AP|
+    FasCtrl* ctrl = nullptr;  // device
+    FasCtrl* h_ctrl = nullptr;  // pinned
     int nlev = 0;
 };
 
@@ -77,6 +89,8 @@ void fas_free(FasState* s) {
     }
     if (s->part) (void)hipFree(s->part);
     if (s->h_red) (void)hipHostFree(s->h_red);
+    if (s->ctrl) (void)hipFree(s->ctrl);
+    if (s->h_ctrl) (void)hipHostFree(s->h_ctrl);
     delete s;
 }
 
@@ -224,7 +238,8 @@ __global__ void k_fas_jac_local(DevMesh m, DevParams p, int nband, const int16_t
 
 // AP = divg (+ tau) on the interior, 0 on the ghosts; block maxima of |AP|.
 __global__ __launch_bounds__(FAS_T) void k_fas_residual(DevMesh m, const double* divg, const double* tau, double* AP,
-                                                        double* part) {
+                                                        double* part, const FasCtrl* ctl) {
+    if (ctl && ctl->done) return;
     __shared__ double s[FAS_T];
     const int64_t n = (int64_t)m.cnx * m.cny * m.cnz;
     double mx = 0.0;
@@ -248,7 +263,11 @@ __global__ __launch_bounds__(FAS_T) void k_fas_residual(DevMesh m, const double*
     if (threadIdx.x == 0) part[blockIdx.x] = s[0];
 }
 
-__global__ __launch_bounds__(FAS_T) void k_fas_max_final(const double* part, int n, double* out) {
+// max of the block maxima into *out; with ctl, the iteration's convergence test
+// (res_iteration's `res < tol`, src/Pressure.jl:819-823) unless the call has already converged.
+__global__ __launch_bounds__(FAS_T) void k_fas_max_final(const double* part, int n, double* out, FasCtrl* ctl,
+                                                         double tol) {
+    if (ctl && ctl->done) return;
     __shared__ double s[FAS_T];
     double mx = 0.0;
     for (int q = threadIdx.x; q < n; q += FAS_T) mx = jmax(mx, part[q]);
@@ -258,14 +277,30 @@ __global__ __launch_bounds__(FAS_T) void k_fas_max_final(const double* part, int
         if ((int)threadIdx.x < w) s[threadIdx.x] = jmax(s[threadIdx.x], s[threadIdx.x + w]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) *out = s[0];
+    if (threadIdx.x == 0) {
+        if (out) *out = s[0];
+        if (ctl) {
+            ctl->res = s[0];
+            ctl->iters += 1;
+            ctl->total += 1;
+            if (s[0] < tol) ctl->done = 1;
+        }
+    }
+}
+
+__global__ void k_fas_ctrl_reset(FasCtrl* ctl, int total_too) {
+    ctl->done = 0;
+    ctl->iters = 0;
+    ctl->res = INFINITY;
+    if (total_too) ctl->total = 0;
 }
 
 // Newton step and history slot: Gn = P - omega AP / max(|J|, 1e-12), Fn = Gn - P into slot
 // `slot`; block partial sums of Fn . F_s for the n filled slots (the new Gram row).
 __global__ __launch_bounds__(FAS_T) void k_fas_step(DevMesh m, const double* P, const double* AP, const double* J,
                                                     double omega, double* hist, int64_t nint, int slot, int n,
-                                                    double* part) {
+                                                    double* part, const FasCtrl* ctl) {
+    if (ctl->done) return;
     __shared__ double s[FAS_M][FAS_T];
     double acc[FAS_M];
 #pragma unroll
@@ -300,7 +335,8 @@ __global__ __launch_bounds__(FAS_T) void k_fas_step(DevMesh m, const double* P, 
 // are filled, the KKT system [G + eps I, 1; 1', 0] a = [0; 1] with eps = 1e-10 max diag G
 // (par_anderson_accel, src/Pressure.jl:747-775) by LU with partial pivoting on one thread.
 __global__ __launch_bounds__(FAS_T) void k_fas_gram_solve(const double* part, int nblocks, int slot, int n,
-                                                          double* small) {
+                                                          double* small, const FasCtrl* ctl) {
+    if (ctl->done) return;
     __shared__ double row[FAS_M];
     double* gram = small;
     double* alpha = small + FAS_M * FAS_M;
@@ -359,7 +395,9 @@ __global__ __launch_bounds__(FAS_T) void k_fas_gram_solve(const double* part, in
 // zeros and adds in slot order), the Newton step Gn of the new slot with one; block partial
 // sums of the new interior P for the mean.
 __global__ __launch_bounds__(FAS_T) void k_fas_mix(DevMesh m, double* P, const double* hist, int64_t nint, int slot,
-                                                   int n, const double* small, double beta, double* part) {
+                                                   int n, const double* small, double beta, double* part,
+                                                   const FasCtrl* ctl) {
+    if (ctl->done) return;
     __shared__ double s[FAS_T];
     const double* G = hist + (int64_t)FAS_M * nint;
     const double* alpha = small + FAS_M * FAS_M;
@@ -390,7 +428,8 @@ __global__ __launch_bounds__(FAS_T) void k_fas_mix(DevMesh m, double* P, const d
 
 // P .-= mean(P[interior]) over the whole array, ghosts included (src/Pressure.jl:881).
 __global__ __launch_bounds__(FAS_T) void k_fas_sub_mean(double* P, int64_t n, const double* part, int nblocks,
-                                                        int64_t count) {
+                                                        int64_t count, const FasCtrl* ctl) {
+    if (ctl->done) return;
     __shared__ double mean;
     if (threadIdx.x == 0) {
         double sum = 0.0;
@@ -562,8 +601,13 @@ static int fas_setup(mist_ctx* ctx) {
         ctx->err = "FAS reduction allocation";
         return MIST_ERR_OOM;
     }
-    if (hipHostMalloc(&S->h_red, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc(&S->h_red, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(&S->h_ctrl, sizeof(FasCtrl), hipHostMallocDefault) != hipSuccess) {
         ctx->err = "FAS pinned allocation";
+        return MIST_ERR_OOM;
+    }
+    if (hipMalloc(&S->ctrl, sizeof(FasCtrl)) != hipSuccess) {
+        ctx->err = "FAS control allocation";
         return MIST_ERR_OOM;
     }
     return MIST_OK;
@@ -595,16 +639,21 @@ static int fas_A(mist_ctx* ctx, FasLevel& L, double* P, double dt) {
     return MIST_OK;
 }
 
-// AP = A!(P) (+ tau) and max |AP| on the host.
-static int fas_residual(mist_ctx* ctx, FasLevel& L, double* P, double* AP, bool use_tau, double dt, double* res) {
+// AP = A!(P) (+ tau). With ctl: one iteration's residual and convergence test on the device
+// (nothing is written once ctl is done). Without: max |AP| into *res on the host when res is
+// given.
+static int fas_residual(mist_ctx* ctx, FasLevel& L, double* P, double* AP, bool use_tau, double dt, double* res,
+                        FasCtrl* ctl = nullptr, double tol = 0.0) {
     FasState* S = ctx->fas;
     RET_IF_ST(fas_A(ctx, L, P, dt));
     const int nb = fgrid(L.ncell, FAS_T);
     FL(k_fas_residual, nb, FAS_T, L.c->m, (const double*)L.c->f[MIST_FIELD_DIVG], use_tau ? (const double*)L.tau : nullptr,
-       AP, S->part);
-    FL(k_fas_max_final, 1, FAS_T, (const double*)S->part, nb, S->h_red);
-    MIST_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    *res = S->h_red[0];
+       AP, S->part, (const FasCtrl*)ctl);
+    FL(k_fas_max_final, 1, FAS_T, (const double*)S->part, nb, ctl ? nullptr : S->h_red, ctl, tol);
+    if (!ctl && res) {
+        MIST_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        *res = S->h_red[0];
+    }
     return MIST_OK;
 }
 
@@ -621,9 +670,13 @@ static int fas_jacobian(mist_ctx* ctx, FasLevel& L, double* P, double dt) {
     return MIST_OK;
 }
 
-// res_iteration (src/Pressure.jl:777-884); *iters the iterations run, *res the last max|AP|.
+// res_iteration (src/Pressure.jl:777-884). The iterations are queued in batches without a host
+// round trip: the convergence test runs on the device (FasCtrl), and the kernels of the
+// iterations queued after convergence change nothing, so the iterates are those of the
+// reference's loop. With `result` the call waits for the device at the end and returns
+// *converged and *res (the last max|AP|); otherwise it returns without waiting.
 static int fas_smooth(mist_ctx* ctx, FasLevel& L, double* P, double* AP, double dt, double tol, int64_t max_iter,
-                      bool use_tau, bool* converged, int64_t* iters, double* res) {
+                      bool use_tau, bool result, bool* converged, double* res) {
     FasState* S = ctx->fas;
     mist_ctx* c = L.c;
     const double omega = 0.8, beta = 0.5;
@@ -633,39 +686,52 @@ static int fas_smooth(mist_ctx* ctx, FasLevel& L, double* P, double* AP, double 
         CHILD(launch_neumann_cells(c, P));
     }
     RET_IF_ST(fas_jacobian(ctx, L, P, dt));
+    FL(k_fas_ctrl_reset, 1, 1, S->ctrl, 0);
     int hist_idx = 0, n_hist = 0;
-    int64_t it = 0;
-    *res = INFINITY;
+    int64_t queued = 0;
+    int batch = 8;
     const int nbi = fgrid(L.nint, FAS_T);
-    while (it < max_iter) {
-        ++it;
-        RET_IF_ST(fas_residual(ctx, L, P, AP, use_tau, dt, res));
-        if (*res < tol) {
-            if (converged) *converged = true;
-            break;
+    while (queued < max_iter) {
+        const int64_t nb = std::min<int64_t>(max_iter <= 16 ? max_iter : batch, max_iter - queued);
+        for (int64_t q = 0; q < nb; ++q) {
+            ++queued;
+            RET_IF_ST(fas_residual(ctx, L, P, AP, use_tau, dt, nullptr, S->ctrl, tol));
+            // the slot and history length of this iteration if it does not converge (the
+            // kernels of a converged call skip, so later values are never used)
+            hist_idx = hist_idx % FAS_M + 1;
+            if (n_hist < FAS_M) ++n_hist;
+            const int slot = hist_idx - 1;
+            FL(k_fas_step, nbi, FAS_T, c->m, (const double*)P, (const double*)AP, (const double*)L.jac, omega, L.hist,
+               L.nint, slot, n_hist, S->part, (const FasCtrl*)S->ctrl);
+            FL(k_fas_gram_solve, 1, FAS_T, (const double*)S->part, nbi, slot, n_hist, L.small, (const FasCtrl*)S->ctrl);
+            FL(k_fas_mix, nbi, FAS_T, c->m, P, (const double*)L.hist, L.nint, slot, n_hist, (const double*)L.small, beta,
+               S->part, (const FasCtrl*)S->ctrl);
+            FL(k_fas_sub_mean, fgrid(L.ncell, FAS_T, 65536), FAS_T, P, L.ncell, (const double*)S->part, nbi, L.nint,
+               (const FasCtrl*)S->ctrl);
         }
-        hist_idx = hist_idx % FAS_M + 1;
-        if (n_hist < FAS_M) ++n_hist;
-        const int slot = hist_idx - 1;
-        FL(k_fas_step, nbi, FAS_T, c->m, (const double*)P, (const double*)AP, (const double*)L.jac, omega, L.hist, L.nint,
-           slot, n_hist, S->part);
-        FL(k_fas_gram_solve, 1, FAS_T, (const double*)S->part, nbi, slot, n_hist, L.small);
-        FL(k_fas_mix, nbi, FAS_T, c->m, P, (const double*)L.hist, L.nint, slot, n_hist, (const double*)L.small, beta,
-           S->part);
-        FL(k_fas_sub_mean, fgrid(L.ncell, FAS_T, 65536), FAS_T, P, L.ncell, (const double*)S->part, nbi, L.nint);
+        if (queued >= max_iter) break;
+        // a long call: wait for the batch and stop once converged
+        MIST_HIP_TRY(hipMemcpyAsync(S->h_ctrl, S->ctrl, sizeof(FasCtrl), hipMemcpyDeviceToHost, ctx->stream));
+        MIST_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        if (S->h_ctrl->done) break;
+        batch = std::min(batch * 2, 32);
     }
-    *iters += it;
+    if (result) {
+        MIST_HIP_TRY(hipMemcpyAsync(S->h_ctrl, S->ctrl, sizeof(FasCtrl), hipMemcpyDeviceToHost, ctx->stream));
+        MIST_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        if (converged) *converged = S->h_ctrl->done != 0;
+        if (res) *res = S->h_ctrl->res;
+    }
     return MIST_OK;
 }
 
 // mg_fas! (src/Multigrid.jl:375-500), lvl 1-based.
-static int fas_cycle(mist_ctx* ctx, int lvl, double dt, double tol, bool* conv, int64_t* iters, double* res_out) {
+static int fas_cycle(mist_ctx* ctx, int lvl, double dt, double tol, bool* conv, double* res_out) {
     FasState* S = ctx->fas;
     FasLevel& L = S->lv[(size_t)lvl - 1];
     const int v1 = 10;
-    double res = 0.0;
-    if (lvl == S->nlev) return fas_smooth(ctx, L, L.P_h, L.AP_f, dt, 1e-10, 50000, true, nullptr, iters, &res);
-    RET_IF_ST(fas_smooth(ctx, L, L.P_h, L.AP_f, dt, tol, v1, lvl != 1, nullptr, iters, &res));
+    if (lvl == S->nlev) return fas_smooth(ctx, L, L.P_h, L.AP_f, dt, 1e-10, 50000, true, false, nullptr, nullptr);
+    RET_IF_ST(fas_smooth(ctx, L, L.P_h, L.AP_f, dt, tol, v1, lvl != 1, false, nullptr, nullptr));
     FasLevel& C = S->lv[(size_t)lvl];
     mist_ctx* cc = C.c;
     const DevMesh& cm = cc->m;
@@ -695,21 +761,19 @@ static int fas_cycle(mist_ctx* ctx, int lvl, double dt, double tol, bool* conv, 
        cm.Nz, cell_ext(cc), cell_ext(L.c), C.uf0, C.vf0, C.wf0, (const double*)L.uf0, (const double*)L.vf0,
        (const double*)L.wf0);
     // tau = R(A_h(P_h)) - A_H(R P_h)
-    {
-        double r = 0.0;
-        RET_IF_ST(fas_residual(ctx, C, C.P_h, C.AP_c, false, dt, &r));
-    }
+    RET_IF_ST(fas_residual(ctx, C, C.P_h, C.AP_c, false, dt, nullptr));
     FL(k_fas_diff, fgrid(C.ncell, 256, 65536), 256, C.tau, (const double*)C.AP_f, (const double*)C.AP_c, C.ncell);
     MIST_HIP_TRY(hipMemcpyAsync(C.P_bar_H, C.P_h, (size_t)C.ncell * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     MIST_HIP_TRY(hipMemcpyAsync(L.P_H, L.P_h, (size_t)L.ncell * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-    RET_IF_ST(fas_cycle(ctx, lvl + 1, dt, tol, conv, iters, res_out));
+    RET_IF_ST(fas_cycle(ctx, lvl + 1, dt, tol, conv, res_out));
     FL(k_fas_axpy, fgrid(C.ncell, 256, 65536), 256, C.P_h, (const double*)C.P_bar_H, -1.0, C.ncell);
     FL(k_fas_zero, fgrid(L.ncell, 256, 65536), 256, L.AP_f, L.ncell);
     FL(k_fas_prolong, fgrid(L.nint, 256, 65536), 256, L.c->m, cm, L.AP_f, (const double*)C.P_h);
     FL(k_fas_interior_add, fgrid(L.nint, 256, 65536), 256, L.c->m, L.P_H, (const double*)L.AP_f);
     FL(k_fas_zero, fgrid(L.ncell, 256, 65536), 256, L.AP_f, L.ncell);
     bool c1 = false;
-    RET_IF_ST(fas_smooth(ctx, L, L.P_H, L.AP_f, dt, tol, v1, lvl != 1, lvl == 1 ? &c1 : nullptr, iters, &res));
+    double res = 0.0;
+    RET_IF_ST(fas_smooth(ctx, L, L.P_H, L.AP_f, dt, tol, v1, lvl != 1, lvl == 1, &c1, &res));
     if (lvl == 1) {
         *conv = c1;
         *res_out = res;
@@ -740,8 +804,7 @@ static int fas_load(mist_ctx* ctx) {
 int pressure_operator_sl(mist_ctx* ctx, double dt) {
     RET_IF_ST(fas_load(ctx));
     FasLevel& F = ctx->fas->lv[0];
-    double r = 0.0;
-    RET_IF_ST(fas_residual(ctx, F, F.P_h, F.AP_f, false, dt, &r));
+    RET_IF_ST(fas_residual(ctx, F, F.P_h, F.AP_f, false, dt, nullptr));
     MIST_HIP_TRY(hipMemcpyAsync(ctx->f[MIST_FIELD_DIVG], F.AP_f, (size_t)F.ncell * sizeof(double),
                                 hipMemcpyDeviceToDevice, ctx->stream));
     MIST_HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -776,25 +839,27 @@ int pressure_solve_sl(mist_ctx* ctx, double dt, double tol, int64_t* cycles, dou
     FasState* S = ctx->fas;
     FasLevel& F = S->lv[0];
     bool conv = false;
-    int64_t it = 0, its = 0;
+    int64_t it = 0;
     double res = INFINITY;
+    FL(k_fas_ctrl_reset, 1, 1, S->ctrl, 1);
     if (S->nlev == 1) {
-        RET_IF_ST(fas_smooth(ctx, F, F.P_h, F.AP_f, dt, tol, 50000, false, &conv, &its, &res));
-        it = its;
+        RET_IF_ST(fas_smooth(ctx, F, F.P_h, F.AP_f, dt, tol, 50000, false, true, &conv, &res));
+        it = S->h_ctrl->iters;
     } else {
         while (it < ctx->p.mg_max_cycles) {
             ++it;
-            RET_IF_ST(fas_cycle(ctx, 1, dt, tol, &conv, &its, &res));
+            RET_IF_ST(fas_cycle(ctx, 1, dt, tol, &conv, &res));
             if (conv) break;
         }
     }
     // P[interior] = P_h[1][interior]; Neumann!; update_borders!
     FL(k_fas_interior_copy, fgrid(F.nint, 256, 65536), 256, ctx->m, (double*)ctx->f[MIST_FIELD_P], (const double*)F.P_h);
     RET_IF_ST(launch_neumann(ctx, MIST_FIELD_P));
+    MIST_HIP_TRY(hipMemcpyAsync(S->h_ctrl, S->ctrl, sizeof(FasCtrl), hipMemcpyDeviceToHost, ctx->stream));
     MIST_HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (cycles) *cycles = it;
     if (resid) *resid = res;
-    if (smooth_iters) *smooth_iters = its;
+    if (smooth_iters) *smooth_iters = S->h_ctrl->total;
     if (!conv) {
         ctx->err = "P-solve did not converge";
         return MIST_ERR_NOCONV;
