@@ -248,6 +248,14 @@ int mist_ns_step(mist_ctx* ctx, double* t, double clamp_to, double* dt_used, dou
  * first step of a sweep). Branches (cut cases, clamps, ELVIRA's choice, cell searches) are
  * those of the forward evaluation. The first call allocates the work arrays. */
 int mist_transport_vjp(mist_ctx* ctx, double dt);
+/* Deferred overflow checks (no reference counterpart; for loops of many stage calls, such as
+ * the checkpointed reverse sweeps). mist_transport and mist_transport_vjp normally wait for the
+ * device after each call to report an overflow of the cut walk (MIST_ERR_OVERFLOW). With
+ * on != 0 they return without waiting; the overflow is then reported by the next call that
+ * waits for the device anyway (mist_download, mist_reduce, mist_compute_dt, mist_vof_step,
+ * mist_ns_step) and by mist_set_deferred_checks(ctx, 0), which waits and returns it. */
+int mist_set_deferred_checks(mist_ctx* ctx, int on);
+
 /* Step checkpoints: copy the state a step starts from (VF with its ghosts, the band flags
  * and band-cell list, the normals, PLIC constants and ELVIRA's choice; those of them that are
  * stale are computed first) to device slot `slot` (0 <= slot < 4096, allocated on first use,

@@ -161,10 +161,14 @@ class SLVOFRun:
         self.set_velocity(amps)
         c.set("VF", self.vf0)
         c.vof_init()
-        for s in range(self.nsteps):
-            if checkpoints and s % self.every == 0:
-                c.checkpoint_save(s // self.every)
-            self._step()
+        c.set_deferred_checks(True)  # one wait for the whole run (the download below)
+        try:
+            for s in range(self.nsteps):
+                if checkpoints and s % self.every == 0:
+                    c.checkpoint_save(s // self.every)
+                self._step()
+        finally:
+            c.set_deferred_checks(False)
         return c.get("VF")
 
     def backward(self, lam_T):
@@ -174,6 +178,16 @@ class SLVOFRun:
         for name in ("uf_bar", "vf_bar", "wf_bar"):
             c.set(name, self._zero[name])
         base = self.nseg  # slots after the checkpoints hold a segment's later states
+        c.set_deferred_checks(True)  # one wait for the whole sweep (the downloads below)
+        try:
+            self._sweep(base)
+        finally:
+            c.set_deferred_checks(False)
+        guf, gvf = c.get("uf_bar"), c.get("vf_bar")
+        return self._Bu @ guf.ravel() + self._Bv @ gvf.ravel()
+
+    def _sweep(self, base):
+        c = self.ctx
         for seg in range(self.nseg - 1, -1, -1):
             s0 = seg * self.every
             s1 = min(self.nsteps, s0 + self.every)
@@ -188,8 +202,6 @@ class SLVOFRun:
             for s in range(s1 - 1, s0 - 1, -1):
                 c.checkpoint_restore(slot(s))
                 c.transport_vjp(self.dt)
-        guf, gvf = c.get("uf_bar"), c.get("vf_bar")
-        return self._Bu @ guf.ravel() + self._Bv @ gvf.ravel()
 
     def loss_and_grad(self, amps, target):
         """L = 1/2 sum over interior cells (VF - target)^2 and dL/d(amps)."""

@@ -883,6 +883,19 @@ static int adj_neumann(mist_ctx* ctx, double* g, int field) {
     return MIST_OK;
 }
 
+// Resident work-groups of the reverse cut pass (queried once; outside any graph capture).
+int vjp_prepare(mist_ctx* ctx) {
+    if (!ctx->tree_blocks_adj) {
+        int per = 0, dev = 0, cus = 0;
+        MIST_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_adj_tree<TreeMain>, TreeMain::NT, 0));
+        MIST_HIP_TRY(hipGetDevice(&dev));
+        MIST_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        ctx->tree_blocks_adj = std::max(1, per) * std::max(1, cus);
+        if (!ctx->tree_cus) ctx->tree_cus = std::max(1, cus);
+    }
+    return MIST_OK;
+}
+
 int launch_transport_vjp(mist_ctx* ctx, double dt) {
     int st;
     const mist::DevMesh& m = ctx->m;
@@ -912,14 +925,7 @@ int launch_transport_vjp(mist_ctx* ctx, double dt) {
     // have a zero adjoint and stay out of the cut list; one work-group per cell, the wider pass
     // for trees that outgrow the first, the general walk for the rest
     if ((st = launch_cut_list(ctx))) return st;
-    if (!ctx->tree_blocks_adj) {
-        int per = 0, dev = 0, cus = 0;
-        MIST_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_adj_tree<TreeMain>, TreeMain::NT, 0));
-        MIST_HIP_TRY(hipGetDevice(&dev));
-        MIST_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        ctx->tree_blocks_adj = std::max(1, per) * std::max(1, cus);
-        if (!ctx->tree_cus) ctx->tree_cus = std::max(1, cus);
-    }
+    if ((st = vjp_prepare(ctx))) return st;
     const CutCell* cut = (const CutCell*)ctx->cut_list;
     ADJL(k_adj_tree<TreeMain>, ctx->tree_blocks_adj, TreeMain::NT, m, (const double*)ctx->pv, cut,
          (const int*)nullptr, (const int*)(ctx->counters + 8), nx, ny, nz, D, (const double*)lam, ctx->adj_pv, gpl[0],

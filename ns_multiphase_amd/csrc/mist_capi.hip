@@ -188,6 +188,8 @@ int mist_ctx_destroy(mist_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     mist::mg_free(ctx->mg);
     mist::fas_free(ctx->fas);
+    if (ctx->vjp_exec) (void)hipGraphExecDestroy(ctx->vjp_exec);
+    if (ctx->vjp_graph) (void)hipGraphDestroy(ctx->vjp_graph);
     for (int f = 0; f < MIST_FIELD_COUNT; ++f)
         if (ctx->f[f]) (void)hipFree(ctx->f[f]);
     void* bufs[] = {ctx->geom, ctx->trig, ctx->pv, ctx->band_list, ctx->iface_list, ctx->defer_list, ctx->cut_list,
@@ -362,6 +364,8 @@ int mist_upload(mist_ctx* ctx, int field, const void* host, size_t nbytes) {
     return MIST_OK;
 }
 
+static int check_overflow(mist_ctx* ctx, bool force = false);
+
 int mist_download(mist_ctx* ctx, int field, void* host, size_t nbytes) {
     CHECK_CTX();
     if (field < 0 || field >= MIST_FIELD_COUNT || !host) return fail(ctx, MIST_ERR_PARAM, "bad field");
@@ -369,6 +373,16 @@ int mist_download(mist_ctx* ctx, int field, void* host, size_t nbytes) {
     MIST_HIP_TRY(hipSetDevice(ctx->device));
     MIST_HIP_TRY(hipMemcpyAsync(host, ctx->f[field], nbytes, hipMemcpyDeviceToHost, ctx->stream));
     MIST_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->defer_checks) RET_IF(check_overflow(ctx, true));
+    return MIST_OK;
+}
+
+int mist_set_deferred_checks(mist_ctx* ctx, int on) {
+    CHECK_CTX();
+    MIST_HIP_TRY(hipSetDevice(ctx->device));
+    const bool was = ctx->defer_checks;
+    ctx->defer_checks = on != 0;
+    if (was && !on) RET_IF(check_overflow(ctx, true));  // what the deferred calls left
     return MIST_OK;
 }
 
@@ -441,6 +455,7 @@ int mist_compute_dt(mist_ctx* ctx, double* dt) {
     if (!dt) return fail(ctx, MIST_ERR_PARAM, "null dt");
     double mx[3];
     RET_IF(mist::launch_maxabs_uvw(ctx, mx));
+    if (ctx->defer_checks) RET_IF(check_overflow(ctx, true));
     *dt = dt_from_max(ctx, mx);
     return MIST_OK;
 }
@@ -472,7 +487,10 @@ int mist_compute_plic(mist_ctx* ctx) {
     return MIST_OK;
 }
 
-static int check_overflow(mist_ctx* ctx) {
+// The cut walk's overflow flag (sticky) read back on the host; with deferred checks, only where
+// the call waits for the device anyway (`force`).
+static int check_overflow(mist_ctx* ctx, bool force) {
+    if (ctx->defer_checks && !force) return MIST_OK;
     MIST_HIP_TRY(hipMemcpyAsync(ctx->h_counters, ctx->counters, 4 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     MIST_HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (ctx->h_counters[2]) {
@@ -546,7 +564,7 @@ int mist_vof_step(mist_ctx* ctx, double* t, double clamp_to, double* dt_used) {
     MIST_HIP_TRY(hipSetDevice(ctx->device));
     double mx[3];
     RET_IF(mist::launch_maxabs_uvw(ctx, mx));  // synchronises: the previous step is complete
-    RET_IF(check_overflow(ctx));
+    RET_IF(check_overflow(ctx, true));
     double dt = dt_from_max(ctx, mx);
     if (clamp_to > 0.0 && *t + dt > clamp_to) dt = clamp_to - *t;
     *t += dt;
@@ -700,7 +718,7 @@ int mist_ns_step(mist_ctx* ctx, double* t, double clamp_to, double* dt_used, dou
     MIST_HIP_TRY(hipSetDevice(ctx->device));
     double mx[3];
     RET_IF(mist::launch_maxabs_uvw(ctx, mx));  // synchronises: the previous step is complete
-    RET_IF(check_overflow(ctx));
+    RET_IF(check_overflow(ctx, true));
     double dt = dt_from_max(ctx, mx);
     if (clamp_to > 0.0 && *t + dt > clamp_to) dt = clamp_to - *t;
     *t += dt;
@@ -793,6 +811,7 @@ int mist_reduce(mist_ctx* ctx, int kind, int field, double* out) {
         field == MIST_FIELD_MASK || face_field(field))
         return fail(ctx, MIST_ERR_PARAM, "bad reduction");
     RET_IF(mist::launch_reduce(ctx, kind, field, out));
+    if (ctx->defer_checks) RET_IF(check_overflow(ctx, true));
     return MIST_OK;
 }
 
@@ -900,7 +919,34 @@ int mist_transport_vjp(mist_ctx* ctx, double dt) {
     RET_IF(mist_compute_normal(ctx));
     if (ctx->plic_version != ctx->vf_version) RET_IF(mist_compute_plic(ctx));
     if (!ctx->band_list_valid) RET_IF(mist_compute_band(ctx));
-    RET_IF(mist::launch_transport_vjp(ctx, dt));
+    if (ctx->prof.on) {
+        RET_IF(mist::launch_transport_vjp(ctx, dt));  // profiled launches are timed one by one
+    } else {
+        // the ~25 launches of the reverse step replayed as one graph, captured for this dt: the
+        // launches read the context's arrays through fixed pointers
+        RET_IF(mist::vjp_prepare(ctx));
+        if (ctx->vjp_exec && ctx->vjp_dt != dt) {
+            (void)hipGraphExecDestroy(ctx->vjp_exec);
+            (void)hipGraphDestroy(ctx->vjp_graph);
+            ctx->vjp_exec = nullptr;
+            ctx->vjp_graph = nullptr;
+        }
+        if (!ctx->vjp_exec) {
+            MIST_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            const int st = mist::launch_transport_vjp(ctx, dt);
+            hipGraph_t g = nullptr;
+            const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+            if (st) {
+                if (g) (void)hipGraphDestroy(g);
+                return st;
+            }
+            MIST_HIP_TRY(ec);
+            ctx->vjp_graph = g;
+            MIST_HIP_TRY(hipGraphInstantiate(&ctx->vjp_exec, g, nullptr, nullptr, 0));
+            ctx->vjp_dt = dt;
+        }
+        MIST_HIP_TRY(hipGraphLaunch(ctx->vjp_exec, ctx->stream));
+    }
     RET_IF(check_overflow(ctx));
     return MIST_OK;
 }

@@ -75,6 +75,7 @@ struct mist_ctx {
     double* ns_sums = nullptr;   // 6 sums per interior cell of the momentum predictor's cut
     double* h_red = nullptr;     // pinned
     uint64_t vf_version = 1, normals_version = 0, band_version = 0;
+    bool defer_checks = false;   // transport / VJP calls skip their host round trip (mist_set_deferred_checks)
     bool band_list_valid = false;
     mist::MGState* mg = nullptr;
     mist::FasState* fas = nullptr;  // SL pressure: multigrid levels (child contexts) and Anderson state
@@ -90,6 +91,9 @@ struct mist_ctx {
     double* adj_pv = nullptr;    // adjoint work: back-traced corners (3 per grid vertex)
     double* adj_sums = nullptr;  // adjoint work: 6 cut sums per interior cell (vol, vLiq used)
     uint64_t plic_version = 0;   // VF version the PLIC constants were computed for
+    hipGraph_t vjp_graph = nullptr;     // the reverse step's launches, captured once per dt
+    hipGraphExec_t vjp_exec = nullptr;
+    double vjp_dt = 0.0;
     std::vector<double*> ckpt;   // VF checkpoints (device)
     std::string err;
 };
@@ -131,6 +135,7 @@ int launch_interp_center(mist_ctx* ctx);
 int launch_divergence(mist_ctx* ctx, double dt);
 // mist_adjoint.hip
 int launch_transport_vjp(mist_ctx* ctx, double dt);
+int vjp_prepare(mist_ctx* ctx);  // what the reverse step's first call sets up on the host
 // mist_mg.hip
 int pressure_solve(mist_ctx* ctx, double dt, double tol, int precision, int64_t* cycles, double* resid);
 // mist_fas.hip

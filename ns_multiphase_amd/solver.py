@@ -82,6 +82,7 @@ def lib():
         "mist_ns_step": (i32, [vp, dp, d, dp, d, i64, ip, dp]),
         "mist_transport_vjp": (i32, [vp, d]),
         "mist_checkpoint_save": (i32, [vp, i32]),
+        "mist_set_deferred_checks": (i32, [vp, i32]),
         "mist_checkpoint_restore": (i32, [vp, i32]),
         "mist_elvira_choice": (i32, [vp, ctypes.POINTER(ctypes.c_int32), ctypes.c_size_t]),
         "mist_pressure_operator": (i32, [vp, d]),
@@ -328,6 +329,12 @@ class Context:
 
     def checkpoint_save(self, slot):
         self._check(lib().mist_checkpoint_save(self.h, slot))
+
+    def set_deferred_checks(self, on=True):
+        """Transport and VJP calls stop waiting for the device to report a cut-walk overflow;
+        the next call that waits (a download, a reduction, dt) reports it, and so does turning
+        the deferral off."""
+        self._check(lib().mist_set_deferred_checks(self.h, 1 if on else 0))
 
     def checkpoint_restore(self, slot):
         self._check(lib().mist_checkpoint_restore(self.h, slot))

@@ -223,3 +223,33 @@ def test_vjp_after_upload_uses_the_new_band(mist):
     for n in ("VF_bar", "uf_bar", "vf_bar", "wf_bar"):
         x, y = a.get(n), b.get(n)
         assert np.abs(x - y).max() <= 1e-12 * max(1.0, np.abs(y).max()), n
+
+
+def test_deferred_checks_change_no_result(mist):
+    """mist_set_deferred_checks: transport and VJP calls that skip their per-call wait give the
+    same fields as calls that wait (the forward bit for bit; the adjoints to the rounding of their
+    atomic sums, whose order of arrival varies from run to run), and turning the deferral off
+    reports no overflow when there was none."""
+    p, ctx, run, amps = _setup(mist, 64, 6, 3)
+    run.set_velocity(amps)
+    out = {}
+    for defer in (False, True):
+        ctx.set("VF", run.vf0)
+        ctx.vof_init()
+        ctx.set_deferred_checks(defer)
+        ctx.checkpoint_save(0)
+        for _ in range(6):
+            run._step()
+        vf = ctx.get("VF")
+        lam = np.zeros_like(vf)
+        _interior(ctx.mesh, lam)[...] = 1.0
+        ctx.set("VF_bar", lam)
+        for name in ("uf_bar", "vf_bar", "wf_bar"):
+            ctx.set(name, np.zeros(tuple(int(v) for v in ctx.extents(name)[1])))
+        ctx.checkpoint_restore(0)
+        ctx.transport_vjp(run.dt)
+        ctx.set_deferred_checks(False)
+        out[defer] = (vf, ctx.get("VF_bar"), ctx.get("uf_bar"), ctx.get("vf_bar"))
+    assert np.array_equal(out[False][0], out[True][0])
+    for a, b in zip(out[False][1:], out[True][1:]):
+        assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max()
