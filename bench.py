@@ -534,6 +534,53 @@ def run_config1(args, torch):
     return out
 
 
+def forward_mode_check(args, N=64, nsteps=64, every=8):
+    """The device's gradient (checkpoints every 8) against the oracle's forward mode (dual
+    numbers through the same steps, oracle/oracle_fwd.cpp) at 64^2, 64 steps, the same 16 modes
+    and seeds as the adjoint workload: both are exact derivatives of the map with its branches
+    held, so no step size enters."""
+    import numpy as np
+    from ns_multiphase_amd import adjoint as A
+    from ns_multiphase_amd import solver
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    pyoracle.set_threads(cpu_threads())
+    p = A.adjoint_params(N)
+    ctx = solver.Context(p)
+    ctx.init_vof("circle", 0.2, 0.5, 0.5)
+    vf0 = ctx.get("VF")
+    basis = A.mode_faces(ctx.mesh)
+    a1 = np.random.default_rng(1).normal(0.0, 0.1, 16)
+    a2 = np.random.default_rng(2).normal(0.0, 0.1, 16)
+    dt = min(A.stable_dt(ctx.mesh, basis, a1), A.stable_dt(ctx.mesh, basis, a2))
+    run = A.SLVOFRun(ctx, vf0, basis, dt, nsteps, every)
+    target = run.forward(a2, checkpoints=False)
+    L, g = run.loss_and_grad(a1, target)
+    o = pyoracle.OracleContext(p)
+    o.set("VF", vf0)
+    Bu = np.stack([b[0] for b in basis])
+    Bv = np.stack([b[1] for b in basis])
+    uf = (a1 @ run._Bu).reshape(run._shape)
+    vf = (a1 @ run._Bv).reshape(run._shape)
+    t0 = time.perf_counter()
+    VF, dVF = o.fwd_vof_run(uf, vf, np.zeros_like(uf), Bu, Bv, nsteps, dt)
+    sec = time.perf_counter() - t0
+    m = ctx.mesh
+    oo = m.nghost + 3
+    sl = (slice(oo, oo + m.Nx), slice(oo, oo + m.Ny), slice(oo, oo + m.Nz))
+    diff = VF[sl] - target[sl]
+    gf = np.array([float((diff * dVF[q][sl]).sum()) for q in range(16)])
+    rel = np.abs(g - gf) / np.maximum(np.abs(gf), 1e-3 * np.abs(gf).max())
+    ctx.close()
+    o.close()
+    out = {"workload": f"{N}^2 x 1, {nsteps} steps, checkpoints every {every}, same modes and seeds",
+           "within_1e-6": int((rel <= 1e-6).sum()), "directions": 16, "max_rel": float(rel.max()),
+           "loss_rel_diff": abs(0.5 * float((diff ** 2).sum()) - L) / L, "oracle_seconds": sec,
+           "oracle_cores": cpu_threads()}
+    log(f"[bench] adjoint vs forward mode at {N}^2: {out['within_1e-6']}/16 within 1e-6, max rel {rel.max():.2e}")
+    return out
+
+
 def run_adjoint(args, torch):
     """Gradient of the SL-VOF loss with respect to 16 stream-function amplitudes: 256^2, 64
     steps, the checkpoint interval chosen by an 8 GiB device-memory budget (every state at this
@@ -590,6 +637,7 @@ def run_adjoint(args, torch):
         run.backward(diff)
         torch.cuda.synchronize()
         t_bwd += (time.perf_counter() - t0) / reps
+    fwd = None if args.no_oracle else forward_mode_check(args)
     # central differences at h = 1e-6 and 1e-7: the map is piecewise smooth, and a kink (an
     # ELVIRA choice or a cut case switching) within h of the point makes a direction disagree
     checks = []
@@ -608,7 +656,7 @@ def run_adjoint(args, torch):
            "reverse_sweep_vs_forward": t_bwd / t_fwd,
            "loss_and_gradient_ms_every8": 1e3 * t_grad8, "gradient_cost_vs_forward_every8": t_grad8 / t_fwd,
            "every8_vs_every_max_rel": float(np.abs(g8 - g).max() / np.abs(g).max()),
-           "fd_check": checks[0], "fd_check_h1e-7": checks[1]}
+           "fd_check": checks[0], "fd_check_h1e-7": checks[1], "forward_mode_check": fwd}
     log(f"[bench] adjoint: checkpoints every {every} ({out['checkpoint_MB']:.0f} MB), forward {out['forward_ms']:.1f} ms, "
         f"loss+gradient {out['loss_and_gradient_ms']:.1f} ms "
         f"({out['gradient_cost_vs_forward']:.2f}x; {out['loss_and_gradient_ms_every8']:.1f} ms = "

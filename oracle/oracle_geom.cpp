@@ -4,6 +4,7 @@
 #include <cstring>
 
 #include "oracle_core.h"
+#include "oracle_tables.h"
 
 namespace ora {
 
@@ -56,46 +57,6 @@ Ext Mesh::face_ext(int dir) const {
     return e;
 }
 
-// ------------------------------------------------------------------------------------
-// Tet-cut look-up tables (src/VFgeom.jl:54-165). Case index c = 0..15 is the reference's
-// case-1; vertex numbers stay 1-based as in the reference (1-4 tet verts, 5-8 cut verts).
-// ------------------------------------------------------------------------------------
-static const int CUT_NVERT[16] = {0, 3, 3, 4, 3, 4, 4, 3, 3, 4, 4, 3, 4, 3, 3, 0};
-static const int CUT_NTETS[16] = {1, 4, 4, 6, 4, 6, 6, 4, 4, 6, 6, 4, 6, 4, 4, 1};
-static const int CUT_NNTET[16] = {1, 2, 2, 4, 2, 4, 4, 4, 2, 4, 4, 4, 4, 4, 4, 2};
-static const int CUT_V1[16][4] = {
-    {-1, -1, -1, -1}, {1, 1, 1, -1}, {2, 2, 2, -1}, {1, 2, 1, 2}, {3, 3, 3, -1}, {1, 3, 1, 3},
-    {2, 3, 2, 3},     {4, 4, 4, -1}, {4, 4, 4, -1}, {1, 4, 1, 4}, {2, 4, 2, 4}, {3, 3, 3, -1},
-    {3, 4, 3, 4},     {2, 2, 2, -1}, {1, 1, 1, -1}, {-1, -1, -1, -1}};
-static const int CUT_V2[16][4] = {
-    {-1, -1, -1, -1}, {2, 3, 4, -1}, {3, 4, 1, -1}, {4, 4, 3, 3}, {4, 1, 2, -1}, {4, 4, 2, 2},
-    {4, 4, 1, 1},     {1, 2, 3, -1}, {1, 2, 3, -1}, {3, 3, 2, 2}, {3, 3, 1, 1}, {4, 1, 2, -1},
-    {2, 2, 1, 1},     {3, 4, 1, -1}, {2, 3, 4, -1}, {-1, -1, -1, -1}};
-static const int CUT_VTET[16][6][4] = {
-    {{1, 2, 3, 4}, {-1, -1, -1, -1}, {-1, -1, -1, -1}, {-1, -1, -1, -1}, {-1, -1, -1, -1}, {-1, -1, -1, -1}},
-    {{5, 7, 6, 1}, {6, 2, 3, 4}, {4, 2, 5, 6}, {5, 6, 7, 4}, {-1, -1, -1, -1}, {-1, -1, -1, -1}},
-    {{7, 5, 6, 2}, {1, 3, 4, 6}, {1, 5, 3, 6}, {5, 7, 6, 1}, {-1, -1, -1, -1}, {-1, -1, -1, -1}},
-    {{5, 8, 6, 2}, {5, 7, 8, 1}, {5, 1, 8, 2}, {5, 6, 8, 4}, {5, 8, 7, 3}, {5, 8, 3, 4}},
-    {{6, 5, 7, 3}, {2, 1, 4, 6}, {6, 5, 4, 2}, {6, 7, 5, 2}, {-1, -1, -1, -1}, {-1, -1, -1, -1}},
-    {{5, 6, 8, 3}, {5, 8, 7, 1}, {5, 8, 1, 3}, {5, 8, 6, 4}, {5, 7, 8, 2}, {5, 8, 4, 2}},
-    {{8, 6, 5, 3}, {5, 7, 8, 2}, {8, 5, 2, 3}, {8, 5, 6, 4}, {5, 8, 7, 1}, {5, 8, 1, 4}},
-    {{1, 2, 3, 7}, {1, 2, 7, 6}, {5, 7, 6, 1}, {5, 6, 7, 4}, {-1, -1, -1, -1}, {-1, -1, -1, -1}},
-    {{5, 6, 7, 4}, {1, 2, 3, 6}, {5, 1, 3, 6}, {5, 7, 6, 3}, {-1, -1, -1, -1}, {-1, -1, -1, -1}},
-    {{5, 8, 6, 4}, {5, 7, 8, 1}, {5, 8, 4, 1}, {5, 6, 8, 3}, {5, 8, 7, 2}, {5, 8, 2, 3}},
-    {{8, 5, 6, 4}, {5, 8, 7, 2}, {8, 2, 5, 4}, {8, 6, 5, 3}, {5, 7, 8, 1}, {5, 8, 3, 1}},
-    {{1, 4, 2, 7}, {4, 1, 6, 7}, {6, 7, 5, 4}, {6, 5, 7, 3}, {-1, -1, -1, -1}, {-1, -1, -1, -1}},
-    {{8, 6, 5, 4}, {5, 7, 8, 3}, {8, 4, 5, 3}, {8, 5, 6, 2}, {5, 8, 7, 1}, {5, 8, 1, 2}},
-    {{3, 4, 1, 7}, {7, 6, 3, 4}, {7, 6, 5, 3}, {7, 5, 6, 2}, {-1, -1, -1, -1}, {-1, -1, -1, -1}},
-    {{7, 4, 2, 3}, {2, 3, 6, 7}, {5, 6, 7, 2}, {5, 7, 6, 1}, {-1, -1, -1, -1}, {-1, -1, -1, -1}},
-    {{1, 2, 3, 4}, {-1, -1, -1, -1}, {-1, -1, -1, -1}, {-1, -1, -1, -1}, {-1, -1, -1, -1}, {-1, -1, -1, -1}}};
-static const int CUT_SIDE[16][6] = {
-    {1, -1, -1, -1, -1, -1}, {2, 1, 1, 1, -1, -1}, {2, 1, 1, 1, -1, -1}, {2, 2, 2, 1, 1, 1},
-    {2, 1, 1, 1, -1, -1},    {2, 2, 2, 1, 1, 1},   {2, 2, 2, 1, 1, 1},   {2, 2, 2, 1, -1, -1},
-    {2, 1, 1, 1, -1, -1},    {2, 2, 2, 1, 1, 1},   {2, 2, 2, 1, 1, 1},   {2, 2, 2, 1, -1, -1},
-    {2, 2, 2, 1, 1, 1},      {2, 2, 2, 1, -1, -1}, {2, 2, 2, 1, -1, -1}, {2, -1, -1, -1, -1, -1}};
-
-// 5-tet decomposition of an 8-vertex hexahedron (src/VFgeom.jl:235-256), 1-based.
-static const int TET5[5][4] = {{1, 2, 4, 6}, {1, 4, 3, 7}, {1, 5, 6, 7}, {4, 7, 6, 8}, {1, 4, 7, 6}};
 
 // tet_vol (src/VFgeom.jl:669-681)
 double tet_vol(const double v[4][3]) {

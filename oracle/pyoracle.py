@@ -85,6 +85,7 @@ def lib():
             "ora_surface_force": (i32, [vp]),
             "ora_ns_init": (i32, [vp]),
             "ora_restart_init": (i32, [vp, dp]),
+            "ora_fwd_vof_run": (i32, [vp, dp, dp, dp, dp, dp, i32, i32, d, dp, dp]),
             "ora_ns_step": (i32, [vp, dp, d, dp, d, i64, ip, dp]),
             "ora_sl_flux_vol": (d, [vp, i32, i64, i64, i64, d, dp]),
             "ora_correction_tets": (i32, [vp, i64, i64, i64, d, dp]),
@@ -305,6 +306,26 @@ class OracleContext:
 
     def ns_init(self):
         self._check(lib().ora_ns_init(self.h))
+
+    def fwd_vof_run(self, uf, vf, wf, Bu, Bv, nsteps, dt):
+        """Forward-mode run (dual numbers) of `nsteps` VOF steps from this context's VF: faces
+        uf, vf, wf (cell extents, indexed [i, j, k]) with derivatives Bu[q], Bv[q] (q < 16).
+        Returns (VF, dVF[q]) over the VF extents, indexed [i, j, k]; the context keeps the values
+        of VF, band and D."""
+        nm = len(Bu)
+        lo, n, _ = self.extents("VF")
+        out = np.zeros(tuple(n[::-1]))
+        dout = np.zeros((nm,) + tuple(n[::-1]))
+
+        def faces(a):  # [i, j, k] -> memory with i fastest
+            return np.ascontiguousarray(np.asarray(a, np.float64).transpose(2, 1, 0))
+
+        bu = np.ascontiguousarray(np.stack([faces(b) for b in Bu]))
+        bv = np.ascontiguousarray(np.stack([faces(b) for b in Bv]))
+        st = lib().ora_fwd_vof_run(self.h, _dp(faces(uf)), _dp(faces(vf)), _dp(faces(wf)), _dp(bu), _dp(bv), nm,
+                                   int(nsteps), float(dt), _dp(out), _dp(dout))
+        self._check(st)
+        return out.transpose(2, 1, 0), dout.transpose(0, 3, 2, 1)
 
     def restart_init(self):
         dt = ctypes.c_double()

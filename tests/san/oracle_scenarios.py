@@ -99,6 +99,22 @@ def helpers():
     pyoracle.anderson_coefficients(g @ g.T + np.eye(4))
 
 
+def forward_mode():
+    step("forward mode 16^2, 2 steps")
+    from ns_multiphase_amd import adjoint as A
+    p = A.adjoint_params(16)
+    o = pyoracle.OracleContext(p)
+    o.init_vof("circle", 0.2, 0.47, 0.52)
+    basis = A.mode_faces(o.mesh)
+    Bu = np.stack([b[0] for b in basis])
+    Bv = np.stack([b[1] for b in basis])
+    a = np.random.default_rng(1).normal(0.0, 0.1, 16)
+    uf = (a @ Bu.reshape(16, -1)).reshape(Bu.shape[1:])
+    vf = (a @ Bv.reshape(16, -1)).reshape(Bv.shape[1:])
+    o.fwd_vof_run(uf, vf, np.zeros_like(uf), Bu, Bv, 2, A.stable_dt(o.mesh, basis, a))
+    o.close()
+
+
 if __name__ == "__main__":
     step(f"oracle library: {pyoracle.LIB_PATH}")
     pyoracle.set_threads(2)
@@ -106,4 +122,5 @@ if __name__ == "__main__":
     vof_cases()
     mg_cases()
     ns_cases()
+    forward_mode()
     step("all scenarios done")
