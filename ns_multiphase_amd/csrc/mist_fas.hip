@@ -333,62 +333,91 @@ __global__ __launch_bounds__(FAS_T) void k_fas_step(DevMesh m, const double* P, 
 
 // The Gram row of the new slot from the partials (fixed order), then, when two or more slots
 // are filled, the KKT system [G + eps I, 1; 1', 0] a = [0; 1] with eps = 1e-10 max diag G
-// (par_anderson_accel, src/Pressure.jl:747-775) by LU with partial pivoting on one thread.
+// (par_anderson_accel, src/Pressure.jl:747-775) by LU with partial pivoting (the first largest
+// pivot): the system lives in LDS, each elimination step updates its rows in parallel (every
+// entry is computed by one thread, as a one-thread elimination computes it), and one thread
+// substitutes back.
 __global__ __launch_bounds__(FAS_T) void k_fas_gram_solve(const double* part, int nblocks, int slot, int n,
                                                           double* small, const FasCtrl* ctl) {
     if (ctl->done) return;
+    constexpr int NM = FAS_M + 1;
     __shared__ double row[FAS_M];
+    __shared__ double K[NM][NM], bv[NM], fr[NM];
+    __shared__ int piv;
     double* gram = small;
     double* alpha = small + FAS_M * FAS_M;
-    if ((int)threadIdx.x < n) {
+    const int tid = threadIdx.x;
+    if (tid < n) {
         double sum = 0.0;
-        for (int b = 0; b < nblocks; ++b) sum += part[(int64_t)b * FAS_M + threadIdx.x];
-        row[threadIdx.x] = sum;
+        for (int b = 0; b < nblocks; ++b) sum += part[(int64_t)b * FAS_M + tid];
+        row[tid] = sum;
     }
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    for (int t = 0; t < n; ++t) {
-        gram[slot * FAS_M + t] = row[t];
-        gram[t * FAS_M + slot] = row[t];
+    if (tid < n) {
+        gram[slot * FAS_M + tid] = row[tid];
+        gram[tid * FAS_M + slot] = row[tid];
     }
     if (n < 2) return;
-    constexpr int NM = FAS_M + 1;
-    double K[NM][NM], b[NM];
-    double mx = gram[0];
-    for (int i = 1; i < n; ++i) mx = jmax(mx, gram[i * FAS_M + i]);
-    const double eps = 1e-10 * mx;
+    __syncthreads();  // the new row and column are in `gram` for the whole block
     const int N = n + 1;
-    for (int i = 0; i < N; ++i) {
-        b[i] = i == n ? 1.0 : 0.0;
-        for (int j = 0; j < N; ++j) {
-            double v;
-            if (i < n && j < n) v = gram[i * FAS_M + j] + (i == j ? eps : 0.0);
-            else if (i == n && j == n) v = 0.0;
-            else v = 1.0;
-            K[i][j] = v;
-        }
+    if (tid == 0) {
+        double mx = gram[0];
+        for (int i = 1; i < n; ++i) mx = jmax(mx, gram[i * FAS_M + i]);
+        fr[0] = 1e-10 * mx;  // eps
     }
+    __syncthreads();
+    const double eps = fr[0];
+    __syncthreads();
+    if (tid < N * N) {
+        const int i = tid / N, j = tid - N * (tid / N);
+        double v;
+        if (i < n && j < n) v = gram[i * FAS_M + j] + (i == j ? eps : 0.0);
+        else if (i == n && j == n) v = 0.0;
+        else v = 1.0;
+        K[i][j] = v;
+        if (j == 0) bv[i] = i == n ? 1.0 : 0.0;
+    }
+    __syncthreads();
     for (int col = 0; col < N; ++col) {
-        int p = col;
-        for (int r = col + 1; r < N; ++r)
-            if (fabs(K[r][col]) > fabs(K[p][col])) p = r;
+        if (tid == 0) {
+            int p = col;
+            for (int r = col + 1; r < N; ++r)
+                if (fabs(K[r][col]) > fabs(K[p][col])) p = r;
+            piv = p;
+        }
+        __syncthreads();
+        const int p = piv;
         if (p != col) {
-            for (int q = 0; q < N; ++q) { const double t = K[col][q]; K[col][q] = K[p][q]; K[p][q] = t; }
-            const double t = b[col]; b[col] = b[p]; b[p] = t;
+            if (tid < N) {
+                const double t = K[col][tid]; K[col][tid] = K[p][tid]; K[p][tid] = t;
+            }
+            if (tid == 0) {
+                const double t = bv[col]; bv[col] = bv[p]; bv[p] = t;
+            }
+            __syncthreads();
         }
-        for (int r = col + 1; r < N; ++r) {
-            const double f = K[r][col] / K[col][col];
-            K[r][col] = f;
-            for (int q = col + 1; q < N; ++q) K[r][q] -= f * K[col][q];
-            b[r] -= f * b[col];
+        if (tid > col && tid < N) fr[tid] = K[tid][col] / K[col][col];
+        __syncthreads();
+        const int w = N - col - 1;  // rows and columns below and right of the pivot
+        if (tid < w * w) {
+            const int r = col + 1 + tid / w, q = col + 1 + tid % w;
+            K[r][q] -= fr[r] * K[col][q];
         }
+        if (tid < w) {
+            const int r = col + 1 + tid;
+            K[r][col] = fr[r];
+            bv[r] -= fr[r] * bv[col];
+        }
+        __syncthreads();
     }
-    for (int r = N - 1; r >= 0; --r) {
-        double s = b[r];
-        for (int q = r + 1; q < N; ++q) s -= K[r][q] * b[q];
-        b[r] = s / K[r][r];
+    if (tid == 0) {
+        for (int r = N - 1; r >= 0; --r) {
+            double s = bv[r];
+            for (int q = r + 1; q < N; ++q) s -= K[r][q] * bv[q];
+            bv[r] = s / K[r][r];
+        }
+        for (int i = 0; i < n; ++i) alpha[i] = bv[i];
     }
-    for (int i = 0; i < n; ++i) alpha[i] = b[i];
 }
 
 // P = beta (sum_s alpha_s G_s) + (1 - beta) P with two or more slots (weighted_sum starts from
@@ -662,7 +691,7 @@ static int fas_residual(mist_ctx* ctx, FasLevel& L, double* P, double* AP, bool 
 static int fas_jacobian(mist_ctx* ctx, FasLevel& L, double* P, double dt) {
     mist_ctx* c = L.c;
     RET_IF_ST(fas_faces(ctx, L, P, dt));
-    FL(k_fas_jac_local, fgrid(L.nint, 256, 65536), 256, c->m, c->dp, (int)c->p.nband,
+    FL(k_fas_jac_local, fgrid(L.nint, 64, 65536), 64, c->m, c->dp, (int)c->p.nband,
        (const int16_t*)c->f[MIST_FIELD_BAND], dt, (const double*)L.uf0, (const double*)L.vf0, (const double*)L.wf0,
        (const double*)c->f[MIST_FIELD_DENX], (const double*)c->f[MIST_FIELD_DENY],
        (const double*)c->f[MIST_FIELD_DENZ], (const double*)P, (const double*)c->f[MIST_FIELD_UF],
