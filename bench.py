@@ -407,17 +407,56 @@ def run_bubble_sl(args, torch):
     ctx.ns_init()
     t = 0.0
     rows = []
-    for _ in range(args.sl_steps):
+    o = None
+    if not args.no_oracle and not args.no_cpu_baseline:  # the oracle's first step from the same state
+        import numpy as np
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        from ns_multiphase_amd.params import FIELD
+        pyoracle.set_threads(cpu_threads())
+        o = pyoracle.OracleContext(param)
+        for name in FIELD:
+            if not name.endswith("_bar"):
+                o.set(name, ctx.get(name))
+        h0 = ctx.elvira_choice()
+    for step in range(args.sl_steps):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        t, dt, cyc, res = ctx.ns_step(t, param.tol)
+        t1, dt, cyc, res = ctx.ns_step(t, param.tol)
         torch.cuda.synchronize()
         rows.append({"ms": 1e3 * (time.perf_counter() - t0), "fas_cycles": cyc, "final_resid": res})
         log(f"[bench] SL bubble {N}^3: {rows[-1]['ms']:.0f} ms, {cyc} FAS cycles, resid {res:.2e}")
+        if step == 0 and o is not None:
+            o.push_hints("elvira", h0)
+            o.push_hints("elvira", ctx.elvira_choice())
+            o.push_hints("curvature", _directions(ctx))
+            c0 = time.perf_counter()
+            _, _, ocyc, ores = o.ns_step(t, param.tol)
+            osec = time.perf_counter() - c0
+            m = ctx.mesh
+            n = m.nghost
+            cin = (slice(n, n + m.Nx), slice(n, n + m.Ny), slice(n, n + m.Nz))
+            Pg, Po = ctx.get("P")[cin], o.get("P")[cin]
+            cmp = {"oracle_ms": 1e3 * osec, "oracle_cores": cpu_threads(), "oracle_fas_cycles": ocyc,
+                   "max_abs_dVF": float(np.abs(ctx.get("VF") - o.get("VF")).max()),
+                   "band_equal": bool(np.array_equal(ctx.get("band"), o.get("band"))),
+                   "max_abs_dP": float(np.abs((Pg - Pg.mean()) - (Po - Po.mean())).max()),
+                   "max_abs_P": float(np.abs(Po - Po.mean()).max()),
+                   "max_rel_du": max(float(np.abs(ctx.get(k) - o.get(k)).max() / max(np.abs(o.get(k)).max(), 1e-300))
+                                     for k in ("u", "v", "w")),
+                   "elvira_hints_refused": o.tie_stats("elvira")["refused"]}
+            log(f"[bench] SL bubble step 0 vs oracle ({cmp['oracle_ms']:.0f} ms on {cmp['oracle_cores']} cores): "
+                f"max|dVF| {cmp['max_abs_dVF']:.2e}, max|dP| {cmp['max_abs_dP']:.2e} of {cmp['max_abs_P']:.2e}")
+        t = t1
     ctx.close()
-    return {"workload": f"rising bubble {N}^3 as above, semi-Lagrangian scheme, SL pressure by FAS "
-                        f"(2 levels, res_iteration smoother) to 1e-10", "steps": rows,
-            "ms_per_step": sum(r["ms"] for r in rows) / len(rows)}
+    out = {"workload": f"rising bubble {N}^3 as above, semi-Lagrangian scheme, SL pressure by FAS "
+                       f"(2 levels, res_iteration smoother) to 1e-10", "steps": rows,
+           "ms_per_step": sum(r["ms"] for r in rows) / len(rows)}
+    if o is not None:
+        out["step0_vs_oracle"] = cmp
+        out["speedup_vs_cpu_step0"] = cmp["oracle_ms"] / rows[0]["ms"]
+        o.close()
+    return out
 
 
 def cpu_bubble(args, param, ic, nsamples=2):

@@ -115,14 +115,15 @@ void compute_band(Ctx& c) {
 // ELVIRA! (src/ELVIRA.jl:6-55)
 // ------------------------------------------------------------------------------------
 // A hinted ELVIRA candidate is a tie of the oracle's own choice when its error exceeds the
-// smallest one by no more than rounding: 1e-9 relative (a genuinely worse candidate differs
-// at O(1) relative), plus 1e-30 absolute for planar interfaces whose errors are rounding
+// smallest one by no more than rounding: 1e-11 relative (the ties of the stated sphere and
+// bubble differ by at most 8e-13 relative; a genuinely worse candidate differs at O(1)
+// relative), plus 1e-30 absolute for planar interfaces whose errors are rounding
 // noise themselves. The errors of mirrored candidates on a symmetric field are equal in exact
 // arithmetic and differ in their last bits between implementations (libm cbrt/acos/cos in
 // computeDist, the association of the transport's volume sums).
 static bool elvira_tie(double e_hint, double e_best) {
     if (!(e_hint == e_hint) || !(e_hint < INFINITY)) return false;
-    return e_hint <= e_best * (1.0 + 1e-9) + 1e-30;
+    return e_hint <= e_best * (1.0 + 1e-11) + 1e-30;
 }
 
 void compute_normal(Ctx& c) {
