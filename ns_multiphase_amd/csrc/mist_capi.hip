@@ -494,7 +494,7 @@ static int check_overflow(mist_ctx* ctx, bool force) {
     MIST_HIP_TRY(hipMemcpyAsync(ctx->h_counters, ctx->counters, 4 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     MIST_HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (ctx->h_counters[2]) {
-        ctx->err = "cutTet: pre-image walk left the mesh or needed more than 100 cut levels (nLevel)";
+        ctx->err = "cutTet: pre-image walk left the mesh or needed more than 16 cut levels";
         return MIST_ERR_OVERFLOW;
     }
     return MIST_OK;

@@ -9,9 +9,7 @@
 namespace mist {
 
 constexpr double PI = 3.141592653589793;
-// Levels of the general cut walk's explicit stack: the reference's nLevel (src/Transport.jl:23).
-// Only the cells the work-group trees cannot take use it (their frames live in scratch memory).
-constexpr int CUT_MAXLVL = 100;
+constexpr int CUT_MAXLVL = 16;
 
 __device__ __forceinline__ void cell_of(const DevMesh& m, int64_t q, int& i, int& j, int& k) {
     const int a = (int)(q % m.cnx);
