@@ -226,6 +226,8 @@ __host__ __device__ __forceinline__ int vert_corner(int v, bool even) { return e
 // vertex arrays with constants and keeps them in registers.
 struct CutTab {
     static constexpr unsigned char nvert[16] = {0, 3, 3, 4, 3, 4, 4, 3, 3, 4, 4, 3, 4, 3, 3, 0};
+    // the 5-tet table TET5 (reference vertex n of tet t in nibble n)
+    static constexpr unsigned short tet5[5] = {0x5310, 0x6230, 0x6540, 0x7563, 0x5630};
     static constexpr unsigned char ntets[16] = {1, 4, 4, 6, 4, 6, 6, 4, 4, 6, 6, 4, 6, 4, 4, 1};
     static constexpr unsigned char nntet[16] = {1, 2, 2, 4, 2, 4, 4, 4, 2, 4, 4, 4, 4, 4, 4, 2};
     // cut vertex n of case c interpolates reference vertices (v1, v2): nibbles v1 | v2 << 4
@@ -415,6 +417,25 @@ __host__ __device__ __forceinline__ void cut_canonical(int cs, const V3 (&P)[4],
     on_liq(L);
     on_gas(G);
 }
+
+// (ts * v) / cellvol, the reference's per-sub-tet term of computePLIC2VF (ts = +-1). When the
+// cell volume is a power of two (vol_is_pow2) the quotient is exact scaling, so v * (ts /
+// cellvol) rounds to the same double (both are the correctly rounded value of ts v 2^-e)
+// without a division; kernels take the form as a template parameter.
+__host__ __device__ inline bool vol_is_pow2(double cellvol) {
+    int e = 0;
+    const double fr = frexp(cellvol, &e);
+    return fr == 0.5 && e > -1000 && e < 1000;
+}
+template <bool POW2>
+struct VolScale {
+    double cv, inv;
+    __device__ explicit VolScale(double cellvol) : cv(cellvol), inv(1.0 / cellvol) {}
+    __device__ __forceinline__ double operator()(double ts, double v) const {
+        if constexpr (POW2) return v * (ts * inv);
+        else return (ts * v) / cv;
+    }
+};
 
 // computePLIC2VF (src/VF.jl:219-278) of the cell [x0,x1]x[y0,y1]x[z0,z1] for the plane
 // (nx,ny,nz,dist): 5 tets (parity `even` selects the vertex numbering and tetsign), d per tet

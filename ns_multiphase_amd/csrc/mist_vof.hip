@@ -414,6 +414,7 @@ __global__ __launch_bounds__(256) void k_elvira_dist(DevMesh m, DevParams p, con
 constexpr int ELV_THREADS = 256;
 constexpr int ELV_PAIRS = 27 * 9;  // candidate-neighbour pairs of one z-plane of neighbours
 
+template <bool POW2>
 __global__ __launch_bounds__(ELV_THREADS, 4) void k_elvira(DevMesh m, DevParams p, const double* VF, const int* list,
                                                         const int* count, int fam_mask, const double* dist,
                                                         double* nxo, double* nyo, double* nzo, int* best) {
@@ -432,7 +433,7 @@ __global__ __launch_bounds__(ELV_THREADS, 4) void k_elvira(DevMesh m, DevParams 
     __shared__ unsigned char sNLiq[16];  // liquid sub-tets per case: ntets - nntet + 1
     const int tid = threadIdx.x;
     const int ncell = *count;
-    const double cellvol = (m.dx * m.dy) * m.dz;
+    const VolScale<POW2> vscale((m.dx * m.dy) * m.dz);
     if (tid < 16) sNLiq[tid] = (unsigned char)(CUT_NTETS[tid] - CUT_NNTET[tid] + 1);
     for (int cidx = blockIdx.x; cidx < ncell; cidx += gridDim.x) {
         int i, j, k;
@@ -448,7 +449,7 @@ __global__ __launch_bounds__(ELV_THREADS, 4) void k_elvira(DevMesh m, DevParams 
             const bool even = ((ii + jj + kk) % 2) == 0;
             V3 P[4];
             tet5_vertices(t, even, m.x[ii], m.x[ii + 1], m.y[jj], m.y[jj + 1], m.z[kk], m.z[kk + 1], P);
-            sT[nb][t] = ((even ? 1.0 : -1.0) * tet_vol3(P[0], P[1], P[2], P[3])) / cellvol;
+            sT[nb][t] = vscale(even ? 1.0 : -1.0, tet_vol3(P[0], P[1], P[2], P[3]));
         }
         __syncthreads();
         if (tid < 27) {
@@ -490,17 +491,26 @@ __global__ __launch_bounds__(ELV_THREADS, 4) void k_elvira(DevMesh m, DevParams 
                     } else {
                         kind = 0;
                         const bool even = ((ii + jj + kk) % 2) == 0;
-                        for (int t = 0; t < 5; ++t) {
-                            V3 P[4];
-                            tet5_vertices(t, even, m.x[ii], m.x[ii + 1], m.y[jj], m.y[jj + 1], m.z[kk], m.z[kk + 1], P);
-                            double d[4];
+                        // d of the cell's 8 reference vertices, each the expression the tets'
+                        // vertices evaluate ((nx x + ny y) + nz z) - dist with the same products
+                        // (odd cells mirror x), then each tet's four taken from them
+                        const double px0 = cnx * m.x[ii], px1 = cnx * m.x[ii + 1];
+                        const double py0 = cny * m.y[jj], py1 = cny * m.y[jj + 1];
+                        const double pz0 = cnz * m.z[kk], pz1 = cnz * m.z[kk + 1];
+                        const double qx0 = even ? px0 : px1, qx1 = even ? px1 : px0;
+                        double e8[8];
 #pragma unroll
-                            for (int n = 0; n < 4; ++n) d[n] = ((cnx * P[n].x + cny * P[n].y) + cnz * P[n].z) - dist;
+                        for (int v = 0; v < 8; ++v)
+                            e8[v] = (((v & 1) ? qx1 : qx0) + ((v & 2) ? py1 : py0) + ((v & 4) ? pz1 : pz0)) - dist;
+                        sfor<0, 5>([&](auto T) {
+                            constexpr int t = decltype(T)::value;
+                            constexpr unsigned tv = CutTab::tet5[t];
+                            double d[4] = {e8[tv & 15], e8[(tv >> 4) & 15], e8[(tv >> 8) & 15], e8[(tv >> 12) & 15]};
                             const int cs = d2case_snap(d);
                             sCase[tid][t] = (unsigned char)cs;
                             if (cs == 0) sTerm[tid][t][0] = sT[nb][t];
                             else if (cs != 15) atomicAdd(&sCount[cs], 1);
-                        }
+                        });
                     }
                 }
                 sKind[tid] = (unsigned char)kind;
@@ -542,7 +552,7 @@ __global__ __launch_bounds__(ELV_THREADS, 4) void k_elvira(DevMesh m, DevParams 
                 const int cs = d2case_snap(d);
                 int w = 0;
                 cut_dispatch<false>(
-                    cs, P, d, [&](double v) { sTerm[pr][t][w++] = (ts * v) / cellvol; }, [](double) {});
+                    cs, P, d, [&](double v) { sTerm[pr][t][w++] = vscale(ts, v); }, [](double) {});
             }
             __syncthreads();
             // (D) computePLIC2VF of each pair in the reference's order, then the errors
@@ -1450,9 +1460,14 @@ int launch_normals(mist_ctx* ctx) {
     hipEvent_t ev = prof_begin(ctx, MIST_PROF_ELVIRA);  // both kernels of ELVIRA
     LAUNCH(k_elvira_dist, grid_for(ncap * 27, 256, 8192), 256, ctx->m, ctx->dp, (const double*)ctx->f[MIST_FIELD_VOF],
            (const int*)ctx->iface_list, (const int*)(ctx->counters + 1), fam, ctx->elv_dist);
-    LAUNCH(k_elvira, grid_for(ncap, 1, 16384), ELV_THREADS, ctx->m, ctx->dp, (const double*)ctx->f[MIST_FIELD_VOF],
-           (const int*)ctx->iface_list, (const int*)(ctx->counters + 1), fam, (const double*)ctx->elv_dist, nx, ny, nz,
-           ctx->elv_best);
+    if (vol_is_pow2((ctx->m.dx * ctx->m.dy) * ctx->m.dz))
+        LAUNCH(k_elvira<true>, grid_for(ncap, 1, 16384), ELV_THREADS, ctx->m, ctx->dp,
+               (const double*)ctx->f[MIST_FIELD_VOF], (const int*)ctx->iface_list, (const int*)(ctx->counters + 1), fam,
+               (const double*)ctx->elv_dist, nx, ny, nz, ctx->elv_best);
+    else
+        LAUNCH(k_elvira<false>, grid_for(ncap, 1, 16384), ELV_THREADS, ctx->m, ctx->dp,
+               (const double*)ctx->f[MIST_FIELD_VOF], (const int*)ctx->iface_list, (const int*)(ctx->counters + 1), fam,
+               (const double*)ctx->elv_dist, nx, ny, nz, ctx->elv_best);
     prof_end(ctx, MIST_PROF_ELVIRA, ev, ctx->counters + 1, 0);
     int st;
     if ((st = launch_borders(ctx, MIST_FIELD_NX))) return st;
