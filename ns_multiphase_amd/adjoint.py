@@ -157,18 +157,27 @@ class SLVOFRun:
         self.ctx.compute_band()
 
     def forward(self, amps, checkpoints=True):
+        """VF after the run. With checkpoints, the state every `every` steps goes to its slot,
+        and the states inside the last segment go to the segment slots the reverse sweep walks
+        first, so that segment is not run forward again."""
         c = self.ctx
         self.set_velocity(amps)
         c.set("VF", self.vf0)
         c.vof_init()
+        self._tail_kept = False
+        last0 = (self.nseg - 1) * self.every
         c.set_deferred_checks(True)  # one wait for the whole run (the download below)
         try:
             for s in range(self.nsteps):
-                if checkpoints and s % self.every == 0:
-                    c.checkpoint_save(s // self.every)
+                if checkpoints:
+                    if s % self.every == 0:
+                        c.checkpoint_save(s // self.every)
+                    elif s > last0:
+                        c.checkpoint_save(self.nseg + (s - last0 - 1))
                 self._step()
         finally:
             c.set_deferred_checks(False)
+        self._tail_kept = bool(checkpoints)
         return c.get("VF")
 
     def backward(self, lam_T):
@@ -178,15 +187,19 @@ class SLVOFRun:
         for name in ("uf_bar", "vf_bar", "wf_bar"):
             c.set(name, self._zero[name])
         base = self.nseg  # slots after the checkpoints hold a segment's later states
+        tail = getattr(self, "_tail_kept", False)
+        self._tail_kept = False  # the sweep reuses the segment slots
         c.set_deferred_checks(True)  # one wait for the whole sweep (the downloads below)
         try:
-            self._sweep(base)
+            self._sweep(base, tail)
         finally:
             c.set_deferred_checks(False)
         guf, gvf = c.get("uf_bar"), c.get("vf_bar")
         return self._Bu @ guf.ravel() + self._Bv @ gvf.ravel()
 
-    def _sweep(self, base):
+    def _sweep(self, base, tail_kept=False):
+        """Segments from the last to the first: each is run forward from its checkpoint into
+        the segment slots (unless the forward pass left the last one there), then walked back."""
         c = self.ctx
         for seg in range(self.nseg - 1, -1, -1):
             s0 = seg * self.every
@@ -194,7 +207,7 @@ class SLVOFRun:
 
             def slot(s):
                 return seg if s == s0 else base + (s - s0 - 1)
-            if s1 - s0 > 1:
+            if s1 - s0 > 1 and not (tail_kept and seg == self.nseg - 1):
                 c.checkpoint_restore(seg)  # VF, band, normals and PLIC constants of state s0
                 for s in range(s0 + 1, s1):
                     self._step()
