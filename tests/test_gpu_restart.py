@@ -101,6 +101,9 @@ def test_restart_ns_matches_oracle(mist, oracle, scheme):
     _close(b.get("VF"), o.get("VF"), 1e-12, "VF")
     for n in ("us", "vs", "ws"):
         _close(b.get(n), o.get(n), 1e-11, n)
+    # the finite-difference pressure solves agree to rounding; the SL-SL FAS solves stop at
+    # different iterates below the same residual (Anderson's small solves are ordered differently)
+    vel_bar = 1e-12 if scheme == "finite-difference" else 1e-7
     for n in ("u", "v", "w"):
-        _close(b.get(n), o.get(n), 1e-7, n)
+        _close(b.get(n), o.get(n), vel_bar, n)
     assert o.tie_stats("elvira")["refused"] == 0
