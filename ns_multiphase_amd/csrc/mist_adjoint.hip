@@ -535,7 +535,7 @@ __global__ __launch_bounds__(C::NT) void k_adj_tree(DevMesh m, const double* pv,
                     ids[n] = (unsigned)c;
                     bits |= (unsigned)((cc.offs >> (6 * c)) & 63ull) << (6 * n);
                 }
-                S.node[0][tid] = TreeNode{ids[0] | (ids[1] << 16), ids[2] | (ids[3] << 16), bits};
+                node_set(S, 0, tid, TreeNode{ids[0] | (ids[1] << 16), ids[2] | (ids[3] << 16), bits});
             }
             __syncthreads();
             // ---- the tree again, recording each pool vertex's edge and plane ---------------
@@ -546,7 +546,7 @@ __global__ __launch_bounds__(C::NT) void k_adj_tree(DevMesh m, const double* pv,
                 for (int rep = 0; rep < 3 && ok; ++rep) {
                     int any = 0;
                     for (int e = tid; e < n; e += NT) {
-                        const unsigned bits = S.node[cur][e].bits;
+                        const unsigned bits = node_bits(S, cur, e);
                         const int o0 = boff(bits, 0, axis), o1 = boff(bits, 1, axis), o2 = boff(bits, 2, axis),
                                   o3 = boff(bits, 3, axis);
                         any |= max(max(o0, o1), max(o2, o3)) - min(min(o0, o1), min(o2, o3)) >= minspan;
@@ -564,7 +564,7 @@ __global__ __launch_bounds__(C::NT) void k_adj_tree(DevMesh m, const double* pv,
                 const int e = b + tid;
                 int cnt = 0;
                 if (e < n) {
-                    const TreeNode N = S.node[cur][e];
+                    const TreeNode N = node_get(S, cur, e);
                     const int o0 = boff(N.bits, 0, 2), o1 = boff(N.bits, 1, 2), o2 = boff(N.bits, 2, 2),
                               o3 = boff(N.bits, 3, 2);
                     const int mx = max(max(o0, o1), max(o2, o3));
@@ -584,10 +584,8 @@ __global__ __launch_bounds__(C::NT) void k_adj_tree(DevMesh m, const double* pv,
                     ok = false;
                     break;
                 }
-                if (e < n) {
-                    S.loff[e] = (unsigned short)(nleaf + o);
-                    for (int c = 0; c < cnt; ++c) S.lnode[nleaf + o + c] = (unsigned short)e;
-                }
+                if (e < n)
+                    for (int c = 0; c < cnt; ++c) S.lnode[nleaf + o + c] = leaf_entry(e, c);
                 nleaf += tot;
             }
             if (!ok) {

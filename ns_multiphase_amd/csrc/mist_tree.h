@@ -39,7 +39,7 @@ struct TreeCfg {
 };
 // Sized from the oracle's counts on the 3D deformation (99.9% of cells need at most ~320
 // pool vertices and ~230 nodes after the y cuts): the main pass takes almost all cells at
-// ~21 KB of LDS per block; the few that overflow it go to a second pass with 256 threads
+// ~17 KB of LDS per block (8-byte nodes); the few that overflow it go to a second pass with 256 threads
 // and ~90 KB.
 using TreeMain = TreeCfg<64, 256, 384, 1152>;
 using TreeWide = TreeCfg<256, 1536, 1536, 6144>;
@@ -48,6 +48,30 @@ struct TreeNode {
     unsigned w0, w1;  // pool ids of vertices 0,1 and 2,3 (16 bits each)
     unsigned bits;    // cell offset from lo of vertex n along axis q at bit 6n + 2q
 };
+// A node as the lists in LDS hold it: with at most 256 pool vertices the four ids take 8 bits
+// each and a node 8 bytes (more blocks fit a compute unit's LDS); otherwise the TreeNode itself.
+struct TreeNode8 {
+    unsigned ids;  // pool id of vertex n at bits 8n
+    unsigned bits;
+};
+template <int PV>
+struct NodeSlot {
+    using type = TreeNode;
+};
+template <>
+struct NodeSlot<256> {
+    using type = TreeNode8;
+};
+__device__ __forceinline__ TreeNode node_unpack(const TreeNode& N) { return N; }
+__device__ __forceinline__ TreeNode node_unpack(const TreeNode8& P) {
+    return TreeNode{(P.ids & 0xFFu) | (((P.ids >> 8) & 0xFFu) << 16), ((P.ids >> 16) & 0xFFu) | ((P.ids >> 24) << 16),
+                    P.bits};
+}
+__device__ __forceinline__ void node_pack(TreeNode& D, const TreeNode& N) { D = N; }
+__device__ __forceinline__ void node_pack(TreeNode8& D, const TreeNode& N) {
+    D.ids = (N.w0 & 0xFFu) | (((N.w0 >> 16) & 0xFFu) << 8) | ((N.w1 & 0xFFu) << 16) | (((N.w1 >> 16) & 0xFFu) << 24);
+    D.bits = N.bits;
+}
 
 // The cut tables in LDS: every lane indexes them with its own case, and LDS answers such a
 // gather with less latency than the constant-memory path.
@@ -61,9 +85,8 @@ template <class C>
 struct TreeShared {
     CutTabS tab;
     double px[C::PV], py[C::PV], pz[C::PV];
-    TreeNode node[2][C::NL];
-    unsigned short loff[C::NL];      // first leaf of each node of the last list
-    unsigned short lnode[C::NLEAF];  // node of each leaf
+    typename NodeSlot<C::PV>::type node[2][C::NL];
+    unsigned short lnode[C::NLEAF];  // node (bits 0-10) and child (bits 11-13) of each leaf
     double plane[3][5];                // faces lo+0 .. lo+span+1 per axis
     double nb[64][4];                  // nx, ny, nz, D of the cells of the index box
     signed char cls[64];               // +1 / -1: the cell lies on the liquid / gas side of its plane
@@ -71,6 +94,21 @@ struct TreeShared {
     int scan[C::NT / 64];
     double red[2][C::NT / 64];
 };
+
+template <class C>
+__device__ __forceinline__ TreeNode node_get(const TreeShared<C>& S, int list, int e) {
+    return node_unpack(S.node[list][e]);
+}
+template <class C>
+__device__ __forceinline__ void node_set(TreeShared<C>& S, int list, int e, const TreeNode& N) {
+    node_pack(S.node[list][e], N);
+}
+template <class C>
+__device__ __forceinline__ unsigned node_bits(const TreeShared<C>& S, int list, int e) {
+    return S.node[list][e].bits;
+}
+// the leaf entry: node e of the last list and child c of its last z cut
+__device__ __forceinline__ unsigned short leaf_entry(int e, int c) { return (unsigned short)(e | (c << 11)); }
 
 __device__ __forceinline__ unsigned tn_id(const TreeNode& N, int n) {
     return ((n < 2 ? N.w0 : N.w1) >> ((n & 1) * 16)) & 0xFFFFu;
@@ -176,6 +214,7 @@ template <class C>
 __device__ bool tree_level(TreeShared<C>& S, int axis, int minspan, int cur, int& n, int& npool, Span3 span,
                            unsigned* par = nullptr) {
     static_assert(C::PV <= 2048, "pool ids are packed in 11 bits");
+    static_assert(C::NL <= 2048, "leaf entries hold the node in 11 bits");
     constexpr int NT = C::NT;
     int nout = 0;
     for (int b = 0; b < n; b += NT) {
@@ -186,7 +225,7 @@ __device__ bool tree_level(TreeShared<C>& S, int axis, int minspan, int cur, int
         bool cut = false;
         double d[4] = {0.0, 0.0, 0.0, 0.0};
         if (active) {
-            N = S.node[cur][e];
+            N = node_get(S, cur, e);
             const int o0 = boff(N.bits, 0, axis), o1 = boff(N.bits, 1, axis), o2 = boff(N.bits, 2, axis),
                       o3 = boff(N.bits, 3, axis);
             mx = max(max(o0, o1), max(o2, o3));
@@ -211,7 +250,7 @@ __device__ bool tree_level(TreeShared<C>& S, int axis, int minspan, int cur, int
         const int oc = block_scan<NT>(nch, tc, S.scan);
         const int ov = block_scan<NT>(nv, tv, S.scan);
         if (nout + tc > C::NL || npool + tv > C::PV) return false;
-        if (active && !cut) S.node[cur ^ 1][nout + oc] = N;
+        if (active && !cut) node_set(S, cur ^ 1, nout + oc, N);
         if (cut) {
             const int vb = npool + ov;
             unsigned cb[4] = {0u, 0u, 0u, 0u};  // side-1 offsets (6 bits) of the cut vertices
@@ -272,7 +311,7 @@ __device__ bool tree_level(TreeShared<C>& S, int axis, int minspan, int cur, int
                 ch.w0 = ids[0] | (ids[1] << 16);
                 ch.w1 = ids[2] | (ids[3] << 16);
                 ch.bits = bits;
-                S.node[cur ^ 1][nout + oc + c] = ch;
+                node_set(S, cur ^ 1, nout + oc + c, ch);
             }
         }
         nout += tc;
@@ -349,9 +388,9 @@ struct TreeLeaf {
 template <class C>
 __device__ __forceinline__ void tree_leaf(const TreeShared<C>& S, int cur, int L, const int (&span)[3], int bx, int by,
                                           TreeLeaf& T) {
-    const int e = S.lnode[L];
-    const int c = L - S.loff[e];
-    const TreeNode N = S.node[cur][e];
+    const unsigned le = S.lnode[L];
+    const int e = (int)(le & 0x7FFu), c = (int)(le >> 11);
+    const TreeNode N = node_get(S, cur, e);
     T.N = N;
     const int o0 = boff(N.bits, 0, 2), o1 = boff(N.bits, 1, 2), o2 = boff(N.bits, 2, 2), o3 = boff(N.bits, 3, 2);
     const int mx = max(max(o0, o1), max(o2, o3));

@@ -900,7 +900,7 @@ __global__ __launch_bounds__(C::NT) void k_sl_tree(DevMesh m, const double* pv, 
                     ob |= (unsigned)(((cc.offs >> (6 * c + 2 * a)) & 3ull) + shift[a]) << (2 * a);
                 bits |= ob << (6 * n);
             }
-            S.node[0][tid] = TreeNode{ids[0] | (ids[1] << 16), ids[2] | (ids[3] << 16), bits};
+            node_set(S, 0, tid, TreeNode{ids[0] | (ids[1] << 16), ids[2] | (ids[3] << 16), bits});
         }
         if constexpr (CORR) {
             if (tid >= 5 && tid < 17 && box_ok) {
@@ -912,7 +912,7 @@ __global__ __launch_bounds__(C::NT) void k_sl_tree(DevMesh m, const double* pv, 
                     for (int a = 0; a < 3; ++a)
                         bits |= (unsigned)(s_cind[4 * t + n][a] - lo[a]) << (6 * n + 2 * a);
                 const unsigned id0 = 8 + 4 * t;
-                S.node[0][tid] = TreeNode{id0 | ((id0 + 1) << 16), (id0 + 2) | ((id0 + 3) << 16), bits};
+                node_set(S, 0, tid, TreeNode{id0 | ((id0 + 1) << 16), (id0 + 2) | ((id0 + 3) << 16), bits});
             }
         }
         __syncthreads();
@@ -924,7 +924,7 @@ __global__ __launch_bounds__(C::NT) void k_sl_tree(DevMesh m, const double* pv, 
             for (int rep = 0; rep < 3 && ok; ++rep) {
                 int any = 0;
                 for (int e = tid; e < n; e += NT) {
-                    const unsigned bits = S.node[cur][e].bits;
+                    const unsigned bits = node_bits(S, cur, e);
                     const int o0 = boff(bits, 0, axis), o1 = boff(bits, 1, axis), o2 = boff(bits, 2, axis),
                               o3 = boff(bits, 3, axis);
                     any |= max(max(o0, o1), max(o2, o3)) - min(min(o0, o1), min(o2, o3)) >= minspan;
@@ -940,7 +940,7 @@ __global__ __launch_bounds__(C::NT) void k_sl_tree(DevMesh m, const double* pv, 
             const int e = b + tid;
             int cnt = 0;
             if (e < n) {
-                const TreeNode N = S.node[cur][e];
+                const TreeNode N = node_get(S, cur, e);
                 const int o0 = boff(N.bits, 0, 2), o1 = boff(N.bits, 1, 2), o2 = boff(N.bits, 2, 2),
                           o3 = boff(N.bits, 3, 2);
                 const int mx = max(max(o0, o1), max(o2, o3));
@@ -960,10 +960,8 @@ __global__ __launch_bounds__(C::NT) void k_sl_tree(DevMesh m, const double* pv, 
                 ok = false;
                 break;
             }
-            if (e < n) {
-                S.loff[e] = (unsigned short)(nleaf + o);
-                for (int c = 0; c < cnt; ++c) S.lnode[nleaf + o + c] = (unsigned short)e;
-            }
+            if (e < n)
+                for (int c = 0; c < cnt; ++c) S.lnode[nleaf + o + c] = leaf_entry(e, c);
             nleaf += tot;
         }
         if (!ok) {  // pool or lists too small for this cell: the next pass
@@ -974,9 +972,9 @@ __global__ __launch_bounds__(C::NT) void k_sl_tree(DevMesh m, const double* pv, 
             double Vs = 0.0, Ls = 0.0;
             double Ms[4] = {0.0, 0.0, 0.0, 0.0};  // rho u vol, rho v vol, rho w vol, rho vol
             for (int L = tid; L < nleaf; L += NT) {
-                const int e = S.lnode[L];
-                const int c = L - S.loff[e];
-                const TreeNode N = S.node[cur][e];
+                const unsigned le = S.lnode[L];
+                const int e = (int)(le & 0x7FFu), c = (int)(le >> 11);
+                const TreeNode N = node_get(S, cur, e);
                 const int o0 = boff(N.bits, 0, 2), o1 = boff(N.bits, 1, 2), o2 = boff(N.bits, 2, 2),
                           o3 = boff(N.bits, 3, 2);
                 const int mx = max(max(o0, o1), max(o2, o3));
