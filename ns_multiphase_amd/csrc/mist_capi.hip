@@ -194,7 +194,7 @@ int mist_ctx_destroy(mist_ctx* ctx) {
         if (ctx->f[f]) (void)hipFree(ctx->f[f]);
     void* bufs[] = {ctx->geom, ctx->trig, ctx->pv, ctx->band_list, ctx->iface_list, ctx->defer_list, ctx->cut_list,
                     ctx->counters, ctx->red, ctx->ns_sums, ctx->elv_best, ctx->adj_lam, ctx->adj_plane, ctx->adj_pv,
-                    ctx->adj_sums, ctx->elv_dist};
+                    ctx->adj_sums, ctx->elv_dist, ctx->vtx_list};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (double* b : ctx->ckpt)
@@ -306,6 +306,7 @@ int mist_ctx_create(const mist_params* pin, int device, mist_ctx** out) {
     const int64_t nvtx = (p.Nx + 1) * (p.Ny + 1) * (p.Nz + 1);
     ctx->red_blocks = 1024;
     if (hipMalloc(&ctx->pv, (size_t)nvtx * 3 * sizeof(double)) != hipSuccess ||
+        hipMalloc(&ctx->vtx_list, (size_t)nvtx * sizeof(int)) != hipSuccess ||
         hipMalloc(&ctx->band_list, (size_t)ncell * sizeof(int)) != hipSuccess ||
         hipMalloc(&ctx->iface_list, (size_t)ncell * sizeof(int)) != hipSuccess ||
         hipMalloc(&ctx->defer_list, (size_t)ncell * sizeof(int)) != hipSuccess ||

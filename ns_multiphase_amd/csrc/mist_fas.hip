@@ -182,7 +182,7 @@ struct CornerVerts {
 // J = (A+ - A-) /^ 2 delta. The divergence at c reads the faces of c and, through the
 // back-trace of c's corners, faces near them: PertFaces gives every face as A! would compute
 // it over the whole mesh with P perturbed at c, and the corners are traced as
-// k_project_vertices traces them (only vertices of band cells are read), so each column equals
+// k_vertex_list selects them (only vertices of band cells are read), so each column equals
 // a whole-mesh evaluation of A! with only c perturbed.
 __global__ void k_fas_jac_local(DevMesh m, DevParams p, int nband, const int16_t* band, double dt, const double* uf0,
                                 const double* vf0, const double* wf0, const double* denx, const double* deny,
@@ -222,7 +222,7 @@ __global__ void k_fas_jac_local(DevMesh m, DevParams p, int nband, const int16_t
                 const int I = i + (c & 1), Jv = j + ((c >> 1) & 1), K = k + (c >> 2);
                 cv.v[c][0] = cv.v[c][1] = cv.v[c][2] = 0.0;
                 if (!trace) continue;
-                bool need = false;  // the vertex selection of k_project_vertices
+                bool need = false;  // the vertex selection of k_vertex_list
                 for (int e = 0; e < 8; ++e)
                     need |= abs((int)band[m.cell(I - 1 + (e & 1), Jv - 1 + ((e >> 1) & 1), K - 1 + (e >> 2))]) <= p.nband;
                 if (!need) continue;

@@ -62,9 +62,11 @@ struct mist_ctx {
     int* iface_list = nullptr;   // interior interface cells (ELVIRA); cells the main cut pass defers
     int* defer_list = nullptr;   // cells for the general cut walk
     void* cut_list = nullptr;    // CutCell entries of the band cells to cut
+    int* vtx_list = nullptr;     // vertices of band cells to back-trace (linear vertex index)
     int* counters = nullptr;     // [0] band cells, [1] interface cells, [2] overflow flag, [4] cells
                                  // deferred by the main cut pass, [5] its work queue, [6] cells for
-                                 // the general walk, [7] the wide pass's work queue, [8] cells to cut
+                                 // the general walk, [7] the wide pass's work queue, [8] cells to
+                                 // cut, [9] vertices to back-trace
     double* red = nullptr;       // reduction partials
     int red_blocks = 0;
     int* h_counters = nullptr;   // pinned

@@ -621,22 +621,39 @@ __global__ void k_plic(DevMesh m, DevParams p, const double* VF, const double* n
     }
 }
 
-__global__ void k_project_vertices(DevMesh m, DevParams p, const int16_t* band, const double* uf, const double* vf,
-                                   const double* wf, double dt, double* pv) {
+// Every vertex of a band cell, ghost cells included (a periodic image of a band cell puts an
+// SL flux on the boundary face of its interior neighbour), appended to `list` (linear vertex
+// index over (Nx+1)(Ny+1)(Nz+1)), so that the back-trace runs on full waves.
+__global__ void k_vertex_list(DevMesh m, DevParams p, const int16_t* band, int* list, int* count) {
+    __shared__ int s_app[2];
     const int64_t n = (int64_t)(m.Nx + 1) * (m.Ny + 1) * (m.Nz + 1);
-    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t nb = (n + blockDim.x - 1) / blockDim.x * blockDim.x;  // whole work-groups take part
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nb; q += (int64_t)gridDim.x * blockDim.x) {
+        bool need = false;
+        if (q < n) {
+            const int I = (int)(q % (m.Nx + 1)) + 1;
+            const int64_t r = q / (m.Nx + 1);
+            const int J = (int)(r % (m.Ny + 1)) + 1;
+            const int K = (int)(r / (m.Ny + 1)) + 1;
+            for (int c = 0; c < 8; ++c) {
+                const int i = I - 1 + (c & 1), j = J - 1 + ((c >> 1) & 1), k = K - 1 + (c >> 2);
+                need |= abs((int)band[m.cell(i, j, k)]) <= p.nband;
+            }
+        }
+        append_group(need, (int)q, list, count, s_app);
+    }
+}
+
+// project! (src/Tools.jl:392-418) of the listed vertices into pv[3 q ..].
+__global__ void k_project_list(DevMesh m, DevParams p, const int* list, const int* count, const double* uf,
+                               const double* vf, const double* wf, double dt, double* pv) {
+    const int n = *count;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        const int64_t q = list[e];
         const int I = (int)(q % (m.Nx + 1)) + 1;
         const int64_t r = q / (m.Nx + 1);
         const int J = (int)(r % (m.Ny + 1)) + 1;
         const int K = (int)(r / (m.Ny + 1)) + 1;
-        // every vertex of a band cell, ghost cells included (a periodic image of a band cell
-        // puts an SL flux on the boundary face of its interior neighbour)
-        bool need = false;
-        for (int c = 0; c < 8; ++c) {
-            const int i = I - 1 + (c & 1), j = J - 1 + ((c >> 1) & 1), k = K - 1 + (c >> 2);
-            need |= abs((int)band[m.cell(i, j, k)]) <= p.nband;
-        }
-        if (!need) continue;
         double pt[3] = {m.x[I], m.y[J], m.z[K]};
         project_pt(m, p, uf, vf, wf, pt, min(I, m.Nx), min(J, m.Ny), min(K, m.Nz), dt);
         pv[3 * q + 0] = pt[0];
@@ -1486,8 +1503,11 @@ int launch_plic(mist_ctx* ctx) {
 int launch_project_vertices(mist_ctx* ctx, double dt) {
     const int64_t nv = (int64_t)(ctx->m.Nx + 1) * (ctx->m.Ny + 1) * (ctx->m.Nz + 1);
     hipEvent_t ev = prof_begin(ctx, MIST_PROF_PROJECT);
-    LAUNCH(k_project_vertices, grid_for(nv, 256), 256, ctx->m, ctx->dp, (const int16_t*)ctx->f[MIST_FIELD_BAND],
-           (const double*)ctx->f[MIST_FIELD_UF], (const double*)ctx->f[MIST_FIELD_VF],
+    LAUNCH(k_fill_int, 1, 64, ctx->counters + 9, 1, 0);
+    LAUNCH(k_vertex_list, grid_for(nv, 256, 8192), 256, ctx->m, ctx->dp, (const int16_t*)ctx->f[MIST_FIELD_BAND],
+           ctx->vtx_list, ctx->counters + 9);
+    LAUNCH(k_project_list, grid_for(nv, 256, 2048), 256, ctx->m, ctx->dp, (const int*)ctx->vtx_list,
+           (const int*)(ctx->counters + 9), (const double*)ctx->f[MIST_FIELD_UF], (const double*)ctx->f[MIST_FIELD_VF],
            (const double*)ctx->f[MIST_FIELD_WF], dt, ctx->pv);
     prof_end(ctx, MIST_PROF_PROJECT, ev, nullptr, nv);
     return MIST_OK;
