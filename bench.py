@@ -121,11 +121,13 @@ def run_vof(args, torch):
         ctx.init_vof(*ic)
         ctx.vof_init()
         ctx.profile(True)
+        ctx.set_deferred_checks(True)  # each step waits for its dt only, not for the previous step
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         tt = 0.0
         for _ in range(steps):
             tt, _ = ctx.vof_step(tt, param.tFinal)
+        ctx.set_deferred_checks(False)  # reports a cut-walk overflow of any step
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
     prof = ctx.profile_read()

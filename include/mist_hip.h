@@ -252,8 +252,11 @@ int mist_transport_vjp(mist_ctx* ctx, double dt);
  * the checkpointed reverse sweeps). mist_transport and mist_transport_vjp normally wait for the
  * device after each call to report an overflow of the cut walk (MIST_ERR_OVERFLOW). With
  * on != 0 they return without waiting; the overflow is then reported by the next call that
- * waits for the device anyway (mist_download, mist_reduce, mist_compute_dt, mist_vof_step,
- * mist_ns_step) and by mist_set_deferred_checks(ctx, 0), which waits and returns it. */
+ * waits for the device anyway (mist_download, mist_reduce, mist_compute_dt, mist_ns_step, a
+ * mist_vof_step that follows another call) and by mist_set_deferred_checks(ctx, 0), which
+ * waits and returns it. Consecutive mist_vof_step calls with no other call in between do not
+ * wait for the previous step either: each reduces the velocity maxima right behind its
+ * defineVelocity! and the next one waits only for those (the same dt bit for bit). */
 int mist_set_deferred_checks(mist_ctx* ctx, int on);
 
 /* Step checkpoints: copy the state a step starts from (VF with its ghosts, the band flags

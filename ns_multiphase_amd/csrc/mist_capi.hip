@@ -62,6 +62,7 @@ static void field_written(mist_ctx* ctx, int field) {
 #define CHECK_CTX()                                                   \
     do {                                                              \
         if (!ctx) return fail(nullptr, MIST_ERR_PARAM, "null context"); \
+        ++ctx->api_seq;                                               \
     } while (0)
 
 #define RET_IF(x)                                 \
@@ -201,6 +202,8 @@ int mist_ctx_destroy(mist_ctx* ctx) {
         if (b) (void)hipFree(b);
     if (ctx->h_counters) (void)hipHostFree(ctx->h_counters);
     if (ctx->h_red) (void)hipHostFree(ctx->h_red);
+    if (ctx->h_vmax) (void)hipHostFree(ctx->h_vmax);
+    if (ctx->vmax_ev) (void)hipEventDestroy(ctx->vmax_ev);
     for (int k = 0; k < MIST_PROF_COUNT; ++k)
         for (auto& pr : ctx->prof.pending[k]) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
     for (hipEvent_t e : ctx->prof.pool) (void)hipEventDestroy(e);
@@ -319,8 +322,11 @@ int mist_ctx_create(const mist_params* pin, int device, mist_ctx** out) {
         return bail(MIST_ERR_OOM, "work allocation");
     if (hipMemset(ctx->counters, 0, 16 * sizeof(int)) != hipSuccess) return bail(MIST_ERR_HIP, "clear");
     if (hipHostMalloc(&ctx->h_counters, 16 * sizeof(int), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc(&ctx->h_red, 16 * sizeof(double), hipHostMallocDefault) != hipSuccess)
+        hipHostMalloc(&ctx->h_red, 16 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(&ctx->h_vmax, 4 * sizeof(double), hipHostMallocDefault) != hipSuccess)
         return bail(MIST_ERR_OOM, "pinned allocation");
+    if (hipEventCreateWithFlags(&ctx->vmax_ev, hipEventDisableTiming) != hipSuccess)
+        return bail(MIST_ERR_HIP, "event");
     *out = ctx;
     return MIST_OK;
 }
@@ -564,14 +570,28 @@ int mist_vof_step(mist_ctx* ctx, double* t, double clamp_to, double* dt_used) {
     if (!t) return fail(ctx, MIST_ERR_PARAM, "null t");
     MIST_HIP_TRY(hipSetDevice(ctx->device));
     double mx[3];
-    RET_IF(mist::launch_maxabs_uvw(ctx, mx));  // synchronises: the previous step is complete
-    RET_IF(check_overflow(ctx, true));
+    // with deferred checks, the maxima the previous step reduced behind its velocity (when no
+    // other call came in between): wait for them, not for the previous step's transport
+    const bool pending = ctx->defer_checks && ctx->vmax_pending && ctx->api_seq == ctx->vmax_seq + 1;
+    ctx->vmax_pending = false;
+    if (pending) {
+        MIST_HIP_TRY(hipEventSynchronize(ctx->vmax_ev));
+        mx[0] = ctx->h_vmax[0]; mx[1] = ctx->h_vmax[1]; mx[2] = ctx->h_vmax[2];
+    } else {
+        RET_IF(mist::launch_maxabs_uvw(ctx, mx));  // synchronises: the previous step is complete
+        RET_IF(check_overflow(ctx, true));
+    }
     double dt = dt_from_max(ctx, mx);
     if (clamp_to > 0.0 && *t + dt > clamp_to) dt = clamp_to - *t;
     *t += dt;
-    RET_IF(mist::launch_define_velocity(ctx, *t));
+    if (ctx->defer_checks) RET_IF(mist::launch_define_velocity_vmax(ctx, *t));
+    else RET_IF(mist::launch_define_velocity(ctx, *t));
     RET_IF(transport_body(ctx, dt));
     RET_IF(mist_compute_band(ctx));
+    if (ctx->defer_checks) {
+        ctx->vmax_pending = true;
+        ctx->vmax_seq = ctx->api_seq;
+    }
     if (dt_used) *dt_used = dt;
     return MIST_OK;
 }

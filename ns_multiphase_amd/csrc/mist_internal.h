@@ -76,6 +76,15 @@ struct mist_ctx {
     int tree_blocks_adj = 0;     // resident blocks of the reverse cut pass
     double* ns_sums = nullptr;   // 6 sums per interior cell of the momentum predictor's cut
     double* h_red = nullptr;     // pinned
+    // mist_vof_step with deferred checks: the velocity maxima the next step's dt needs are
+    // reduced right after the velocity is defined, copied to pinned memory and marked by an
+    // event, so the next step waits for that event instead of for the whole stream. They are
+    // used only when no other call came in between (api_seq counts the entry points).
+    uint64_t api_seq = 0;
+    uint64_t vmax_seq = 0;
+    bool vmax_pending = false;
+    double* h_vmax = nullptr;    // pinned, 3 doubles
+    hipEvent_t vmax_ev = nullptr;
     uint64_t vf_version = 1, normals_version = 0, band_version = 0;
     bool defer_checks = false;   // transport / VJP calls skip their host round trip (mist_set_deferred_checks)
     bool band_list_valid = false;
@@ -107,6 +116,7 @@ void prof_end(mist_ctx* ctx, int kernel, hipEvent_t a, const int* device_units, 
 // mist_vof.hip
 int launch_trig_tables(mist_ctx* ctx, int64_t n);
 int launch_define_velocity(mist_ctx* ctx, double t);
+int launch_define_velocity_vmax(mist_ctx* ctx, double t);
 int launch_maxabs_uvw(mist_ctx* ctx, double out[3]);
 int launch_neumann(mist_ctx* ctx, int field);
 int launch_neumann_cells(mist_ctx* ctx, double* A);  // Neumann! of a cell-extent array of ctx's mesh

@@ -74,22 +74,35 @@ __device__ __forceinline__ void vel_at(int kind, double T, double t, double ct, 
     }
 }
 
-__global__ void k_define_velocity(DevMesh m, VelTab V, int kind, double T, double t, double* u, double* v, double* w,
-                                  double* uf, double* vf, double* wf) {
+__device__ double block_reduce(int kind, double v);
+
+// With `part`, each block also writes the maxima of |u|, |v|, |w| over its cells to
+// part[3 blockIdx + c] (the partials k_maxabs3 would give: a maximum does not depend on order).
+__global__ __launch_bounds__(256) void k_define_velocity(DevMesh m, VelTab V, int kind, double T, double t, double* u,
+                                                         double* v, double* w, double* uf, double* vf, double* wf,
+                                                         double* part) {
     const int64_t n = (int64_t)m.cnx * m.cny * m.cnz;
     const double ct = cos((PI * t) / T);
+    double r[3] = {0.0, 0.0, 0.0};
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
         int i, j, k;
         cell_of(m, q, i, j, k);
         double a, b, c;
         vel_at<1, 1, 1>(kind, T, t, ct, m, V, i, j, k, a, b, c);
         u[q] = a; v[q] = b; w[q] = c;
+        r[0] = jmax(r[0], fabs(a)); r[1] = jmax(r[1], fabs(b)); r[2] = jmax(r[2], fabs(c));
         vel_at<0, 1, 1>(kind, T, t, ct, m, V, i, j, k, a, b, c);
         uf[q] = a;
         vel_at<1, 0, 1>(kind, T, t, ct, m, V, i, j, k, a, b, c);
         vf[q] = b;
         vel_at<1, 1, 0>(kind, T, t, ct, m, V, i, j, k, a, b, c);
         wf[q] = c;
+    }
+    if (part) {
+        for (int c = 0; c < 3; ++c) {
+            const double bm = block_reduce(MIST_REDUCE_MAX, r[c]);
+            if (threadIdx.x == 0) part[3 * blockIdx.x + c] = bm;
+        }
     }
 }
 
@@ -1362,7 +1375,23 @@ int launch_define_velocity(mist_ctx* ctx, double t) {
     const int64_t n = (int64_t)ctx->m.cnx * ctx->m.cny * ctx->m.cnz;
     LAUNCH(k_define_velocity, grid_for(n, 256), 256, ctx->m, ctx->vt, (int)ctx->p.vf_velocity, ctx->p.vel_period, t,
            (double*)ctx->f[MIST_FIELD_U], (double*)ctx->f[MIST_FIELD_V], (double*)ctx->f[MIST_FIELD_W],
-           (double*)ctx->f[MIST_FIELD_UF], (double*)ctx->f[MIST_FIELD_VF], (double*)ctx->f[MIST_FIELD_WF]);
+           (double*)ctx->f[MIST_FIELD_UF], (double*)ctx->f[MIST_FIELD_VF], (double*)ctx->f[MIST_FIELD_WF],
+           (double*)nullptr);
+    return MIST_OK;
+}
+
+// defineVelocity! with the maxima of |u|, |v|, |w| reduced behind it into ctx->h_vmax (pinned);
+// ctx->vmax_ev marks their arrival.
+int launch_define_velocity_vmax(mist_ctx* ctx, double t) {
+    const int64_t n = (int64_t)ctx->m.cnx * ctx->m.cny * ctx->m.cnz;
+    const int g = std::min(ctx->red_blocks, grid_for(n, RED_THREADS));
+    LAUNCH(k_define_velocity, g, RED_THREADS, ctx->m, ctx->vt, (int)ctx->p.vf_velocity, ctx->p.vel_period, t,
+           (double*)ctx->f[MIST_FIELD_U], (double*)ctx->f[MIST_FIELD_V], (double*)ctx->f[MIST_FIELD_W],
+           (double*)ctx->f[MIST_FIELD_UF], (double*)ctx->f[MIST_FIELD_VF], (double*)ctx->f[MIST_FIELD_WF], ctx->red);
+    LAUNCH(k_reduce_partials, 3, RED_THREADS, ctx->red, g, 3, (int)MIST_REDUCE_MAX, ctx->red + 3 * ctx->red_blocks);
+    MIST_HIP_TRY(hipMemcpyAsync(ctx->h_vmax, ctx->red + 3 * ctx->red_blocks, 3 * sizeof(double),
+                                hipMemcpyDeviceToHost, ctx->stream));
+    MIST_HIP_TRY(hipEventRecord(ctx->vmax_ev, ctx->stream));
     return MIST_OK;
 }
 

@@ -242,6 +242,32 @@ def test_deformation_steps(mist, oracle, case):
     _vof_state_equal(g, o, g.mesh)
 
 
+def test_deferred_vof_steps_bit_identical(mist):
+    """mist_vof_step under deferred checks takes dt from the velocity maxima the previous step
+    reduced behind its defineVelocity! (waiting on an event, not on the whole stream), and falls
+    back to the synchronising reduction when another call came in between. Both give the same
+    t, dt and VF bit for bit as the plain steps."""
+    param, ic = W.deformation3d(32)
+    runs = []
+    for deferred in (False, True):
+        c = mist.Context(param)
+        c.init_vof(*ic)
+        c.vof_init()
+        c.set_deferred_checks(deferred)
+        t, seq = 0.0, []
+        for s in range(10):
+            if s == 6:
+                seq.append(c.reduce("sum", "VF"))  # a call in between: the next dt is reduced again
+            t, dt = c.vof_step(t, param.tFinal)
+            seq.append((t, dt))
+        c.set_deferred_checks(False)
+        runs.append((seq, c.get("VF"), c.get("u")))
+        c.close()
+    (s0, vf0, u0), (s1, vf1, u1) = runs
+    assert s0 == s1
+    assert np.array_equal(vf0, vf1) and np.array_equal(u0, u1)
+
+
 @pytest.mark.parametrize("variant", ["Euler", "Midpoint", "Zalesak"])
 def test_deformation_backtrace_variants(mist, oracle, variant):
     """The other back-traces of project! (src/Tools.jl:392-418): explicit Euler, the midpoint
