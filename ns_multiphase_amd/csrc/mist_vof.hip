@@ -74,10 +74,8 @@ __device__ __forceinline__ void vel_at(int kind, double T, double t, double ct, 
     }
 }
 
-__device__ double block_reduce(int kind, double v);
-
-// With `part`, each block also writes the maxima of |u|, |v|, |w| over its cells to
-// part[3 blockIdx + c] (the partials k_maxabs3 would give: a maximum does not depend on order).
+// With `part` (blocks of 256 threads), each block also writes the maxima of |u|, |v|, |w| over
+// its cells to part[3 blockIdx + c] (what k_maxabs3 gives: a maximum does not depend on order).
 __global__ __launch_bounds__(256) void k_define_velocity(DevMesh m, VelTab V, int kind, double T, double t, double* u,
                                                          double* v, double* w, double* uf, double* vf, double* wf,
                                                          double* part) {
@@ -98,10 +96,16 @@ __global__ __launch_bounds__(256) void k_define_velocity(DevMesh m, VelTab V, in
         vel_at<1, 1, 0>(kind, T, t, ct, m, V, i, j, k, a, b, c);
         wf[q] = c;
     }
-    if (part) {
-        for (int c = 0; c < 3; ++c) {
-            const double bm = block_reduce(MIST_REDUCE_MAX, r[c]);
-            if (threadIdx.x == 0) part[3 * blockIdx.x + c] = bm;
+    if (part) {  // wave maxima by shuffles, then the block's four waves
+        __shared__ double s_w[4][3];
+        for (int c = 0; c < 3; ++c)
+            for (int off = 32; off >= 1; off >>= 1) r[c] = jmax(r[c], __shfl_xor(r[c], off));
+        if ((threadIdx.x & 63) == 0)
+            for (int c = 0; c < 3; ++c) s_w[threadIdx.x >> 6][c] = r[c];
+        __syncthreads();
+        if (threadIdx.x < 3) {
+            const int c = threadIdx.x;
+            part[3 * blockIdx.x + c] = jmax(jmax(s_w[0][c], s_w[1][c]), jmax(s_w[2][c], s_w[3][c]));
         }
     }
 }
@@ -238,32 +242,6 @@ __global__ void k_borders_axis(T* A, Ext3 e, int d, int mn, int mx, int mno, int
 // ------------------------------------------------------------------------------------
 // computeBand! (src/VF.jl:16-74)
 // ------------------------------------------------------------------------------------
-__global__ void k_band_classify(DevMesh m, DevParams p, const double* VF, int16_t* band) {
-    const int n = m.Nx * m.Ny * m.Nz;
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-        int i, j, k;
-        interior_of(m, q, i, j, k);
-        const double v = VF[m.vof(i, j, k)];
-        int b;
-        if (v < p.VFlo) {
-            b = -p.nband - 1;
-            for (int kk = k - 1; kk <= k + 1; ++kk)
-                for (int jj = j - 1; jj <= j + 1; ++jj)
-                    for (int ii = i - 1; ii <= i + 1; ++ii)
-                        if (VF[m.vof(ii, jj, kk)] >= p.VFhi) b = 0;
-        } else if (v > p.VFhi) {
-            b = p.nband + 1;
-            for (int kk = k - 1; kk <= k + 1; ++kk)
-                for (int jj = j - 1; jj <= j + 1; ++jj)
-                    for (int ii = i - 1; ii <= i + 1; ++ii)
-                        if (VF[m.vof(ii, jj, kk)] <= p.VFlo) b = 0;
-        } else {
-            b = 0;
-        }
-        band[m.cell(i, j, k)] = (int16_t)b;
-    }
-}
-
 // One sweep of the band growth. In place: a sweep only lowers |band| from >n to n and only
 // compares neighbours with n-1, so concurrent updates do not change any outcome.
 __global__ void k_band_sweep(DevMesh m, int nstep, int16_t* band) {
@@ -283,6 +261,7 @@ __global__ void k_band_sweep(DevMesh m, int nstep, int16_t* band) {
     }
 }
 
+// Cells with |band| <= nband (interior linear index) appended to `list`.
 __global__ void k_band_list(DevMesh m, int nband, const int16_t* band, int* list, int* count) {
     __shared__ int s_app[2];
     const int n = m.Nx * m.Ny * m.Nz;
@@ -296,6 +275,59 @@ __global__ void k_band_list(DevMesh m, int nband, const int16_t* band, int* list
         }
         append_group(f, q, list, count, s_app);
     }
+}
+
+// Classification: a cell takes band 0 when it is an interface cell or one of its 27 neighbours
+// lies in the other phase, +-(nband+1) otherwise. One work-group of 256 threads per tile of
+// interior cells (8x8x4, 16x8x2 or 16x16x1 by the mesh's depth), the tile's VF and a one-cell
+// halo staged in LDS, so a cell's 27 neighbours come from LDS. (The same staging of the int16
+// band values made the sweep slower than its 27 coalesced loads per wave: 32 against 15 us.)
+struct BandTile {
+    int tx, ty, tz;  // tile extents
+    int nbx, nby;    // tiles along x and y
+};
+constexpr int BAND_TILE_LDS = 1024;  // >= (tx + 2)(ty + 2)(tz + 2) of every tile shape
+
+__device__ __forceinline__ void band_tile_origin(const BandTile& T, int& i0, int& j0, int& k0) {
+    const int b = blockIdx.x;
+    i0 = 1 + (b % T.nbx) * T.tx;
+    j0 = 1 + ((b / T.nbx) % T.nby) * T.ty;
+    k0 = 1 + (b / (T.nbx * T.nby)) * T.tz;
+}
+
+__global__ __launch_bounds__(256) void k_band_classify_t(DevMesh m, DevParams p, BandTile T, const double* VF,
+                                                         int16_t* band) {
+    __shared__ double s[BAND_TILE_LDS];
+    int i0, j0, k0;
+    band_tile_origin(T, i0, j0, k0);
+    const int hx = T.tx + 2, hy = T.ty + 2, nh = hx * hy * (T.tz + 2);
+    for (int e = threadIdx.x; e < nh; e += blockDim.x) {
+        const int i = i0 - 1 + e % hx, j = j0 - 1 + (e / hx) % hy, k = k0 - 1 + e / (hx * hy);
+        if (i <= m.Nx + 1 && j <= m.Ny + 1 && k <= m.Nz + 1) s[e] = VF[m.vof(i, j, k)];
+    }
+    __syncthreads();
+    const int a = threadIdx.x % T.tx, b = (threadIdx.x / T.tx) % T.ty, c = threadIdx.x / (T.tx * T.ty);
+    const int i = i0 + a, j = j0 + b, k = k0 + c;
+    if (c >= T.tz || i > m.Nx || j > m.Ny || k > m.Nz) return;
+    const double* o = s + (a + 1) + hx * ((b + 1) + hy * (c + 1));
+    const double v = o[0];
+    int bb;
+    if (v < p.VFlo) {
+        bb = -p.nband - 1;
+        for (int dk = -1; dk <= 1; ++dk)
+            for (int dj = -1; dj <= 1; ++dj)
+                for (int di = -1; di <= 1; ++di)
+                    if (o[di + hx * (dj + hy * dk)] >= p.VFhi) bb = 0;
+    } else if (v > p.VFhi) {
+        bb = p.nband + 1;
+        for (int dk = -1; dk <= 1; ++dk)
+            for (int dj = -1; dj <= 1; ++dj)
+                for (int di = -1; di <= 1; ++di)
+                    if (o[di + hx * (dj + hy * dk)] <= p.VFlo) bb = 0;
+    } else {
+        bb = 0;
+    }
+    band[m.cell(i, j, k)] = (int16_t)bb;
 }
 
 __global__ void k_fill_int(int* a, int n, int v) {
@@ -1474,18 +1506,25 @@ int launch_borders(mist_ctx* ctx, int field) {
 }
 
 int launch_band(mist_ctx* ctx) {
-    const int n = (int)(ctx->m.Nx * ctx->m.Ny * ctx->m.Nz);
+    const DevMesh& m = ctx->m;
+    const int n = (int)(m.Nx * m.Ny * m.Nz);
     int16_t* band = (int16_t*)ctx->f[MIST_FIELD_BAND];
-    LAUNCH(k_band_classify, grid_for(n, 256), 256, ctx->m, ctx->dp, (const double*)ctx->f[MIST_FIELD_VOF], band);
+    BandTile T{8, 8, 4, 0, 0};
+    if (m.Nz < 2) T = BandTile{16, 16, 1, 0, 0};
+    else if (m.Nz < 4) T = BandTile{16, 8, 2, 0, 0};
+    T.nbx = (m.Nx + T.tx - 1) / T.tx;
+    T.nby = (m.Ny + T.ty - 1) / T.ty;
+    const int64_t tiles = (int64_t)T.nbx * T.nby * ((m.Nz + T.tz - 1) / T.tz);
+    LAUNCH(k_fill_int, 1, 64, ctx->counters, 1, 0);
+    LAUNCH(k_band_classify_t, (int)tiles, 256, m, ctx->dp, T, (const double*)ctx->f[MIST_FIELD_VOF], band);
     int st = launch_neumann(ctx, MIST_FIELD_BAND);
     if (st) return st;
     if ((st = launch_borders(ctx, MIST_FIELD_BAND))) return st;
     for (int s = 1; s <= ctx->dp.nband; ++s) {
-        LAUNCH(k_band_sweep, grid_for(n, 256), 256, ctx->m, s, band);
+        LAUNCH(k_band_sweep, grid_for(n, 256), 256, m, s, band);
         if ((st = launch_borders(ctx, MIST_FIELD_BAND))) return st;
     }
-    LAUNCH(k_fill_int, 1, 64, ctx->counters, 1, 0);
-    LAUNCH(k_band_list, grid_for(n, 256), 256, ctx->m, ctx->dp.nband, (const int16_t*)band, ctx->band_list,
+    LAUNCH(k_band_list, grid_for(n, 256), 256, m, ctx->dp.nband, (const int16_t*)band, ctx->band_list,
            ctx->counters + 0);
     return MIST_OK;
 }
