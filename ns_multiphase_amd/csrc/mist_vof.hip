@@ -1539,7 +1539,12 @@ int launch_normals(mist_ctx* ctx) {
            ctx->iface_list, ctx->counters + 1);
     const int fam = (ctx->m.Nz > 1 ? 1 : 0) | (ctx->m.Ny > 1 ? 2 : 0) | (ctx->m.Nx > 1 ? 4 : 0);
     const int64_t ncap = (int64_t)ctx->m.Nx * ctx->m.Ny * ctx->m.Nz;
-    if (!ctx->elv_dist) MIST_HIP_TRY(hipMalloc(&ctx->elv_dist, (size_t)ncap * 27 * sizeof(double)));
+    if (!ctx->elv_dist && hipMalloc(&ctx->elv_dist, (size_t)ncap * 27 * sizeof(double)) != hipSuccess) {
+        (void)hipGetLastError();
+        ctx->elv_dist = nullptr;
+        ctx->err = "ELVIRA plane constants: device allocation failed";
+        return MIST_ERR_OOM;
+    }
     hipEvent_t ev = prof_begin(ctx, MIST_PROF_ELVIRA);  // both kernels of ELVIRA
     LAUNCH(k_elvira_dist, grid_for(ncap * 27, 256, 8192), 256, ctx->m, ctx->dp, (const double*)ctx->f[MIST_FIELD_VOF],
            (const int*)ctx->iface_list, (const int*)(ctx->counters + 1), fam, ctx->elv_dist);
