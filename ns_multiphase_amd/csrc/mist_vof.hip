@@ -476,6 +476,9 @@ __global__ __launch_bounds__(ELV_THREADS, 4) void k_elvira(DevMesh m, DevParams 
     __shared__ double sE[ELV_PAIRS];
     __shared__ double sErr[27];
     __shared__ unsigned char sNLiq[16];  // liquid sub-tets per case: ntets - nntet + 1
+    // faces x[i-1..i+2], y[j-1..j+2], z[k-1..k+2] and centres xm[i-1..i+1], ym, zm of the cell's
+    // neighbourhood: the pair and cut steps read them from LDS
+    __shared__ double sF[3][4], sM[3][3];
     const int tid = threadIdx.x;
     const int ncell = *count;
     const VolScale<POW2> vscale((m.dx * m.dy) * m.dz);
@@ -487,6 +490,16 @@ __global__ __launch_bounds__(ELV_THREADS, 4) void k_elvira(DevMesh m, DevParams 
             const int ii = i - 1 + tid % 3, jj = j - 1 + (tid / 3) % 3, kk = k - 1 + tid / 9;
             sVF[tid] = VF[m.vof(ii, jj, kk)];
             sErr[tid] = 0.0;
+        }
+        if (tid >= 160 && tid < 181) {
+            const int e = tid - 160;
+            if (e < 12) {
+                const int a = e / 4, o = e % 4;
+                sF[a][o] = (a == 0 ? m.x[i - 1 + o] : (a == 1 ? m.y[j - 1 + o] : m.z[k - 1 + o]));
+            } else {
+                const int a = (e - 12) / 3, o = (e - 12) % 3;
+                sM[a][o] = (a == 0 ? m.xm[i - 1 + o] : (a == 1 ? m.ym[j - 1 + o] : m.zm[k - 1 + o]));
+            }
         }
         if (tid < 27 * 5) {
             const int nb = tid / 5, t = tid % 5;
@@ -523,12 +536,13 @@ __global__ __launch_bounds__(ELV_THREADS, 4) void k_elvira(DevMesh m, DevParams 
                 int kind = 1;
                 if (sOk[c]) {
                     const double cnx = sC[c][0], cny = sC[c][1], cnz = sC[c][2], dist = sC[c][3];
-                    const int ii = i - 1 + nb % 3, jj = j - 1 + (nb / 3) % 3, kk = k - 1 + nb / 9;
+                    const int oi = nb % 3, oj = (nb / 3) % 3, ol = nb / 9;
+                    const int ii = i - 1 + oi, jj = j - 1 + oj, kk = k - 1 + ol;
                     const double hx = fabs(cnx) * m.dx, hy = fabs(cny) * m.dy, hz = fabs(cnz) * m.dz;
-                    const double dcen = ((cnx * m.xm[ii] + cny * m.ym[jj]) + cnz * m.zm[kk]) - dist;
+                    const double xc = sM[0][oi], yc = sM[1][oj], zc = sM[2][ol];
+                    const double dcen = ((cnx * xc + cny * yc) + cnz * zc) - dist;
                     const double hext = 0.5 * ((hx + hy) + hz);
-                    const double margin =
-                        1e-9 * (((1.0 + fabs(dist)) + fabs(m.xm[ii])) + (fabs(m.ym[jj]) + fabs(m.zm[kk])));
+                    const double margin = 1e-9 * (((1.0 + fabs(dist)) + fabs(xc)) + (fabs(yc) + fabs(zc)));
                     if (dcen - hext > margin) {
                         kind = 1;
                     } else if (dcen + hext < -margin) {
@@ -539,9 +553,9 @@ __global__ __launch_bounds__(ELV_THREADS, 4) void k_elvira(DevMesh m, DevParams 
                         // d of the cell's 8 reference vertices, each the expression the tets'
                         // vertices evaluate ((nx x + ny y) + nz z) - dist with the same products
                         // (odd cells mirror x), then each tet's four taken from them
-                        const double px0 = cnx * m.x[ii], px1 = cnx * m.x[ii + 1];
-                        const double py0 = cny * m.y[jj], py1 = cny * m.y[jj + 1];
-                        const double pz0 = cnz * m.z[kk], pz1 = cnz * m.z[kk + 1];
+                        const double px0 = cnx * sF[0][oi], px1 = cnx * sF[0][oi + 1];
+                        const double py0 = cny * sF[1][oj], py1 = cny * sF[1][oj + 1];
+                        const double pz0 = cnz * sF[2][ol], pz1 = cnz * sF[2][ol + 1];
                         const double qx0 = even ? px0 : px1, qx1 = even ? px1 : px0;
                         double e8[8];
 #pragma unroll
@@ -586,11 +600,11 @@ __global__ __launch_bounds__(ELV_THREADS, 4) void k_elvira(DevMesh m, DevParams 
                 const int pr = item / 5, t = item - 5 * pr;
                 const int c = pr % 27, nb = 9 * plane + pr / 27;
                 const double cnx = sC[c][0], cny = sC[c][1], cnz = sC[c][2], dist = sC[c][3];
-                const int ii = i - 1 + nb % 3, jj = j - 1 + (nb / 3) % 3, kk = k - 1 + nb / 9;
-                const bool even = ((ii + jj + kk) % 2) == 0;
+                const int oi = nb % 3, oj = (nb / 3) % 3, ol = nb / 9;
+                const bool even = (((i - 1 + oi) + (j - 1 + oj) + (k - 1 + ol)) % 2) == 0;
                 const double ts = even ? 1.0 : -1.0;
                 V3 P[4];
-                tet5_vertices(t, even, m.x[ii], m.x[ii + 1], m.y[jj], m.y[jj + 1], m.z[kk], m.z[kk + 1], P);
+                tet5_vertices(t, even, sF[0][oi], sF[0][oi + 1], sF[1][oj], sF[1][oj + 1], sF[2][ol], sF[2][ol + 1], P);
                 double d[4];
 #pragma unroll
                 for (int n = 0; n < 4; ++n) d[n] = ((cnx * P[n].x + cny * P[n].y) + cnz * P[n].z) - dist;
