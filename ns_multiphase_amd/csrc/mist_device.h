@@ -369,20 +369,10 @@ __host__ __device__ __forceinline__ unsigned canon_case(int cs) {
     }
 }
 
-template <class FL, class FG>
-__host__ __device__ __forceinline__ void cut_canonical(int cs, const V3 (&P)[4], const double (&dm)[4], FL&& on_liq,
-                                                       FG&& on_gas) {
-    if (cs == 0 || cs == 15) {  // one piece, the leaf itself
-        const double tv = tet_vol3(P[0], P[1], P[2], P[3]);
-        on_liq(cs == 0 ? tv : 0.0);
-        on_gas(cs == 0 ? 0.0 : tv);
-        return;
-    }
-    const unsigned c = canon_case(cs);
-    V3 Q[4];
-    double e[4];
-    // the renumbering as register copies per case (an index taken from the table would make the
-    // compiler load the vertices from a private array in scratch memory)
+// The renumbering of canon_case as register copies per case (an index taken from the table
+// would make the compiler load the vertices from a private array in scratch memory).
+__host__ __device__ __forceinline__ void canon_perm(int cs, const V3 (&P)[4], const double (&dm)[4], V3 (&Q)[4],
+                                                    double (&e)[4]) {
     auto perm = [&](auto A, auto B, auto C, auto D) {
         constexpr int a = decltype(A)::value, b = decltype(B)::value, cc = decltype(C)::value,
                       d = decltype(D)::value;
@@ -400,6 +390,21 @@ __host__ __device__ __forceinline__ void cut_canonical(int cs, const V3 (&P)[4],
         case 12: perm(IC<2>{}, IC<3>{}, IC<0>{}, IC<1>{}); break;
         default: perm(IC<0>{}, IC<1>{}, IC<2>{}, IC<3>{}); break;
     }
+}
+
+template <class FL, class FG>
+__host__ __device__ __forceinline__ void cut_canonical(int cs, const V3 (&P)[4], const double (&dm)[4], FL&& on_liq,
+                                                       FG&& on_gas) {
+    if (cs == 0 || cs == 15) {  // one piece, the leaf itself
+        const double tv = tet_vol3(P[0], P[1], P[2], P[3]);
+        on_liq(cs == 0 ? tv : 0.0);
+        on_gas(cs == 0 ? 0.0 : tv);
+        return;
+    }
+    const unsigned c = canon_case(cs);
+    V3 Q[4];
+    double e[4];
+    canon_perm(cs, P, dm, Q, e);
     double L = 0.0, G = 0.0;
     auto lq = [&](double v) { L += v; };
     auto gs = [&](double v) { G += v; };
@@ -416,6 +421,53 @@ __host__ __device__ __forceinline__ void cut_canonical(int cs, const V3 (&P)[4],
     }
     on_liq(L);
     on_gas(G);
+}
+
+// One side of canonical case CS (1 or 3): the liquid sub-tets (LIQ) or the gas ones, added in
+// the order cut_case hands them to on_liq / on_gas.
+template <int CS, bool LIQ>
+__host__ __device__ __forceinline__ double cut_side(const V3 (&P)[4], const double (&dm)[4]) {
+    constexpr int NV = CutTab::nvert[CS], NT = CutTab::ntets[CS], NN = CutTab::nntet[CS];
+    V3 Q[8];
+    Q[0] = P[0]; Q[1] = P[1]; Q[2] = P[2]; Q[3] = P[3];
+    sfor<0, NV>([&](auto I) {
+        constexpr int n = decltype(I)::value;
+        constexpr int v1 = CutTab::edge[CS][n] & 15, v2 = CutTab::edge[CS][n] >> 4;
+        const double mu = jmin(1.0, jmax(0.0, sdiv(-dm[v1], dm[v2] - dm[v1])));
+        Q[4 + n] = lerp3(Q[v1], Q[v2], mu);
+    });
+    double S = 0.0;
+    if constexpr (LIQ) {
+        sfor<0, NT - NN + 1>([&](auto I) {
+            constexpr int n = NT - decltype(I)::value;
+            S += tet_vol3(Q[vtet_of(CS, n - 1, 0)], Q[vtet_of(CS, n - 1, 1)], Q[vtet_of(CS, n - 1, 2)],
+                          Q[vtet_of(CS, n - 1, 3)]);
+        });
+    } else {
+        sfor<1, NN>([&](auto I) {
+            constexpr int n = decltype(I)::value;
+            S += tet_vol3(Q[vtet_of(CS, n - 1, 0)], Q[vtet_of(CS, n - 1, 1)], Q[vtet_of(CS, n - 1, 2)],
+                          Q[vtet_of(CS, n - 1, 3)]);
+        });
+    }
+    return S;
+}
+
+// The liquid volume cut_canonical gives (its on_liq argument, bit for bit), computing only the
+// side it needs: case 1 or case 3 on the renumbered vertices, the gas side of case 1 when the
+// lone vertex lies below the plane.
+__host__ __device__ __forceinline__ double cut_canonical_liquid(int cs, const V3 (&P)[4], const double (&dm)[4]) {
+    if (cs == 0) return tet_vol3(P[0], P[1], P[2], P[3]);
+    if (cs == 15) return 0.0;
+    const unsigned c = canon_case(cs);
+    V3 Q[4];
+    double e[4];
+    canon_perm(cs, P, dm, Q, e);
+    double v;
+    if (c & (1u << 18)) v = cut_side<3, true>(Q, e);
+    else if (c & (1u << 17)) v = cut_side<1, false>(Q, e);
+    else v = cut_side<1, true>(Q, e);
+    return (c & (1u << 16)) ? -v : v;
 }
 
 // (ts * v) / cellvol, the reference's per-sub-tet term of computePLIC2VF (ts = +-1). When the

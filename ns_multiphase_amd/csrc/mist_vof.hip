@@ -451,11 +451,14 @@ __global__ __launch_bounds__(256) void k_elvira_dist(DevMesh m, DevParams p, con
 // time. (A) one thread per pair finds the d2case case of its five tets; a plane missing the
 // cell by a wide margin gives the exact all-gas / all-liquid value at once, and an all-liquid
 // tet's single sub-tet is its whole volume, computed once per neighbour. (B) the other tets
-// are bucketed by case, so (C) the threads that cut them run mostly the same case in a wave.
-// Every sub-tet term lands in its own slot; (D) one thread per pair adds the terms in the
-// reference's tet and sub-tet order, and one thread per candidate adds the squared errors in
-// neighbour order. The best candidate is the lowest-numbered one with the smallest finite
-// error, which is what the reference's strict `<` scan keeps.
+// are bucketed by case, in the order of the three bodies below, so (C) the threads that cut
+// them mostly run one body per wave: a tet's liquid volume is that of the reference's sub-tets
+// on vertices renumbered onto case 1 or case 3 (cut_canonical_liquid; each side needs one
+// body), equal to the reference's fourteen-case sum to rounding. (D) one thread per pair adds
+// the five tets' terms in the reference's tet order, and one thread per candidate adds the
+// squared errors in neighbour order. The best candidate is the lowest-numbered one with the
+// smallest finite error, which is what the reference's strict `<` scan keeps; candidates whose
+// errors tie to rounding may be picked differently from the reference (either is its result).
 constexpr int ELV_THREADS = 256;
 constexpr int ELV_PAIRS = 27 * 9;  // candidate-neighbour pairs of one z-plane of neighbours
 
@@ -468,21 +471,19 @@ __global__ __launch_bounds__(ELV_THREADS, 4) void k_elvira(DevMesh m, DevParams 
     __shared__ int sOk[27];
     __shared__ double sT[27][5];      // all-liquid term (ts * vol(tet)) / cellvol per neighbour and tet
     __shared__ double sFull[27];      // computePLIC2VF of an all-liquid neighbour
-    __shared__ double sTerm[ELV_PAIRS][5][3];
+    __shared__ double sTerm[ELV_PAIRS][5];
     __shared__ unsigned char sCase[ELV_PAIRS][5];
     __shared__ unsigned char sKind[ELV_PAIRS];  // 0: cut tets, 1: all gas, 2: all liquid
     __shared__ unsigned short sItem[ELV_PAIRS * 5];
     __shared__ int sCount[16], sBase[16];
     __shared__ double sE[ELV_PAIRS];
     __shared__ double sErr[27];
-    __shared__ unsigned char sNLiq[16];  // liquid sub-tets per case: ntets - nntet + 1
     // faces x[i-1..i+2], y[j-1..j+2], z[k-1..k+2] and centres xm[i-1..i+1], ym, zm of the cell's
     // neighbourhood: the pair and cut steps read them from LDS
     __shared__ double sF[3][4], sM[3][3];
     const int tid = threadIdx.x;
     const int ncell = *count;
     const VolScale<POW2> vscale((m.dx * m.dy) * m.dz);
-    if (tid < 16) sNLiq[tid] = (unsigned char)(CUT_NTETS[tid] - CUT_NNTET[tid] + 1);
     for (int cidx = blockIdx.x; cidx < ncell; cidx += gridDim.x) {
         int i, j, k;
         interior_of(m, list[cidx], i, j, k);
@@ -567,7 +568,7 @@ __global__ __launch_bounds__(ELV_THREADS, 4) void k_elvira(DevMesh m, DevParams 
                             double d[4] = {e8[tv & 15], e8[(tv >> 4) & 15], e8[(tv >> 8) & 15], e8[(tv >> 12) & 15]};
                             const int cs = d2case_snap(d);
                             sCase[tid][t] = (unsigned char)cs;
-                            if (cs == 0) sTerm[tid][t][0] = sT[nb][t];
+                            if (cs == 0) sTerm[tid][t] = sT[nb][t];
                             else if (cs != 15) atomicAdd(&sCount[cs], 1);
                         });
                     }
@@ -578,7 +579,11 @@ __global__ __launch_bounds__(ELV_THREADS, 4) void k_elvira(DevMesh m, DevParams 
             // (B) bucket the cut tets by case
             if (tid == 0) {
                 int b = 0;
-                for (int q = 0; q < 16; ++q) {
+                // buckets in the order of the three case bodies: three cut vertices with the
+                // lone vertex above the plane, with it below, four cut vertices
+                constexpr unsigned char ord[16] = {0, 1, 2, 4, 8, 14, 13, 11, 7, 3, 5, 6, 9, 10, 12, 15};
+                for (int r = 0; r < 16; ++r) {
+                    const int q = ord[r];
                     sBase[q] = b;
                     b += sCount[q];
                 }
@@ -609,9 +614,7 @@ __global__ __launch_bounds__(ELV_THREADS, 4) void k_elvira(DevMesh m, DevParams 
 #pragma unroll
                 for (int n = 0; n < 4; ++n) d[n] = ((cnx * P[n].x + cny * P[n].y) + cnz * P[n].z) - dist;
                 const int cs = d2case_snap(d);
-                int w = 0;
-                cut_dispatch<false>(
-                    cs, P, d, [&](double v) { sTerm[pr][t][w++] = vscale(ts, v); }, [](double) {});
+                sTerm[pr][t] = vscale(ts, cut_canonical_liquid(cs, P, d));
             }
             __syncthreads();
             // (D) computePLIC2VF of each pair in the reference's order, then the errors
@@ -622,11 +625,8 @@ __global__ __launch_bounds__(ELV_THREADS, 4) void k_elvira(DevMesh m, DevParams 
                 if (kind == 2) {
                     vf_ = sFull[nb];
                 } else if (kind == 0) {
-                    for (int t = 0; t < 5; ++t) {
-                        const int cs = sCase[tid][t];
-                        const int nt = sNLiq[cs];
-                        for (int w = 0; w < nt; ++w) vf_ += sTerm[tid][t][w];
-                    }
+                    for (int t = 0; t < 5; ++t)
+                        if (sCase[tid][t] != 15) vf_ += sTerm[tid][t];
                 }
                 const double df = sVF[nb] - vf_;
                 sE[tid] = df * df;

@@ -100,7 +100,8 @@ void h_leaf_adj(const double* x, const double* n, double D, double gL, double gV
 }
 
 // liquid and total volume of tet x cut by n.x = D: out[0..1] by cut_dispatch, out[2..3] by
-// cut_canonical, out[4] the snapped case
+// cut_canonical, out[4] the snapped case, out[5] the liquid volume by cut_canonical_liquid
+// (out holds 6 doubles)
 void h_cut_compare(const double* x, const double* n, double D, double* out) {
     V3 P[4];
     double d[4], dd[4];
@@ -114,6 +115,7 @@ void h_cut_compare(const double* x, const double* n, double D, double* out) {
     cut_dispatch<true>(cs, P, d, [&](double v) { L += v; }, [&](double v) { G += v; });
     cut_canonical(cs, P, d, [&](double v) { Lc += v; }, [&](double v) { Gc += v; });
     out[0] = L; out[1] = L + G; out[2] = Lc; out[3] = Lc + Gc; out[4] = cs;
+    out[5] = cut_canonical_liquid(cs, P, d);
 }
 
 // adjoint of gL * liquid + gV * total volume of the cut of h_cut_compare, with respect to

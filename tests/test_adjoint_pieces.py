@@ -177,7 +177,8 @@ def test_elvira_candidate_adjoint(H, cand):
 def test_canonical_leaf_cut_matches_reference_cases(H):
     """cut_canonical (two case bodies on renumbered vertices) against cut_dispatch (the
     reference's fourteen case bodies): liquid and total volumes agree to rounding for every cut
-    case, in both orientations of the tet."""
+    case, in both orientations of the tet. cut_canonical_liquid (one side of the canonical case,
+    what ELVIRA's scoring needs) gives cut_canonical's liquid volume bit for bit."""
     rng = np.random.default_rng(21)
     seen = set()
     for trial in range(3000):
@@ -187,8 +188,9 @@ def test_canonical_leaf_cut_matches_reference_cases(H):
         n = rng.standard_normal(3)
         n /= np.linalg.norm(n)
         D = float(n @ x[rng.integers(0, 4)] + 0.3 * rng.standard_normal() * 0.2)
-        out = np.zeros(5)
+        out = np.zeros(6)
         H.h_cut_compare(P(x), P(n), D, P(out))
+        assert out[5] == out[2], (int(out[4]), out)
         cs = int(out[4])
         seen.add(cs)
         # rounding of a determinant of unit-box coordinates: absolute, not relative to a sliver's volume
@@ -215,16 +217,16 @@ def test_canonical_leaf_cut_adjoint(H):
         gL, gV = rng.standard_normal(2)
         out = np.zeros(32)
         H.h_cut_adj_compare(P(x), P(n), D, gL, gV, P(out))
-        f5 = np.zeros(5)
-        H.h_cut_compare(P(x), P(n), D, P(f5))
-        cs = int(f5[4])
+        f6 = np.zeros(6)
+        H.h_cut_compare(P(x), P(n), D, P(f6))
+        cs = int(f6[4])
         seen.add(cs)
         scale = np.abs(out[:16]).max() + 1e-12
         assert np.abs(out[16:] - out[:16]).max() <= 1e-12 * scale, (cs, out[:16], out[16:])
         d = x @ n - D
         if fd_checked < 300 and np.abs(d).min() > 1e-3:
             def F(z):
-                o = np.zeros(5)
+                o = np.zeros(6)
                 H.h_cut_compare(P(z[:12]), P(z[12:15]), float(z[15]), P(o))
                 return gL * o[2] + gV * o[3]
             z = np.concatenate([x.ravel(), n, [D]])
