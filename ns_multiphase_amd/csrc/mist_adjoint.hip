@@ -619,13 +619,8 @@ __global__ __launch_bounds__(C::NT) void k_adj_tree(DevMesh m, const double* pv,
                         dd[v] = d[v];
                     }
                     const int cs = d2case_snap(dd);
-                    if (cs == 0 || cs == 15) {
-                        const double tv = tet_vol3(T.Q[0], T.Q[1], T.Q[2], T.Q[3]);
-                        if (cs == 0) on_liq(tv);
-                        else on_gas(tv);
-                    } else {
-                        cut_canonical(cs, T.Q, d, on_liq, on_gas);
-                    }
+                    Vs += tet_vol3(T.Q[0], T.Q[1], T.Q[2], T.Q[3]);
+                    Ls += cut_canonical_liquid(cs, T.Q, d);
                 }
 #pragma unroll
                 for (int off = 32; off >= 1; off >>= 1) {
