@@ -421,10 +421,16 @@ def run_bubble_sl(args, torch):
             if not name.endswith("_bar"):
                 o.set(name, ctx.get(name))
         h0 = ctx.elvira_choice()
+    failed = None
     for step in range(args.sl_steps):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        t1, dt, cyc, res = ctx.ns_step(t, param.tol)
+        try:
+            t1, dt, cyc, res = ctx.ns_step(t, param.tol)
+        except solver.MistError as e:  # the FAS cycles stalled above tol (mg_max_cycles): reported, not fatal
+            failed = {"step": step, "error": str(e), "ms": 1e3 * (time.perf_counter() - t0)}
+            log(f"[bench] SL bubble {N}^3 step {step}: {e}")
+            break
         torch.cuda.synchronize()
         rows.append({"ms": 1e3 * (time.perf_counter() - t0), "fas_cycles": cyc, "final_resid": res})
         log(f"[bench] SL bubble {N}^3: {rows[-1]['ms']:.0f} ms, {cyc} FAS cycles, resid {res:.2e}")
@@ -453,8 +459,9 @@ def run_bubble_sl(args, torch):
     ctx.close()
     out = {"workload": f"rising bubble {N}^3 as above, semi-Lagrangian scheme, SL pressure by FAS "
                        f"(2 levels, res_iteration smoother) to 1e-10", "steps": rows,
-           "ms_per_step": sum(r["ms"] for r in rows) / len(rows)}
-    if o is not None:
+           "ms_per_step": sum(r["ms"] for r in rows) / len(rows) if rows else None,
+           "not_converged": failed}
+    if o is not None and rows:
         out["step0_vs_oracle"] = cmp
         out["speedup_vs_cpu_step0"] = cmp["oracle_ms"] / rows[0]["ms"]
         o.close()
@@ -764,6 +771,16 @@ def run_large(args, torch):
     return out
 
 
+def guarded(name, fn):
+    """A section of the line: its result, or its error (the other sections and the headline still
+    print)."""
+    try:
+        return fn()
+    except Exception as e:  # noqa: BLE001 - reported in the JSON line
+        log(f"[bench] {name} failed: {e!r}")
+        return {"error": repr(e)}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -825,73 +842,85 @@ def main():
 
     mg = None
     if not args.no_mg:
-        log(f"[bench] multigrid MMS N={args.mg_N}")
-        mg_raw = run_mg(args, torch)
-        mg = {}
-        for prec, r in mg_raw.items():
-            rb_ms, rb_n, rb_units = r["prof"]["mg_rbgs"]
-            s = 8 if prec == "fp64" else 4
-            entry = {"solve_ms": r["ms"], "cycles": r["cycles"], "final_resid": r["resid"],
-                     "linf_err_vs_exact": r["linf_err"]}
-            if rb_n:
-                avg = rb_ms / rb_n / 1e3
-                bytes_launch = 6.0 * s * rb_units / rb_n  # p read + write, b, 3 face coefficients
-                entry["rbgs_fine"] = {"avg_launch_us": 1e6 * avg, "achieved_GBs": bytes_launch / avg / 1e9,
-                                      "frac_hbm": bytes_launch / avg / 1e9 / PEAK_HBM_GBS,
-                                      "bytes_per_launch": bytes_launch}
-            mg[prec] = entry
-        if not args.no_oracle:
-            P_ora, oinfo = oracle_mms(args.mg_N, 1e-3)
+        def _mg_section():
+            log(f"[bench] multigrid MMS N={args.mg_N}")
+            mg_raw = run_mg(args, torch)
+            mg = {}
+            for prec, r in mg_raw.items():
+                rb_ms, rb_n, rb_units = r["prof"]["mg_rbgs"]
+                s = 8 if prec == "fp64" else 4
+                entry = {"solve_ms": r["ms"], "cycles": r["cycles"], "final_resid": r["resid"],
+                         "linf_err_vs_exact": r["linf_err"]}
+                if rb_n:
+                    avg = rb_ms / rb_n / 1e3
+                    bytes_launch = 6.0 * s * rb_units / rb_n  # p read + write, b, 3 face coefficients
+                    entry["rbgs_fine"] = {"avg_launch_us": 1e6 * avg, "achieved_GBs": bytes_launch / avg / 1e9,
+                                          "frac_hbm": bytes_launch / avg / 1e9 / PEAK_HBM_GBS,
+                                          "bytes_per_launch": bytes_launch}
+                mg[prec] = entry
+            if not args.no_oracle:
+                P_ora, oinfo = oracle_mms(args.mg_N, 1e-3)
+                for prec in ("fp64", "mixed"):
+                    mg[prec]["vs_oracle"] = mms_vs_oracle(mg_raw[prec]["P"], P_ora, oinfo)
+                log(f"[bench] mg vs oracle: max|dP| fp64 {mg['fp64']['vs_oracle']['max_abs_dP']:.2e}, "
+                    f"mixed {mg['mixed']['vs_oracle']['max_abs_dP']:.2e} (oracle {oinfo['seconds']:.1f} s)")
+            mg["speedup_mixed_vs_fp64"] = mg["fp64"]["solve_ms"] / mg["mixed"]["solve_ms"]
+            # measured copy bandwidths beside the HBM peak: 2 x 64 MiB stays in the 256 MB Infinity
+            # Cache (where the 128^3 levels live), 2 x 2 GiB does not
+            mg["copy_GBs"] = {"cache_resident_2x64MiB": copy_bandwidth(torch, 64 << 20),
+                              "hbm_2x2GiB": copy_bandwidth(torch, 2 << 30)}
             for prec in ("fp64", "mixed"):
-                mg[prec]["vs_oracle"] = mms_vs_oracle(mg_raw[prec]["P"], P_ora, oinfo)
-            log(f"[bench] mg vs oracle: max|dP| fp64 {mg['fp64']['vs_oracle']['max_abs_dP']:.2e}, "
-                f"mixed {mg['mixed']['vs_oracle']['max_abs_dP']:.2e} (oracle {oinfo['seconds']:.1f} s)")
-        mg["speedup_mixed_vs_fp64"] = mg["fp64"]["solve_ms"] / mg["mixed"]["solve_ms"]
-        # measured copy bandwidths beside the HBM peak: 2 x 64 MiB stays in the 256 MB Infinity
-        # Cache (where the 128^3 levels live), 2 x 2 GiB does not
-        mg["copy_GBs"] = {"cache_resident_2x64MiB": copy_bandwidth(torch, 64 << 20),
-                          "hbm_2x2GiB": copy_bandwidth(torch, 2 << 30)}
-        for prec in ("fp64", "mixed"):
-            if "rbgs_fine" in mg[prec]:
-                rf = mg[prec]["rbgs_fine"]
-                rf["frac_cache_copy"] = rf["achieved_GBs"] / mg["copy_GBs"]["cache_resident_2x64MiB"]
-        log(f"[bench] mg fp64 {mg['fp64']['solve_ms']:.2f} ms, mixed {mg['mixed']['solve_ms']:.2f} ms")
+                if "rbgs_fine" in mg[prec]:
+                    rf = mg[prec]["rbgs_fine"]
+                    rf["frac_cache_copy"] = rf["achieved_GBs"] / mg["copy_GBs"]["cache_resident_2x64MiB"]
+            log(f"[bench] mg fp64 {mg['fp64']['solve_ms']:.2f} ms, mixed {mg['mixed']['solve_ms']:.2f} ms")
+            return mg
+
+        mg = guarded("mg", _mg_section)
 
     bubble = None
     if not args.no_bubble:
-        log(f"[bench] rising bubble N={args.bubble_N}, {args.bubble_steps} steps")
-        bparam, bic, bubble = run_bubble(args, torch)
-        if not args.no_cpu_baseline:
-            bubble["cpu_baseline"], bubble["per_step_vs_oracle"] = cpu_bubble(args, bparam, bic)
-            bubble["speedup_vs_cpu"] = {prec: bubble["cpu_baseline"]["ms_per_step"] / bubble[prec]["ms_per_step"]
-                                        for prec in ("mixed", "fp64")}
-        bubble["workload"] = (f"rising bubble {args.bubble_N}^3, r=0.125 at (0.5,0.5,0.3), rho 1000/1, "
-                              f"mu 1e-3/1e-5, sigma 0.07, g=-9.81 z, no-slip walls, dt=1e-3, CFL 0.5, "
-                              f"finite-difference scheme with correction tets, MG tol 1e-10")
+        def _bubble_section():
+            log(f"[bench] rising bubble N={args.bubble_N}, {args.bubble_steps} steps")
+            bparam, bic, bubble = run_bubble(args, torch)
+            if not args.no_cpu_baseline:
+                bubble["cpu_baseline"], bubble["per_step_vs_oracle"] = cpu_bubble(args, bparam, bic)
+                bubble["speedup_vs_cpu"] = {prec: bubble["cpu_baseline"]["ms_per_step"] / bubble[prec]["ms_per_step"]
+                                            for prec in ("mixed", "fp64")}
+            bubble["workload"] = (f"rising bubble {args.bubble_N}^3, r=0.125 at (0.5,0.5,0.3), rho 1000/1, "
+                                  f"mu 1e-3/1e-5, sigma 0.07, g=-9.81 z, no-slip walls, dt=1e-3, CFL 0.5, "
+                                  f"finite-difference scheme with correction tets, MG tol 1e-10")
+            return bubble
+
+        bubble = guarded("rising_bubble", _bubble_section)
 
     bubble_sl = None
     if not args.no_bubble and args.sl_steps > 0:
-        bubble_sl = run_bubble_sl(args, torch)
+        bubble_sl = guarded("rising_bubble_sl", lambda: run_bubble_sl(args, torch))
 
     config1 = None
     if not args.no_cpu_baseline and not args.no_oracle:
         log("[bench] config 1: 2D deformation 128^2 (oracle and device)")
-        config1 = run_config1(args, torch)
+        config1 = guarded("deformation2d_128", lambda: run_config1(args, torch))
 
     adjoint = None
     if not args.no_adjoint:
         log(f"[bench] adjoint {args.adj_N}^2")
-        adjoint = run_adjoint(args, torch)
+        adjoint = guarded("adjoint", lambda: run_adjoint(args, torch))
     large = None
     if not args.no_large:
         log(f"[bench] large {args.large_N}^3")
-        large = run_large(args, torch)
+        large = guarded("large", lambda: run_large(args, torch))
 
     cpu = accuracy = None
     roofs = {}
     if info is not None and not args.no_oracle:
         log("[bench] oracle on the sampled states (cpu baseline, per-step error, operation counts)")
-        cpu, accuracy, fc = cpu_baseline(args, param, ic, info, time_cpu=not args.no_cpu_baseline)
+        try:
+            cpu, accuracy, fc = cpu_baseline(args, param, ic, info, time_cpu=not args.no_cpu_baseline)
+        except Exception as e:  # noqa: BLE001 - reported in the JSON line; no rooflines then
+            log(f"[bench] cpu_baseline failed: {e!r}")
+            cpu, accuracy = {"error": repr(e)}, None
     # rooflines of the two FP64 kernels of the step; the dominant one by time is `roofline`
     for key, kern, per, unit_name in (("elvira", "k_elvira_dist + k_elvira", fc["elvira_per_interface_cell"],
                                        "interface_cells"),

@@ -195,7 +195,7 @@ int mist_ctx_destroy(mist_ctx* ctx) {
         if (ctx->f[f]) (void)hipFree(ctx->f[f]);
     void* bufs[] = {ctx->geom, ctx->trig, ctx->pv, ctx->band_list, ctx->iface_list, ctx->defer_list, ctx->cut_list,
                     ctx->counters, ctx->red, ctx->ns_sums, ctx->elv_best, ctx->adj_lam, ctx->adj_plane, ctx->adj_pv,
-                    ctx->adj_sums, ctx->elv_dist, ctx->vtx_list};
+                    ctx->adj_sums, ctx->elv_dist, ctx->vtx_list, ctx->elv_ties};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (double* b : ctx->ckpt)

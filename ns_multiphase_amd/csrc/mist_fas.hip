@@ -24,6 +24,7 @@
 //   * the Jacobian's columns are independent: each cell's two perturbed evaluations start from
 //     the unperturbed P (the reference adds and subtracts delta in place); one thread per cell
 //     evaluates both at that cell.
+#include <cstdio>
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -890,7 +891,10 @@ int pressure_solve_sl(mist_ctx* ctx, double dt, double tol, int64_t* cycles, dou
     if (resid) *resid = res;
     if (smooth_iters) *smooth_iters = S->h_ctrl->total;
     if (!conv) {
-        ctx->err = "P-solve did not converge";
+        char buf[160];
+        std::snprintf(buf, sizeof(buf), "P-solve did not converge: max|A(P)| = %.3e after %lld cycles (tol %.1e)", res,
+                      (long long)it, tol);
+        ctx->err = buf;
         return MIST_ERR_NOCONV;
     }
     return MIST_OK;

@@ -63,10 +63,11 @@ struct mist_ctx {
     int* defer_list = nullptr;   // cells for the general cut walk
     void* cut_list = nullptr;    // CutCell entries of the band cells to cut
     int* vtx_list = nullptr;     // vertices of band cells to back-trace (linear vertex index)
+    int* elv_ties = nullptr;     // ELVIRA's near-tied cells: (interface-list index, candidate mask)
     int* counters = nullptr;     // [0] band cells, [1] interface cells, [2] overflow flag, [4] cells
                                  // deferred by the main cut pass, [5] its work queue, [6] cells for
                                  // the general walk, [7] the wide pass's work queue, [8] cells to
-                                 // cut, [9] vertices to back-trace
+                                 // cut, [9] vertices to back-trace, [10] ELVIRA's near-tied cells
     double* red = nullptr;       // reduction partials
     int red_blocks = 0;
     int* h_counters = nullptr;   // pinned

@@ -465,7 +465,8 @@ constexpr int ELV_PAIRS = 27 * 9;  // candidate-neighbour pairs of one z-plane o
 template <bool POW2>
 __global__ __launch_bounds__(ELV_THREADS, 4) void k_elvira(DevMesh m, DevParams p, const double* VF, const int* list,
                                                         const int* count, int fam_mask, const double* dist,
-                                                        double* nxo, double* nyo, double* nzo, int* best) {
+                                                        double* nxo, double* nyo, double* nzo, int* best, int* ties,
+                                                        int* ntie) {
     __shared__ double sVF[27];
     __shared__ double sC[27][4];
     __shared__ int sOk[27];
@@ -657,12 +658,83 @@ __global__ __launch_bounds__(ELV_THREADS, 4) void k_elvira(DevMesh m, DevParams 
                               sC[lane][1] == sC[bl][1] && sC[lane][2] == sC[bl][2];
             const unsigned long long tmask = __ballot(same);
             const int tie = tmask ? __ffsll((long long)tmask) - 1 : -1;
+            // candidates whose errors lie within the rounding of the cut volumes of the minimum:
+            // the reference's own arithmetic decides between them (k_elvira_ties)
+            const double slack = 1e-11 * be + 1e-15 * sqrt(be) + 1e-30;
+            const unsigned long long fmask = __ballot(bl < 64 && lane < 27 && sOk[lane] && err <= be + slack);
             if (lane == 0) {
                 if (bl < 64) {
                     const int64_t c = m.cell(i, j, k);
                     nxo[c] = sC[bl][0]; nyo[c] = sC[bl][1]; nzo[c] = sC[bl][2];
                 }
                 best[list[cidx]] = bl < 64 ? (bl | ((tie + 1) << 8)) : -1;
+                if (__popcll(fmask) > 1) {
+                    const int slot = atomicAdd(ntie, 1);
+                    ties[2 * slot] = cidx;
+                    ties[2 * slot + 1] = (int)fmask;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// The cells k_elvira flagged, their near-tied candidates scored again in the reference's own
+// arithmetic: computePLIC2VF of each neighbour with the fourteen case bodies and the sub-tets
+// in the reference's order (plic2vf_reg), the squared errors added in neighbour order. Between
+// candidates whose errors are equal in exact arithmetic (mirrored candidates of a symmetric
+// interface) this keeps the one the reference keeps; the rest of the cell's result is
+// k_elvira's. One work-group of 64 threads per flagged cell, a thread per candidate.
+__global__ __launch_bounds__(64) void k_elvira_ties(DevMesh m, const double* VF, const int* list, int fam_mask,
+                                                    const double* dist, const int* ties, const int* ntie,
+                                                    double* nxo, double* nyo, double* nzo, int* best) {
+    __shared__ double sErr[27];
+    __shared__ double sN[27][3];
+    const double cellvol = (m.dx * m.dy) * m.dz;
+    const int n = *ntie;
+    for (int e = blockIdx.x; e < n; e += gridDim.x) {
+        const int cidx = ties[2 * e];
+        const unsigned fmask = (unsigned)ties[2 * e + 1];
+        int i, j, k;
+        interior_of(m, list[cidx], i, j, k);
+        const int c = threadIdx.x;
+        auto at = [&](int di, int dj, int dk) { return VF[m.vof(i + di, j + dj, k + dk)]; };
+        if (c < 27) {
+            double cnx = 0.0, cny = 0.0, cnz = 0.0;
+            if ((fam_mask >> (c / 9)) & 1) elvira_normal(m, i, j, k, c, at, cnx, cny, cnz);
+            sN[c][0] = cnx; sN[c][1] = cny; sN[c][2] = cnz;
+            double err = __builtin_huge_val();
+            if ((fmask >> c) & 1) {
+                const double dc = dist[27 * (int64_t)cidx + c];
+                double acc = 0.0;
+                for (int nb = 0; nb < 27; ++nb) {
+                    const int oi = nb % 3, oj = (nb / 3) % 3, ol = nb / 9;
+                    const int ii = i - 1 + oi, jj = j - 1 + oj, kk = k - 1 + ol;
+                    const bool even = ((ii + jj + kk) % 2) == 0;
+                    const double vf_ = plic2vf_reg(m.x[ii], m.x[ii + 1], m.y[jj], m.y[jj + 1], m.z[kk], m.z[kk + 1],
+                                                   cnx, cny, cnz, dc, even, cellvol);
+                    const double df = at(oi - 1, oj - 1, ol - 1) - vf_;
+                    acc += df * df;
+                }
+                err = acc;
+            }
+            sErr[c] = err;
+        }
+        __syncthreads();
+        if (c == 0) {
+            int bl = -1;
+            double be = __builtin_huge_val();
+            for (int q = 0; q < 27; ++q)  // the reference's strict < scan over the flagged candidates
+                if (((fmask >> q) & 1) && sErr[q] < be) { be = sErr[q]; bl = q; }
+            if (bl >= 0) {
+                int tie = -1;  // another candidate with the very same normal (k_elvira's rule)
+                for (int q = 0; q < 27 && tie < 0; ++q)
+                    if (q != bl && ((fam_mask >> (q / 9)) & 1) && sN[q][0] == sN[bl][0] && sN[q][1] == sN[bl][1] &&
+                        sN[q][2] == sN[bl][2])
+                        tie = q;
+                const int64_t cc = m.cell(i, j, k);
+                nxo[cc] = sN[bl][0]; nyo[cc] = sN[bl][1]; nzo[cc] = sN[bl][2];
+                best[list[cidx]] = bl | ((tie + 1) << 8);
             }
         }
         __syncthreads();
@@ -1558,6 +1630,12 @@ int launch_normals(mist_ctx* ctx) {
            ctx->iface_list, ctx->counters + 1);
     const int fam = (ctx->m.Nz > 1 ? 1 : 0) | (ctx->m.Ny > 1 ? 2 : 0) | (ctx->m.Nx > 1 ? 4 : 0);
     const int64_t ncap = (int64_t)ctx->m.Nx * ctx->m.Ny * ctx->m.Nz;
+    if (!ctx->elv_ties && hipMalloc(&ctx->elv_ties, (size_t)ncap * 2 * sizeof(int)) != hipSuccess) {
+        (void)hipGetLastError();
+        ctx->elv_ties = nullptr;
+        ctx->err = "ELVIRA tie list: device allocation failed";
+        return MIST_ERR_OOM;
+    }
     if (!ctx->elv_dist && hipMalloc(&ctx->elv_dist, (size_t)ncap * 27 * sizeof(double)) != hipSuccess) {
         (void)hipGetLastError();
         ctx->elv_dist = nullptr;
@@ -1567,14 +1645,18 @@ int launch_normals(mist_ctx* ctx) {
     hipEvent_t ev = prof_begin(ctx, MIST_PROF_ELVIRA);  // both kernels of ELVIRA
     LAUNCH(k_elvira_dist, grid_for(ncap * 27, 256, 8192), 256, ctx->m, ctx->dp, (const double*)ctx->f[MIST_FIELD_VOF],
            (const int*)ctx->iface_list, (const int*)(ctx->counters + 1), fam, ctx->elv_dist);
+    LAUNCH(k_fill_int, 1, 64, ctx->counters + 10, 1, 0);  // near-tied cells
     if (vol_is_pow2((ctx->m.dx * ctx->m.dy) * ctx->m.dz))
         LAUNCH(k_elvira<true>, grid_for(ncap, 1, 16384), ELV_THREADS, ctx->m, ctx->dp,
                (const double*)ctx->f[MIST_FIELD_VOF], (const int*)ctx->iface_list, (const int*)(ctx->counters + 1), fam,
-               (const double*)ctx->elv_dist, nx, ny, nz, ctx->elv_best);
+               (const double*)ctx->elv_dist, nx, ny, nz, ctx->elv_best, ctx->elv_ties, ctx->counters + 10);
     else
         LAUNCH(k_elvira<false>, grid_for(ncap, 1, 16384), ELV_THREADS, ctx->m, ctx->dp,
                (const double*)ctx->f[MIST_FIELD_VOF], (const int*)ctx->iface_list, (const int*)(ctx->counters + 1), fam,
-               (const double*)ctx->elv_dist, nx, ny, nz, ctx->elv_best);
+               (const double*)ctx->elv_dist, nx, ny, nz, ctx->elv_best, ctx->elv_ties, ctx->counters + 10);
+    LAUNCH(k_elvira_ties, grid_for(ncap, 1, 2048), 64, ctx->m, (const double*)ctx->f[MIST_FIELD_VOF],
+           (const int*)ctx->iface_list, fam, (const double*)ctx->elv_dist, (const int*)ctx->elv_ties,
+           (const int*)(ctx->counters + 10), nx, ny, nz, ctx->elv_best);
     prof_end(ctx, MIST_PROF_ELVIRA, ev, ctx->counters + 1, 0);
     int st;
     if ((st = launch_borders(ctx, MIST_FIELD_NX))) return st;
