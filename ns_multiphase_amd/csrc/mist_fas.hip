@@ -665,7 +665,7 @@ static int fas_faces(mist_ctx* ctx, FasLevel& L, double* P, double dt) {
 static int fas_A(mist_ctx* ctx, FasLevel& L, double* P, double dt) {
     mist_ctx* c = L.c;
     RET_IF_ST(fas_faces(ctx, L, P, dt));
-    CHILD(launch_divergence(c, dt));
+    CHILD(launch_divergence(c, dt, true));  // the level's band is fixed during a solve
     return MIST_OK;
 }
 
@@ -827,6 +827,7 @@ static int fas_load(mist_ctx* ctx) {
         const size_t bytes = (size_t)(ctx->fn[f][0] * ctx->fn[f][1] * ctx->fn[f][2] * ctx->felem[f]);
         MIST_HIP_TRY(hipMemcpyAsync(c->f[f], ctx->f[f], bytes, hipMemcpyDeviceToDevice, ctx->stream));
     }
+    c->vtx_list_ok = false;  // a new band: the next A! lists its vertices again
     return MIST_OK;
 }
 

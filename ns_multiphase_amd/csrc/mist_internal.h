@@ -63,6 +63,7 @@ struct mist_ctx {
     int* defer_list = nullptr;   // cells for the general cut walk
     void* cut_list = nullptr;    // CutCell entries of the band cells to cut
     int* vtx_list = nullptr;     // vertices of band cells to back-trace (linear vertex index)
+    bool vtx_list_ok = false;    // vtx_list matches the band (set only by calls that reuse the list)
     int* elv_ties = nullptr;     // ELVIRA's near-tied cells: (interface-list index, candidate mask)
     int* counters = nullptr;     // [0] band cells, [1] interface cells, [2] overflow flag, [4] cells
                                  // deferred by the main cut pass, [5] its work queue, [6] cells for
@@ -126,7 +127,9 @@ int launch_borders(mist_ctx* ctx, int field);
 int launch_band(mist_ctx* ctx);
 int launch_normals(mist_ctx* ctx);
 int launch_plic(mist_ctx* ctx);
-int launch_project_vertices(mist_ctx* ctx, double dt);
+// reuse_list: keep the band cells' vertex list of the previous call when vtx_list_ok says the
+// band has not changed since (the SL pressure's repeated A! evaluations on one band)
+int launch_project_vertices(mist_ctx* ctx, double dt, bool reuse_list = false);
 int launch_transport(mist_ctx* ctx, double dt);
 int launch_tree_momentum(mist_ctx* ctx, bool corr, double dt);
 int launch_cut_list(mist_ctx* ctx);
@@ -145,7 +148,7 @@ int launch_surface_force(mist_ctx* ctx);
 int launch_reflect_bc(mist_ctx* ctx, int fu, int fv, int fw);
 int launch_interp_face(mist_ctx* ctx, int fu, int fv, int fw);
 int launch_interp_center(mist_ctx* ctx);
-int launch_divergence(mist_ctx* ctx, double dt);
+int launch_divergence(mist_ctx* ctx, double dt, bool reuse_vertex_list = false);
 // mist_adjoint.hip
 int launch_transport_vjp(mist_ctx* ctx, double dt);
 int vjp_prepare(mist_ctx* ctx);  // what the reverse step's first call sets up on the host

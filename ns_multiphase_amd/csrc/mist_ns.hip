@@ -511,10 +511,10 @@ int launch_interp_center(mist_ctx* ctx) {
     return MIST_OK;
 }
 
-int launch_divergence(mist_ctx* ctx, double dt) {
+int launch_divergence(mist_ctx* ctx, double dt, bool reuse_vertex_list) {
     const bool sl = ctx->p.pressure_scheme == MIST_PS_SEMI_LAGRANGIAN;
     int st;
-    if (sl && (st = launch_project_vertices(ctx, dt))) return st;
+    if (sl && (st = launch_project_vertices(ctx, dt, reuse_vertex_list))) return st;
     const int64_t n = (int64_t)ctx->m.cnx * ctx->m.cny * ctx->m.cnz;
     NSL(k_divergence, ns_grid(n, 256), 256, ctx->m, (int)ctx->p.nband, sl ? 1 : 0,
         (const int16_t*)ctx->f[MIST_FIELD_BAND], (const double*)ctx->pv, (const double*)FLD(MIST_FIELD_UF),

@@ -1597,6 +1597,7 @@ int launch_borders(mist_ctx* ctx, int field) {
 }
 
 int launch_band(mist_ctx* ctx) {
+    ctx->vtx_list_ok = false;
     const DevMesh& m = ctx->m;
     const int n = (int)(m.Nx * m.Ny * m.Nz);
     int16_t* band = (int16_t*)ctx->f[MIST_FIELD_BAND];
@@ -1674,12 +1675,15 @@ int launch_plic(mist_ctx* ctx) {
     return launch_borders(ctx, MIST_FIELD_PLIC_D);
 }
 
-int launch_project_vertices(mist_ctx* ctx, double dt) {
+int launch_project_vertices(mist_ctx* ctx, double dt, bool reuse_list) {
     const int64_t nv = (int64_t)(ctx->m.Nx + 1) * (ctx->m.Ny + 1) * (ctx->m.Nz + 1);
     hipEvent_t ev = prof_begin(ctx, MIST_PROF_PROJECT);
-    LAUNCH(k_fill_int, 1, 64, ctx->counters + 9, 1, 0);
-    LAUNCH(k_vertex_list, grid_for(nv, 256, 8192), 256, ctx->m, ctx->dp, (const int16_t*)ctx->f[MIST_FIELD_BAND],
-           ctx->vtx_list, ctx->counters + 9);
+    if (!(reuse_list && ctx->vtx_list_ok)) {
+        LAUNCH(k_fill_int, 1, 64, ctx->counters + 9, 1, 0);
+        LAUNCH(k_vertex_list, grid_for(nv, 256, 8192), 256, ctx->m, ctx->dp, (const int16_t*)ctx->f[MIST_FIELD_BAND],
+               ctx->vtx_list, ctx->counters + 9);
+    }
+    ctx->vtx_list_ok = reuse_list;
     LAUNCH(k_project_list, grid_for(nv, 256, 2048), 256, ctx->m, ctx->dp, (const int*)ctx->vtx_list,
            (const int*)(ctx->counters + 9), (const double*)ctx->f[MIST_FIELD_UF], (const double*)ctx->f[MIST_FIELD_VF],
            (const double*)ctx->f[MIST_FIELD_WF], dt, ctx->pv);
