@@ -426,7 +426,8 @@ __device__ __forceinline__ void elvira_normal(const DevMesh& m, int i, int j, in
 // dist[27 cidx + c]. Its own kernel: the closed-form root's registers would otherwise spill
 // in the scoring kernel's loops.
 __global__ __launch_bounds__(256) void k_elvira_dist(DevMesh m, DevParams p, const double* VF, const int* list,
-                                                     const int* count, int fam_mask, double* dist) {
+                                                     const int* count, int fam_mask, double* dist, int* ntie) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *ntie = 0;  // k_elvira, which follows, appends the near ties
     const int64_t n = (int64_t)(*count) * 27;
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
         const int cidx = (int)(q / 27), c = (int)(q - 27 * (int64_t)cidx);
@@ -658,10 +659,14 @@ __global__ __launch_bounds__(ELV_THREADS, 4) void k_elvira(DevMesh m, DevParams 
                               sC[lane][1] == sC[bl][1] && sC[lane][2] == sC[bl][2];
             const unsigned long long tmask = __ballot(same);
             const int tie = tmask ? __ffsll((long long)tmask) - 1 : -1;
-            // candidates whose errors lie within the rounding of the cut volumes of the minimum:
-            // the reference's own arithmetic decides between them (k_elvira_ties)
+            // candidates with another normal whose errors lie within the rounding of the cut
+            // volumes of the minimum: the reference's own arithmetic decides between them and the
+            // best (k_elvira_ties). A candidate with the very same normal has the very same error in
+            // any arithmetic, and the lowest-numbered one is kept either way.
             const double slack = 1e-11 * be + 1e-15 * sqrt(be) + 1e-30;
-            const unsigned long long fmask = __ballot(bl < 64 && lane < 27 && sOk[lane] && err <= be + slack);
+            const bool near = bl < 64 && lane < 27 && lane != bl && sOk[lane] && err <= be + slack &&
+                              !(sC[lane][0] == sC[bl][0] && sC[lane][1] == sC[bl][1] && sC[lane][2] == sC[bl][2]);
+            const unsigned long long fmask = __ballot(near) | (bl < 64 ? (1ull << bl) : 0ull);
             if (lane == 0) {
                 if (bl < 64) {
                     const int64_t c = m.cell(i, j, k);
@@ -1645,8 +1650,7 @@ int launch_normals(mist_ctx* ctx) {
     }
     hipEvent_t ev = prof_begin(ctx, MIST_PROF_ELVIRA);  // both kernels of ELVIRA
     LAUNCH(k_elvira_dist, grid_for(ncap * 27, 256, 8192), 256, ctx->m, ctx->dp, (const double*)ctx->f[MIST_FIELD_VOF],
-           (const int*)ctx->iface_list, (const int*)(ctx->counters + 1), fam, ctx->elv_dist);
-    LAUNCH(k_fill_int, 1, 64, ctx->counters + 10, 1, 0);  // near-tied cells
+           (const int*)ctx->iface_list, (const int*)(ctx->counters + 1), fam, ctx->elv_dist, ctx->counters + 10);
     if (vol_is_pow2((ctx->m.dx * ctx->m.dy) * ctx->m.dz))
         LAUNCH(k_elvira<true>, grid_for(ncap, 1, 16384), ELV_THREADS, ctx->m, ctx->dp,
                (const double*)ctx->f[MIST_FIELD_VOF], (const int*)ctx->iface_list, (const int*)(ctx->counters + 1), fam,
