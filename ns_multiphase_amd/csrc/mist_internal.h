@@ -68,7 +68,8 @@ struct mist_ctx {
     int* counters = nullptr;     // [0] band cells, [1] interface cells, [2] overflow flag, [4] cells
                                  // deferred by the main cut pass, [5] its work queue, [6] cells for
                                  // the general walk, [7] the wide pass's work queue, [8] cells to
-                                 // cut, [9] vertices to back-trace, [10] ELVIRA's near-tied cells
+                                 // cut, [9] vertices to back-trace, [10] ELVIRA's near-tied cells,
+                                 // [11] ELVIRA's work queue
     double* red = nullptr;       // reduction partials
     int red_blocks = 0;
     int* h_counters = nullptr;   // pinned

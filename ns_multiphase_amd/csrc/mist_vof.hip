@@ -426,8 +426,12 @@ __device__ __forceinline__ void elvira_normal(const DevMesh& m, int i, int j, in
 // dist[27 cidx + c]. Its own kernel: the closed-form root's registers would otherwise spill
 // in the scoring kernel's loops.
 __global__ __launch_bounds__(256) void k_elvira_dist(DevMesh m, DevParams p, const double* VF, const int* list,
-                                                     const int* count, int fam_mask, double* dist, int* ntie) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *ntie = 0;  // k_elvira, which follows, appends the near ties
+                                                     const int* count, int fam_mask, double* dist, int* ntie,
+                                                     int* queue) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *ntie = 0;   // k_elvira, which follows, appends the near ties
+        *queue = 0;  // and takes its cells from this counter
+    }
     const int64_t n = (int64_t)(*count) * 27;
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
         const int cidx = (int)(q / 27), c = (int)(q - 27 * (int64_t)cidx);
@@ -462,12 +466,13 @@ __global__ __launch_bounds__(256) void k_elvira_dist(DevMesh m, DevParams p, con
 // errors tie to rounding may be picked differently from the reference (either is its result).
 constexpr int ELV_THREADS = 256;
 constexpr int ELV_PAIRS = 27 * 9;  // candidate-neighbour pairs of one z-plane of neighbours
+constexpr int ELV_GRID = 1024;     // work-groups of the scoring kernel: four per CU of 256
 
 template <bool POW2>
 __global__ __launch_bounds__(ELV_THREADS, 4) void k_elvira(DevMesh m, DevParams p, const double* VF, const int* list,
                                                         const int* count, int fam_mask, const double* dist,
                                                         double* nxo, double* nyo, double* nzo, int* best, int* ties,
-                                                        int* ntie) {
+                                                        int* ntie, int* queue) {
     __shared__ double sVF[27];
     __shared__ double sC[27][4];
     __shared__ int sOk[27];
@@ -483,10 +488,17 @@ __global__ __launch_bounds__(ELV_THREADS, 4) void k_elvira(DevMesh m, DevParams 
     // faces x[i-1..i+2], y[j-1..j+2], z[k-1..k+2] and centres xm[i-1..i+1], ym, zm of the cell's
     // neighbourhood: the pair and cut steps read them from LDS
     __shared__ double sF[3][4], sM[3][3];
+    __shared__ int sCell;
     const int tid = threadIdx.x;
     const int ncell = *count;
     const VolScale<POW2> vscale((m.dx * m.dy) * m.dz);
-    for (int cidx = blockIdx.x; cidx < ncell; cidx += gridDim.x) {
+    for (;;) {
+        // the next cell from the queue: a work-group per resident slot, each taking cells until
+        // the list is done (the barrier at the loop's end orders this write after every read)
+        if (tid == 0) sCell = atomicAdd(queue, 1);
+        __syncthreads();
+        const int cidx = sCell;
+        if (cidx >= ncell) break;
         int i, j, k;
         interior_of(m, list[cidx], i, j, k);
         if (tid < 27) {
@@ -1650,15 +1662,21 @@ int launch_normals(mist_ctx* ctx) {
     }
     hipEvent_t ev = prof_begin(ctx, MIST_PROF_ELVIRA);  // both kernels of ELVIRA
     LAUNCH(k_elvira_dist, grid_for(ncap * 27, 256, 8192), 256, ctx->m, ctx->dp, (const double*)ctx->f[MIST_FIELD_VOF],
-           (const int*)ctx->iface_list, (const int*)(ctx->counters + 1), fam, ctx->elv_dist, ctx->counters + 10);
+           (const int*)ctx->iface_list, (const int*)(ctx->counters + 1), fam, ctx->elv_dist, ctx->counters + 10,
+           ctx->counters + 11);
+    // the scoring kernel's work-groups take cells from a queue: as many as stay resident (its
+    // registers allow four per CU), each paying its register-spill prologue once
+    const int elv_grid = grid_for(ncap, 1, ELV_GRID);
     if (vol_is_pow2((ctx->m.dx * ctx->m.dy) * ctx->m.dz))
-        LAUNCH(k_elvira<true>, grid_for(ncap, 1, 16384), ELV_THREADS, ctx->m, ctx->dp,
+        LAUNCH(k_elvira<true>, elv_grid, ELV_THREADS, ctx->m, ctx->dp,
                (const double*)ctx->f[MIST_FIELD_VOF], (const int*)ctx->iface_list, (const int*)(ctx->counters + 1), fam,
-               (const double*)ctx->elv_dist, nx, ny, nz, ctx->elv_best, ctx->elv_ties, ctx->counters + 10);
+               (const double*)ctx->elv_dist, nx, ny, nz, ctx->elv_best, ctx->elv_ties, ctx->counters + 10,
+               ctx->counters + 11);
     else
-        LAUNCH(k_elvira<false>, grid_for(ncap, 1, 16384), ELV_THREADS, ctx->m, ctx->dp,
+        LAUNCH(k_elvira<false>, elv_grid, ELV_THREADS, ctx->m, ctx->dp,
                (const double*)ctx->f[MIST_FIELD_VOF], (const int*)ctx->iface_list, (const int*)(ctx->counters + 1), fam,
-               (const double*)ctx->elv_dist, nx, ny, nz, ctx->elv_best, ctx->elv_ties, ctx->counters + 10);
+               (const double*)ctx->elv_dist, nx, ny, nz, ctx->elv_best, ctx->elv_ties, ctx->counters + 10,
+               ctx->counters + 11);
     LAUNCH(k_elvira_ties, grid_for(ncap, 1, 2048), 64, ctx->m, (const double*)ctx->f[MIST_FIELD_VOF],
            (const int*)ctx->iface_list, fam, (const double*)ctx->elv_dist, (const int*)ctx->elv_ties,
            (const int*)(ctx->counters + 10), nx, ny, nz, ctx->elv_best);
