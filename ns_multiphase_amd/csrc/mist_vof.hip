@@ -445,8 +445,8 @@ __global__ __launch_bounds__(256) void k_elvira_dist(DevMesh m, DevParams p, con
     }
 }
 
-// ELVIRA_calc! + check_normal! (src/ELVIRA.jl:62-137) with one interface cell per work-group
-// of 256 threads. Threads 0..26 load the 3x3x3 liquid fractions and build the 27 candidate
+// ELVIRA_calc! + check_normal! (src/ELVIRA.jl:62-137) with one interface cell at a time per
+// work-group of 256 threads, the cells taken from a queue. Threads 0..26 load the 3x3x3 liquid fractions and build the 27 candidate
 // normals once; their PLIC constants (computeDist) come from k_elvira_dist, which evaluates the
 // same normals bit for bit. Each candidate's error is the sum over
 // the 27 neighbours, in the reference's order, of (VF - computePLIC2VF)^2, where
