@@ -470,6 +470,43 @@ __host__ __device__ __forceinline__ double cut_canonical_liquid(int cs, const V3
     return (c & (1u << 16)) ? -v : v;
 }
 
+// The share of a tet's volume on the liquid side (d < 0) of the plane, from its four vertex
+// distances snapped by d2case_snap (so none is zero) for a cut case cs (1..14). The point of
+// the tet with barycentric weights w lies at distance sum w_v d_v, and the liquid share is the
+// volume of {sum w_v d_v < 0}: with one liquid vertex (liquid depth a, gas heights g) it is
+// a^3 / prod(a + g); with one gas vertex it is 1 minus the same corner share of that vertex;
+// with two liquid vertices (depths p, q) and two gas ones (heights s, t) it is
+//   (p^2 q^2 + p q (p + q)(s + t) + s t (p^2 + p q + q^2)) / ((p + s)(p + t)(q + s)(q + t)),
+// the sum of the two liquid corner shares over their common denominator. Every term is
+// positive, so the value is accurate to a few rounding errors; times the tet's volume it
+// equals, in exact arithmetic, the sum of the reference's liquid sub-tets (src/VFgeom.jl:55-165).
+__host__ __device__ __forceinline__ double tet_liquid_fraction(int cs, const double (&d)[4]) {
+    const int ngas = __builtin_popcount((unsigned)cs);
+    if (ngas == 2) {
+        // depths of the two liquid vertices and heights of the two gas ones, in vertex order
+        const bool l0 = d[0] < 0.0, l1 = d[1] < 0.0, l2 = d[2] < 0.0;
+        const double p = -(l0 ? d[0] : (l1 ? d[1] : d[2]));
+        const double q = -(d[3] < 0.0 ? d[3] : (l2 ? d[2] : d[1]));
+        const double s = (!l0 ? d[0] : (!l1 ? d[1] : d[2]));
+        const double t = (d[3] > 0.0 ? d[3] : (!l2 ? d[2] : d[1]));
+        const double pq = p * q;
+        const double num = (pq * pq + (pq * (p + q)) * (s + t)) + (s * t) * ((p * p + pq) + q * q);
+        const double den = ((p + s) * (p + t)) * ((q + s) * (q + t));
+        return num / den;
+    }
+    // one vertex alone on its side: its depth or height a and the corner share a^3 / prod(a + |d|)
+    const bool lone_liquid = ngas == 3;
+    double a = 0.0;
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+        if ((d[v] < 0.0) == lone_liquid) a = fabs(d[v]);
+    double den = 1.0;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) den *= ((d[v] < 0.0) == lone_liquid) ? 1.0 : (a + fabs(d[v]));
+    const double corner = ((a * a) * a) / den;
+    return lone_liquid ? corner : 1.0 - corner;
+}
+
 // (ts * v) / cellvol, the reference's per-sub-tet term of computePLIC2VF (ts = +-1). When the
 // cell volume is a power of two (vol_is_pow2) the quotient is exact scaling, so v * (ts /
 // cellvol) rounds to the same double (both are the correctly rounded value of ts v 2^-e)

@@ -77,6 +77,7 @@ struct mist_ctx {
     int tree_cus = 0;            // compute units of the device
     int tree_blocks_mom = 0;     // resident blocks of the main cut pass with momentum sums
     int tree_blocks_adj = 0;     // resident blocks of the reverse cut pass
+    int elv_blocks = 0;          // resident blocks of ELVIRA's scoring kernel
     double* ns_sums = nullptr;   // 6 sums per interior cell of the momentum predictor's cut
     double* h_red = nullptr;     // pinned
     // mist_vof_step with deferred checks: the velocity maxima the next step's dt needs are

@@ -446,30 +446,32 @@ __global__ __launch_bounds__(256) void k_elvira_dist(DevMesh m, DevParams p, con
 }
 
 // ELVIRA_calc! + check_normal! (src/ELVIRA.jl:62-137) with one interface cell at a time per
-// work-group of 256 threads, the cells taken from a queue. Threads 0..26 load the 3x3x3 liquid fractions and build the 27 candidate
-// normals once; their PLIC constants (computeDist) come from k_elvira_dist, which evaluates the
-// same normals bit for bit. Each candidate's error is the sum over
-// the 27 neighbours, in the reference's order, of (VF - computePLIC2VF)^2, where
+// work-group of 256 threads, the cells taken from a queue. Threads 0..26 load the 3x3x3 liquid
+// fractions and build the 27 candidate normals once; their PLIC constants (computeDist) come
+// from k_elvira_dist, which evaluates the same normals bit for bit. Each candidate's error is
+// the sum over the 27 neighbours, in the reference's order, of (VF - computePLIC2VF)^2, where
 // computePLIC2VF adds, tet by tet, the liquid sub-tet volumes of the cut (src/VF.jl:219-278).
 //
-// The neighbours are scored a z-plane (9 neighbours, 243 candidate-neighbour pairs) at a
-// time. (A) one thread per pair finds the d2case case of its five tets; a plane missing the
-// cell by a wide margin gives the exact all-gas / all-liquid value at once, and an all-liquid
-// tet's single sub-tet is its whole volume, computed once per neighbour. (B) the other tets
-// are bucketed by case, in the order of the three bodies below, so (C) the threads that cut
-// them mostly run one body per wave: a tet's liquid volume is that of the reference's sub-tets
-// on vertices renumbered onto case 1 or case 3 (cut_canonical_liquid; each side needs one
-// body), equal to the reference's fourteen-case sum to rounding. (D) one thread per pair adds
-// the five tets' terms in the reference's tet order, and one thread per candidate adds the
-// squared errors in neighbour order. The best candidate is the lowest-numbered one with the
-// smallest finite error, which is what the reference's strict `<` scan keeps; candidates whose
-// errors tie to rounding may be picked differently from the reference (either is its result).
+// The threads score the 729 candidate-neighbour pairs in turn. A plane that misses the
+// neighbour by a wide margin gives the exact all-gas / all-liquid value at once. Otherwise the
+// plane's distances at the neighbour's 8 corners (the products and sums the reference forms at
+// each tet vertex, so each tet's four distances are its values bit for bit) give each of the
+// five tets its d2case case: an all-liquid tet adds its whole term (ts vol / cellvol, computed
+// once per neighbour) and a cut tet that term times its liquid share in closed form
+// (tet_liquid_fraction). That equals the reference's sub-tet sum in exact arithmetic; in
+// floating point the two differ by the rounding of the reference's sub-tet determinants, about
+// 1e-16 |x| / h per neighbour. One thread per candidate adds the squared errors in neighbour
+// order. The best candidate is the lowest-numbered one with the smallest finite error (the
+// reference's strict `<` scan); a candidate with another normal whose error lies within what
+// that deviation can change of the best error is a near tie, and the reference's own
+// arithmetic decides between them (k_elvira_ties).
 constexpr int ELV_THREADS = 256;
-constexpr int ELV_PAIRS = 27 * 9;  // candidate-neighbour pairs of one z-plane of neighbours
-constexpr int ELV_GRID = 1024;     // work-groups of the scoring kernel: four per CU of 256
+constexpr int ELV_PAIRS = 27 * 27;  // candidate-neighbour pairs
+// Six work-groups per CU (80 VGPRs): the kernel arguments spill to scratch in the prologue,
+// once per resident work-group; four (123 VGPRs, no spills) were 2% slower.
 
 template <bool POW2>
-__global__ __launch_bounds__(ELV_THREADS, 4) void k_elvira(DevMesh m, DevParams p, const double* VF, const int* list,
+__global__ __launch_bounds__(ELV_THREADS, 6) void k_elvira(DevMesh m, DevParams p, const double* VF, const int* list,
                                                         const int* count, int fam_mask, const double* dist,
                                                         double* nxo, double* nyo, double* nzo, int* best, int* ties,
                                                         int* ntie, int* queue) {
@@ -478,20 +480,16 @@ __global__ __launch_bounds__(ELV_THREADS, 4) void k_elvira(DevMesh m, DevParams 
     __shared__ int sOk[27];
     __shared__ double sT[27][5];      // all-liquid term (ts * vol(tet)) / cellvol per neighbour and tet
     __shared__ double sFull[27];      // computePLIC2VF of an all-liquid neighbour
-    __shared__ double sTerm[ELV_PAIRS][5];
-    __shared__ unsigned char sCase[ELV_PAIRS][5];
-    __shared__ unsigned char sKind[ELV_PAIRS];  // 0: cut tets, 1: all gas, 2: all liquid
-    __shared__ unsigned short sItem[ELV_PAIRS * 5];
-    __shared__ int sCount[16], sBase[16];
-    __shared__ double sE[ELV_PAIRS];
+    __shared__ double sE[ELV_PAIRS];  // squared error of each candidate-neighbour pair
     __shared__ double sErr[27];
     // faces x[i-1..i+2], y[j-1..j+2], z[k-1..k+2] and centres xm[i-1..i+1], ym, zm of the cell's
-    // neighbourhood: the pair and cut steps read them from LDS
+    // neighbourhood: the pair step reads them from LDS
     __shared__ double sF[3][4], sM[3][3];
     __shared__ int sCell;
     const int tid = threadIdx.x;
     const int ncell = *count;
     const VolScale<POW2> vscale((m.dx * m.dy) * m.dz);
+    const double hmin = jmin(jmin(m.dx, m.dy), m.dz);
     for (;;) {
         // the next cell from the queue: a work-group per resident slot, each taking cells until
         // the list is done (the barrier at the loop's end orders this write after every read)
@@ -504,7 +502,6 @@ __global__ __launch_bounds__(ELV_THREADS, 4) void k_elvira(DevMesh m, DevParams 
         if (tid < 27) {
             const int ii = i - 1 + tid % 3, jj = j - 1 + (tid / 3) % 3, kk = k - 1 + tid / 9;
             sVF[tid] = VF[m.vof(ii, jj, kk)];
-            sErr[tid] = 0.0;
         }
         if (tid >= 160 && tid < 181) {
             const int e = tid - 160;
@@ -542,117 +539,52 @@ __global__ __launch_bounds__(ELV_THREADS, 4) void k_elvira(DevMesh m, DevParams 
             sOk[tid] = ok;
         }
         __syncthreads();
-        for (int plane = 0; plane < 3; ++plane) {
-            if (tid < 16) sCount[tid] = 0;
-            __syncthreads();
-            // (A) classify the five tets of each pair
-            if (tid < ELV_PAIRS) {
-                const int c = tid % 27, nb = 9 * plane + tid / 27;
-                int kind = 1;
-                if (sOk[c]) {
-                    const double cnx = sC[c][0], cny = sC[c][1], cnz = sC[c][2], dist = sC[c][3];
-                    const int oi = nb % 3, oj = (nb / 3) % 3, ol = nb / 9;
-                    const int ii = i - 1 + oi, jj = j - 1 + oj, kk = k - 1 + ol;
-                    const double hx = fabs(cnx) * m.dx, hy = fabs(cny) * m.dy, hz = fabs(cnz) * m.dz;
-                    const double xc = sM[0][oi], yc = sM[1][oj], zc = sM[2][ol];
-                    const double dcen = ((cnx * xc + cny * yc) + cnz * zc) - dist;
-                    const double hext = 0.5 * ((hx + hy) + hz);
-                    const double margin = 1e-9 * (((1.0 + fabs(dist)) + fabs(xc)) + (fabs(yc) + fabs(zc)));
-                    if (dcen - hext > margin) {
-                        kind = 1;
-                    } else if (dcen + hext < -margin) {
-                        kind = 2;
-                    } else {
-                        kind = 0;
-                        const bool even = ((ii + jj + kk) % 2) == 0;
-                        // d of the cell's 8 reference vertices, each the expression the tets'
-                        // vertices evaluate ((nx x + ny y) + nz z) - dist with the same products
-                        // (odd cells mirror x), then each tet's four taken from them
-                        const double px0 = cnx * sF[0][oi], px1 = cnx * sF[0][oi + 1];
-                        const double py0 = cny * sF[1][oj], py1 = cny * sF[1][oj + 1];
-                        const double pz0 = cnz * sF[2][ol], pz1 = cnz * sF[2][ol + 1];
-                        const double qx0 = even ? px0 : px1, qx1 = even ? px1 : px0;
-                        double e8[8];
-#pragma unroll
-                        for (int v = 0; v < 8; ++v)
-                            e8[v] = (((v & 1) ? qx1 : qx0) + ((v & 2) ? py1 : py0) + ((v & 4) ? pz1 : pz0)) - dist;
-                        sfor<0, 5>([&](auto T) {
-                            constexpr int t = decltype(T)::value;
-                            constexpr unsigned tv = CutTab::tet5[t];
-                            double d[4] = {e8[tv & 15], e8[(tv >> 4) & 15], e8[(tv >> 8) & 15], e8[(tv >> 12) & 15]};
-                            const int cs = d2case_snap(d);
-                            sCase[tid][t] = (unsigned char)cs;
-                            if (cs == 0) sTerm[tid][t] = sT[nb][t];
-                            else if (cs != 15) atomicAdd(&sCount[cs], 1);
-                        });
-                    }
-                }
-                sKind[tid] = (unsigned char)kind;
-            }
-            __syncthreads();
-            // (B) bucket the cut tets by case
-            if (tid == 0) {
-                int b = 0;
-                // buckets in the order of the three case bodies: three cut vertices with the
-                // lone vertex above the plane, with it below, four cut vertices
-                constexpr unsigned char ord[16] = {0, 1, 2, 4, 8, 14, 13, 11, 7, 3, 5, 6, 9, 10, 12, 15};
-                for (int r = 0; r < 16; ++r) {
-                    const int q = ord[r];
-                    sBase[q] = b;
-                    b += sCount[q];
-                }
-                sCount[0] = b;  // total; sCount[1..14] become the fill cursors
-                for (int q = 1; q < 15; ++q) sCount[q] = sBase[q];
-            }
-            __syncthreads();
-            const int nitem = sCount[0];
-            if (tid < ELV_PAIRS && sKind[tid] == 0) {
-                for (int t = 0; t < 5; ++t) {
-                    const int cs = sCase[tid][t];
-                    if (cs != 0 && cs != 15) sItem[atomicAdd(&sCount[cs], 1)] = (unsigned short)(tid * 5 + t);
-                }
-            }
-            __syncthreads();
-            // (C) cut the bucketed tets
-            for (int q = tid; q < nitem; q += ELV_THREADS) {
-                const int item = sItem[q];
-                const int pr = item / 5, t = item - 5 * pr;
-                const int c = pr % 27, nb = 9 * plane + pr / 27;
+        for (int pr = tid; pr < ELV_PAIRS; pr += ELV_THREADS) {
+            const int c = pr % 27, nb = pr / 27;
+            double vf_ = 0.0;
+            if (sOk[c]) {
                 const double cnx = sC[c][0], cny = sC[c][1], cnz = sC[c][2], dist = sC[c][3];
                 const int oi = nb % 3, oj = (nb / 3) % 3, ol = nb / 9;
-                const bool even = (((i - 1 + oi) + (j - 1 + oj) + (k - 1 + ol)) % 2) == 0;
-                const double ts = even ? 1.0 : -1.0;
-                V3 P[4];
-                tet5_vertices(t, even, sF[0][oi], sF[0][oi + 1], sF[1][oj], sF[1][oj + 1], sF[2][ol], sF[2][ol + 1], P);
-                double d[4];
-#pragma unroll
-                for (int n = 0; n < 4; ++n) d[n] = ((cnx * P[n].x + cny * P[n].y) + cnz * P[n].z) - dist;
-                const int cs = d2case_snap(d);
-                sTerm[pr][t] = vscale(ts, cut_canonical_liquid(cs, P, d));
-            }
-            __syncthreads();
-            // (D) computePLIC2VF of each pair in the reference's order, then the errors
-            if (tid < ELV_PAIRS) {
-                const int nb = 9 * plane + tid / 27;
-                const int kind = sKind[tid];
-                double vf_ = 0.0;
-                if (kind == 2) {
+                const double hx = fabs(cnx) * m.dx, hy = fabs(cny) * m.dy, hz = fabs(cnz) * m.dz;
+                const double xc = sM[0][oi], yc = sM[1][oj], zc = sM[2][ol];
+                const double dcen = ((cnx * xc + cny * yc) + cnz * zc) - dist;
+                const double hext = 0.5 * ((hx + hy) + hz);
+                const double margin = 1e-9 * (((1.0 + fabs(dist)) + fabs(xc)) + (fabs(yc) + fabs(zc)));
+                if (dcen + hext < -margin) {
                     vf_ = sFull[nb];
-                } else if (kind == 0) {
-                    for (int t = 0; t < 5; ++t)
-                        if (sCase[tid][t] != 15) vf_ += sTerm[tid][t];
+                } else if (!(dcen - hext > margin)) {
+                    const bool even = (((i - 1 + oi) + (j - 1 + oj) + (k - 1 + ol)) % 2) == 0;
+                    // d of the cell's 8 reference vertices, each the expression the tets'
+                    // vertices evaluate ((nx x + ny y) + nz z) - dist with the same products
+                    // (odd cells mirror x), then each tet's four taken from them
+                    const double px0 = cnx * sF[0][oi], px1 = cnx * sF[0][oi + 1];
+                    const double py0 = cny * sF[1][oj], py1 = cny * sF[1][oj + 1];
+                    const double pz0 = cnz * sF[2][ol], pz1 = cnz * sF[2][ol + 1];
+                    const double qx0 = even ? px0 : px1, qx1 = even ? px1 : px0;
+                    double e8[8];
+#pragma unroll
+                    for (int v = 0; v < 8; ++v)
+                        e8[v] = (((v & 1) ? qx1 : qx0) + ((v & 2) ? py1 : py0) + ((v & 4) ? pz1 : pz0)) - dist;
+                    sfor<0, 5>([&](auto T) {
+                        constexpr int t = decltype(T)::value;
+                        constexpr unsigned tv = CutTab::tet5[t];
+                        double d[4] = {e8[tv & 15], e8[(tv >> 4) & 15], e8[(tv >> 8) & 15], e8[(tv >> 12) & 15]};
+                        const int cs = d2case_snap(d);
+                        if (cs == 0) vf_ += sT[nb][t];
+                        else if (cs != 15) vf_ += sT[nb][t] * tet_liquid_fraction(cs, d);
+                    });
                 }
-                const double df = sVF[nb] - vf_;
-                sE[tid] = df * df;
             }
-            __syncthreads();
-            if (tid < 27) {
-                double e = sErr[tid];
-                for (int q = 0; q < 9; ++q) e += sE[q * 27 + tid];
-                sErr[tid] = e;
-            }
-            __syncthreads();
+            const double df = sVF[nb] - vf_;
+            sE[pr] = df * df;
         }
+        __syncthreads();
+        if (tid < 27) {
+            double e = 0.0;
+            for (int nb = 0; nb < 27; ++nb) e += sE[27 * nb + tid];
+            sErr[tid] = e;
+        }
+        __syncthreads();
         if (tid < 32) {
             const int lane = tid;
             double err = __builtin_huge_val();
@@ -671,11 +603,17 @@ __global__ __launch_bounds__(ELV_THREADS, 4) void k_elvira(DevMesh m, DevParams 
                               sC[lane][1] == sC[bl][1] && sC[lane][2] == sC[bl][2];
             const unsigned long long tmask = __ballot(same);
             const int tie = tmask ? __ffsll((long long)tmask) - 1 : -1;
-            // candidates with another normal whose errors lie within the rounding of the cut
-            // volumes of the minimum: the reference's own arithmetic decides between them and the
-            // best (k_elvira_ties). A candidate with the very same normal has the very same error in
-            // any arithmetic, and the lowest-numbered one is kept either way.
-            const double slack = 1e-11 * be + 1e-15 * sqrt(be) + 1e-30;
+            // candidates with another normal whose errors lie within what the volume fractions'
+            // deviation from the reference's arithmetic can change of the minimum: the reference's
+            // own arithmetic decides between them and the best (k_elvira_ties). A deviation dv
+            // per neighbour moves an error E = sum of 27 squares by at most 2 dv sqrt(27 E) +
+            // 27 dv^2, and two candidates' errors twice that; dv is ten times the measured
+            // 1e-16 |x| / h. A candidate with the very same normal has the very same error in any
+            // arithmetic, and the lowest-numbered one is kept either way.
+            double R = 0.0;
+            for (int a = 0; a < 3; ++a) R = jmax(R, jmax(fabs(sF[a][0]), fabs(sF[a][3])));
+            const double dv = 1e-15 * (R / hmin + 1.0);
+            const double slack = (1e-11 * be + (21.0 * dv) * sqrt(be)) + (54.0 * dv) * dv;
             const bool near = bl < 64 && lane < 27 && lane != bl && sOk[lane] && err <= be + slack &&
                               !(sC[lane][0] == sC[bl][0] && sC[lane][1] == sC[bl][1] && sC[lane][2] == sC[bl][2]);
             const unsigned long long fmask = __ballot(near) | (bl < 64 ? (1ull << bl) : 0ull);
@@ -1664,9 +1602,15 @@ int launch_normals(mist_ctx* ctx) {
     LAUNCH(k_elvira_dist, grid_for(ncap * 27, 256, 8192), 256, ctx->m, ctx->dp, (const double*)ctx->f[MIST_FIELD_VOF],
            (const int*)ctx->iface_list, (const int*)(ctx->counters + 1), fam, ctx->elv_dist, ctx->counters + 10,
            ctx->counters + 11);
-    // the scoring kernel's work-groups take cells from a queue: as many as stay resident (its
-    // registers allow four per CU), each paying its register-spill prologue once
-    const int elv_grid = grid_for(ncap, 1, ELV_GRID);
+    // the scoring kernel's work-groups take cells from a queue: as many as stay resident
+    if (!ctx->elv_blocks) {
+        int per = 0, dev = 0, cus = 0;
+        MIST_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_elvira<true>, ELV_THREADS, 0));
+        MIST_HIP_TRY(hipGetDevice(&dev));
+        MIST_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        ctx->elv_blocks = std::max(1, per) * std::max(1, cus);
+    }
+    const int elv_grid = grid_for(ncap, 1, ctx->elv_blocks);
     if (vol_is_pow2((ctx->m.dx * ctx->m.dy) * ctx->m.dz))
         LAUNCH(k_elvira<true>, elv_grid, ELV_THREADS, ctx->m, ctx->dp,
                (const double*)ctx->f[MIST_FIELD_VOF], (const int*)ctx->iface_list, (const int*)(ctx->counters + 1), fam,

@@ -100,8 +100,9 @@ void h_leaf_adj(const double* x, const double* n, double D, double gL, double gV
 }
 
 // liquid and total volume of tet x cut by n.x = D: out[0..1] by cut_dispatch, out[2..3] by
-// cut_canonical, out[4] the snapped case, out[5] the liquid volume by cut_canonical_liquid
-// (out holds 6 doubles)
+// cut_canonical, out[4] the snapped case, out[5] the liquid volume by cut_canonical_liquid,
+// out[6] tet_liquid_fraction times the tet's volume and out[7] cut_dispatch's liquid volume,
+// both on the snapped distances (out holds 8 doubles)
 void h_cut_compare(const double* x, const double* n, double D, double* out) {
     V3 P[4];
     double d[4], dd[4];
@@ -116,6 +117,36 @@ void h_cut_compare(const double* x, const double* n, double D, double* out) {
     cut_canonical(cs, P, d, [&](double v) { Lc += v; }, [&](double v) { Gc += v; });
     out[0] = L; out[1] = L + G; out[2] = Lc; out[3] = Lc + Gc; out[4] = cs;
     out[5] = cut_canonical_liquid(cs, P, d);
+    const double tv = tet_vol3(P[0], P[1], P[2], P[3]);
+    out[6] = cs == 0 ? tv : (cs == 15 ? 0.0 : tv * tet_liquid_fraction(cs, dd));
+    double Ls = 0.0;
+    cut_dispatch<false>(cs, P, dd, [&](double v) { Ls += v; }, [](double) {});
+    out[7] = Ls;
+}
+
+// computePLIC2VF of the cell [x0,x1]x[y0,y1]x[z0,z1] (corners numbered x fastest, x mirrored
+// in odd cells, the reference's 5-tet decomposition) for the plane n.x = D: out[0] with the
+// reference's fourteen case bodies on the snapped distances, out[1] as the ELVIRA scoring
+// kernel forms it, each cut tet's liquid share (tet_liquid_fraction) times its whole term
+void h_cell_vf_compare(const double* box, const double* n, double D, int even, double* out) {
+    const double cellvol = ((box[1] - box[0]) * (box[3] - box[2])) * (box[5] - box[4]);
+    const double ts = even ? 1.0 : -1.0;
+    double ref = 0.0, frac = 0.0;
+    for (int t = 0; t < 5; ++t) {
+        V3 P[4];
+        double d[4];
+        for (int v = 0; v < 4; ++v) {
+            const int c = vert_corner((CutTab::tet5[t] >> (4 * v)) & 15, even != 0);
+            P[v] = V3{(c & 1) ? box[1] : box[0], (c & 2) ? box[3] : box[2], (c & 4) ? box[5] : box[4]};
+            d[v] = ((n[0] * P[v].x + n[1] * P[v].y) + n[2] * P[v].z) - D;
+        }
+        const int cs = d2case_snap(d);
+        cut_dispatch<false>(cs, P, d, [&](double v) { ref += (ts * v) / cellvol; }, [](double) {});
+        const double whole = (ts * tet_vol3(P[0], P[1], P[2], P[3])) / cellvol;
+        frac += cs == 0 ? whole : (cs == 15 ? 0.0 : whole * tet_liquid_fraction(cs, d));
+    }
+    out[0] = ref;
+    out[1] = frac;
 }
 
 // adjoint of gL * liquid + gV * total volume of the cut of h_cut_compare, with respect to

@@ -51,6 +51,7 @@ def _bind(L):
     L.h_elvira_cand_adj.argtypes = [dp] + [ctypes.c_int] * 4 + [dp, dp]
     L.h_cut_compare.argtypes = [dp, dp, ctypes.c_double, dp]
     L.h_cut_adj_compare.argtypes = [dp, dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp]
+    L.h_cell_vf_compare.argtypes = [dp, dp, ctypes.c_double, ctypes.c_int, dp]
     return L
 
 
@@ -188,7 +189,7 @@ def test_canonical_leaf_cut_matches_reference_cases(H):
         n = rng.standard_normal(3)
         n /= np.linalg.norm(n)
         D = float(n @ x[rng.integers(0, 4)] + 0.3 * rng.standard_normal() * 0.2)
-        out = np.zeros(6)
+        out = np.zeros(8)
         H.h_cut_compare(P(x), P(n), D, P(out))
         assert out[5] == out[2], (int(out[4]), out)
         cs = int(out[4])
@@ -217,7 +218,7 @@ def test_canonical_leaf_cut_adjoint(H):
         gL, gV = rng.standard_normal(2)
         out = np.zeros(32)
         H.h_cut_adj_compare(P(x), P(n), D, gL, gV, P(out))
-        f6 = np.zeros(6)
+        f6 = np.zeros(8)
         H.h_cut_compare(P(x), P(n), D, P(f6))
         cs = int(f6[4])
         seen.add(cs)
@@ -226,7 +227,7 @@ def test_canonical_leaf_cut_adjoint(H):
         d = x @ n - D
         if fd_checked < 300 and np.abs(d).min() > 1e-3:
             def F(z):
-                o = np.zeros(6)
+                o = np.zeros(8)
                 H.h_cut_compare(P(z[:12]), P(z[12:15]), float(z[15]), P(o))
                 return gL * o[2] + gV * o[3]
             z = np.concatenate([x.ravel(), n, [D]])
@@ -235,3 +236,62 @@ def test_canonical_leaf_cut_adjoint(H):
             fd_checked += 1
     assert seen >= set(range(0, 16)), sorted(seen)
     assert fd_checked >= 100
+
+
+def test_tet_liquid_fraction_matches_reference_cases(H):
+    """tet_liquid_fraction (the liquid share of a cut tet in closed form, what ELVIRA's scoring
+    kernel uses) times the tet's volume against the reference's fourteen case bodies
+    (cut_dispatch) on the same snapped distances, for every cut case in both orientations; and
+    against two textbook volumes of the unit right tet: {x < c} (one vertex alone on the gas
+    side) is (1 - (1 - c)^3) / 6 and {x + y < c} (two on each side) is c^2 / 2 - c^3 / 3."""
+    rng = np.random.default_rng(23)
+    seen = set()
+    for trial in range(4000):
+        x = rng.random((4, 3))
+        if trial % 2:
+            x = x[[1, 0, 2, 3]]
+        n = rng.standard_normal(3)
+        n /= np.linalg.norm(n)
+        D = float(n @ x[rng.integers(0, 4)] + 0.06 * rng.standard_normal())
+        out = np.zeros(8)
+        H.h_cut_compare(P(x), P(n), D, P(out))
+        cs = int(out[4])
+        seen.add(cs)
+        tv = out[1]
+        assert abs(out[6] - out[7]) <= 1e-13 * abs(tv) + 1e-16, (cs, out)
+    assert seen >= set(range(1, 15)), sorted(seen)
+    unit = np.array([[0.0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]])
+    for c in (0.1, 0.37, 0.5, 0.83):
+        out = np.zeros(8)
+        H.h_cut_compare(P(unit), P(np.array([1.0, 0, 0])), c, P(out))
+        assert abs(abs(out[6]) - (1 - (1 - c) ** 3) / 6) <= 1e-16, (c, out)
+        r = np.sqrt(0.5)
+        H.h_cut_compare(P(unit), P(np.array([r, r, 0.0])), c * r, P(out))
+        assert abs(abs(out[6]) - (c * c / 2 - c ** 3 / 3)) <= 1e-15, (c, out)
+
+
+@pytest.mark.parametrize("N", [64, 128, 256])
+def test_cell_vf_fraction_form_against_reference_arithmetic(H, N):
+    """computePLIC2VF of a cell as ELVIRA's scoring kernel forms it (each cut tet's liquid
+    share in closed form) against the reference's sub-tet sums, on cells of a unit N^3 mesh with
+    planes through them: the two agree to DV_BOUND, the bound the kernel's near-tie test
+    assumes for one neighbour's volume fraction."""
+    rng = np.random.default_rng(N)
+    h = 1.0 / N
+    worst = 0.0
+    for trial in range(3000):
+        i, j, k = (int(v) for v in rng.integers(0, N, 3))
+        box = np.array([i * h, (i + 1) * h, j * h, (j + 1) * h, k * h, (k + 1) * h])
+        n = rng.standard_normal(3)
+        if trial % 3 == 0:
+            n[rng.integers(0, 3)] *= 1e-6  # nearly axis-aligned planes
+        n /= np.linalg.norm(n)
+        pt = box[::2] + h * rng.random(3)
+        out = np.zeros(2)
+        H.h_cell_vf_compare(P(box), P(n), float(n @ pt), (i + j + k) % 2 == 0, P(out))
+        worst = max(worst, abs(out[1] - out[0]))
+    print(f"N={N}: max |VF(closed form) - VF(reference)| = {worst:.3e}")
+    assert worst <= DV_BOUND * N, worst
+
+
+DV_BOUND = 1e-15  # per unit of N (the reference's sub-tet determinants lose |x| / h digits)
