@@ -619,8 +619,9 @@ __global__ __launch_bounds__(C::NT) void k_adj_tree(DevMesh m, const double* pv,
                         dd[v] = d[v];
                     }
                     const int cs = d2case_snap(dd);
-                    Vs += tet_vol3(T.Q[0], T.Q[1], T.Q[2], T.Q[3]);
-                    Ls += cut_canonical_liquid(cs, T.Q, d);
+                    const double tv = tet_vol3(T.Q[0], T.Q[1], T.Q[2], T.Q[3]);
+                    Vs += tv;
+                    Ls += cs == 0 ? tv : (cs == 15 ? 0.0 : tv * tet_liquid_fraction(cs, dd));
                 }
 #pragma unroll
                 for (int off = 32; off >= 1; off >>= 1) {

@@ -1168,10 +1168,12 @@ __global__ __launch_bounds__(C::NT) void k_sl_tree(DevMesh m, const double* pv, 
                 }
                 const int cs = d2case_snap(dd);
                 if constexpr (!MOM) {
-                    // the leaf's volume is its own tet's (the sum of its pieces to rounding); only
-                    // the liquid side is cut
-                    Vs += tet_vol3(Q[0], Q[1], Q[2], Q[3]);
-                    Ls += cut_canonical_liquid(cs, Q, d);
+                    // the leaf's volume is its own tet's (the sum of its pieces to rounding) and its
+                    // liquid volume that times its liquid share in closed form (the sum of the
+                    // reference's liquid sub-tets in exact arithmetic)
+                    const double tv = tet_vol3(Q[0], Q[1], Q[2], Q[3]);
+                    Vs += tv;
+                    Ls += cs == 0 ? tv : (cs == 15 ? 0.0 : tv * tet_liquid_fraction(cs, dd));
                 } else if (cs == 0 || cs == 15) {  // one sub-tet, the leaf itself (vertex order 1,2,3,4)
                     const double tv = tet_vol3(Q[0], Q[1], Q[2], Q[3]);
                     if (cs == 0) on_liq(tv);
