@@ -13,8 +13,8 @@ MMS in both precisions and 20 deformation steps at 256^3.
 Beside it (`mg`): pressure multigrid solve time to max|b - A p| <= 1e-10 on the 128^3
 variable-density problem (density ratio 1000, exact p = cos cos cos), FP32 V-cycles in FP64
 defect correction vs FP64 V-cycles.
-`roofline`: the ELVIRA kernel (the dominant kernel of the step) in FP64 FLOP/s from HIP
-events on the library's stream over the timed steps; `cpu_baseline`: the oracle on the
+`roofline`: the dominant FP64 kernel of the step by time (the cut pass; `rooflines` holds
+ELVIRA beside it) in FP64 FLOP/s from HIP events on the library's stream over the timed steps; `cpu_baseline`: the oracle on the
 host cores on a bounded sample of the same workload.
 """
 from __future__ import annotations
@@ -45,7 +45,7 @@ def log(msg):
 # pass) from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per dispatch on the timed steps of this
 # workload, FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0 (what the counters read for a plain
 # 256 MiB device copy); see profiles/pmc_traffic_deformation3d_128.md.
-TRAFFIC_PER_UNIT = {"k_elvira_dist + k_elvira": 3420.1, "k_sl_tree": 735.0}
+TRAFFIC_PER_UNIT = {"k_elvira_dist + k_elvira": 3801.9, "k_sl_tree": 721.8}
 
 
 def _sample_steps(nper, k):
