@@ -562,14 +562,27 @@ __global__ __launch_bounds__(ELV_THREADS, 6) void k_elvira(DevMesh m, DevParams 
                     const double pz0 = cnz * sF[2][ol], pz1 = cnz * sF[2][ol + 1];
                     const double qx0 = even ? px0 : px1, qx1 = even ? px1 : px0;
                     double e8[8];
+                    unsigned pos8 = 0;  // corners on the gas side
+                    bool tiny = false;  // a corner d2case would snap
 #pragma unroll
-                    for (int v = 0; v < 8; ++v)
+                    for (int v = 0; v < 8; ++v) {
                         e8[v] = (((v & 1) ? qx1 : qx0) + ((v & 2) ? py1 : py0) + ((v & 4) ? pz1 : pz0)) - dist;
+                        pos8 |= (e8[v] > 0.0 ? 1u : 0u) << v;
+                        tiny = tiny || fabs(e8[v]) < EPS;
+                    }
                     sfor<0, 5>([&](auto T) {
                         constexpr int t = decltype(T)::value;
                         constexpr unsigned tv = CutTab::tet5[t];
                         double d[4] = {e8[tv & 15], e8[(tv >> 4) & 15], e8[(tv >> 8) & 15], e8[(tv >> 12) & 15]};
-                        const int cs = d2case_snap(d);
+                        // with no corner below d2case's snapping threshold its case is the tet's
+                        // corners' sides and the distances stay as they are
+                        int cs;
+                        if (!tiny)
+                            cs = (int)(((pos8 >> (tv & 15)) & 1u) | (((pos8 >> ((tv >> 4) & 15)) & 1u) << 1) |
+                                       (((pos8 >> ((tv >> 8) & 15)) & 1u) << 2) |
+                                       (((pos8 >> ((tv >> 12) & 15)) & 1u) << 3));
+                        else
+                            cs = d2case_snap(d);
                         if (cs == 0) vf_ += sT[nb][t];
                         else if (cs != 15) vf_ += sT[nb][t] * tet_liquid_fraction(cs, d);
                     });
