@@ -136,8 +136,12 @@ __host__ __device__ __forceinline__ double tet_vol(const double* p0, const doubl
     return (-1.0 / 6.0) * ((a1 * (b2 * c3 - c2 * b3) - a2 * (b1 * c3 - c1 * b3)) + a3 * (b1 * c2 - c1 * b2));
 }
 
-// d2case (src/VFgeom.jl:693-708) on a copy held in registers; d is snapped in place.
+// d2case (src/VFgeom.jl:693-708) on a copy held in registers; d is snapped in place. With no
+// distance below the snapping threshold (the common case) the snap changes nothing and the
+// case is the vertices' sides.
 __host__ __device__ __forceinline__ int d2case_snap(double d[4]) {
+    if (!(fabs(d[0]) < EPS || fabs(d[1]) < EPS || fabs(d[2]) < EPS || fabs(d[3]) < EPS))
+        return (d[0] > 0.0) | ((d[1] > 0.0) << 1) | ((d[2] > 0.0) << 2) | ((d[3] > 0.0) << 3);
     int npos = 0, nneg = 0;
 #pragma unroll
     for (int n = 0; n < 4; ++n) { npos += d[n] > 0.0; nneg += d[n] < 0.0; }
