@@ -243,21 +243,33 @@ __global__ void k_borders_axis(T* A, Ext3 e, int d, int mn, int mx, int mno, int
 // computeBand! (src/VF.jl:16-74)
 // ------------------------------------------------------------------------------------
 // One sweep of the band growth. In place: a sweep only lowers |band| from >n to n and only
-// compares neighbours with n-1, so concurrent updates do not change any outcome.
-__global__ void k_band_sweep(DevMesh m, int nstep, int16_t* band) {
+// compares neighbours with n-1, so concurrent updates do not change any outcome. With `list`
+// (the last sweep, n = nband) each cell's value is final once its own update is done, and the
+// cells with |band| <= nband are appended to the band-cell list (what k_band_list gives).
+__global__ void k_band_sweep(DevMesh m, int nstep, int16_t* band, int* list, int* count) {
+    __shared__ int s_app[2];
     const int n = m.Nx * m.Ny * m.Nz;
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-        int i, j, k;
-        interior_of(m, q, i, j, k);
-        const int b = band[m.cell(i, j, k)];
-        if (abs(b) > nstep) {
-            bool hit = false;
-            for (int kk = k - 1; kk <= k + 1; ++kk)
-                for (int jj = j - 1; jj <= j + 1; ++jj)
-                    for (int ii = i - 1; ii <= i + 1; ++ii)
-                        hit |= abs((int)band[m.cell(ii, jj, kk)]) == nstep - 1;
-            if (hit) band[m.cell(i, j, k)] = (int16_t)(nstep * (b > 0 ? 1 : -1));
+    const int nb = (n + blockDim.x - 1) / blockDim.x * blockDim.x;  // whole work-groups take part
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nb; q += gridDim.x * blockDim.x) {
+        bool f = false;
+        if (q < n) {
+            int i, j, k;
+            interior_of(m, q, i, j, k);
+            int b = band[m.cell(i, j, k)];
+            if (abs(b) > nstep) {
+                bool hit = false;
+                for (int kk = k - 1; kk <= k + 1; ++kk)
+                    for (int jj = j - 1; jj <= j + 1; ++jj)
+                        for (int ii = i - 1; ii <= i + 1; ++ii)
+                            hit |= abs((int)band[m.cell(ii, jj, kk)]) == nstep - 1;
+                if (hit) {
+                    b = nstep * (b > 0 ? 1 : -1);
+                    band[m.cell(i, j, k)] = (int16_t)b;
+                }
+            }
+            f = abs(b) <= nstep;
         }
+        if (list) append_group(f, q, list, count, s_app);
     }
 }
 
@@ -1582,12 +1594,16 @@ int launch_band(mist_ctx* ctx) {
     int st = launch_neumann(ctx, MIST_FIELD_BAND);
     if (st) return st;
     if ((st = launch_borders(ctx, MIST_FIELD_BAND))) return st;
+    // the last sweep also lists the band cells
     for (int s = 1; s <= ctx->dp.nband; ++s) {
-        LAUNCH(k_band_sweep, grid_for(n, 256), 256, m, s, band);
+        const bool last = s == ctx->dp.nband;
+        LAUNCH(k_band_sweep, grid_for(n, 256), 256, m, s, band, last ? ctx->band_list : nullptr,
+               last ? ctx->counters + 0 : nullptr);
         if ((st = launch_borders(ctx, MIST_FIELD_BAND))) return st;
     }
-    LAUNCH(k_band_list, grid_for(n, 256), 256, m, ctx->dp.nband, (const int16_t*)band, ctx->band_list,
-           ctx->counters + 0);
+    if (ctx->dp.nband < 1)
+        LAUNCH(k_band_list, grid_for(n, 256), 256, m, ctx->dp.nband, (const int16_t*)band, ctx->band_list,
+               ctx->counters + 0);
     return MIST_OK;
 }
 
