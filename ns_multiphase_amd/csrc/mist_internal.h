@@ -131,7 +131,7 @@ int launch_normals(mist_ctx* ctx);
 int launch_plic(mist_ctx* ctx);
 // reuse_list: keep the band cells' vertex list of the previous call when vtx_list_ok says the
 // band has not changed since (the SL pressure's repeated A! evaluations on one band)
-int launch_project_vertices(mist_ctx* ctx, double dt, bool reuse_list = false);
+int launch_project_vertices(mist_ctx* ctx, double dt, bool reuse_list = false, bool count_cleared = false);
 int launch_transport(mist_ctx* ctx, double dt);
 int launch_tree_momentum(mist_ctx* ctx, bool corr, double dt);
 int launch_cut_list(mist_ctx* ctx);

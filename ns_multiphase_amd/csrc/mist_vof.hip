@@ -308,8 +308,9 @@ __device__ __forceinline__ void band_tile_origin(const BandTile& T, int& i0, int
 }
 
 __global__ __launch_bounds__(256) void k_band_classify_t(DevMesh m, DevParams p, BandTile T, const double* VF,
-                                                         int16_t* band) {
+                                                         int16_t* band, int* count) {
     __shared__ double s[BAND_TILE_LDS];
+    if (blockIdx.x == 0 && threadIdx.x == 0) *count = 0;  // the band-cell list, built after the sweeps
     int i0, j0, k0;
     band_tile_origin(T, i0, j0, k0);
     const int hx = T.tx + 2, hy = T.ty + 2, nh = hx * hy * (T.tz + 2);
@@ -1589,8 +1590,8 @@ int launch_band(mist_ctx* ctx) {
     T.nbx = (m.Nx + T.tx - 1) / T.tx;
     T.nby = (m.Ny + T.ty - 1) / T.ty;
     const int64_t tiles = (int64_t)T.nbx * T.nby * ((m.Nz + T.tz - 1) / T.tz);
-    LAUNCH(k_fill_int, 1, 64, ctx->counters, 1, 0);
-    LAUNCH(k_band_classify_t, (int)tiles, 256, m, ctx->dp, T, (const double*)ctx->f[MIST_FIELD_VOF], band);
+    LAUNCH(k_band_classify_t, (int)tiles, 256, m, ctx->dp, T, (const double*)ctx->f[MIST_FIELD_VOF], band,
+           ctx->counters + 0);
     int st = launch_neumann(ctx, MIST_FIELD_BAND);
     if (st) return st;
     if ((st = launch_borders(ctx, MIST_FIELD_BAND))) return st;
@@ -1672,11 +1673,11 @@ int launch_plic(mist_ctx* ctx) {
     return launch_borders(ctx, MIST_FIELD_PLIC_D);
 }
 
-int launch_project_vertices(mist_ctx* ctx, double dt, bool reuse_list) {
+int launch_project_vertices(mist_ctx* ctx, double dt, bool reuse_list, bool count_cleared) {
     const int64_t nv = (int64_t)(ctx->m.Nx + 1) * (ctx->m.Ny + 1) * (ctx->m.Nz + 1);
     hipEvent_t ev = prof_begin(ctx, MIST_PROF_PROJECT);
     if (!(reuse_list && ctx->vtx_list_ok)) {
-        LAUNCH(k_fill_int, 1, 64, ctx->counters + 9, 1, 0);
+        if (!count_cleared) LAUNCH(k_fill_int, 1, 64, ctx->counters + 9, 1, 0);
         LAUNCH(k_vertex_list, grid_for(nv, 256, 8192), 256, ctx->m, ctx->dp, (const int16_t*)ctx->f[MIST_FIELD_BAND],
                ctx->vtx_list, ctx->counters + 9);
     }
@@ -1689,11 +1690,12 @@ int launch_project_vertices(mist_ctx* ctx, double dt, bool reuse_list) {
 }
 
 int launch_transport(mist_ctx* ctx, double dt) {
-    if (int st = launch_project_vertices(ctx, dt)) return st;
+    // deferred cells, work queues and cut-list size [4..8], and the back-trace's vertex count [9]
+    LAUNCH(k_fill_int, 1, 64, ctx->counters + 4, 6, 0);
+    if (int st = launch_project_vertices(ctx, dt, false, true)) return st;
     hipEvent_t ev;
     const int64_t ncap = (int64_t)ctx->m.Nx * ctx->m.Ny * ctx->m.Nz;
     ev = prof_begin(ctx, MIST_PROF_SL_CUT);
-    LAUNCH(k_fill_int, 1, 64, ctx->counters + 4, 5, 0);  // deferred cells, work queues, cut-list size
     if (!ctx->tree_blocks) {
         int per = 0, dev = 0, cus = 0;
         MIST_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_sl_tree<TreeMain, false>, TreeMain::NT, 0));
