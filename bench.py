@@ -45,7 +45,7 @@ def log(msg):
 # pass) from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per dispatch on the timed steps of this
 # workload, FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0 (what the counters read for a plain
 # 256 MiB device copy); see profiles/pmc_traffic_deformation3d_128.md.
-TRAFFIC_PER_UNIT = {"k_elvira_dist + k_elvira": 3189.6, "k_sl_tree": 723.1}
+TRAFFIC_PER_UNIT = {"k_elvira_dist + k_elvira": 3190.0, "k_sl_tree": 720.7}
 
 
 def _sample_steps(nper, k):
