@@ -653,29 +653,28 @@ def run_adjoint(args, torch):
     target = run.forward(a2, checkpoints=False)
     run.loss_and_grad(a1, target)  # warm-up
     run8.loss_and_grad(a1, target)
-    reps = 3
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        L = run.loss(a1, target)
-    torch.cuda.synchronize()
-    t_fwd = (time.perf_counter() - t0) / reps
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        L2, g = run.loss_and_grad(a1, target)
-    torch.cuda.synchronize()
-    t_grad = (time.perf_counter() - t0) / reps
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        _, g8 = run8.loss_and_grad(a1, target)
-    torch.cuda.synchronize()
-    t_grad8 = (time.perf_counter() - t0) / reps
+    # each time is the median of `reps` synchronised calls (host-side jitter of single calls
+    # of 20-60 ms is a few per cent)
+    reps = 7
+
+    def timed(fn):
+        ts, out = [], None
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            out = fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)), out
+    t_fwd, L = timed(lambda: run.loss(a1, target))
+    t_grad, (L2, g) = timed(lambda: run.loss_and_grad(a1, target))
+    t_grad8, (_, g8) = timed(lambda: run8.loss_and_grad(a1, target))
     # the reverse sweep alone (recomputing each segment from its checkpoint, when the interval
     # is above 1, then walking it back), after a forward that stored the checkpoints
     m = ctx.mesh
     o = m.nghost + 3
     sl = (slice(o, o + m.Nx), slice(o, o + m.Ny), slice(o, o + m.Nz))
-    t_bwd = 0.0
+    tb = []
     for _ in range(reps):
         VF = run.forward(a1)
         diff = np.zeros_like(VF)
@@ -684,7 +683,8 @@ def run_adjoint(args, torch):
         t0 = time.perf_counter()
         run.backward(diff)
         torch.cuda.synchronize()
-        t_bwd += (time.perf_counter() - t0) / reps
+        tb.append(time.perf_counter() - t0)
+    t_bwd = float(np.median(tb))
     fwd = None if args.no_oracle else forward_mode_check(args)
     # central differences at h = 1e-6 and 1e-7: the map is piecewise smooth, and a kink (an
     # ELVIRA choice or a cut case switching) within h of the point makes a direction disagree
@@ -698,6 +698,7 @@ def run_adjoint(args, torch):
     out = {"workload": f"{N}^2 x 1, circle r=0.2 at (0.5,0.5), 16-mode stream function (seed 1 control, seed 2 "
                        f"target), {nsteps} steps, dt={dt:.3e}, checkpoints every {every}",
            "checkpoint_every": every,
+           "timing": f"medians of {reps} synchronised calls each",
            "checkpoint_MB": (run.nseg + every - 1) * A.checkpoint_bytes(m) / 1e6,
            "loss": L, "forward_ms": 1e3 * t_fwd, "loss_and_gradient_ms": 1e3 * t_grad,
            "gradient_cost_vs_forward": t_grad / t_fwd, "reverse_sweep_ms": 1e3 * t_bwd,
